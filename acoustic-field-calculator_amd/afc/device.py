@@ -1,0 +1,81 @@
+"""Device-resident entry points of libafc_hip.so (include/afc_hip.h, part 2).
+
+torch is used only for device memory and streams; every computation runs in
+the library's HIP kernels through the C ABI.
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import byref, c_void_p
+
+import numpy as np
+
+from . import AFCError, dll
+from .nativemethods import AFH_PointSet
+
+SOURCE_STRIDE = 12
+FIELD_PRESSURE, FIELD_POWER, FIELD_COMPLEX = 0, 1, 2
+PREC_FAST, PREC_ACCURATE = 0, 1
+
+
+def _err(what: str, rc: int):
+    if rc != 0:
+        raise AFCError(f"{what} failed ({rc}): {dll().AFC_GetLastError().decode()}")
+
+
+def packed_count(n: int) -> int:
+    return int(dll().AFH_PackedCount(n))
+
+
+def pack_sources(source_type: int, sources: np.ndarray, wavenums: np.ndarray, attens: np.ndarray) -> np.ndarray:
+    """Reference source structs (n, 6|9) + k, alpha -> packed (packed_count(n), 12) float32."""
+    src = np.ascontiguousarray(sources, np.float32)
+    k = np.ascontiguousarray(wavenums, np.float32)
+    a = np.ascontiguousarray(attens, np.float32)
+    n = k.shape[0]
+    out = np.empty((packed_count(n), SOURCE_STRIDE), np.float32)
+    rc = dll().AFH_PackSources(src.ctypes.data_as(c_void_p), k.ctypes.data_as(c_void_p), a.ctypes.data_as(c_void_p),
+                               n, source_type, out.ctypes.data_as(c_void_p))
+    _err("AFH_PackSources", rc)
+    return out
+
+
+def grid_pointset(order, counts, origin, resolution) -> AFH_PointSet:
+    """order: axes fastest-first (0=x,1=y,2=z); counts along that order; origin by axis."""
+    ps = AFH_PointSet()
+    ps.kind = 0
+    for i in range(3):
+        ps.order[i] = int(order[i])
+        ps.count[i] = int(counts[i])
+        ps.origin[i] = float(origin[i])
+    ps.resolution = float(resolution)
+    ps.points = None
+    ps.n_points = int(counts[0]) * int(counts[1]) * int(counts[2])
+    return ps
+
+
+def scatter_pointset(points_dev_ptr: int, n: int) -> AFH_PointSet:
+    ps = AFH_PointSet()
+    ps.kind = 1
+    ps.points = points_dev_ptr
+    ps.n_points = n
+    return ps
+
+
+def field_eval(packed_dev_ptr: int, n_packed: int, source_type: int, pointset: AFH_PointSet, field_type: int,
+               precision: int, out_dev_ptr: int, stream_ptr: int = 0) -> None:
+    rc = dll().AFH_FieldEval(c_void_p(packed_dev_ptr), n_packed, source_type, byref(pointset), field_type, precision,
+                             c_void_p(out_dev_ptr), c_void_p(stream_ptr))
+    _err("AFH_FieldEval", rc)
+
+
+def grid_from_area(order, ranges, res):
+    """(order of ranged axes, ranges by axis, res) as the reference builder takes them ->
+    a device grid point set with the builder's counts (builder.rs:198-200)."""
+    cnt = []
+    for a in range(3):
+        lo, hi = np.float32(ranges[a][0]), np.float32(ranges[a][1])
+        q = np.float32(hi - lo) / np.float32(res)
+        cnt.append(int(q) + 1 if q > 0 else 1)
+    full = list(order) + [a for a in range(3) if a not in order]
+    return grid_pointset(full, [cnt[a] for a in full], [ranges[a][0] for a in range(3)], res), tuple(cnt)

@@ -1,0 +1,160 @@
+// Device-side source-to-point term for gfx950.
+//
+// Reference: system/uniform_medium.rs:111-120 (same as
+// acoustic-field-optimizer/src/multiple_foci/macros.rs:17-30):
+//     diff = t - s; r = |diff|; theta = acos(dir.diff / r); D = directivity(theta)
+//     term = amp * D * exp(-r * alpha) / r * exp(i (phase + k r))
+//
+// Formulation here (FP32 throughout):
+//   * r comes from the hardware rsqrt plus one Newton residual, kept as a
+//     (r, r_lo) pair so the phase sees a distance accurate to well below an ulp;
+//   * the phase is carried in revolutions: k r / (2 pi) with k / (2 pi) split
+//     into hi + lo floats and the product's rounding error recovered with an
+//     fma, the integer part dropped exactly with v_fract; phase / (2 pi) is
+//     precomputed per source modulo 1;
+//   * exp(-alpha r) = exp2(-(alpha log2 e) r) on v_exp_f32;
+//   * sin / cos: v_sin_f32 / v_cos_f32 (take revolutions) in the FAST mode, a
+//     quadrant-reduced Taylor polynomial in the ACCURATE mode;
+//   * T4010A1 directivity (t4010a1.rs:107-120): D = 1 for cos(theta) >= 0.94
+//     (theta <= 20 deg, segments 0-2 are identically 1), D = 0 for
+//     cos(theta) <= -0.001 (theta > 90 deg), otherwise theta = acos(c) and the
+//     reference's piecewise cubic with its ceil(theta / (pi/18)) binning.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace afh {
+
+enum SourceModel { kSphere = 0, kT4010A1 = 1 };
+enum Precision { kFast = 0, kAccurate = 1 };
+
+// Packed per-source parameters (AFH_SOURCE_STRIDE = 12 floats).
+struct Src {
+    float x, y, z, nx, ny, nz, amp, ph, kh, kl, al, pad;
+};
+
+// f32 constants of t4010a1.rs: ANGLE_DIV = PI_f32 / 18 in f32 arithmetic.
+constexpr float kPiF = 3.14159265358979323846f;
+constexpr float kAngleDiv = kPiF / 18.0f;
+constexpr float kPio2Hi = 1.57079637050628662109375f;  // nearest f32 to pi/2
+constexpr float kPio2Lo = -4.37113900018624283e-8f;    // pi/2 - kPio2Hi
+constexpr float kCosBandHi = 0.94f;                    // > cos(20 deg) = 0.93969
+constexpr float kCosBandLo = -0.001f;                  // < cos(90 deg)
+
+// Directivity table entry i (i = ceil(theta / ANGLE_DIV) clamped to 0..10):
+// {a, b, c, d} and the segment start (i - 1) * ANGLE_DIV computed in f32.
+struct DirEntry {
+    float a, b, c, d, start;
+};
+
+// acos for c in [-0.01, 1]: asin polynomial of the Cephes Math Library
+// (S. L. Moshier, asinf.c, minimax on [0, 0.5]) with the half-angle identity
+// acos(c) = 2 asin(sqrt((1 - c) / 2)) for c > 0.5 and pi/2 - asin(c) below.
+__host__ __device__ inline float acos_band(float c) {
+    const bool big = c > 0.5f;
+    const float z = big ? fmaf(-0.5f, c, 0.5f) : c * c;
+#if defined(__HIP_DEVICE_COMPILE__)
+    const float s = big ? __builtin_amdgcn_sqrtf(z) : c;
+#else
+    const float s = big ? sqrtf(z) : c;
+#endif
+    float p = fmaf(4.2163199048e-2f, z, 2.4181311049e-2f);
+    p = fmaf(p, z, 4.5470025998e-2f);
+    p = fmaf(p, z, 7.4953002686e-2f);
+    p = fmaf(p, z, 1.6666752422e-1f);
+    const float a = fmaf(p * z, s, s);  // asin(s)
+    return big ? 2.0f * a : (kPio2Hi - a) + kPio2Lo;
+}
+
+// sin and cos of 2 pi u: u reduced to the nearest quarter turn, then Taylor
+// series of sin (to x^9) and cos (to x^10) on |x| <= pi/4, whose truncation
+// errors are below 2e-9.
+__host__ __device__ inline void sincos_rev_poly(float u, float* s, float* c) {
+    const float q = rintf(4.0f * u);
+    const float w = fmaf(-0.25f, q, u);  // exact
+    const float x = w * 6.28318530717958647692f;
+    const float x2 = x * x;
+    float sp = fmaf(x2, 2.7557319223985893e-6f, -1.9841269841269841e-4f);
+    sp = fmaf(sp, x2, 8.3333333333333333e-3f);
+    sp = fmaf(sp, x2, -1.6666666666666667e-1f);
+    sp = fmaf(sp * x2, x, x);
+    float cp = fmaf(x2, -2.7557319223985888e-7f, 2.4801587301587302e-5f);
+    cp = fmaf(cp, x2, -1.3888888888888889e-3f);
+    cp = fmaf(cp, x2, 4.1666666666666667e-2f);
+    cp = fmaf(cp, x2, -0.5f);
+    cp = fmaf(cp, x2, 1.0f);
+    const int qi = ((int)q) & 3;
+    const float s0 = (qi & 1) ? cp : sp;
+    const float c0 = (qi & 1) ? sp : cp;
+    *s = (qi & 2) ? -s0 : s0;
+    *c = ((qi + 1) & 2) ? -c0 : c0;
+}
+
+#if defined(__HIPCC__)
+
+// Exact reference binning of theta (t4010a1.rs:108): i = ceil(theta / ANGLE_DIV)
+// with Rust's saturating cast, NaN -> 0. Division is IEEE (correctly rounded).
+__device__ inline float directivity_exact(float c, const DirEntry* tab) {
+    const float theta = acos_band(c);
+    const float q = ceilf(theta / kAngleDiv);
+    int i = (q > 0.0f) ? (int)fminf(q, 10.0f) : 0;
+    const DirEntry e = tab[i];
+    const float x = theta - e.start;
+    return e.a + (e.b + (e.c + e.d * x) * x) * x;
+}
+
+__device__ inline float directivity(float c, const DirEntry* tab) {
+    float d = 1.0f;  // also the NaN case (c undefined at r = 0)
+    if (c < kCosBandHi) {
+        d = 0.0f;
+        if (c > kCosBandLo) d = directivity_exact(c, tab);
+    }
+    return d;
+}
+
+// Adds one source's term at one point to (re, im).
+template <int SRC, int PREC>
+__device__ __forceinline__ void add_term(const Src& s, float px, float py, float pz, const DirEntry* tab, float& re,
+                                         float& im) {
+    const float dx = px - s.x;
+    const float dy = py - s.y;
+    const float dz = pz - s.z;
+    const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    const float rinv = __builtin_amdgcn_rsqf(d2);
+    const float r = d2 * rinv;
+    const float r_lo = fmaf(-r, r, d2) * (0.5f * rinv);
+    float amp = s.amp;
+    if (SRC == kT4010A1) {
+        const float c = fmaf(s.nx, dx, fmaf(s.ny, dy, s.nz * dz)) * rinv;
+        amp *= directivity(c, tab);
+    }
+    const float att = __builtin_amdgcn_exp2f(-s.al * r);
+    const float a = amp * att * rinv;
+    const float t_hi = r * s.kh;
+    const float t_lo = fmaf(r, s.kh, -t_hi) + fmaf(r_lo, s.kh, r * s.kl);
+    const float u = __builtin_amdgcn_fractf(__builtin_amdgcn_fractf(t_hi) + (t_lo + s.ph));
+    float sn, cs;
+    if (PREC == kFast) {
+        sn = __builtin_amdgcn_sinf(u);
+        cs = __builtin_amdgcn_cosf(u);
+    } else {
+        sincos_rev_poly(u, &sn, &cs);
+    }
+    re = fmaf(a, cs, re);
+    im = fmaf(a, sn, im);
+}
+
+// Error-free addition of x into (s, c): Knuth's TwoSum, the rounding error of
+// s + x accumulated into c.
+__device__ __forceinline__ void two_sum_acc(float& s, float& c, float x) {
+    const float t = s + x;
+    const float bp = t - s;
+    const float err = (s - (t - bp)) + (x - bp);
+    s = t;
+    c += err;
+}
+
+#endif  // __HIPCC__
+
+}  // namespace afh

@@ -1,0 +1,117 @@
+// Optimizer exports of the C ABI (capi/src/optimizer.rs).
+//
+// GS, GS-PAT, Naive and LM run on the GPU (holo.hip / lm.hip). The other
+// optimizers of the reference are outside this backend: they record
+// "unsupported" and leave the system untouched.
+#include <math.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "afh_host.hpp"
+#include "afh_internal.hpp"
+#include "afh_holo.hpp"
+
+#pragma clang fp contract(off)
+
+using namespace afh;
+
+namespace {
+
+System* as_system(void* h, int32_t source_type, const char* who) {
+    System* s = static_cast<System*>(h);
+    if (!s || s->source_type != source_type) {
+        set_error(std::string(who) + ": null system or source type mismatch");
+        return nullptr;
+    }
+    return s;
+}
+
+void unsupported(const char* who) {
+    set_error(std::string(who) + " is not provided by libafc_hip (outside the GPU hot path)");
+}
+
+// Shared front end of AFO_GS / AFO_GSPAT / AFO_Naive / AFO_LM: one instance
+// through the batched device solver, then amp / phase written back into the
+// system's sources the way the reference does (gs.rs:71-76, gs_pat.rs:75-80,
+// naive.rs:45-50, lm.rs:102-104).
+void run_single(void* handle, const void* foci, const float* amps, uint64_t size, int32_t source_type, int algo,
+                const char* who) {
+    clear_error();
+    System* s = as_system(handle, source_type, who);
+    if (!s) return;
+    if (size == 0 || !foci || !amps) {
+        set_error(std::string(who) + ": no foci");
+        return;
+    }
+    const uint64_t n = s->count();
+    if (n == 0) return;
+    std::vector<float> amp_out(n), phase_out(n);
+    HoloProblem p;
+    p.source_type = source_type;
+    p.sources = s->sources.data();
+    p.wavenums = s->wavenums.data();
+    p.attens = s->attens.data();
+    p.n = n;
+    p.foci = static_cast<const float*>(foci);
+    p.amps = amps;
+    p.m = size;
+    if (!holo_solve_single(p, algo, amp_out.data(), phase_out.data())) return;
+    const size_t st = s->stride();
+    const size_t ia = source_type == kT4010A1 ? 6 : 3;
+    for (uint64_t j = 0; j < n; ++j) {
+        float* src = s->sources.data() + st * j;
+        if (algo != kAlgoLM) src[ia] = amp_out[j];
+        src[ia + 1] = phase_out[j];
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+// acoustic-field-optimizer/src/focal_point.rs:28-39 (host-side input set-up,
+// as in the reference: phi_j = -2 PI ((|s_j - f| mod lambda) / lambda))
+void AFO_FocalPoint(void* handle, AFC_Vector3 point, int32_t source_type) {
+    clear_error();
+    System* s = as_system(handle, source_type, "AFO_FocalPoint");
+    if (!s) return;
+    const size_t st = s->stride();
+    for (uint64_t j = 0; j < s->count(); ++j) {
+        float* src = s->sources.data() + st * j;
+        const float dx = src[0] - point.x, dy = src[1] - point.y, dz = src[2] - point.z;
+        const float d = sqrtf((dx * dx + dy * dy) + dz * dz);
+        const float wl = s->sound_speed / src[st - 1];
+        const float ph = fmodf(d, wl) / wl;
+        src[st - 2] = -2.0f * kPiF * ph;
+    }
+}
+
+void AFO_GSPAT(void* handle, const void* foci, const float* amps, uint64_t size, int32_t source_type) {
+    run_single(handle, foci, amps, size, source_type, kAlgoGSPAT, "AFO_GSPAT");
+}
+void AFO_GS(void* handle, const void* foci, const float* amps, uint64_t size, int32_t source_type) {
+    run_single(handle, foci, amps, size, source_type, kAlgoGS, "AFO_GS");
+}
+void AFO_Naive(void* handle, const void* foci, const float* amps, uint64_t size, int32_t source_type) {
+    run_single(handle, foci, amps, size, source_type, kAlgoNaive, "AFO_Naive");
+}
+void AFO_LM(void* handle, const void* foci, const float* amps, uint64_t size, int32_t source_type) {
+    run_single(handle, foci, amps, size, source_type, kAlgoLM, "AFO_LM");
+}
+
+void AFO_BesselBeam(void*, AFC_Vector3, AFC_Vector3, float, int32_t) { unsupported("AFO_BesselBeam"); }
+void AFO_IFFT(void*, const char*, AFC_Vector3, AFC_Vector3, AFC_Vector3, float, float, int32_t) {
+    unsupported("AFO_IFFT");
+}
+void AFO_Horn(void*, const void*, const float*, uint64_t, int32_t) { unsupported("AFO_Horn"); }
+void AFO_Long(void*, const void*, const float*, uint64_t, int32_t) { unsupported("AFO_Long"); }
+void AFO_APO(void*, const void*, const float*, uint64_t, float, int32_t) { unsupported("AFO_APO"); }
+void AFO_GaussNewton(void*, const void*, const float*, uint64_t, int32_t) { unsupported("AFO_GaussNewton"); }
+void AFO_GradientDescent(void*, const void*, const float*, uint64_t, int32_t) {
+    unsupported("AFO_GradientDescent");
+}
+void AFO_Greedy(void*, const void*, const float*, uint64_t, uint64_t, int32_t) { unsupported("AFO_Greedy"); }
+
+}  // extern "C"

@@ -1,0 +1,201 @@
+/*
+ * afc_hip.h -- C ABI of libafc_hip.so, the MI355X (gfx950) backend for the
+ * acoustic field sum and the GS / GS-PAT / LM hologram solvers.
+ *
+ * Part 1 is the drop-in surface: the same symbols, argument order, value
+ * structs, type tags and ownership rules as the reference's cdylib `afccapi`
+ * (capi/src/*.rs; ctypes argtypes in python/afc/nativemethods.py:113-234).
+ * Each declaration cites the reference export it replaces.
+ *
+ * Part 2 adds entry points the reference has no counterpart for: device-
+ * resident evaluation, batched solvers and the request server. They take
+ * plain pointers and sizes and return an int32 status (AFH_OK on success).
+ *
+ * Differences from the reference (intentional):
+ *   - CalculatorType 0 (CpuCalculator) is not provided by this library: the
+ *     CPU path stays in the reference's own library. AFC_Calculate with it
+ *     returns 0 and sets the error string. Types 1 (Accurate) and 2 (Gpu) run
+ *     on the GPU; type 3 (AFC_CALC_HIP) is this backend's own tag.
+ *   - Errors never abort the process: calls that fail record a message
+ *     retrievable with AFC_GetLastError() (thread-local).
+ *   - Optimizers outside the hot path (BesselBeam, IFFT, Horn, Long, APO,
+ *     GaussNewton, GradientDescent, Greedy) are exported so that bindings
+ *     load, but they only record "unsupported" and leave the system as is.
+ */
+#ifndef AFC_HIP_H
+#define AFC_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- value types */
+/* nalgebra Vector3<f32> as passed by value (nativemethods.py:18-25) */
+typedef struct {
+    float x, y, z;
+} AFC_Vector3;
+
+/* core/wave_sources/sphere_sources.rs:19-27 (repr(C), 24 bytes) */
+typedef struct {
+    float x, y, z;
+    float amp, phase, freq;
+} AFC_SphereWaveSource;
+
+/* core/wave_sources/t4010a1.rs:73-82 (repr(C), 36 bytes) */
+typedef struct {
+    float x, y, z;
+    float nx, ny, nz;
+    float amp, phase, freq;
+} AFC_T4010A1;
+
+/* type tags: capi/src/type_inference_aux.rs:53-103, grid/dimension.rs:23-28 */
+enum { AFC_CALC_CPU = 0, AFC_CALC_ACCURATE = 1, AFC_CALC_GPU = 2, AFC_CALC_HIP = 3 };
+enum { AFC_SOURCE_SPHERE = 0, AFC_SOURCE_T4010A1 = 1 };
+enum { AFC_AREA_GRID1D = 0, AFC_AREA_GRID2D = 1, AFC_AREA_GRID3D = 2, AFC_AREA_SCATTER = 3 };
+enum { AFC_FIELD_PRESSURE = 0, AFC_FIELD_POWER = 1, AFC_FIELD_COMPLEX = 2 };
+enum { AFC_AXIS_X = 1, AFC_AXIS_Y = 2, AFC_AXIS_Z = 3 };
+
+/* ======================================================= part 1: capi surface */
+
+/* capi/src/calculator.rs:21 */
+void AFC_CreateCalculator(void** out, int32_t calc_type);
+/* capi/src/calculator.rs:39 */
+void AFC_FreeCalculator(void* handle, int32_t calc_type);
+/* capi/src/calculator.rs:53-63. Returns the number of observation points and
+ * points *out at the field handle's own buffer (1 float per point, or an
+ * interleaved (re, im) pair for AFC_FIELD_COMPLEX), valid until the next
+ * calculate on that field or its free. 0 with *out = NULL on error. */
+uint64_t AFC_Calculate(void* calc_handle, void* system_handle, void* obs_handle, void* field_handle, const void** out,
+                       int32_t calc_type, int32_t source_type, int32_t obs_type, int32_t field_type);
+
+/* capi/src/field_buffer.rs:37-49 */
+void AFC_CreatePressureField(void** out);
+void AFC_FreePressureField(void* handle);
+void AFC_CreatePowerField(void** out);
+void AFC_FreePowerField(void* handle);
+void AFC_CreateComplexPressureField(void** out);
+void AFC_FreeComplexPressureField(void* handle);
+
+/* capi/src/observe_area.rs:21 */
+void AFC_CreateScatterArea(void** out);
+/* capi/src/observe_area.rs:31 */
+void AFC_ScatterAddObservePoint(void* handle, AFC_Vector3 p);
+/* capi/src/observe_area.rs:38 */
+void AFC_FreeScatterArea(void* handle);
+/* capi/src/observe_area.rs:43-53 */
+void AFC_CreateGridArea1D(void** out, float first_start, float first_end, float second, float third, float resolution,
+                          int32_t first_axis, int32_t second_axis, int32_t third_axis);
+/* capi/src/observe_area.rs:88-99 */
+void AFC_CreateGridArea2D(void** out, float first_start, float first_end, float second_start, float second_end,
+                          float third, float resolution, int32_t first_axis, int32_t second_axis, int32_t third_axis);
+/* capi/src/observe_area.rs:134-146 */
+void AFC_CreateGridArea3D(void** out, float first_start, float first_end, float second_start, float second_end,
+                          float third_start, float third_end, float resolution, int32_t first_axis,
+                          int32_t second_axis, int32_t third_axis);
+/* capi/src/observe_area.rs:181 */
+void AFC_FreeGridArea(void* handle, int32_t dimension);
+/* capi/src/observe_area.rs:197-203 -- counts by axis (x, y, z), not by order */
+void AFC_GetGridSize(void* handle, int32_t dimension, uint32_t* first, uint32_t* second, uint32_t* third);
+
+/* capi/src/system.rs:22-26 */
+void AFC_CreateUniformSystem(void** out, float temperature, int32_t source_type);
+/* capi/src/system.rs:44 */
+void AFC_FreeUniformSystem(void* handle, int32_t source_type);
+/* capi/src/system.rs:58 -- mm/s */
+float AFC_UniformSystemSoundSpeed(void* handle, int32_t source_type);
+/* capi/src/system.rs:75 -- string owned by the handle, valid until the next info call on it */
+char* AFC_UniformSystemInfo(void* handle, int32_t source_type);
+/* capi/src/system.rs:95 */
+char* AFC_UniformSystemSourceInfo(void* handle, uint64_t source_idx, int32_t source_type);
+/* capi/src/system.rs:117 */
+void AFC_AddSphereWaveSource(void* handle, AFC_SphereWaveSource source);
+/* capi/src/system.rs:125 */
+void AFC_AddT4010A1(void* handle, AFC_T4010A1 source);
+/* capi/src/system.rs:132-136 -- mutable pointer into the system's own source
+ * array; callers may edit it in place and the next calculate sees the edits */
+uint64_t AFC_GetWaveSources(void* handle, void** out, int32_t source_type);
+
+/* capi/src/optimizer.rs:43 */
+void AFO_FocalPoint(void* handle, AFC_Vector3 point, int32_t source_type);
+/* capi/src/optimizer.rs:80-95 -- foci packed as 3 floats per focus */
+void AFO_GSPAT(void* handle, const void* foci, const float* amps, uint64_t size, int32_t source_type);
+/* capi/src/optimizer.rs:98-109 */
+void AFO_GS(void* handle, const void* foci, const float* amps, uint64_t size, int32_t source_type);
+/* capi/src/optimizer.rs:112-127 */
+void AFO_Naive(void* handle, const void* foci, const float* amps, uint64_t size, int32_t source_type);
+/* capi/src/optimizer.rs:213-224 */
+void AFO_LM(void* handle, const void* foci, const float* amps, uint64_t size, int32_t source_type);
+/* Exported for binding compatibility only; each records "unsupported".
+ * capi/src/optimizer.rs:49, :61, :130, :144, :158, :177, :195, :227 */
+void AFO_BesselBeam(void* handle, AFC_Vector3 point, AFC_Vector3 dir, float theta, int32_t source_type);
+void AFO_IFFT(void* handle, const char* path, AFC_Vector3 bottom_left, AFC_Vector3 top_left,
+              AFC_Vector3 bottom_right, float spacing, float z, int32_t source_type);
+void AFO_Horn(void* handle, const void* foci, const float* amps, uint64_t size, int32_t source_type);
+void AFO_Long(void* handle, const void* foci, const float* amps, uint64_t size, int32_t source_type);
+void AFO_APO(void* handle, const void* foci, const float* amps, uint64_t size, float lambda, int32_t source_type);
+void AFO_GaussNewton(void* handle, const void* foci, const float* amps, uint64_t size, int32_t source_type);
+void AFO_GradientDescent(void* handle, const void* foci, const float* amps, uint64_t size, int32_t source_type);
+void AFO_Greedy(void* handle, const void* foci, const float* amps, uint64_t size, uint64_t phase_div,
+                int32_t source_type);
+
+/* ============================================== part 2: no reference counterpart */
+
+enum {
+    AFH_OK = 0,
+    AFH_ERR_ARG = 1,      /* bad tag, size or pointer */
+    AFH_ERR_HIP = 2,      /* HIP runtime error */
+    AFH_ERR_NO_DEVICE = 3,/* no GPU visible */
+    AFH_ERR_UNSUPPORTED = 4,
+    AFH_ERR_BUDGET = 5    /* request does not fit the HBM budget */
+};
+
+/* Thread-local message of the last failing call ("" if none). */
+const char* AFC_GetLastError(void);
+
+/* Copies the per-source wavenumber [1/mm] and attenuation [Np/mm] the system
+ * computed when each source was added (system/uniform_medium.rs:102-107).
+ * Returns the number of sources; either output may be NULL. */
+uint64_t AFH_GetSourceConstants(void* system_handle, float* wavenums, float* attens);
+
+/* Floats per packed source in device memory (see AFH_PackSources). */
+#define AFH_SOURCE_STRIDE 12
+/* Packed source arrays are padded to a multiple of this many entries. */
+#define AFH_SOURCE_PAD 32
+
+/* Packs n reference structs (AFC_SphereWaveSource or AFC_T4010A1 by
+ * source_type) with their wavenumbers [1/mm] and attenuations [Np/mm] into
+ * the device layout: per source {x, y, z, nx, ny, nz, amp, phase/(2pi) mod 1,
+ * k/(2pi) hi, k/(2pi) lo, alpha*log2(e), 0}. Writes AFH_PackedCount(n)
+ * entries (the tail is zero-amplitude padding) to out (host memory). */
+uint64_t AFH_PackedCount(uint64_t n);
+int32_t AFH_PackSources(const void* sources, const float* wavenums, const float* attens, uint64_t n,
+                        int32_t source_type, float* out);
+
+/* Observation points of a field evaluation. kind 0: a grid generated on the
+ * device from (origin, resolution, counts in axis order); kind 1: scatter
+ * points read from `points` (device pointer, 3 floats per point). */
+typedef struct {
+    int32_t kind;
+    int32_t order[3];   /* axes 0=x 1=y 2=z, fastest-varying first */
+    uint32_t count[3];  /* points along order[0], order[1], order[2] */
+    float origin[3];    /* by axis (x, y, z) */
+    float resolution;
+    const float* points;
+    uint64_t n_points;
+} AFH_PointSet;
+
+enum { AFH_PREC_FAST = 0, AFH_PREC_ACCURATE = 1 };
+
+/* Field of packed device sources at a point set, written to device `out`
+ * (field_type as AFC_FIELD_*: 1 float per point, or 2 for complex). All
+ * pointers are device memory; the call is asynchronous on `stream`
+ * (a hipStream_t, NULL = default stream). */
+int32_t AFH_FieldEval(const float* packed_sources_dev, uint64_t n_packed, int32_t source_type, const AFH_PointSet* pts,
+                      int32_t field_type, int32_t precision, float* out_dev, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AFC_HIP_H */

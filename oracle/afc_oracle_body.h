@@ -1,0 +1,542 @@
+/*
+ * Oracle body, instantiated twice by afc_oracle.c: once with R = float (the
+ * reference's FP32 CPU path, same operation order, no FMA contraction) and
+ * once with R = double (the FP64 twin used to measure error).
+ *
+ * TEST INFRASTRUCTURE ONLY. Nothing in the product library links or calls
+ * this code; it is the checker for tests/, the smoke check and the CPU
+ * baseline leg of bench.py.
+ *
+ * Macros expected from the includer:
+ *   R         real type
+ *   SFX(name) name mangling (name##_f32 / name##_f64)
+ *   RACOS RCOS RSIN REXP RSQRT RCEIL RHYPOT RATAN2 RFMOD RPOW RABS  libm calls
+ */
+
+typedef struct {
+    R re, im;
+} SFX(cplx);
+
+static inline SFX(cplx) SFX(c_make)(R re, R im) {
+    SFX(cplx) z;
+    z.re = re;
+    z.im = im;
+    return z;
+}
+/* num-complex Mul: (a.re*b.re - a.im*b.im, a.re*b.im + a.im*b.re) */
+static inline SFX(cplx) SFX(c_mul)(SFX(cplx) a, SFX(cplx) b) {
+    return SFX(c_make)(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+}
+static inline SFX(cplx) SFX(c_add)(SFX(cplx) a, SFX(cplx) b) {
+    return SFX(c_make)(a.re + b.re, a.im + b.im);
+}
+static inline SFX(cplx) SFX(c_conj)(SFX(cplx) a) { return SFX(c_make)(a.re, -a.im); }
+static inline SFX(cplx) SFX(c_divr)(SFX(cplx) a, R r) { return SFX(c_make)(a.re / r, a.im / r); }
+/* nalgebra ComplexField::abs for Complex = num-complex norm() = hypot */
+static inline R SFX(c_abs)(SFX(cplx) a) { return RHYPOT(a.re, a.im); }
+static inline R SFX(c_arg)(SFX(cplx) a) { return RATAN2(a.im, a.re); }
+
+/* ---------------------------------------------------------------------------
+ * T4010A1 directivity: core/wave_sources/t4010a1.rs:107-120, tables :18-69.
+ * The tables are the reference's f32 literals; the f64 twin promotes them.
+ * ------------------------------------------------------------------------- */
+R SFX(orc_directivity_t4010a1)(R theta) {
+    const R angle_div = SFX(ORC_ANGLE_DIV);
+    R q = RCEIL(theta / angle_div);
+    /* Rust `as usize` saturates: NaN and negatives become 0. */
+    size_t i;
+    if (!(q > 0)) {
+        i = 0;
+    } else if (q > (R)1e9) {
+        i = 1000000000u;
+    } else {
+        i = (size_t)q;
+    }
+    if (i == 0) return (R)1.0;
+    if (i > 9) return (R)0.0;
+    {
+        R a = (R)ORC_DIR_A[i];
+        R b = (R)ORC_DIR_B[i];
+        R c = (R)ORC_DIR_C[i];
+        R d = (R)ORC_DIR_D[i];
+        R x = theta - ((R)i - (R)1.0) * angle_div;
+        return a + (b + ((c + d * x) * x)) * x;
+    }
+}
+
+/* core/sound_speed.rs:22-24, in mm/s */
+R SFX(orc_sound_speed)(R t) { return (R)331.3 * RSQRT(t / (R)273.15) * (R)1000.0; }
+
+/* core/attenuation.rs:31-48 (Bass et al. 1995), Np/mm */
+R SFX(orc_attenuation_coef)(R freq, R hr, R ps, R ps0, R t) {
+    const R T0 = (R)293.15;
+    const R T01 = (R)273.16;
+    R ten = (R)10.0;
+    R psat = ps0 * RPOW(ten, (R)-6.8346 * RPOW(T01 / t, (R)1.261) + (R)4.6151);
+    R h = ps0 * (hr / ps) * (psat / ps0);
+    R f_ro = ((R)24. + (R)4.04e4 * h * ((R)0.02 + h) / ((R)0.391 + h)) / ps0;
+    R f_rn = ((R)1. / ps0) * ((R)9. + (R)280. * h * REXP((R)-4.17 * (RPOW(T0 / t, (R)1. / (R)3.) - (R)1.))) *
+             RPOW(T0 / t, (R)1. / (R)2.);
+    R f = freq / ps;
+    R alpha = (f * f) / ps0 * ps *
+              ((R)1.84 * RPOW(t / T0, (R)1. / (R)2.) * (R)1e-11 +
+               RPOW(t / T0, (R)-5. / (R)2.) * ((R)0.01278 * REXP((R)-2239.1 / t) / (f_ro + f * f / f_ro) +
+                                                (R)0.1068 * REXP((R)-3352. / t) / (f_rn + f * f / f_rn)));
+    return alpha * (R)1e-3;
+}
+
+/* ---------------------------------------------------------------------------
+ * One source-to-point term: system/uniform_medium.rs:111-120
+ * (same as acoustic-field-optimizer/src/multiple_foci/macros.rs:17-30).
+ * src points at the reference struct: SphereWaveSource = {pos[3], amp, phase,
+ * freq} (sphere_sources.rs:19-27); T4010A1 = {pos[3], dir[3], amp, phase, freq}
+ * (t4010a1.rs:73-82).
+ * ------------------------------------------------------------------------- */
+static inline SFX(cplx) SFX(term)(int source_type, const float* src, R wavenum, R atten, R tx, R ty, R tz,
+                                  R phase_override, int use_override) {
+    R px = (R)src[0], py = (R)src[1], pz = (R)src[2];
+    R dirx = 0, diry = 0, dirz = 0, amp, phase;
+    R dx, dy, dz, dist, cos_t, theta, d, r, phi;
+    if (source_type == 1) {
+        dirx = (R)src[3];
+        diry = (R)src[4];
+        dirz = (R)src[5];
+        amp = (R)src[6];
+        phase = (R)src[7];
+    } else {
+        amp = (R)src[3];
+        phase = (R)src[4];
+    }
+    if (use_override) phase = phase_override;
+    /* fmath::sub is plain subtraction in the capi build (fmath gated off by `accurate`) */
+    dx = tx - px;
+    dy = ty - py;
+    dz = tz - pz;
+    /* nalgebra norm of a 3-vector: sqrt((x*x + y*y) + z*z) */
+    dist = RSQRT_F((dx * dx + dy * dy) + dz * dz);
+    cos_t = ((dirx * dx + diry * dy) + dirz * dz) / dist;
+    theta = RACOS(cos_t);
+    d = (source_type == 1) ? SFX(orc_directivity_t4010a1)(theta) : (R)1.0;
+    r = amp * d * REXP(-dist * atten) / dist;
+    phi = phase + wavenum * dist;
+    return SFX(c_make)(r * RCOS(phi), r * RSIN(phi));
+}
+
+/* ---------------------------------------------------------------------------
+ * Field: CpuCalculator::calculate (calculator/cpu_calculator.rs:43-65) over
+ * UniformSystem::propagate_all (system/uniform_medium.rs:122-128): per point a
+ * serial complex sum in source order. out holds interleaved (re, im).
+ * ------------------------------------------------------------------------- */
+void SFX(orc_field)(int source_type, const float* sources, const float* wavenums, const float* attens,
+                    uint64_t n_sources, const float* points, uint64_t n_points, R* out, int threads) {
+    const size_t st = orc_stride(source_type);
+    int64_t p;
+    if (threads <= 0) threads = 1;
+#pragma omp parallel for schedule(static) num_threads(threads)
+    for (p = 0; p < (int64_t)n_points; ++p) {
+        SFX(cplx) c = SFX(c_make)(0, 0);
+        R tx = (R)points[3 * p], ty = (R)points[3 * p + 1], tz = (R)points[3 * p + 2];
+        uint64_t i;
+        for (i = 0; i < n_sources; ++i) {
+            SFX(cplx) t = SFX(term)(source_type, sources + st * i, (R)wavenums[i], (R)attens[i], tx, ty, tz, 0, 0);
+            c.re += t.re;
+            c.im += t.im;
+        }
+        out[2 * p] = c.re;
+        out[2 * p + 1] = c.im;
+    }
+}
+
+static int SFX(cmp_asc)(const void* a, const void* b) {
+    R x = *(const R*)a, y = *(const R*)b;
+    return (x > y) - (x < y);
+}
+
+/* AccurateCalculator: accurate/system_acc.rs:26-54 -- each term's re and im
+ * are summed in ascending signed order (min-heap drain), starting from 0. */
+void SFX(orc_field_accurate)(int source_type, const float* sources, const float* wavenums, const float* attens,
+                             uint64_t n_sources, const float* points, uint64_t n_points, R* out, int threads) {
+    const size_t st = orc_stride(source_type);
+    int64_t p;
+    if (threads <= 0) threads = 1;
+#pragma omp parallel num_threads(threads)
+    {
+        R* re = (R*)malloc(sizeof(R) * (n_sources + 1));
+        R* im = (R*)malloc(sizeof(R) * (n_sources + 1));
+#pragma omp for schedule(static)
+        for (p = 0; p < (int64_t)n_points; ++p) {
+            R tx = (R)points[3 * p], ty = (R)points[3 * p + 1], tz = (R)points[3 * p + 2];
+            R sr = 0, si = 0;
+            uint64_t i;
+            for (i = 0; i < n_sources; ++i) {
+                SFX(cplx) t = SFX(term)(source_type, sources + st * i, (R)wavenums[i], (R)attens[i], tx, ty, tz, 0, 0);
+                re[i] = t.re;
+                im[i] = t.im;
+            }
+            qsort(re, n_sources, sizeof(R), SFX(cmp_asc));
+            qsort(im, n_sources, sizeof(R), SFX(cmp_asc));
+            for (i = 0; i < n_sources; ++i) sr += re[i];
+            for (i = 0; i < n_sources; ++i) si += im[i];
+            out[2 * p] = sr;
+            out[2 * p + 1] = si;
+        }
+        free(re);
+        free(im);
+    }
+}
+
+/* ---------------------------------------------------------------------------
+ * Propagation matrix G (M x N, column-major, element (i, j) at i + j*M):
+ * acoustic-field-optimizer/src/multiple_foci/macros.rs:32-53. The caller's
+ * phases are used as stored (the optimizers zero them first).
+ * ------------------------------------------------------------------------- */
+void SFX(orc_propagation_matrix)(int source_type, const float* sources, const float* wavenums, const float* attens,
+                                 uint64_t n, const float* foci, uint64_t m, R* out_interleaved) {
+    const size_t st = orc_stride(source_type);
+    uint64_t i, j;
+    for (j = 0; j < n; ++j) {
+        for (i = 0; i < m; ++i) {
+            SFX(cplx) t = SFX(term)(source_type, sources + st * j, (R)wavenums[j], (R)attens[j], (R)foci[3 * i],
+                                    (R)foci[3 * i + 1], (R)foci[3 * i + 2], 0, 0);
+            out_interleaved[2 * (i + j * m)] = t.re;
+            out_interleaved[2 * (i + j * m) + 1] = t.im;
+        }
+    }
+}
+
+/* G with every phase forced to 0, as GS / GS-PAT / LM do before building it
+ * (gs.rs:37-39, gs_pat.rs:39-41, lm.rs:41-43). */
+static SFX(cplx) * SFX(make_G)(int source_type, const float* sources, const float* wavenums, const float* attens,
+                               uint64_t n, const float* foci, uint64_t m) {
+    const size_t st = orc_stride(source_type);
+    SFX(cplx)* G = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * m * n);
+    uint64_t i, j;
+    for (j = 0; j < n; ++j)
+        for (i = 0; i < m; ++i)
+            G[i + j * m] = SFX(term)(source_type, sources + st * j, (R)wavenums[j], (R)attens[j], (R)foci[3 * i],
+                                     (R)foci[3 * i + 1], (R)foci[3 * i + 2], 0, 1);
+    return G;
+}
+
+static R SFX(src_amp)(int source_type, const float* sources, uint64_t j) {
+    return (R)sources[orc_stride(source_type) * j + (source_type == 1 ? 6 : 3)];
+}
+
+/* y = A x for column-major A (rows x cols): nalgebra gemv accumulates column by
+ * column, y += x_j * A[:, j]. */
+static void SFX(gemv)(const SFX(cplx) * A, uint64_t rows, uint64_t cols, const SFX(cplx) * x, SFX(cplx) * y) {
+    uint64_t i, j;
+    for (i = 0; i < rows; ++i) y[i] = SFX(c_make)(0, 0);
+    for (j = 0; j < cols; ++j)
+        for (i = 0; i < rows; ++i) y[i] = SFX(c_add)(SFX(c_mul)(x[j], A[i + j * rows]), y[i]);
+}
+
+/* ---------------------------------------------------------------------------
+ * GS: acoustic-field-optimizer/src/multiple_foci/linear_synthesis/gs.rs:36-77.
+ * Outputs per-source amp = |q| and phase = arg q.
+ * ------------------------------------------------------------------------- */
+void SFX(orc_gs)(int source_type, const float* sources, const float* wavenums, const float* attens, uint64_t n,
+                 const float* foci, const float* amps, uint64_t m, uint32_t repeat, R* out_amp, R* out_phase) {
+    SFX(cplx)* G = SFX(make_G)(source_type, sources, wavenums, attens, n, foci, m);
+    SFX(cplx)* Gh = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * m * n); /* N x M column-major */
+    SFX(cplx)* q0 = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * n);
+    SFX(cplx)* q = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * n);
+    SFX(cplx)* xi = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * n);
+    SFX(cplx)* p0 = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * m);
+    SFX(cplx)* p = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * m);
+    SFX(cplx)* gamma = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * m);
+    uint64_t i, j;
+    uint32_t it;
+    for (j = 0; j < n; ++j)
+        for (i = 0; i < m; ++i) Gh[j + i * n] = SFX(c_conj)(G[i + j * m]);
+    for (i = 0; i < m; ++i) p0[i] = SFX(c_make)((R)amps[i], 0);
+    for (j = 0; j < n; ++j) q0[j] = SFX(c_make)(SFX(src_amp)(source_type, sources, j), 0);
+    for (j = 0; j < n; ++j) q[j] = q0[j];
+    for (it = 0; it < repeat; ++it) {
+        SFX(gemv)(G, m, n, q, gamma);
+        for (i = 0; i < m; ++i) p[i] = SFX(c_mul)(SFX(c_divr)(gamma[i], SFX(c_abs)(gamma[i])), p0[i]);
+        SFX(gemv)(Gh, n, m, p, xi);
+        for (j = 0; j < n; ++j) q[j] = SFX(c_mul)(SFX(c_divr)(xi[j], SFX(c_abs)(xi[j])), q0[j]);
+    }
+    for (j = 0; j < n; ++j) {
+        out_amp[j] = SFX(c_abs)(q[j]);
+        out_phase[j] = SFX(c_arg)(q[j]);
+    }
+    free(G);
+    free(Gh);
+    free(q0);
+    free(q);
+    free(xi);
+    free(p0);
+    free(p);
+    free(gamma);
+}
+
+/* ---------------------------------------------------------------------------
+ * GS-PAT: acoustic-field-optimizer/src/multiple_foci/linear_synthesis/gs_pat.rs:38-81.
+ * denomi_j = sum_i |G_ij| (row_sum: one entry per transducer column),
+ * B = (conj(G_ij) / denomi_j^2)^T (N x M), R = G B (M x M).
+ * ------------------------------------------------------------------------- */
+void SFX(orc_gspat)(int source_type, const float* sources, const float* wavenums, const float* attens, uint64_t n,
+                    const float* foci, const float* amps, uint64_t m, uint32_t repeat, R* out_amp, R* out_phase) {
+    SFX(cplx)* G = SFX(make_G)(source_type, sources, wavenums, attens, n, foci, m);
+    SFX(cplx)* B = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * m * n); /* N x M column-major */
+    SFX(cplx)* Rm = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * m * m);
+    SFX(cplx)* p0 = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * m);
+    SFX(cplx)* p = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * m);
+    SFX(cplx)* gamma = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * m);
+    SFX(cplx)* q = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * n);
+    R* denomi = (R*)malloc(sizeof(R) * n);
+    uint64_t i, j, k;
+    uint32_t it;
+    for (j = 0; j < n; ++j) {
+        R s = 0;
+        for (i = 0; i < m; ++i) s += SFX(c_abs)(G[i + j * m]);
+        denomi[j] = s;
+    }
+    for (j = 0; j < n; ++j)
+        for (i = 0; i < m; ++i) B[j + i * n] = SFX(c_divr)(SFX(c_conj)(G[i + j * m]), denomi[j] * denomi[j]);
+    /* R = G * B, column by column (gemm as repeated gemv) */
+    for (k = 0; k < m; ++k) SFX(gemv)(G, m, n, B + k * n, Rm + k * m);
+    for (i = 0; i < m; ++i) p0[i] = SFX(c_make)((R)amps[i], 0);
+    for (i = 0; i < m; ++i) p[i] = p0[i];
+    SFX(gemv)(Rm, m, m, p, gamma);
+    for (it = 0; it < repeat; ++it) {
+        for (i = 0; i < m; ++i) p[i] = SFX(c_mul)(SFX(c_divr)(gamma[i], SFX(c_abs)(gamma[i])), p0[i]);
+        SFX(gemv)(Rm, m, m, p, gamma);
+    }
+    for (i = 0; i < m; ++i) {
+        R a = SFX(c_abs)(gamma[i]);
+        p[i] = SFX(c_mul)(SFX(c_mul)(SFX(c_divr)(gamma[i], a * a), p0[i]), p0[i]);
+    }
+    SFX(gemv)(B, n, m, p, q);
+    for (j = 0; j < n; ++j) {
+        out_amp[j] = SFX(c_abs)(q[j]);
+        out_phase[j] = SFX(c_arg)(q[j]);
+    }
+    free(G);
+    free(B);
+    free(Rm);
+    free(p0);
+    free(p);
+    free(gamma);
+    free(q);
+    free(denomi);
+}
+
+/* Naive linear synthesis: linear_synthesis/naive.rs:32-51, q = G^H p. */
+void SFX(orc_naive)(int source_type, const float* sources, const float* wavenums, const float* attens, uint64_t n,
+                    const float* foci, const float* amps, uint64_t m, R* out_amp, R* out_phase) {
+    SFX(cplx)* G = SFX(make_G)(source_type, sources, wavenums, attens, n, foci, m);
+    uint64_t i, j;
+    for (j = 0; j < n; ++j) {
+        SFX(cplx) q = SFX(c_make)(0, 0);
+        for (i = 0; i < m; ++i) q = SFX(c_add)(SFX(c_mul)(SFX(c_make)((R)amps[i], 0), SFX(c_conj)(G[i + j * m])), q);
+        out_amp[j] = SFX(c_abs)(q);
+        out_phase[j] = SFX(c_arg)(q);
+    }
+    free(G);
+}
+
+/* ---------------------------------------------------------------------------
+ * Levenberg-Marquardt: acoustic-field-optimizer/src/multiple_foci/nls/lm.rs:40-105
+ * with make_BhB / make_T / calc_JtJ_Jtf / calc_Fx from nls/macros.rs:17-58.
+ * The damped system is solved by Householder QR (nalgebra qr().solve), which
+ * reports failure when a diagonal entry of R is zero.
+ * ------------------------------------------------------------------------- */
+static int SFX(qr_solve)(R* A, uint64_t n, R* b) {
+    /* A column-major n x n, overwritten; b overwritten by the solution */
+    uint64_t k, i, j;
+    R* diag = (R*)malloc(sizeof(R) * n);
+    for (k = 0; k < n; ++k) {
+        R norm = 0, alpha, s;
+        for (i = k; i < n; ++i) norm = RHYPOT(norm, A[i + k * n]);
+        if (norm == 0) {
+            diag[k] = 0;
+            continue;
+        }
+        alpha = (A[k + k * n] > 0) ? -norm : norm;
+        /* v = x - alpha e1 stored in A[k:, k] */
+        A[k + k * n] -= alpha;
+        s = 0;
+        for (i = k; i < n; ++i) s += A[i + k * n] * A[i + k * n];
+        if (s != 0) {
+            for (j = k + 1; j < n; ++j) {
+                R dot = 0;
+                for (i = k; i < n; ++i) dot += A[i + k * n] * A[i + j * n];
+                dot = (R)2 * dot / s;
+                for (i = k; i < n; ++i) A[i + j * n] -= dot * A[i + k * n];
+            }
+            {
+                R dot = 0;
+                for (i = k; i < n; ++i) dot += A[i + k * n] * b[i];
+                dot = (R)2 * dot / s;
+                for (i = k; i < n; ++i) b[i] -= dot * A[i + k * n];
+            }
+        }
+        diag[k] = alpha;
+    }
+    for (k = 0; k < n; ++k)
+        if (diag[k] == 0) {
+            free(diag);
+            return 0;
+        }
+    /* back substitution with R (diag on diag[], strict upper in A) */
+    for (i = n; i-- > 0;) {
+        R s = b[i];
+        for (j = i + 1; j < n; ++j) s -= A[i + j * n] * b[j];
+        b[i] = s / diag[i];
+    }
+    free(diag);
+    return 1;
+}
+
+static void SFX(lm_A_g)(const SFX(cplx) * BhB, const SFX(cplx) * T, uint64_t np, R* A, R* g) {
+    uint64_t k, l;
+    for (k = 0; k < np; ++k) g[k] = 0;
+    /* column_sum: g = sum over columns l of Im(BhB o T T^H)[:, l], column by column */
+    for (l = 0; l < np; ++l) {
+        SFX(cplx) tl = SFX(c_conj)(T[l]);
+        for (k = 0; k < np; ++k) {
+            SFX(cplx) tt = SFX(c_mul)(tl, T[k]);
+            SFX(cplx) e = SFX(c_mul)(BhB[k + l * np], tt);
+            A[k + l * np] = e.re;
+            g[k] += e.im;
+        }
+    }
+}
+
+static R SFX(lm_Fx)(const SFX(cplx) * BhB, const R* x, uint64_t np, SFX(cplx) * t, SFX(cplx) * row) {
+    uint64_t k, l;
+    SFX(cplx) acc = SFX(c_make)(0, 0);
+    for (k = 0; k < np; ++k) t[k] = SFX(c_make)(RCOS(x[k]), RSIN(x[k]));
+    /* row = t^H BhB (1 x np), then row * t */
+    for (l = 0; l < np; ++l) {
+        SFX(cplx) s = SFX(c_make)(0, 0);
+        for (k = 0; k < np; ++k) s = SFX(c_add)(SFX(c_mul)(SFX(c_conj)(t[k]), BhB[k + l * np]), s);
+        row[l] = s;
+    }
+    for (l = 0; l < np; ++l) acc = SFX(c_add)(SFX(c_mul)(t[l], row[l]), acc);
+    return acc.re;
+}
+
+/* Returns the number of iterations taken; out_phase gets x[0..n). */
+uint32_t SFX(orc_lm)(int source_type, const float* sources, const float* wavenums, const float* attens, uint64_t n,
+                     const float* foci, const float* amps, uint64_t m, uint32_t k_max, R* out_phase, R* out_x) {
+    const R EPS_1 = (R)1e-8, EPS_2 = (R)1e-8, TAU = (R)1e-3;
+    const uint64_t np = n + m;
+    SFX(cplx)* G = SFX(make_G)(source_type, sources, wavenums, attens, n, foci, m);
+    SFX(cplx)* Bm = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * m * np); /* M x np */
+    SFX(cplx)* BhB = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * np * np);
+    SFX(cplx)* T = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * np);
+    SFX(cplx)* tv = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * np);
+    SFX(cplx)* row = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * np);
+    R* A = (R*)malloc(sizeof(R) * np * np);
+    R* M = (R*)malloc(sizeof(R) * np * np);
+    R* g = (R*)malloc(sizeof(R) * np);
+    R* h = (R*)malloc(sizeof(R) * np);
+    R* x = (R*)calloc(np, sizeof(R));
+    R* xn = (R*)malloc(sizeof(R) * np);
+    R nu = 2, mu, Fx, amax, gmax;
+    uint64_t i, j, k;
+    uint32_t it = 0;
+    int found;
+    memset(Bm, 0, sizeof(SFX(cplx)) * m * np);
+    for (j = 0; j < n; ++j)
+        for (i = 0; i < m; ++i) Bm[i + j * m] = G[i + j * m];
+    for (i = 0; i < m; ++i) Bm[i + (n + i) * m] = SFX(c_make)(-(R)amps[i], 0);
+    /* BhB = B^H B: column l = sum_i B^H[:, i] * B[i, l] */
+    for (j = 0; j < np; ++j) {
+        for (k = 0; k < np; ++k) BhB[k + j * np] = SFX(c_make)(0, 0);
+        for (i = 0; i < m; ++i) {
+            SFX(cplx) bil = Bm[i + j * m];
+            for (k = 0; k < np; ++k)
+                BhB[k + j * np] = SFX(c_add)(SFX(c_mul)(bil, SFX(c_conj)(Bm[i + k * m])), BhB[k + j * np]);
+        }
+    }
+    for (k = 0; k < np; ++k) T[k] = SFX(c_make)(RCOS(x[k]), -RSIN(x[k]));
+    SFX(lm_A_g)(BhB, T, np, A, g);
+    amax = A[0];
+    for (k = 1; k < np; ++k)
+        if (A[k + k * np] > amax) amax = A[k + k * np];
+    gmax = g[0];
+    for (k = 1; k < np; ++k)
+        if (g[k] > gmax) gmax = g[k];
+    mu = TAU * amax;
+    found = gmax <= EPS_1;
+    Fx = SFX(lm_Fx)(BhB, x, np, tv, row);
+    for (it = 0; it < k_max; ++it) {
+        R hn = 0, xnorm = 0;
+        if (found) break;
+        memcpy(M, A, sizeof(R) * np * np);
+        for (k = 0; k < np; ++k) M[k + k * np] += mu;
+        for (k = 0; k < np; ++k) h[k] = g[k];
+        if (!SFX(qr_solve)(M, np, h)) break;
+        for (k = 0; k < np; ++k) h[k] = -h[k];
+        for (k = 0; k < np; ++k) hn += h[k] * h[k];
+        for (k = 0; k < np; ++k) xnorm += x[k] * x[k];
+        hn = RSQRT(hn);
+        xnorm = RSQRT(xnorm);
+        if (hn <= EPS_2 * (xnorm + EPS_2)) {
+            found = 1;
+        } else {
+            R Fx_new, L, rho;
+            for (k = 0; k < np; ++k) xn[k] = x[k] + h[k];
+            Fx_new = SFX(lm_Fx)(BhB, xn, np, tv, row);
+            L = 0;
+            for (k = 0; k < np; ++k) L += h[k] * (mu * h[k] - g[k]);
+            L = (R)0.5 * L;
+            rho = (Fx - Fx_new) / L;
+            Fx = Fx_new; /* assigned even when the step is rejected (lm.rs:85) */
+            if (rho > 0) {
+                R f;
+                memcpy(x, xn, sizeof(R) * np);
+                for (k = 0; k < np; ++k) T[k] = SFX(c_make)(RCOS(x[k]), -RSIN(x[k]));
+                SFX(lm_A_g)(BhB, T, np, A, g);
+                gmax = g[0];
+                for (k = 1; k < np; ++k)
+                    if (g[k] > gmax) gmax = g[k];
+                found = gmax <= EPS_1;
+                f = (R)1 - RPOW((R)2 * rho - (R)1, (R)3);
+                mu *= ((R)1 / (R)3 > f) ? (R)1 / (R)3 : f;
+                nu = 2;
+            } else {
+                mu *= nu;
+                nu *= 2;
+            }
+        }
+    }
+    for (j = 0; j < n; ++j) out_phase[j] = x[j];
+    if (out_x)
+        for (k = 0; k < np; ++k) out_x[k] = x[k];
+    free(G);
+    free(Bm);
+    free(BhB);
+    free(T);
+    free(tv);
+    free(row);
+    free(A);
+    free(M);
+    free(g);
+    free(h);
+    free(x);
+    free(xn);
+    return it;
+}
+
+/* FocalPoint phases: acoustic-field-optimizer/src/focal_point.rs:28-39 */
+void SFX(orc_focal_point_phases)(int source_type, const float* sources, uint64_t n, R sound_speed, R fx, R fy, R fz,
+                                 R* out_phase) {
+    const size_t st = orc_stride(source_type);
+    const R pi = SFX(ORC_PI);
+    uint64_t j;
+    for (j = 0; j < n; ++j) {
+        const float* s = sources + st * j;
+        R dx = (R)s[0] - fx, dy = (R)s[1] - fy, dz = (R)s[2] - fz;
+        R d = RSQRT_F((dx * dx + dy * dy) + dz * dz);
+        R freq = (R)s[st - 1];
+        R wl = sound_speed / freq;
+        R ph = RFMOD(d, wl) / wl;
+        out_phase[j] = (R)-2.0 * pi * ph;
+    }
+}

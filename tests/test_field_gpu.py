@@ -1,0 +1,249 @@
+"""GPU parity of the field path against the oracle (oracle/afc_oracle.c).
+
+Tolerances (relative L2 over all points unless stated):
+  * GPU vs the reference's FP32 CPU path (the oracle's *_f32): <= 1e-5;
+  * GPU FAST mode vs FP64: <= 5e-6; ACCURATE mode vs FP64: <= 2e-6;
+  * reductions |p|, |p|^2 and grid-vs-scatter consistency: bit-exact where the
+    inputs are bit-identical.
+Reference: calculator/cpu_calculator.rs:43-65, system/uniform_medium.rs:111-128,
+field_buffer/*.rs, observe_area/grid/*.rs, capi/src/calculator.rs:53-117.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.complex128)
+    b = np.asarray(b, np.complex128)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def log(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def build_system(afc, src, temperature, source_type=1):
+    s = afc.UniformSystem(temperature)
+    for row in src:
+        if source_type == 1:
+            s.add_wave_source(afc.T4010A1(row[0:3], row[3:6], row[6], row[7], row[8]))
+        else:
+            s.add_wave_source(afc.SphereWaveSource(row[0:3], row[3], row[4], row[5]))
+    return s
+
+
+def consts(afc, s, n):
+    import ctypes
+    k = np.zeros(n, np.float32)
+    a = np.zeros(n, np.float32)
+    afc.dll().AFH_GetSourceConstants(s.handle, k.ctypes.data_as(ctypes.c_void_p), a.ctypes.data_as(ctypes.c_void_p))
+    return k, a
+
+
+@pytest.fixture(scope="module")
+def cfg1(afc, oracle, torch_cuda):
+    src, k, a, area = oracle.config1()
+    s = build_system(afc, src, oracle.TEMP_340)
+    kk, aa = consts(afc, s, src.shape[0])
+    pts, bounds = oracle.grid_points(**area)
+    ref32 = oracle.field(1, src, kk, aa, pts, "f32")
+    ref64 = oracle.field(1, src, kk, aa, pts, "f64")
+    return dict(src=src, sys=s, k=kk, a=aa, pts=pts, bounds=bounds, ref32=ref32, ref64=ref64)
+
+
+def config1_area(afc):
+    return afc.GridAreaBuilder().x_range((40, 139)).y_range((20, 119)).z_at(150).resolution(1.0).generate()
+
+
+def test_config1_pressure_capi(afc, oracle, cfg1):
+    log("case: config1 PressureField via AFC_Calculate (GpuCalculator)")
+    calc = afc.GpuCalculator()
+    area = config1_area(afc)
+    assert area.get_size() == cfg1["bounds"]
+    got = calc.calculate(cfg1["sys"], area, afc.PressureFieldBuffer())
+    assert got.shape == (10000,)
+    want32 = oracle.reduce_f32(0, cfg1["ref32"])
+    e32 = float(np.linalg.norm(got - want32) / np.linalg.norm(want32))
+    e64 = float(np.linalg.norm(got - np.abs(cfg1["ref64"])) / np.linalg.norm(np.abs(cfg1["ref64"])))
+    log(f"  |p| rel-L2 vs FP32 CPU path {e32:.3e}, vs FP64 {e64:.3e}")
+    assert e32 <= 1e-5
+    assert e64 <= 5e-6
+    # the focus is where FocalPoint put it
+    assert np.argmax(got) == np.where((cfg1["pts"][:, 0] == 90) & (cfg1["pts"][:, 1] == 70))[0][0]
+
+
+def test_config1_complex_and_power(afc, oracle, cfg1):
+    log("case: config1 Complex/Power via AFC_Calculate")
+    calc = afc.GpuCalculator()
+    area = config1_area(afc)
+    cp = calc.calculate(cfg1["sys"], area, afc.ComplexPressureFieldBuffer())
+    e32 = rel_l2(cp, cfg1["ref32"])
+    e64 = rel_l2(cp, cfg1["ref64"])
+    log(f"  complex rel-L2 vs FP32 CPU {e32:.3e}, vs FP64 {e64:.3e}; CPU FP32 vs FP64 "
+        f"{rel_l2(cfg1['ref32'], cfg1['ref64']):.3e}")
+    assert e32 <= 1e-5 and e64 <= 5e-6
+    pw = calc.calculate(cfg1["sys"], area, afc.PowerFieldBuffer())
+    pr = calc.calculate(cfg1["sys"], area, afc.PressureFieldBuffer())
+    # the reductions are applied to the same complex sum: |p|^2 and sqrt(|p|^2)
+    c32 = cp.astype(np.complex64)
+    ns = (c32.real * c32.real + c32.imag * c32.imag).astype(np.float32)
+    assert np.array_equal(pw, ns)
+    assert np.array_equal(pr, np.sqrt(ns))
+
+
+def test_config1_accurate_mode(afc, oracle, cfg1):
+    log("case: config1 AccurateCalculator (polynomial sincos)")
+    calc = afc.AccurateCalculator()
+    cp = calc.calculate(cfg1["sys"], config1_area(afc), afc.ComplexPressureFieldBuffer())
+    e64 = rel_l2(cp, cfg1["ref64"])
+    log(f"  accurate rel-L2 vs FP64 {e64:.3e}")
+    assert e64 <= 2e-6
+    assert rel_l2(cp, cfg1["ref32"]) <= 1e-5
+
+
+def test_hip_tag_equals_gpu_tag(afc, cfg1):
+    a = afc.GpuCalculator().calculate(cfg1["sys"], config1_area(afc), afc.ComplexPressureFieldBuffer())
+    b = afc.HipCalculator().calculate(cfg1["sys"], config1_area(afc), afc.ComplexPressureFieldBuffer())
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("axes", [(1, 2, 3), (3, 1, 2), (2, 3, 1), (1, 3, 2), (2, 1, 3), (3, 2, 1)])
+def test_grid3d_orders(afc, oracle, cfg1, axes):
+    log(f"case: grid3d axes {axes}")
+    rng = {1: (60.0, 75.0), 2: (55.0, 66.0), 3: (140.0, 149.0)}
+    b = afc.GridAreaBuilder()
+    for ax in axes:
+        getattr(b, {1: "x_range", 2: "y_range", 3: "z_range"}[ax])(rng[ax])
+    area = b.resolution(1.5).generate()
+    got = afc.GpuCalculator().calculate(cfg1["sys"], area, afc.ComplexPressureFieldBuffer())
+    pts, _ = oracle.grid_points([a - 1 for a in axes], (rng[1], rng[2], rng[3]), 1.5)
+    want = oracle.field(1, cfg1["src"], cfg1["k"], cfg1["a"], pts, "f32")
+    assert got.shape == want.shape
+    assert rel_l2(got, want) <= 1e-5
+
+
+@pytest.mark.parametrize("builder", ["x", "y", "z", "xz", "zy", "yx"])
+def test_grid_1d_2d(afc, oracle, cfg1, builder):
+    log(f"case: grid {builder}")
+    rng = {"x": (60.0, 120.0), "y": (40.0, 100.0), "z": (100.0, 180.0)}
+    at = {"x": 90.0, "y": 70.0, "z": 150.0}
+    b = afc.GridAreaBuilder()
+    order = [("xyz".index(c)) for c in builder]
+    for c in builder:
+        getattr(b, f"{c}_range")(rng[c])
+    for c in "xyz":
+        if c not in builder:
+            getattr(b, f"{c}_at")(at[c])
+    area = b.resolution(0.75).generate()
+    got = afc.GpuCalculator().calculate(cfg1["sys"], area, afc.ComplexPressureFieldBuffer())
+    ranges = [rng[c] if c in builder else (at[c], at[c]) for c in "xyz"]
+    pts, _ = oracle.grid_points(order, ranges, 0.75)
+    want = oracle.field(1, cfg1["src"], cfg1["k"], cfg1["a"], pts, "f32")
+    assert rel_l2(got, want) <= 1e-5
+
+
+def test_scatter_equals_grid_bitwise(afc, oracle, cfg1):
+    """Grid points are generated on the device with the reference's rounding, so
+    the same points given as a scatter area produce identical sums."""
+    log("case: scatter vs grid")
+    area = config1_area(afc)
+    grid = afc.GpuCalculator().calculate(cfg1["sys"], area, afc.ComplexPressureFieldBuffer())
+    sc = afc.ScatterArea()
+    for p in cfg1["pts"]:
+        sc.add_observe_point(tuple(p))
+    scat = afc.GpuCalculator().calculate(cfg1["sys"], sc, afc.ComplexPressureFieldBuffer())
+    assert np.array_equal(grid, scat)
+
+
+def test_sphere_sources_random(afc, oracle, torch_cuda):
+    log("case: sphere sources, random geometry, scatter points")
+    rng = np.random.default_rng(11)
+    n = 301
+    src = np.zeros((n, 6), np.float32)
+    src[:, 0:2] = rng.uniform(0, 150, (n, 2))
+    src[:, 3] = rng.uniform(0.2, 1.0, n)
+    src[:, 4] = rng.uniform(-10, 10, n)  # phases outside [0, 2 pi) too
+    src[:, 5] = 40e3
+    s = build_system(afc, src, 300.0, source_type=0)
+    k, a = consts(afc, s, n)
+    pts = rng.uniform((0, 0, 20), (150, 150, 300), (3000, 3)).astype(np.float32)
+    sc = afc.ScatterArea()
+    for p in pts:
+        sc.add_observe_point(tuple(p))
+    got = afc.GpuCalculator().calculate(s, sc, afc.ComplexPressureFieldBuffer())
+    want = oracle.field(0, src, k, a, pts, "f32")
+    w64 = oracle.field(0, src, k, a, pts, "f64")
+    log(f"  rel-L2 vs FP32 {rel_l2(got, want):.3e} vs FP64 {rel_l2(got, w64):.3e}")
+    assert rel_l2(got, want) <= 1e-5 and rel_l2(got, w64) <= 5e-6
+
+
+def test_t4010a1_random_directions_all_angles(afc, oracle, torch_cuda):
+    """Observation points on all sides of tilted transducers: exercises every
+    directivity segment, including theta > 90 degrees (D = 0)."""
+    log("case: T4010A1 random directions")
+    rng = np.random.default_rng(5)
+    n = 97
+    pos = rng.uniform(-40, 40, (n, 3)).astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    src = oracle.t4010a1_sources(pos, amp=rng.uniform(0.5, 1, n), phase=rng.uniform(0, 6.3, n),
+                                 direction=d.astype(np.float32))
+    s = build_system(afc, src, 300.0)
+    k, a = consts(afc, s, n)
+    pts = rng.uniform(-120, 120, (4000, 3)).astype(np.float32)
+    sc = afc.ScatterArea()
+    for p in pts:
+        sc.add_observe_point(tuple(p))
+    got = afc.GpuCalculator().calculate(s, sc, afc.ComplexPressureFieldBuffer())
+    want = oracle.field(1, src, k, a, pts, "f32")
+    w64 = oracle.field(1, src, k, a, pts, "f64")
+    log(f"  rel-L2 vs FP32 {rel_l2(got, want):.3e} vs FP64 {rel_l2(got, w64):.3e}")
+    assert rel_l2(got, want) <= 1e-5 and rel_l2(got, w64) <= 5e-6
+    # pointwise: the largest error relative to the local field magnitude scale
+    scale = np.abs(w64).max()
+    assert np.max(np.abs(got - w64)) <= 1e-5 * scale
+
+
+def test_in_place_source_edit_is_seen(afc, oracle, cfg1):
+    """AFC_GetWaveSources hands out the system's own array (python/main.py:58-63)."""
+    log("case: in-place phase edit")
+    s = build_system(afc, cfg1["src"], oracle.TEMP_340)
+    calc = afc.GpuCalculator()
+    area = config1_area(afc)
+    before = calc.calculate(s, area, afc.ComplexPressureFieldBuffer())
+    srcs = s.get_wave_sources()
+    for i in range(len(srcs)):
+        srcs[i].phase = 0.0
+    after = calc.calculate(s, area, afc.ComplexPressureFieldBuffer())
+    src0 = cfg1["src"].copy()
+    src0[:, 7] = 0
+    want = oracle.field(1, src0, cfg1["k"], cfg1["a"], cfg1["pts"], "f32")
+    assert rel_l2(after, want) <= 1e-5 and rel_l2(before, after) > 0.1
+
+
+def test_empty_inputs(afc, oracle, cfg1):
+    log("case: empty source list / empty area")
+    calc = afc.GpuCalculator()
+    empty = afc.UniformSystem(300.0)
+    empty.add_wave_source(afc.T4010A1((0, 0, 0), (0, 0, 1), 1.0, 0.0, 40e3))
+    # no points
+    assert calc.calculate(cfg1["sys"], afc.ScatterArea(), afc.PressureFieldBuffer()).shape == (0,)
+    # amplitude zero everywhere -> exactly zero field
+    srcs = empty.get_wave_sources()
+    srcs[0].amp = 0.0
+    out = calc.calculate(empty, config1_area(afc), afc.PressureFieldBuffer())
+    assert np.all(out == 0)
+
+
+def test_point_on_a_source_is_not_finite(afc, oracle, cfg1):
+    """At r = 0 the reference divides by zero (uniform_medium.rs:117): non-finite."""
+    sc = afc.ScatterArea()
+    sc.add_observe_point(tuple(cfg1["src"][5, 0:3]))
+    sc.add_observe_point((90.0, 70.0, 150.0))
+    out = afc.GpuCalculator().calculate(cfg1["sys"], sc, afc.PressureFieldBuffer())
+    assert not np.isfinite(out[0]) and np.isfinite(out[1])
