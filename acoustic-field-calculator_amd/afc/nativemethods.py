@@ -89,6 +89,13 @@ def load(path: str = LIB_PATH):
     global AFC_DLL
     if AFC_DLL is not None:
         return AFC_DLL
+    try:
+        # One HIP runtime per process: when torch (which bundles its own
+        # libamdhip64.so.7) is present, load it first so that this library's
+        # libamdhip64.so.7 dependency resolves to the copy already mapped.
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(path):
         raise ImportError(f"libafc_hip.so not found at {path}: build it with "
                           f"`python acoustic-field-calculator_amd/build.py` (or __graft_entry__.build())")

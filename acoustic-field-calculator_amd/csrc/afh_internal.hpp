@@ -11,6 +11,9 @@
 
 namespace afh {
 
+// Point indices are 32-bit in the kernels (block base + lane stays below 2^32).
+constexpr uint64_t kMaxPointsPerCall = 1ull << 31;
+
 enum PointKind { kGrid = 0, kScatter = 1 };
 enum OutKind { kAbs = 0, kAbs2 = 1, kComplex = 2 };  // = AFC_FIELD_* tags
 

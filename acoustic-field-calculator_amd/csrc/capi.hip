@@ -117,9 +117,10 @@ bool DevBuf::reserve(size_t bytes) {
     ptr = nullptr;
     cap = 0;
     size_t want = std::max<size_t>(bytes, 256);
-    if (hipMalloc(&ptr, want) != hipSuccess) {
+    const hipError_t e = hipMalloc(&ptr, want);
+    if (e != hipSuccess) {
         (void)hipGetLastError();
-        set_error("hipMalloc failed for " + std::to_string(want) + " bytes");
+        set_error("hipMalloc failed for " + std::to_string(want) + " bytes: " + hipGetErrorString(e));
         ptr = nullptr;
         return false;
     }
@@ -138,9 +139,10 @@ bool HostBuf::reserve(size_t bytes) {
     ptr = nullptr;
     cap = 0;
     size_t want = std::max<size_t>(bytes, 256);
-    if (hipHostMalloc(&ptr, want, hipHostMallocDefault) != hipSuccess) {
+    const hipError_t e = hipHostMalloc(&ptr, want, hipHostMallocDefault);
+    if (e != hipSuccess) {
         (void)hipGetLastError();
-        set_error("hipHostMalloc failed for " + std::to_string(want) + " bytes");
+        set_error("hipHostMalloc failed for " + std::to_string(want) + " bytes: " + hipGetErrorString(e));
         ptr = nullptr;
         return false;
     }
@@ -294,8 +296,8 @@ uint64_t AFC_Calculate(void* calc_handle, void* system_handle, void* obs_handle,
         }
         n = grid->n;
     }
-    if (n > 0xFFFFFFFFull) {
-        set_error("AFC_Calculate: more than 2^32-1 observation points");
+    if (n > kMaxPointsPerCall) {
+        set_error("AFC_Calculate: more than 2^31 observation points in one call");
         return 0;
     }
     const size_t per_pt = field_type == AFC_FIELD_COMPLEX ? 2 : 1;
@@ -555,8 +557,8 @@ int32_t AFH_FieldEval(const float* packed_sources_dev, uint64_t n_packed, int32_
         set_error("AFH_FieldEval: bad point-set kind");
         return AFH_ERR_ARG;
     }
-    if (n > 0xFFFFFFFFull) {
-        set_error("AFH_FieldEval: more than 2^32-1 points");
+    if (n > kMaxPointsPerCall) {
+        set_error("AFH_FieldEval: more than 2^31 points in one call");
         return AFH_ERR_ARG;
     }
     FieldArgs a = field_args_for(gp, pts->points, n);

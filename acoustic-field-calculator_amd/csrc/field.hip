@@ -52,15 +52,17 @@ __device__ __forceinline__ void point_at(const FieldArgs& a, uint32_t idx, float
         y = a.points[3 * (uint64_t)idx + 1];
         z = a.points[3 * (uint64_t)idx + 2];
     } else {
+#pragma clang fp contract(off)
         const uint32_t n0 = idx % a.count0;
         const uint32_t rest = idx / a.count0;
         const uint32_t n1 = rest % a.count1;
         const uint32_t n2 = rest / a.count1;
         float p[3] = {a.origin[0], a.origin[1], a.origin[2]};
         // grid_area.rs: origin[axis] + (n as Float * resolution), two roundings
-        p[a.ax2] = __fadd_rn(a.origin[a.ax2], __fmul_rn((float)n2, a.res));
-        p[a.ax1] = __fadd_rn(a.origin[a.ax1], __fmul_rn((float)n1, a.res));
-        p[a.ax0] = __fadd_rn(a.origin[a.ax0], __fmul_rn((float)n0, a.res));
+        // (contraction is off in this block, so no fma)
+        p[a.ax2] = a.origin[a.ax2] + (float)n2 * a.res;
+        p[a.ax1] = a.origin[a.ax1] + (float)n1 * a.res;
+        p[a.ax0] = a.origin[a.ax0] + (float)n0 * a.res;
         x = p[0];
         y = p[1];
         z = p[2];
@@ -121,9 +123,10 @@ __global__ __launch_bounds__(kBlock) void field_kernel(FieldArgs a) {
         if (OUT == kComplex) {
             reinterpret_cast<float2*>(a.out)[idx] = make_float2(re, im);
         } else {
+#pragma clang fp contract(off)
             // pressure_field.rs:35-37 norm_sqr().sqrt(); power_field.rs:35-37 norm_sqr()
-            const float ns = __fadd_rn(__fmul_rn(re, re), __fmul_rn(im, im));
-            a.out[idx] = (OUT == kAbs) ? __fsqrt_rn(ns) : ns;
+            const float ns = re * re + im * im;
+            a.out[idx] = (OUT == kAbs) ? sqrtf(ns) : ns;  // correctly rounded (HIP default)
         }
     }
 }
@@ -132,7 +135,7 @@ template <int SRC, int PTS, int OUT, int PREC>
 static hipError_t launch4(const FieldArgs& a, hipStream_t stream) {
     constexpr int PPL = 4;
     const uint32_t per_block = kBlock * PPL;
-    const uint32_t blocks = (uint32_t)((a.n_points + per_block - 1) / per_block);
+    const uint32_t blocks = (uint32_t)(((uint64_t)a.n_points + per_block - 1) / per_block);
     hipLaunchKernelGGL((field_kernel<SRC, PTS, OUT, PREC, PPL>), dim3(blocks), dim3(kBlock), 0, stream, a);
     return hipGetLastError();
 }
