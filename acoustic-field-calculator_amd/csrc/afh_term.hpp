@@ -15,10 +15,11 @@
 //   * exp(-alpha r) = exp2(-(alpha log2 e) r) on v_exp_f32;
 //   * sin / cos: v_sin_f32 / v_cos_f32 (take revolutions) in the FAST mode, a
 //     quadrant-reduced Taylor polynomial in the ACCURATE mode;
-//   * T4010A1 directivity (t4010a1.rs:107-120): D = 1 for cos(theta) >= 0.94
-//     (theta <= 20 deg, segments 0-2 are identically 1), D = 0 for
-//     cos(theta) <= -0.001 (theta > 90 deg), otherwise theta = acos(c) and the
-//     reference's piecewise cubic with its ceil(theta / (pi/18)) binning.
+//   * T4010A1 directivity (t4010a1.rs:107-120): the ACCURATE mode takes
+//     theta = acos(c) and the reference's piecewise cubic with its
+//     ceil(theta / (pi/18)) binning (D = 1 for c >= 0.94, i.e. theta <= 20 deg
+//     where segments 0-2 are identically 1; D = 0 for c <= -0.001); the FAST
+//     mode reads D(c) from a cell table of quadratic pieces (DirCell).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -47,6 +48,22 @@ constexpr float kCosBandLo = -0.001f;                  // < cos(90 deg)
 struct DirEntry {
     float a, b, c, d, start;
 };
+
+// Directivity as a function of c = cos(theta), for the FAST mode. Cells of
+// width exactly 1/512 in c: cell j (1..kDirCells-2) covers
+// [(j - 2) / 512, (j - 1) / 512); cell 0 holds c < -1/512 (theta > 90 deg,
+// D = 0) and the last cell c >= 482/512 (theta < 20 deg, D = 1). Inside a
+// cell, with t = 512 c - floor(512 c) in [0, 1), D is a quadratic in t; a
+// cell that contains a segment boundary of the reference's piecewise cubic
+// (theta = 20, 30, ..., 90 deg) holds two quadratics split at t = ts, so the
+// jumps of the table are kept. Max deviation from the reference formula
+// evaluated in FP64: below 3e-8.
+struct alignas(16) DirCell {
+    float a0, a1, a2, ts;  // piece for t < ts
+    float b0, b1, b2, pad; // piece for t >= ts
+};
+constexpr int kDirCells = 485;
+constexpr float kDirScale = 512.0f;
 
 // acos for c in [-0.01, 1]: asin polynomial of the Cephes Math Library
 // (S. L. Moshier, asinf.c, minimax on [0, 0.5]) with the half-angle identity
@@ -91,11 +108,25 @@ __host__ __device__ inline void sincos_rev_poly(float u, float* s, float* c) {
     *c = ((qi + 1) & 2) ? -c0 : c0;
 }
 
-#if defined(__HIPCC__)
+// Reference coefficient table (t4010a1.rs:18-69) with the segment start
+// (i - 1) * ANGLE_DIV in f32; entry 10 stands for theta beyond 90 degrees.
+#define AFH_DIR_ENTRIES(E)                                                                                   \
+    E(0, 1.0f, 0.f, 0.f, 0.f)                                                                                \
+    E(1, 1.0f, 0.f, 0.f, 0.f)                                                                                \
+    E(2, 1.0f, 0.f, 0.f, 0.f)                                                                                \
+    E(3, 1.0f, -0.2633589359691162f, -2.586746661956269f, 0.0009174516976956062f)                           \
+    E(4, 0.891250938f, -0.8910683499820542f, -1.0097633564074366f, 0.0001704413361662757f)                  \
+    E(5, 0.707945784f, -1.1924098538381465f, -0.7167962799004945f, 4.362034715924911f)                      \
+    E(6, 0.501187234f, -1.0439934287916925f, 0.027352097655185065f, -2.238476800444034f)                    \
+    E(7, 0.354813389f, -0.7015151838490918f, 0.39509604454751995f, 1.274774620435337f)                      \
+    E(8, 0.251188643f, -0.4471050803642494f, 1.0625664749725618f, -1.1276955684895724f)                     \
+    E(9, 0.199526231f, -0.179254124486545f, 0.008242027161100222f, -0.9016569137759131f)                    \
+    E(10, 0.0f, 0.f, 0.f, 0.f)
+#define AFH_DIR_ENTRY(i, A, B, C, D) {A, B, C, D, (float(i) - 1.0f) * kAngleDiv},
 
 // Exact reference binning of theta (t4010a1.rs:108): i = ceil(theta / ANGLE_DIV)
 // with Rust's saturating cast, NaN -> 0. Division is IEEE (correctly rounded).
-__device__ inline float directivity_exact(float c, const DirEntry* tab) {
+__host__ __device__ inline float directivity_exact_t(float c, const DirEntry* tab) {
     const float theta = acos_band(c);
     const float q = ceilf(theta / kAngleDiv);
     int i = (q > 0.0f) ? (int)fminf(q, 10.0f) : 0;
@@ -104,19 +135,35 @@ __device__ inline float directivity_exact(float c, const DirEntry* tab) {
     return e.a + (e.b + (e.c + e.d * x) * x) * x;
 }
 
-__device__ inline float directivity(float c, const DirEntry* tab) {
+__host__ __device__ inline float directivity_band(float c, const DirEntry* tab) {
     float d = 1.0f;  // also the NaN case (c undefined at r = 0)
     if (c < kCosBandHi) {
         d = 0.0f;
-        if (c > kCosBandLo) d = directivity_exact(c, tab);
+        if (c > kCosBandLo) d = directivity_exact_t(c, tab);
     }
     return d;
 }
 
-// Adds one source's term at one point to (re, im).
+__host__ __device__ inline float directivity_table(float c, const DirCell* cells) {
+    const float cs = c * kDirScale;  // exact
+    const float fl = floorf(cs);
+    const float t = cs - fl;  // exact
+    int j = (int)fl + 2;
+    j = j < 0 ? 0 : (j > kDirCells - 1 ? kDirCells - 1 : j);
+    const DirCell& e = cells[j];
+    const float pa = fmaf(fmaf(e.a2, t, e.a1), t, e.a0);
+    const float pb = fmaf(fmaf(e.b2, t, e.b1), t, e.b0);
+    return t >= e.ts ? pb : pa;
+}
+
+#if defined(__HIPCC__)
+
+// Adds one source's term at one point to (re, im). The FAST mode takes the
+// directivity from the cell table, the ACCURATE mode evaluates the reference's
+// acos / ceil binning / cubic.
 template <int SRC, int PREC>
-__device__ __forceinline__ void add_term(const Src& s, float px, float py, float pz, const DirEntry* tab, float& re,
-                                         float& im) {
+__device__ __forceinline__ void add_term(const Src& s, float px, float py, float pz, const DirEntry* tab,
+                                         const DirCell* cells, float& re, float& im) {
     const float dx = px - s.x;
     const float dy = py - s.y;
     const float dz = pz - s.z;
@@ -127,7 +174,7 @@ __device__ __forceinline__ void add_term(const Src& s, float px, float py, float
     float amp = s.amp;
     if (SRC == kT4010A1) {
         const float c = fmaf(s.nx, dx, fmaf(s.ny, dy, s.nz * dz)) * rinv;
-        amp *= directivity(c, tab);
+        amp *= (PREC == kFast) ? directivity_table(c, cells) : directivity_band(c, tab);
     }
     const float att = __builtin_amdgcn_exp2f(-s.al * r);
     const float a = amp * att * rinv;

@@ -17,6 +17,11 @@
 //     source per wave;
 //   * sums: plain FP32 over chunks of kChunk sources, then TwoSum-compensated
 //     across chunks, so the rounding error does not grow with N.
+#include <math.h>
+#include <stdlib.h>
+
+#include <mutex>
+
 #include "afh_internal.hpp"
 
 namespace afh {
@@ -27,22 +32,113 @@ constexpr int kChunk = 32;
 static_assert(kTile % kChunk == 0, "tile must hold whole chunks");
 static_assert(AFH_SOURCE_PAD % kChunk == 0, "packed sources must be whole chunks");
 
-#define AFH_DIR_ENTRY(i, A, B, C, D) \
-    { A, B, C, D, (float(i) - 1.0f) * kAngleDiv }
-// t4010a1.rs:18-69, entries 0..9, and entry 10 for theta beyond 90 degrees.
-__constant__ DirEntry c_dir_tab[11] = {
-    AFH_DIR_ENTRY(0, 1.0f, 0.f, 0.f, 0.f),
-    AFH_DIR_ENTRY(1, 1.0f, 0.f, 0.f, 0.f),
-    AFH_DIR_ENTRY(2, 1.0f, 0.f, 0.f, 0.f),
-    AFH_DIR_ENTRY(3, 1.0f, -0.2633589359691162f, -2.586746661956269f, 0.0009174516976956062f),
-    AFH_DIR_ENTRY(4, 0.891250938f, -0.8910683499820542f, -1.0097633564074366f, 0.0001704413361662757f),
-    AFH_DIR_ENTRY(5, 0.707945784f, -1.1924098538381465f, -0.7167962799004945f, 4.362034715924911f),
-    AFH_DIR_ENTRY(6, 0.501187234f, -1.0439934287916925f, 0.027352097655185065f, -2.238476800444034f),
-    AFH_DIR_ENTRY(7, 0.354813389f, -0.7015151838490918f, 0.39509604454751995f, 1.274774620435337f),
-    AFH_DIR_ENTRY(8, 0.251188643f, -0.4471050803642494f, 1.0625664749725618f, -1.1276955684895724f),
-    AFH_DIR_ENTRY(9, 0.199526231f, -0.179254124486545f, 0.008242027161100222f, -0.9016569137759131f),
-    AFH_DIR_ENTRY(10, 0.0f, 0.f, 0.f, 0.f),
-};
+// t4010a1.rs:18-69 as device constants (ACCURATE mode) and a host copy.
+__constant__ DirEntry c_dir_tab[11] = {AFH_DIR_ENTRIES(AFH_DIR_ENTRY)};
+static const DirEntry h_dir_tab[11] = {AFH_DIR_ENTRIES(AFH_DIR_ENTRY)};
+
+// Directivity cells of the FAST mode, filled once per device from the host.
+__device__ DirCell g_dir_cells[kDirCells];
+
+// Reference directivity (t4010a1.rs:107-120) in double on the f32 constants.
+static double dir_ref(double theta, int seg) {
+    static const float A[10] = {0.f, 1.0f, 1.0f, 1.0f, 0.891250938f, 0.707945784f, 0.501187234f, 0.354813389f,
+                                0.251188643f, 0.199526231f};
+    static const float B[10] = {0.f, 0.f, 0.f, -0.2633589359691162f, -0.8910683499820542f, -1.1924098538381465f,
+                                -1.0439934287916925f, -0.7015151838490918f, -0.4471050803642494f,
+                                -0.179254124486545f};
+    static const float C[10] = {0.f, 0.f, 0.f, -2.586746661956269f, -1.0097633564074366f, -0.7167962799004945f,
+                                0.027352097655185065f, 0.39509604454751995f, 1.0625664749725618f,
+                                0.008242027161100222f};
+    static const float D[10] = {0.f, 0.f, 0.f, 0.0009174516976956062f, 0.0001704413361662757f,
+                                4.362034715924911f, -2.238476800444034f, 1.274774620435337f,
+                                -1.1276955684895724f, -0.9016569137759131f};
+    if (seg <= 0) return 1.0;
+    if (seg > 9) return 0.0;
+    const double x = theta - (seg - 1) * (double)kAngleDiv;
+    return A[seg] + (B[seg] + (C[seg] + D[seg] * x) * x) * x;
+}
+
+// Quadratic in t through Chebyshev nodes of [t0, t1] of D on segment `seg`,
+// with c = (cell_lo + t) / 512.
+static void fit_piece(double cell_lo, double t0, double t1, int seg, float* p) {
+    double tn[3], v[3];
+    for (int k = 0; k < 3; ++k) {
+        tn[k] = 0.5 * (t0 + t1) + 0.5 * (t1 - t0) * cos((2 * k + 1) * M_PI / 6.0);
+        const double c = (cell_lo + tn[k]) / 512.0;
+        v[k] = dir_ref(acos(c < -1 ? -1 : (c > 1 ? 1 : c)), seg);
+    }
+    // Newton divided differences -> monomial coefficients
+    const double d01 = (v[1] - v[0]) / (tn[1] - tn[0]);
+    const double d12 = (v[2] - v[1]) / (tn[2] - tn[1]);
+    const double d012 = (d12 - d01) / (tn[2] - tn[0]);
+    // v0 + d01 (t - t0) + d012 (t - t0)(t - t1)
+    p[2] = (float)d012;
+    p[1] = (float)(d01 - d012 * (tn[0] + tn[1]));
+    p[0] = (float)(v[0] - d01 * tn[0] + d012 * tn[0] * tn[1]);
+}
+
+// Segment of the reference for c (theta = acos c), i = ceil(theta / ANGLE_DIV).
+static int segment_of(double c) {
+    const double th = acos(c < -1 ? -1 : (c > 1 ? 1 : c));
+    const double q = ceil(th / (double)kAngleDiv);
+    return q <= 0 ? 0 : (q > 10 ? 10 : (int)q);
+}
+
+static void build_dir_cells(DirCell* cells) {
+    const float inf = __builtin_inff();
+    for (int j = 0; j < kDirCells; ++j) {
+        DirCell& e = cells[j];
+        e.ts = inf;
+        e.pad = 0.f;
+        e.a1 = e.a2 = e.b1 = e.b2 = 0.f;
+        if (j == 0) {
+            e.a0 = e.b0 = 0.f;
+            continue;
+        }
+        if (j == kDirCells - 1) {
+            e.a0 = e.b0 = 1.f;
+            continue;
+        }
+        const double lo = (double)(j - 2);  // in units of 1/512
+        // boundaries cos(k * ANGLE_DIV), k = 2..9, inside this cell
+        double split = -1;
+        for (int k = 2; k <= 9; ++k) {
+            const double cb = cos(k * (double)kAngleDiv) * 512.0 - lo;
+            if (cb > 0.0 && cb < 1.0) split = cb;
+        }
+        if (split < 0) {
+            const int seg = segment_of((lo + 0.5) / 512.0);
+            fit_piece(lo, 0.0, 1.0, seg, &e.a0);
+            e.b0 = e.a0;
+            e.b1 = e.a1;
+            e.b2 = e.a2;
+        } else {
+            fit_piece(lo, 0.0, split, segment_of((lo + 0.5 * split) / 512.0), &e.a0);
+            fit_piece(lo, split, 1.0, segment_of((lo + 0.5 * (1.0 + split)) / 512.0), &e.b0);
+            e.ts = (float)split;
+        }
+    }
+}
+
+static std::mutex g_dir_mu;
+static bool g_dir_ready[64];
+
+static hipError_t ensure_dir_cells() {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> g(g_dir_mu);
+    if (dev < 64 && g_dir_ready[dev]) return hipSuccess;
+    static DirCell host[kDirCells];
+    static bool built = false;
+    if (!built) {
+        build_dir_cells(host);
+        built = true;
+    }
+    e = hipMemcpyToSymbol(HIP_SYMBOL(g_dir_cells), host, sizeof(host), 0, hipMemcpyHostToDevice);
+    if (e == hipSuccess && dev < 64) g_dir_ready[dev] = true;
+    return e;
+}
 
 // Coordinates of point idx (< n_points).
 template <int PTS>
@@ -73,8 +169,13 @@ template <int SRC, int PTS, int OUT, int PREC, int PPL>
 __global__ __launch_bounds__(kBlock) void field_kernel(FieldArgs a) {
     __shared__ float4 s_src[kTile * 3];
     __shared__ DirEntry s_dir[11];
+    __shared__ DirCell s_cells[(SRC == kT4010A1 && PREC == kFast) ? kDirCells : 1];
     const int tid = threadIdx.x;
-    if (SRC == kT4010A1 && tid < 11) s_dir[tid] = c_dir_tab[tid];
+    if (SRC == kT4010A1 && PREC == kAccurate && tid < 11) s_dir[tid] = c_dir_tab[tid];
+    if (SRC == kT4010A1 && PREC == kFast) {
+        for (int v = tid; v < kDirCells * 2; v += kBlock)
+            reinterpret_cast<float4*>(s_cells)[v] = reinterpret_cast<const float4*>(g_dir_cells)[v];
+    }
 
     const uint32_t base = blockIdx.x * (uint32_t)(kBlock * PPL) + tid;
     float px[PPL], py[PPL], pz[PPL];
@@ -104,7 +205,7 @@ __global__ __launch_bounds__(kBlock) void field_kernel(FieldArgs a) {
                 const float4 q2 = s_src[(j0 + j) * 3 + 2];
                 const Src s = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
 #pragma unroll
-                for (int k = 0; k < PPL; ++k) add_term<SRC, PREC>(s, px[k], py[k], pz[k], s_dir, tr[k], ti[k]);
+                for (int k = 0; k < PPL; ++k) add_term<SRC, PREC>(s, px[k], py[k], pz[k], s_dir, s_cells, tr[k], ti[k]);
             }
 #pragma unroll
             for (int k = 0; k < PPL; ++k) {
@@ -131,13 +232,26 @@ __global__ __launch_bounds__(kBlock) void field_kernel(FieldArgs a) {
     }
 }
 
-template <int SRC, int PTS, int OUT, int PREC>
-static hipError_t launch4(const FieldArgs& a, hipStream_t stream) {
-    constexpr int PPL = 4;
+template <int SRC, int PTS, int OUT, int PREC, int PPL>
+static hipError_t launch_ppl(const FieldArgs& a, hipStream_t stream) {
     const uint32_t per_block = kBlock * PPL;
     const uint32_t blocks = (uint32_t)(((uint64_t)a.n_points + per_block - 1) / per_block);
     hipLaunchKernelGGL((field_kernel<SRC, PTS, OUT, PREC, PPL>), dim3(blocks), dim3(kBlock), 0, stream, a);
     return hipGetLastError();
+}
+
+static int points_per_lane() {
+    static int ppl = [] {
+        const char* e = getenv("AFH_FIELD_PPL");
+        return (e && atoi(e) == 8) ? 8 : 4;
+    }();
+    return ppl;
+}
+
+template <int SRC, int PTS, int OUT, int PREC>
+static hipError_t launch4(const FieldArgs& a, hipStream_t stream) {
+    return points_per_lane() == 8 ? launch_ppl<SRC, PTS, OUT, PREC, 8>(a, stream)
+                                  : launch_ppl<SRC, PTS, OUT, PREC, 4>(a, stream);
 }
 
 template <int SRC, int PTS, int OUT>
@@ -159,8 +273,27 @@ static hipError_t launch_pts(const FieldArgs& a, int pts, int out, int prec, hip
 
 hipError_t launch_field(const FieldArgs& a, int src, int pts, int out, int prec, hipStream_t stream) {
     if (a.n_points == 0) return hipSuccess;
+    if (src == kT4010A1 && prec == kFast) {
+        const hipError_t e = ensure_dir_cells();
+        if (e != hipSuccess) return e;
+    }
     return src == kT4010A1 ? launch_pts<kT4010A1>(a, pts, out, prec, stream)
                            : launch_pts<kSphere>(a, pts, out, prec, stream);
 }
 
 }  // namespace afh
+
+// Host evaluation of the device directivity formulas, for tests: mode
+// AFH_PREC_FAST reads the cell table, AFH_PREC_ACCURATE evaluates the
+// reference binning and cubic (both exactly as the kernels do).
+extern "C" int32_t AFH_DirectivityT4010A1(const float* cos_theta, uint64_t n, int32_t mode, float* out) {
+    using namespace afh;
+    static DirCell cells[kDirCells];
+    static std::once_flag once;
+    std::call_once(once, [] { build_dir_cells(cells); });
+    if ((!cos_theta || !out) && n) return AFH_ERR_ARG;
+    for (uint64_t i = 0; i < n; ++i)
+        out[i] = mode == AFH_PREC_ACCURATE ? directivity_band(cos_theta[i], h_dir_tab)
+                                           : directivity_table(cos_theta[i], cells);
+    return AFH_OK;
+}

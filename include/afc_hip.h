@@ -195,6 +195,12 @@ enum { AFH_PREC_FAST = 0, AFH_PREC_ACCURATE = 1 };
 int32_t AFH_FieldEval(const float* packed_sources_dev, uint64_t n_packed, int32_t source_type, const AFH_PointSet* pts,
                       int32_t field_type, int32_t precision, float* out_dev, void* stream);
 
+/* T4010A1 directivity as a function of cos(theta), evaluated on the host with
+ * exactly the formulas the kernels use in each precision mode (cell table
+ * for AFH_PREC_FAST, reference binning for AFH_PREC_ACCURATE). For tests and
+ * inspection; reference formula: core/wave_sources/t4010a1.rs:107-120. */
+int32_t AFH_DirectivityT4010A1(const float* cos_theta, uint64_t n, int32_t mode, float* out);
+
 #ifdef __cplusplus
 }
 #endif
