@@ -79,3 +79,38 @@ def grid_from_area(order, ranges, res):
         cnt.append(int(q) + 1 if q > 0 else 1)
     full = list(order) + [a for a in range(3) if a not in order]
     return grid_pointset(full, [cnt[a] for a in full], [ranges[a][0] for a in range(3)], res), tuple(cnt)
+
+
+ALGO_GSPAT, ALGO_GS, ALGO_NAIVE = 0, 1, 2
+HOLO_FP32, HOLO_MIXED, HOLO_MIXED_CLOSING, HOLO_MIXED_SPLIT = 0, 1, 2, 3
+
+
+class Context:
+    """A HIP stream plus grow-only workspace owned by the library (AFH_CreateContext)."""
+
+    def __init__(self):
+        self.handle = c_void_p()
+        _err("AFH_CreateContext", dll().AFH_CreateContext(byref(self.handle)))
+
+    def close(self):
+        if self.handle:
+            dll().AFH_DestroyContext(self.handle)
+            self.handle = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def holo_solve(ctx: Context, packed_dev_ptr: int, n_packed: int, n_sources: int, source_type: int, foci_dev_ptr: int,
+               amps_dev_ptr: int, offsets: np.ndarray, algo: int, precision: int, iterations: int,
+               out_amp_dev_ptr: int, out_phase_dev_ptr: int, stream_ptr: int = 0) -> None:
+    """Batched GS-PAT / GS / Naive on device arrays; offsets (host, uint32, batch + 1)."""
+    offs = np.ascontiguousarray(offsets, np.uint32)
+    rc = dll().AFH_HoloSolve(ctx.handle, c_void_p(packed_dev_ptr), n_packed, n_sources, source_type,
+                             c_void_p(foci_dev_ptr), c_void_p(amps_dev_ptr), offs.ctypes.data_as(c_void_p),
+                             offs.shape[0] - 1, algo, precision, iterations, c_void_p(out_amp_dev_ptr),
+                             c_void_p(out_phase_dev_ptr), c_void_p(stream_ptr))
+    _err("AFH_HoloSolve", rc)

@@ -1,7 +1,12 @@
 // Hologram solvers (GS, GS-PAT, Naive, LM) on the GPU.
 #pragma once
 
+#include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <mutex>
+
+#include "afh_host.hpp"
 
 namespace afh {
 
@@ -20,8 +25,51 @@ struct HoloProblem {
     uint64_t m;
 };
 
+// A batch of instances on the device (all pointers device memory except
+// offsets_host): instance b owns foci [offsets[b], offsets[b+1]).
+struct HoloBatch {
+    const float* sources;
+    uint32_t n, n_packed;
+    int source_type;
+    const float* foci;
+    const float* amps;
+    const uint32_t* offsets_host;
+    uint32_t batch;
+    int algo, precision;
+    uint32_t iterations;
+    float* out_amp;
+    float* out_phase;
+};
+
+// Stream + grow-only device workspace; calls on one context are serialized.
+struct Context {
+    std::mutex mu;
+    hipStream_t stream = nullptr;
+    DevBuf work, io;
+    HostBuf staging, io_host;
+    hipEvent_t upload_done = nullptr;
+    bool ensure_stream() {
+        if (stream) return true;
+        return hip_ok(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+    }
+    bool reserve_staging(size_t bytes);
+    ~Context() {
+        if (upload_done) (void)hipEventDestroy(upload_done);
+        if (stream) (void)hipStreamDestroy(stream);
+        work.release();
+        io.release();
+        staging.release();
+        io_host.release();
+    }
+};
+
+int32_t holo_solve(Context* ctx, const HoloBatch& hb, hipStream_t st);
+Context& default_context();
+
 // Solves one instance on the device and returns per-source amplitude and
 // phase (host arrays of n floats). false on error (message set).
 bool holo_solve_single(const HoloProblem& p, int algo, float* amp_out, float* phase_out);
+// LM (lm.rs:40-105): phases of the n sources.
+bool lm_solve_single(const HoloProblem& p, float* phase_out);
 
 }  // namespace afh

@@ -29,7 +29,12 @@ struct FieldArgs {
     float res;
     const float* points;  // scatter: 3 floats per point
     float* out;
+    const DirCell* cells;  // FAST-mode directivity table (set by launch_field)
 };
+
+// Host builder of the FAST-mode directivity cells and the per-device copy.
+void build_dir_cells(DirCell* cells);
+const DirCell* dir_cells_device();
 
 hipError_t launch_field(const FieldArgs& a, int src, int pts, int out, int prec, hipStream_t stream);
 

@@ -158,12 +158,12 @@ __host__ __device__ inline float directivity_table(float c, const DirCell* cells
 
 #if defined(__HIPCC__)
 
-// Adds one source's term at one point to (re, im). The FAST mode takes the
-// directivity from the cell table, the ACCURATE mode evaluates the reference's
-// acos / ceil binning / cubic.
+// One source's term at one point, (re, im) = A (cos phi, sin phi). The FAST
+// mode takes the directivity from the cell table, the ACCURATE mode evaluates
+// the reference's acos / ceil binning / cubic.
 template <int SRC, int PREC>
-__device__ __forceinline__ void add_term(const Src& s, float px, float py, float pz, const DirEntry* tab,
-                                         const DirCell* cells, float& re, float& im) {
+__device__ __forceinline__ void term_value(const Src& s, float px, float py, float pz, const DirEntry* tab,
+                                           const DirCell* cells, float& a, float& cs, float& sn) {
     const float dx = px - s.x;
     const float dy = py - s.y;
     const float dz = pz - s.z;
@@ -177,17 +177,24 @@ __device__ __forceinline__ void add_term(const Src& s, float px, float py, float
         amp *= (PREC == kFast) ? directivity_table(c, cells) : directivity_band(c, tab);
     }
     const float att = __builtin_amdgcn_exp2f(-s.al * r);
-    const float a = amp * att * rinv;
+    a = amp * att * rinv;
     const float t_hi = r * s.kh;
     const float t_lo = fmaf(r, s.kh, -t_hi) + fmaf(r_lo, s.kh, r * s.kl);
     const float u = __builtin_amdgcn_fractf(__builtin_amdgcn_fractf(t_hi) + (t_lo + s.ph));
-    float sn, cs;
     if (PREC == kFast) {
         sn = __builtin_amdgcn_sinf(u);
         cs = __builtin_amdgcn_cosf(u);
     } else {
         sincos_rev_poly(u, &sn, &cs);
     }
+}
+
+// Adds one source's term at one point to (re, im).
+template <int SRC, int PREC>
+__device__ __forceinline__ void add_term(const Src& s, float px, float py, float pz, const DirEntry* tab,
+                                         const DirCell* cells, float& re, float& im) {
+    float a, cs, sn;
+    term_value<SRC, PREC>(s, px, py, pz, tab, cells, a, cs, sn);
     re = fmaf(a, cs, re);
     im = fmaf(a, sn, im);
 }

@@ -36,9 +36,6 @@ static_assert(AFH_SOURCE_PAD % kChunk == 0, "packed sources must be whole chunks
 __constant__ DirEntry c_dir_tab[11] = {AFH_DIR_ENTRIES(AFH_DIR_ENTRY)};
 static const DirEntry h_dir_tab[11] = {AFH_DIR_ENTRIES(AFH_DIR_ENTRY)};
 
-// Directivity cells of the FAST mode, filled once per device from the host.
-__device__ DirCell g_dir_cells[kDirCells];
-
 // Reference directivity (t4010a1.rs:107-120) in double on the f32 constants.
 static double dir_ref(double theta, int seg) {
     static const float A[10] = {0.f, 1.0f, 1.0f, 1.0f, 0.891250938f, 0.707945784f, 0.501187234f, 0.354813389f,
@@ -84,7 +81,7 @@ static int segment_of(double c) {
     return q <= 0 ? 0 : (q > 10 ? 10 : (int)q);
 }
 
-static void build_dir_cells(DirCell* cells) {
+void build_dir_cells(DirCell* cells) {
     const float inf = __builtin_inff();
     for (int j = 0; j < kDirCells; ++j) {
         DirCell& e = cells[j];
@@ -120,24 +117,29 @@ static void build_dir_cells(DirCell* cells) {
     }
 }
 
-static std::mutex g_dir_mu;
-static bool g_dir_ready[64];
-
-static hipError_t ensure_dir_cells() {
-    int dev = 0;
-    hipError_t e = hipGetDevice(&dev);
-    if (e != hipSuccess) return e;
-    std::lock_guard<std::mutex> g(g_dir_mu);
-    if (dev < 64 && g_dir_ready[dev]) return hipSuccess;
+// Per-device copy of the cell table (device memory, uploaded once; the
+// kernels stage it into LDS).
+const DirCell* dir_cells_device() {
+    static std::mutex mu;
+    static DirCell* dev_tab[64] = {};
     static DirCell host[kDirCells];
     static bool built = false;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+    std::lock_guard<std::mutex> g(mu);
+    if (dev_tab[dev]) return dev_tab[dev];
     if (!built) {
         build_dir_cells(host);
         built = true;
     }
-    e = hipMemcpyToSymbol(HIP_SYMBOL(g_dir_cells), host, sizeof(host), 0, hipMemcpyHostToDevice);
-    if (e == hipSuccess && dev < 64) g_dir_ready[dev] = true;
-    return e;
+    DirCell* p = nullptr;
+    if (hipMalloc(&p, sizeof(host)) != hipSuccess) return nullptr;
+    if (hipMemcpy(p, host, sizeof(host), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(p);
+        return nullptr;
+    }
+    dev_tab[dev] = p;
+    return p;
 }
 
 // Coordinates of point idx (< n_points).
@@ -174,7 +176,7 @@ __global__ __launch_bounds__(kBlock) void field_kernel(FieldArgs a) {
     if (SRC == kT4010A1 && PREC == kAccurate && tid < 11) s_dir[tid] = c_dir_tab[tid];
     if (SRC == kT4010A1 && PREC == kFast) {
         for (int v = tid; v < kDirCells * 2; v += kBlock)
-            reinterpret_cast<float4*>(s_cells)[v] = reinterpret_cast<const float4*>(g_dir_cells)[v];
+            reinterpret_cast<float4*>(s_cells)[v] = reinterpret_cast<const float4*>(a.cells)[v];
     }
 
     const uint32_t base = blockIdx.x * (uint32_t)(kBlock * PPL) + tid;
@@ -271,11 +273,12 @@ static hipError_t launch_pts(const FieldArgs& a, int pts, int out, int prec, hip
     return pts == kScatter ? launch_out<SRC, kScatter>(a, out, prec, s) : launch_out<SRC, kGrid>(a, out, prec, s);
 }
 
-hipError_t launch_field(const FieldArgs& a, int src, int pts, int out, int prec, hipStream_t stream) {
-    if (a.n_points == 0) return hipSuccess;
+hipError_t launch_field(const FieldArgs& a_in, int src, int pts, int out, int prec, hipStream_t stream) {
+    if (a_in.n_points == 0) return hipSuccess;
+    FieldArgs a = a_in;
     if (src == kT4010A1 && prec == kFast) {
-        const hipError_t e = ensure_dir_cells();
-        if (e != hipSuccess) return e;
+        a.cells = dir_cells_device();
+        if (!a.cells) return hipErrorOutOfMemory;
     }
     return src == kT4010A1 ? launch_pts<kT4010A1>(a, pts, out, prec, stream)
                            : launch_pts<kSphere>(a, pts, out, prec, stream);

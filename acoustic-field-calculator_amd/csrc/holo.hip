@@ -1,12 +1,843 @@
-// Hologram solvers on the GPU (work in progress).
+// Hologram solvers on the GPU: GS-PAT, GS and naive linear synthesis, batched
+// over instances that share one transducer array.
+//
+// Reference algorithms (acoustic-field-optimizer/src/multiple_foci/):
+//   macros.rs:32-53            G_ij = term(source j -> focus i), phases forced to 0
+//   linear_synthesis/gs_pat.rs:38-81
+//       d_j = sum_i |G_ij|; B = (conj(G_ij) / d_j^2)^T; R = G B;
+//       gamma = R p0; 100 x {p = gamma / |gamma| p0; gamma = R p};
+//       p = gamma / |gamma|^2 p0^2; q = B p; amp = |q|, phase = arg q
+//   linear_synthesis/gs.rs:36-77
+//       100 x {gamma = G q; p = gamma / |gamma| p0; xi = G^H p; q = xi / |xi| q0}
+//   linear_synthesis/naive.rs:32-51   q = G^H p0
+//
+// GS-PAT formulation here: H_ij = G_ij / d_j (|H_ij| <= 1, the per-column
+// scaling), so R = H H^H. Three kernels per bucket of instances with the same
+// padded focus count MP:
+//   A  generates G column chunks in registers (never stored), reduces d_j in
+//      LDS and accumulates R -- in FP32 on the VALU (precision FP32), or on
+//      MFMA v_mfma_f32_16x16x32_f16 with FP32 accumulation from an FP16 copy
+//      of H (MIXED, MIXED_CLOSING) or from H split into two FP16 halves with
+//      three products hi hi^H + hi lo^H + lo hi^H (MIXED_SPLIT);
+//   B  runs the iterations with R held in registers, one workgroup per
+//      instance, reductions by fixed-order lane shuffles;
+//   C  regenerates G in FP32 for q = B p; in MIXED_CLOSING it first redoes the
+//      last gamma = R p exactly as G (B p) in FP32 (the closing FP32 iteration).
+// Every reduction is per instance and in a fixed order, so an instance's result
+// does not depend on the other instances of the batch.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <vector>
+
 #include "afh_holo.hpp"
+#include "afh_host.hpp"
 #include "afh_internal.hpp"
 
 namespace afh {
 
-bool holo_solve_single(const HoloProblem&, int, float*, float*) {
-    set_error("hologram solvers: not built yet");
-    return false;
+constexpr int kHB = 256;  // threads per workgroup
+constexpr int kKC = 32;   // G columns per chunk
+
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+struct HoloArgs {
+    const float* sources;  // packed, AFH_SOURCE_STRIDE floats each
+    uint32_t n;            // sources (outputs per instance)
+    uint32_t n_packed;     // multiple of kKC
+    const float* foci;     // 3 floats per focus, all instances
+    const float* amps;     // target amplitude per focus
+    const uint32_t* offsets;  // batch + 1
+    const uint32_t* inst;     // instance ids of this bucket
+    const DirCell* cells;
+    float2* R;   // bucket: MP * MP per instance (row-major)
+    float2* p;   // bucket: MP per instance
+    float2* G;   // GS: M x n_packed per instance (row-major), bucket-relative
+    float2* q;   // GS: n_packed per instance
+    float* out_amp;    // batch x n
+    float* out_phase;  // batch x n
+};
+
+__device__ __forceinline__ Src load_src(const float* sources, uint32_t j) {
+    const float4* p = reinterpret_cast<const float4*>(sources) + 3 * (uint64_t)j;
+    const float4 q0 = p[0], q1 = p[1], q2 = p[2];
+    Src s = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
+    s.ph = 0.0f;  // the optimizers zero every phase before building G
+    return s;
+}
+
+// Per-instance foci in LDS; rows >= M are padding.
+template <int MP>
+struct Foci {
+    float x[MP], y[MP], z[MP], a[MP];
+};
+
+template <int SRC, int MP>
+__device__ __forceinline__ void load_instance(const HoloArgs& a, uint32_t b, Foci<MP>& f, DirCell* s_cells,
+                                              uint32_t& M, uint32_t& base) {
+    base = a.offsets[b];
+    M = a.offsets[b + 1] - base;
+    for (int i = threadIdx.x; i < MP; i += kHB) {
+        const bool v = i < (int)M;
+        f.x[i] = v ? a.foci[3 * (uint64_t)(base + i)] : 0.f;
+        f.y[i] = v ? a.foci[3 * (uint64_t)(base + i) + 1] : 0.f;
+        f.z[i] = v ? a.foci[3 * (uint64_t)(base + i) + 2] : 1.f;
+        f.a[i] = v ? a.amps[base + i] : 0.f;
+    }
+    if (SRC == kT4010A1)
+        for (int v = threadIdx.x; v < kDirCells * 2; v += kHB)
+            reinterpret_cast<float4*>(s_cells)[v] = reinterpret_cast<const float4*>(a.cells)[v];
+}
+
+// G for column j0 + (t & 31) at rows (t >> 5) + 8 r, r < MP / 8 (zero for
+// padding rows and padding columns).
+template <int SRC, int MP>
+__device__ __forceinline__ void g_tile(const HoloArgs& a, uint32_t j0, const Foci<MP>& f, uint32_t M,
+                                       const DirCell* cells, float2 (&g)[MP / 8]) {
+    const int t = threadIdx.x;
+    const uint32_t j = j0 + (t & 31);
+    const int grp = t >> 5;
+    const Src s = load_src(a.sources, j);
+#pragma unroll
+    for (int r = 0; r < MP / 8; ++r) {
+        const int i = grp + 8 * r;
+        float amp, cs, sn;
+        term_value<SRC, kFast>(s, f.x[i], f.y[i], f.z[i], nullptr, cells, amp, cs, sn);
+        const bool v = i < (int)M;
+        g[r] = make_float2(v ? amp * cs : 0.f, v ? amp * sn : 0.f);
+    }
+}
+
+// 1 / d_j (or 1 / d_j^2) of the chunk's columns into s_inv[0..31]; d_j =
+// sum_i |G_ij| summed in a fixed order. Columns beyond n get 0.
+template <int MP>
+__device__ __forceinline__ void column_scale(const float2 (&g)[MP / 8], uint32_t j0, uint32_t n, bool squared,
+                                             float (*s_part)[kKC], float* s_inv) {
+    const int t = threadIdx.x;
+    float part = 0.f;
+#pragma unroll
+    for (int r = 0; r < MP / 8; ++r) part += sqrtf(g[r].x * g[r].x + g[r].y * g[r].y);
+    s_part[t >> 5][t & 31] = part;
+    __syncthreads();
+    if (t < kKC) {
+        float d = 0.f;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) d += s_part[q][t];
+        // a real column with d = 0 gives NaN, as conj(G) / 0 does in the reference
+        s_inv[t] = (j0 + t < n) ? (squared ? 1.0f / (d * d) : 1.0f / d) : 0.f;
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------- kernel A (FP32)
+template <int SRC, int MP>
+__global__ __launch_bounds__(kHB) void gspat_r_fp32(HoloArgs a) {
+    __shared__ DirCell s_cells[SRC == kT4010A1 ? kDirCells : 1];
+    __shared__ Foci<MP> f;
+    __shared__ float s_part[8][kKC];
+    __shared__ float s_inv[kKC];
+    __shared__ float2 s_h[MP][kKC + 1];
+    const int t = threadIdx.x;
+    const uint32_t k = blockIdx.x, b = a.inst[k];
+    uint32_t M, base;
+    load_instance<SRC, MP>(a, b, f, s_cells, M, base);
+    __syncthreads();
+    constexpr int BS = MP / 16;
+    const int rb = (t >> 4) * BS, cb = (t & 15) * BS;
+    float2 acc[BS][BS];
+#pragma unroll
+    for (int x = 0; x < BS; ++x)
+#pragma unroll
+        for (int y = 0; y < BS; ++y) acc[x][y] = make_float2(0.f, 0.f);
+
+    for (uint32_t j0 = 0; j0 < a.n_packed; j0 += kKC) {
+        float2 g[MP / 8];
+        g_tile<SRC, MP>(a, j0, f, M, s_cells, g);
+        column_scale<MP>(g, j0, a.n, false, s_part, s_inv);
+        const float inv = s_inv[t & 31];
+#pragma unroll
+        for (int r = 0; r < MP / 8; ++r) s_h[(t >> 5) + 8 * r][t & 31] = make_float2(g[r].x * inv, g[r].y * inv);
+        __syncthreads();
+        for (int jj = 0; jj < kKC; ++jj) {
+            float2 hr[BS], hc[BS];
+#pragma unroll
+            for (int x = 0; x < BS; ++x) hr[x] = s_h[rb + x][jj];
+#pragma unroll
+            for (int y = 0; y < BS; ++y) hc[y] = s_h[cb + y][jj];
+#pragma unroll
+            for (int x = 0; x < BS; ++x)
+#pragma unroll
+                for (int y = 0; y < BS; ++y) {
+                    // R_ik += H_ij conj(H_kj)
+                    acc[x][y].x = fmaf(hr[x].x, hc[y].x, fmaf(hr[x].y, hc[y].y, acc[x][y].x));
+                    acc[x][y].y = fmaf(hr[x].y, hc[y].x, fmaf(-hr[x].x, hc[y].y, acc[x][y].y));
+                }
+        }
+        __syncthreads();
+    }
+    float2* R = a.R + (uint64_t)k * MP * MP;
+#pragma unroll
+    for (int x = 0; x < BS; ++x)
+#pragma unroll
+        for (int y = 0; y < BS; ++y) R[(rb + x) * MP + cb + y] = acc[x][y];
+}
+
+// ---------------------------------------------------------------- kernel A (MFMA)
+// v_mfma_f32_16x16x32_f16: lane l supplies A[l % 16][8 (l / 16) + v] and
+// B[8 (l / 16) + v][l % 16], v = 0..7; it receives D[4 (l / 16) + e][l % 16].
+// With both operands taken as rows of H, D is a 16 x 16 block of H_a H_b^T.
+// SPLIT: H = hi + lo with both halves in FP16 (lo = H - hi rounded again) and
+// R = hi hi^H + hi lo^H + lo hi^H, FP32 accumulation; H is first scaled by a
+// per-instance power of two s ~ M / 2 (undone exactly on output) so that lo
+// stays in the normal FP16 range.
+template <int SRC, int MP, bool SPLIT>
+__global__ __launch_bounds__(kHB) void gspat_r_mfma(HoloArgs a) {
+    static_assert(MP >= 32, "one 16x16 tile per wave at least");
+    constexpr int T = MP / 16;
+    constexpr int TPW = T * T / 4;  // tiles per wave
+    constexpr int HS = kKC + 8;     // row stride in halves (80 bytes)
+    constexpr int NH = SPLIT ? 2 : 1;
+    __shared__ DirCell s_cells[SRC == kT4010A1 ? kDirCells : 1];
+    __shared__ Foci<MP> f;
+    __shared__ float s_part[8][kKC];
+    __shared__ float s_inv[kKC];
+    __shared__ __attribute__((aligned(16))) _Float16 s_hr[NH][MP * HS];
+    __shared__ __attribute__((aligned(16))) _Float16 s_hi[NH][MP * HS];
+    const int t = threadIdx.x;
+    const int w = t >> 6, l = t & 63;
+    const uint32_t k = blockIdx.x, b = a.inst[k];
+    uint32_t M, base;
+    load_instance<SRC, MP>(a, b, f, s_cells, M, base);
+    __syncthreads();
+    // s = 2^floor(log2(M)) / 2 (>= 1) for SPLIT, 1 otherwise
+    const float scale = SPLIT ? fmaxf(1.0f, exp2f(floorf(log2f((float)M)) - 1.0f)) : 1.0f;
+    floatx4 accRe[TPW], accIR[TPW], accRI[TPW];
+#pragma unroll
+    for (int s = 0; s < TPW; ++s) {
+        accRe[s] = floatx4{0.f, 0.f, 0.f, 0.f};
+        accIR[s] = accRe[s];
+        accRI[s] = accRe[s];
+    }
+    for (uint32_t j0 = 0; j0 < a.n_packed; j0 += kKC) {
+        float2 g[MP / 8];
+        g_tile<SRC, MP>(a, j0, f, M, s_cells, g);
+        column_scale<MP>(g, j0, a.n, false, s_part, s_inv);
+        const float inv = s_inv[t & 31] * scale;
+#pragma unroll
+        for (int r = 0; r < MP / 8; ++r) {
+            const int i = (t >> 5) + 8 * r;
+            const float hre = g[r].x * inv, him = g[r].y * inv;
+            const _Float16 rh = (_Float16)hre, ih = (_Float16)him;
+            s_hr[0][i * HS + (t & 31)] = rh;
+            s_hi[0][i * HS + (t & 31)] = ih;
+            if (SPLIT) {
+                s_hr[NH - 1][i * HS + (t & 31)] = (_Float16)(hre - (float)rh);
+                s_hi[NH - 1][i * HS + (t & 31)] = (_Float16)(him - (float)ih);
+            }
+        }
+        __syncthreads();
+        const int kb = (l >> 4) * 8;
+#pragma unroll
+        for (int s = 0; s < TPW; ++s) {
+            const int tau = w * TPW + s;
+            const int arow = (tau / T) * 16 + (l & 15);
+            const int brow = (tau % T) * 16 + (l & 15);
+            const half8 ar = *reinterpret_cast<const half8*>(&s_hr[0][arow * HS + kb]);
+            const half8 ai = *reinterpret_cast<const half8*>(&s_hi[0][arow * HS + kb]);
+            const half8 br = *reinterpret_cast<const half8*>(&s_hr[0][brow * HS + kb]);
+            const half8 bi = *reinterpret_cast<const half8*>(&s_hi[0][brow * HS + kb]);
+            if (SPLIT) {
+                const half8 arl = *reinterpret_cast<const half8*>(&s_hr[NH - 1][arow * HS + kb]);
+                const half8 ail = *reinterpret_cast<const half8*>(&s_hi[NH - 1][arow * HS + kb]);
+                const half8 brl = *reinterpret_cast<const half8*>(&s_hr[NH - 1][brow * HS + kb]);
+                const half8 bil = *reinterpret_cast<const half8*>(&s_hi[NH - 1][brow * HS + kb]);
+                // cross terms first: they are small, the hi products dominate the sum
+                accRe[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ar, brl, accRe[s], 0, 0, 0);
+                accRe[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(arl, br, accRe[s], 0, 0, 0);
+                accRe[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ai, bil, accRe[s], 0, 0, 0);
+                accRe[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ail, bi, accRe[s], 0, 0, 0);
+                accIR[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ai, brl, accIR[s], 0, 0, 0);
+                accIR[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ail, br, accIR[s], 0, 0, 0);
+                accRI[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ar, bil, accRI[s], 0, 0, 0);
+                accRI[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(arl, bi, accRI[s], 0, 0, 0);
+            }
+            accRe[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ar, br, accRe[s], 0, 0, 0);
+            accRe[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ai, bi, accRe[s], 0, 0, 0);
+            accIR[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ai, br, accIR[s], 0, 0, 0);
+            accRI[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ar, bi, accRI[s], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    const float unscale = 1.0f / (scale * scale);  // exact: scale is a power of two
+    float2* R = a.R + (uint64_t)k * MP * MP;
+#pragma unroll
+    for (int s = 0; s < TPW; ++s) {
+        const int tau = w * TPW + s;
+        const int col = (tau % T) * 16 + (l & 15);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int row = (tau / T) * 16 + 4 * (l >> 4) + e;
+            R[row * MP + col] = make_float2(accRe[s][e] * unscale, (accIR[s][e] - accRI[s][e]) * unscale);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- kernel B
+// gamma = R p0; iters x {p = gamma / |gamma| p0; gamma = R p}. Writes the
+// last p (closing) or p = gamma / |gamma|^2 p0^2 to a.p.
+template <int MP>
+__global__ __launch_bounds__(kHB) void gspat_iter(HoloArgs a, uint32_t iters, int closing) {
+    constexpr int TPR = kHB / MP;  // lanes per row (consecutive, same wave)
+    constexpr int CPT = MP / TPR;  // columns per lane
+    static_assert(TPR >= 1 && TPR <= 64, "row group within a wave");
+    __shared__ float2 s_p[2][MP];
+    const int t = threadIdx.x;
+    const int row = t / TPR, h = t % TPR;
+    const uint32_t k = blockIdx.x, b = a.inst[k];
+    const uint32_t base = a.offsets[b], M = a.offsets[b + 1] - base;
+    const float2* R = a.R + (uint64_t)k * MP * MP + (uint64_t)row * MP + h * CPT;
+    float2 r[CPT];
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) r[c] = R[c];
+    const bool valid = row < (int)M;
+    const float p0 = valid ? a.amps[base + row] : 0.f;
+    if (h == 0) s_p[0][row] = make_float2(p0, 0.f);
+    __syncthreads();
+    int cur = 0;
+    auto matvec = [&](int buf) {
+        float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int c = 0; c < CPT; ++c) {
+            const float2 pv = s_p[buf][h * CPT + c];
+            acc.x = fmaf(r[c].x, pv.x, fmaf(-r[c].y, pv.y, acc.x));
+            acc.y = fmaf(r[c].x, pv.y, fmaf(r[c].y, pv.x, acc.y));
+        }
+#pragma unroll
+        for (int off = TPR / 2; off >= 1; off >>= 1) {
+            acc.x += __shfl_xor(acc.x, off);
+            acc.y += __shfl_xor(acc.y, off);
+        }
+        return acc;
+    };
+    float2 gam = matvec(cur);
+    for (uint32_t it = 0; it < iters; ++it) {
+        if (h == 0) {
+            const float inv = p0 / sqrtf(gam.x * gam.x + gam.y * gam.y);
+            s_p[cur ^ 1][row] = valid ? make_float2(gam.x * inv, gam.y * inv) : make_float2(0.f, 0.f);
+        }
+        __syncthreads();
+        cur ^= 1;
+        gam = matvec(cur);
+    }
+    if (h == 0) {
+        float2 out;
+        if (closing) {
+            out = s_p[cur][row];
+        } else {
+            const float inv = p0 * p0 / (gam.x * gam.x + gam.y * gam.y);
+            out = valid ? make_float2(gam.x * inv, gam.y * inv) : make_float2(0.f, 0.f);
+        }
+        a.p[(uint64_t)k * MP + row] = out;
+    }
+}
+
+// ---------------------------------------------------------------- kernel C
+template <int SRC, int MP>
+__global__ __launch_bounds__(kHB) void gspat_q(HoloArgs a, int closing) {
+    __shared__ DirCell s_cells[SRC == kT4010A1 ? kDirCells : 1];
+    __shared__ Foci<MP> f;
+    __shared__ float s_part[8][kKC];
+    __shared__ float s_inv[kKC];
+    __shared__ float2 s_qp[8][kKC];
+    __shared__ float2 s_q[kKC];
+    __shared__ float2 s_p[MP];
+    const int t = threadIdx.x;
+    const int grp = t >> 5, jj = t & 31;
+    const uint32_t k = blockIdx.x, b = a.inst[k];
+    uint32_t M, base;
+    load_instance<SRC, MP>(a, b, f, s_cells, M, base);
+    for (int i = t; i < MP; i += kHB) s_p[i] = a.p[(uint64_t)k * MP + i];
+    __syncthreads();
+
+    // q_j partials of the chunk: w_j sum_i conj(G_ij) p_i, reduced over the 8
+    // row groups in a fixed order into s_q
+    auto chunk_q = [&](const float2 (&g)[MP / 8]) {
+        float2 part = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int r = 0; r < MP / 8; ++r) {
+            const float2 pv = s_p[grp + 8 * r];
+            part.x = fmaf(g[r].x, pv.x, fmaf(g[r].y, pv.y, part.x));
+            part.y = fmaf(g[r].x, pv.y, fmaf(-g[r].y, pv.x, part.y));
+        }
+        s_qp[grp][jj] = part;
+        __syncthreads();
+        if (t < kKC) {
+            float2 q = make_float2(0.f, 0.f);
+#pragma unroll
+            for (int q8 = 0; q8 < 8; ++q8) {
+                q.x += s_qp[q8][t].x;
+                q.y += s_qp[q8][t].y;
+            }
+            s_q[t] = make_float2(q.x * s_inv[t], q.y * s_inv[t]);
+        }
+        __syncthreads();
+    };
+
+    if (closing) {
+        float2 gam[MP / 8];
+#pragma unroll
+        for (int r = 0; r < MP / 8; ++r) gam[r] = make_float2(0.f, 0.f);
+        for (uint32_t j0 = 0; j0 < a.n_packed; j0 += kKC) {
+            float2 g[MP / 8];
+            g_tile<SRC, MP>(a, j0, f, M, s_cells, g);
+            column_scale<MP>(g, j0, a.n, true, s_part, s_inv);
+            chunk_q(g);
+            const float2 qj = s_q[jj];
+#pragma unroll
+            for (int r = 0; r < MP / 8; ++r) {
+                gam[r].x = fmaf(g[r].x, qj.x, fmaf(-g[r].y, qj.y, gam[r].x));
+                gam[r].y = fmaf(g[r].x, qj.y, fmaf(g[r].y, qj.x, gam[r].y));
+            }
+            __syncthreads();
+        }
+        // reduce over the 32 columns held by the half-wave of this row group
+#pragma unroll
+        for (int r = 0; r < MP / 8; ++r) {
+#pragma unroll
+            for (int off = 16; off >= 1; off >>= 1) {
+                gam[r].x += __shfl_xor(gam[r].x, off);
+                gam[r].y += __shfl_xor(gam[r].y, off);
+            }
+        }
+        if (jj == 0) {
+#pragma unroll
+            for (int r = 0; r < MP / 8; ++r) {
+                const int i = grp + 8 * r;
+                const float p0 = f.a[i];
+                const float inv = p0 * p0 / (gam[r].x * gam[r].x + gam[r].y * gam[r].y);
+                s_p[i] = i < (int)M ? make_float2(gam[r].x * inv, gam[r].y * inv) : make_float2(0.f, 0.f);
+            }
+        }
+        __syncthreads();
+    }
+    for (uint32_t j0 = 0; j0 < a.n_packed; j0 += kKC) {
+        float2 g[MP / 8];
+        g_tile<SRC, MP>(a, j0, f, M, s_cells, g);
+        column_scale<MP>(g, j0, a.n, true, s_part, s_inv);
+        chunk_q(g);
+        if (t < kKC && j0 + t < a.n) {
+            const float2 q = s_q[t];
+            a.out_amp[(uint64_t)b * a.n + j0 + t] = hypotf(q.x, q.y);
+            a.out_phase[(uint64_t)b * a.n + j0 + t] = atan2f(q.y, q.x);
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- GS
+// G (FP32, M x n_packed row-major) of every instance of the bucket.
+template <int SRC>
+__global__ __launch_bounds__(kHB) void gen_g(HoloArgs a, uint32_t g_stride_inst) {
+    __shared__ DirCell s_cells[SRC == kT4010A1 ? kDirCells : 1];
+    if (SRC == kT4010A1)
+        for (int v = threadIdx.x; v < kDirCells * 2; v += kHB)
+            reinterpret_cast<float4*>(s_cells)[v] = reinterpret_cast<const float4*>(a.cells)[v];
+    __syncthreads();
+    const uint32_t k = blockIdx.y, b = a.inst[k];
+    const uint32_t j = blockIdx.x * kHB + threadIdx.x;
+    if (j >= a.n_packed) return;
+    const uint32_t base = a.offsets[b], M = a.offsets[b + 1] - base;
+    const Src s = load_src(a.sources, j);
+    float2* G = a.G + (uint64_t)k * g_stride_inst;
+    for (uint32_t i = 0; i < M; ++i) {
+        float amp, cs, sn;
+        term_value<SRC, kFast>(s, a.foci[3 * (uint64_t)(base + i)], a.foci[3 * (uint64_t)(base + i) + 1],
+                               a.foci[3 * (uint64_t)(base + i) + 2], nullptr, s_cells, amp, cs, sn);
+        G[(uint64_t)i * a.n_packed + j] = make_float2(amp * cs, amp * sn);
+    }
+}
+
+__device__ __forceinline__ float2 wave_sum(float2 v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        v.x += __shfl_xor(v.x, off);
+        v.y += __shfl_xor(v.y, off);
+    }
+    return v;
+}
+
+template <int MP>
+__global__ __launch_bounds__(kHB) void gs_iter(HoloArgs a, uint32_t iters, uint32_t g_stride_inst) {
+    __shared__ float2 s_gam[MP];
+    __shared__ float2 s_p[MP];
+    const int t = threadIdx.x, w = t >> 6, l = t & 63;
+    const uint32_t k = blockIdx.x, b = a.inst[k];
+    const uint32_t base = a.offsets[b], M = a.offsets[b + 1] - base;
+    const uint32_t n = a.n, np = a.n_packed;
+    const float2* G = a.G + (uint64_t)k * g_stride_inst;
+    float2* q = a.q + (uint64_t)k * np;
+    // q0 = current source amplitudes (gs.rs:48-54)
+    for (uint32_t j = t; j < np; j += kHB) q[j] = make_float2(j < n ? a.sources[AFH_SOURCE_STRIDE * (uint64_t)j + 6] : 0.f, 0.f);
+    __syncthreads();
+    for (uint32_t it = 0; it < iters; ++it) {
+        // gamma = G q, a wave per row, lanes over columns
+        for (uint32_t i = w; i < M; i += 4) {
+            float2 acc = make_float2(0.f, 0.f);
+            for (uint32_t j = l; j < np; j += 64) {
+                const float2 gv = G[(uint64_t)i * np + j], qv = q[j];
+                acc.x = fmaf(gv.x, qv.x, fmaf(-gv.y, qv.y, acc.x));
+                acc.y = fmaf(gv.x, qv.y, fmaf(gv.y, qv.x, acc.y));
+            }
+            acc = wave_sum(acc);
+            if (l == 0) s_gam[i] = acc;
+        }
+        __syncthreads();
+        if (t < (int)M) {
+            const float2 g = s_gam[t];
+            const float inv = a.amps[base + t] / sqrtf(g.x * g.x + g.y * g.y);
+            s_p[t] = make_float2(g.x * inv, g.y * inv);
+        }
+        __syncthreads();
+        // xi = G^H p, q = xi / |xi| q0
+        for (uint32_t j = t; j < n; j += kHB) {
+            float2 xi = make_float2(0.f, 0.f);
+            for (uint32_t i = 0; i < M; ++i) {
+                const float2 gv = G[(uint64_t)i * np + j], pv = s_p[i];
+                xi.x = fmaf(gv.x, pv.x, fmaf(gv.y, pv.y, xi.x));
+                xi.y = fmaf(gv.x, pv.y, fmaf(-gv.y, pv.x, xi.y));
+            }
+            const float q0 = a.sources[AFH_SOURCE_STRIDE * (uint64_t)j + 6];
+            const float inv = q0 / sqrtf(xi.x * xi.x + xi.y * xi.y);
+            q[j] = make_float2(xi.x * inv, xi.y * inv);
+        }
+        __threadfence_block();
+        __syncthreads();
+    }
+    for (uint32_t j = t; j < n; j += kHB) {
+        const float2 qv = q[j];
+        a.out_amp[(uint64_t)b * n + j] = hypotf(qv.x, qv.y);
+        a.out_phase[(uint64_t)b * n + j] = atan2f(qv.y, qv.x);
+    }
+}
+
+// ---------------------------------------------------------------- naive
+template <int SRC>
+__global__ __launch_bounds__(kHB) void naive_q(HoloArgs a) {
+    __shared__ DirCell s_cells[SRC == kT4010A1 ? kDirCells : 1];
+    if (SRC == kT4010A1)
+        for (int v = threadIdx.x; v < kDirCells * 2; v += kHB)
+            reinterpret_cast<float4*>(s_cells)[v] = reinterpret_cast<const float4*>(a.cells)[v];
+    __syncthreads();
+    const uint32_t b = blockIdx.y;
+    const uint32_t j = blockIdx.x * kHB + threadIdx.x;
+    if (j >= a.n) return;
+    const uint32_t base = a.offsets[b], M = a.offsets[b + 1] - base;
+    const Src s = load_src(a.sources, j);
+    float2 q = make_float2(0.f, 0.f);
+    for (uint32_t i = 0; i < M; ++i) {
+        float amp, cs, sn;
+        term_value<SRC, kFast>(s, a.foci[3 * (uint64_t)(base + i)], a.foci[3 * (uint64_t)(base + i) + 1],
+                               a.foci[3 * (uint64_t)(base + i) + 2], nullptr, s_cells, amp, cs, sn);
+        const float p = a.amps[base + i];
+        // q += conj(G_ij) p_i
+        q.x = fmaf(amp * cs, p, q.x);
+        q.y = fmaf(-amp * sn, p, q.y);
+    }
+    a.out_amp[(uint64_t)b * a.n + j] = hypotf(q.x, q.y);
+    a.out_phase[(uint64_t)b * a.n + j] = atan2f(q.y, q.x);
+}
+
+// ---------------------------------------------------------------- host side
+static int padded_m(uint32_t m, bool mfma) {
+    if (m <= 16) return mfma ? 32 : 16;
+    if (m <= 32) return 32;
+    if (m <= 64) return 64;
+    if (m <= 128) return 128;
+    return 0;
+}
+
+template <int MP>
+static hipError_t launch_gspat_bucket(HoloArgs a, int src, int precision, uint32_t nb, uint32_t iters,
+                                      hipStream_t st) {
+    const dim3 grid(nb), blk(kHB);
+    if (precision == AFH_HOLO_FP32) {
+        if (src == kT4010A1)
+            hipLaunchKernelGGL((gspat_r_fp32<kT4010A1, MP>), grid, blk, 0, st, a);
+        else
+            hipLaunchKernelGGL((gspat_r_fp32<kSphere, MP>), grid, blk, 0, st, a);
+    } else {
+        if constexpr (MP >= 32) {
+            const bool split = precision == AFH_HOLO_MIXED_SPLIT;
+            if (src == kT4010A1) {
+                if (split)
+                    hipLaunchKernelGGL((gspat_r_mfma<kT4010A1, MP, true>), grid, blk, 0, st, a);
+                else
+                    hipLaunchKernelGGL((gspat_r_mfma<kT4010A1, MP, false>), grid, blk, 0, st, a);
+            } else {
+                if (split)
+                    hipLaunchKernelGGL((gspat_r_mfma<kSphere, MP, true>), grid, blk, 0, st, a);
+                else
+                    hipLaunchKernelGGL((gspat_r_mfma<kSphere, MP, false>), grid, blk, 0, st, a);
+            }
+        } else {
+            return hipErrorInvalidValue;
+        }
+    }
+    const int closing = precision == AFH_HOLO_MIXED_CLOSING ? 1 : 0;
+    hipLaunchKernelGGL((gspat_iter<MP>), grid, blk, 0, st, a, iters, closing);
+    if (src == kT4010A1)
+        hipLaunchKernelGGL((gspat_q<kT4010A1, MP>), grid, blk, 0, st, a, closing);
+    else
+        hipLaunchKernelGGL((gspat_q<kSphere, MP>), grid, blk, 0, st, a, closing);
+    return hipGetLastError();
+}
+
+template <int MP>
+static hipError_t launch_gs_bucket(HoloArgs a, int src, uint32_t nb, uint32_t iters, uint32_t g_stride,
+                                   hipStream_t st) {
+    const dim3 ggrid((a.n_packed + kHB - 1) / kHB, nb);
+    if (src == kT4010A1)
+        hipLaunchKernelGGL((gen_g<kT4010A1>), ggrid, dim3(kHB), 0, st, a, g_stride);
+    else
+        hipLaunchKernelGGL((gen_g<kSphere>), ggrid, dim3(kHB), 0, st, a, g_stride);
+    hipLaunchKernelGGL((gs_iter<MP>), dim3(nb), dim3(kHB), 0, st, a, iters, g_stride);
+    return hipGetLastError();
+}
+
+bool Context::reserve_staging(size_t bytes) {
+    if (upload_done) {
+        // the previous call's uploads must have left the staging buffer
+        if (!hip_ok(hipEventSynchronize(upload_done), "hipEventSynchronize")) return false;
+    }
+    return staging.reserve(bytes);
+}
+
+int32_t holo_solve(Context* ctx, const HoloBatch& hb, hipStream_t st) {
+    const uint32_t B = hb.batch;
+    if (B == 0) return AFH_OK;
+    const bool mfma = hb.precision != AFH_HOLO_FP32;
+    // bucket the instances by padded focus count
+    const int kinds[4] = {16, 32, 64, 128};
+    std::vector<uint32_t> lists[4];
+    for (uint32_t b = 0; b < B; ++b) {
+        const uint32_t m = hb.offsets_host[b + 1] - hb.offsets_host[b];
+        if (m == 0) {
+            set_error("holo: an instance has no foci");
+            return AFH_ERR_ARG;
+        }
+        const int mp = padded_m(m, mfma && hb.algo == kAlgoGSPAT);
+        if (mp == 0) {
+            set_error("holo: more than 128 foci in one instance is not supported");
+            return AFH_ERR_UNSUPPORTED;
+        }
+        for (int q = 0; q < 4; ++q)
+            if (kinds[q] == mp) lists[q].push_back(b);
+    }
+    const uint32_t np = hb.n_packed;
+    // workspace layout: offsets | instance lists | R | p | G | q
+    size_t off_bytes = (B + 1) * sizeof(uint32_t);
+    size_t list_bytes = B * sizeof(uint32_t);
+    size_t r_elems = 0, p_elems = 0, g_elems = 0, q_elems = 0;
+    for (int q = 0; q < 4; ++q) {
+        const size_t nb = lists[q].size(), mp = kinds[q];
+        if (hb.algo == kAlgoGSPAT) {
+            r_elems += nb * mp * mp;
+            p_elems += nb * mp;
+        } else if (hb.algo == kAlgoGS) {
+            g_elems += nb * mp * np;
+            q_elems += nb * np;
+        }
+    }
+    auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t o_off = 0, o_list = align(off_bytes), o_r = o_list + align(list_bytes);
+    const size_t o_p = o_r + align(r_elems * sizeof(float2)), o_g = o_p + align(p_elems * sizeof(float2));
+    const size_t o_q = o_g + align(g_elems * sizeof(float2)), total = o_q + align(q_elems * sizeof(float2));
+    if (!ctx->work.reserve(total)) return AFH_ERR_HIP;
+    if (!ctx->reserve_staging(off_bytes + list_bytes)) return AFH_ERR_HIP;
+    uint32_t* stage = static_cast<uint32_t*>(ctx->staging.ptr);
+    memcpy(stage, hb.offsets_host, off_bytes);
+    {
+        uint32_t* lp = stage + (B + 1);
+        for (int q = 0; q < 4; ++q)
+            for (uint32_t b : lists[q]) *lp++ = b;
+    }
+    char* wbase = static_cast<char*>(ctx->work.ptr);
+    if (!hip_ok(hipMemcpyAsync(wbase + o_off, stage, off_bytes, hipMemcpyHostToDevice, st), "H2D offsets") ||
+        !hip_ok(hipMemcpyAsync(wbase + o_list, stage + (B + 1), list_bytes, hipMemcpyHostToDevice, st), "H2D lists"))
+        return AFH_ERR_HIP;
+    if (!ctx->upload_done && !hip_ok(hipEventCreateWithFlags(&ctx->upload_done, hipEventDisableTiming), "event"))
+        return AFH_ERR_HIP;
+    if (!hip_ok(hipEventRecord(ctx->upload_done, st), "hipEventRecord")) return AFH_ERR_HIP;
+
+    HoloArgs a;
+    memset(&a, 0, sizeof(a));
+    a.sources = hb.sources;
+    a.n = hb.n;
+    a.n_packed = np;
+    a.foci = hb.foci;
+    a.amps = hb.amps;
+    a.offsets = reinterpret_cast<const uint32_t*>(wbase + o_off);
+    a.cells = dir_cells_device();
+    a.out_amp = hb.out_amp;
+    a.out_phase = hb.out_phase;
+    if (!a.cells) {
+        set_error("holo: directivity table allocation failed");
+        return AFH_ERR_HIP;
+    }
+    size_t list_pos = 0, r_pos = 0, p_pos = 0, g_pos = 0, q_pos = 0;
+    for (int q = 0; q < 4; ++q) {
+        const uint32_t nb = (uint32_t)lists[q].size();
+        const int mp = kinds[q];
+        if (nb == 0) continue;
+        a.inst = reinterpret_cast<const uint32_t*>(wbase + o_list) + list_pos;
+        a.R = reinterpret_cast<float2*>(wbase + o_r) + r_pos;
+        a.p = reinterpret_cast<float2*>(wbase + o_p) + p_pos;
+        a.G = reinterpret_cast<float2*>(wbase + o_g) + g_pos;
+        a.q = reinterpret_cast<float2*>(wbase + o_q) + q_pos;
+        hipError_t e = hipSuccess;
+        if (hb.algo == kAlgoGSPAT) {
+            switch (mp) {
+                case 16: e = launch_gspat_bucket<16>(a, hb.source_type, hb.precision, nb, hb.iterations, st); break;
+                case 32: e = launch_gspat_bucket<32>(a, hb.source_type, hb.precision, nb, hb.iterations, st); break;
+                case 64: e = launch_gspat_bucket<64>(a, hb.source_type, hb.precision, nb, hb.iterations, st); break;
+                default: e = launch_gspat_bucket<128>(a, hb.source_type, hb.precision, nb, hb.iterations, st); break;
+            }
+            r_pos += (size_t)nb * mp * mp;
+            p_pos += (size_t)nb * mp;
+        } else if (hb.algo == kAlgoGS) {
+            const uint32_t gstride = (uint32_t)(mp * np);
+            switch (mp) {
+                case 16: e = launch_gs_bucket<16>(a, hb.source_type, nb, hb.iterations, gstride, st); break;
+                case 32: e = launch_gs_bucket<32>(a, hb.source_type, nb, hb.iterations, gstride, st); break;
+                case 64: e = launch_gs_bucket<64>(a, hb.source_type, nb, hb.iterations, gstride, st); break;
+                default: e = launch_gs_bucket<128>(a, hb.source_type, nb, hb.iterations, gstride, st); break;
+            }
+            g_pos += (size_t)nb * mp * np;
+            q_pos += (size_t)nb * np;
+        }
+        if (!hip_ok(e, "holo kernel launch")) return AFH_ERR_HIP;
+        list_pos += nb;
+    }
+    if (hb.algo == kAlgoNaive) {
+        const dim3 grid((hb.n + kHB - 1) / kHB, B);
+        if (hb.source_type == kT4010A1)
+            hipLaunchKernelGGL((naive_q<kT4010A1>), grid, dim3(kHB), 0, st, a);
+        else
+            hipLaunchKernelGGL((naive_q<kSphere>), grid, dim3(kHB), 0, st, a);
+        if (!hip_ok(hipGetLastError(), "naive kernel launch")) return AFH_ERR_HIP;
+    }
+    return AFH_OK;
+}
+
+// ---------------------------------------------------------------- single instance (AFO_*)
+Context& default_context() {
+    static Context c;
+    return c;
+}
+
+bool holo_solve_single(const HoloProblem& p, int algo, float* amp_out, float* phase_out) {
+    if (algo == kAlgoLM) return lm_solve_single(p, phase_out);
+    if (!device().ensure()) return false;
+    Context& ctx = default_context();
+    std::lock_guard<std::mutex> g(ctx.mu);
+    if (!ctx.ensure_stream()) return false;
+    const uint64_t n = p.n, np = packed_count(n), m = p.m;
+    const size_t src_b = np * AFH_SOURCE_STRIDE * sizeof(float), foci_b = m * 3 * sizeof(float);
+    const size_t amps_b = m * sizeof(float), out_b = n * sizeof(float);
+    auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t o_src = 0, o_foci = align(src_b), o_amps = o_foci + align(foci_b), o_amp = o_amps + align(amps_b);
+    const size_t o_ph = o_amp + align(out_b), total = o_ph + align(out_b);
+    if (!ctx.io.reserve(total) || !ctx.io_host.reserve(total)) return false;
+    char* h = static_cast<char*>(ctx.io_host.ptr);
+    char* d = static_cast<char*>(ctx.io.ptr);
+    pack_sources(p.sources, p.wavenums, p.attens, n, p.source_type, reinterpret_cast<float*>(h + o_src));
+    memcpy(h + o_foci, p.foci, foci_b);
+    memcpy(h + o_amps, p.amps, amps_b);
+    if (!hip_ok(hipMemcpyAsync(d, h, o_amp, hipMemcpyHostToDevice, ctx.stream), "H2D")) return false;
+    const uint32_t offs[2] = {0, (uint32_t)m};
+    HoloBatch hb;
+    hb.sources = reinterpret_cast<const float*>(d + o_src);
+    hb.n = (uint32_t)n;
+    hb.n_packed = (uint32_t)np;
+    hb.source_type = p.source_type;
+    hb.foci = reinterpret_cast<const float*>(d + o_foci);
+    hb.amps = reinterpret_cast<const float*>(d + o_amps);
+    hb.offsets_host = offs;
+    hb.batch = 1;
+    hb.algo = algo;
+    hb.precision = AFH_HOLO_FP32;
+    hb.iterations = 100;  // REPEAT in gs.rs:17 and gs_pat.rs:17
+    hb.out_amp = reinterpret_cast<float*>(d + o_amp);
+    hb.out_phase = reinterpret_cast<float*>(d + o_ph);
+    if (holo_solve(&ctx, hb, ctx.stream) != AFH_OK) return false;
+    if (!hip_ok(hipMemcpyAsync(h + o_amp, d + o_amp, o_ph + out_b - o_amp, hipMemcpyDeviceToHost, ctx.stream), "D2H"))
+        return false;
+    if (!hip_ok(hipStreamSynchronize(ctx.stream), "hipStreamSynchronize")) return false;
+    memcpy(amp_out, h + o_amp, out_b);
+    memcpy(phase_out, h + o_ph, out_b);
+    return true;
 }
 
 }  // namespace afh
+
+using namespace afh;
+
+extern "C" {
+
+int32_t AFH_CreateContext(void** out) {
+    clear_error();
+    if (!out) return AFH_ERR_ARG;
+    *out = nullptr;
+    if (!device().ensure()) return AFH_ERR_NO_DEVICE;
+    Context* c = new Context();
+    if (!c->ensure_stream()) {
+        delete c;
+        return AFH_ERR_HIP;
+    }
+    *out = c;
+    return AFH_OK;
+}
+
+void AFH_DestroyContext(void* ctx) {
+    Context* c = static_cast<Context*>(ctx);
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    delete c;
+}
+
+int32_t AFH_HoloSolve(void* ctx, const float* packed_sources_dev, uint64_t n_packed, uint64_t n_sources,
+                      int32_t source_type, const float* foci_dev, const float* amps_dev, const uint32_t* offsets_host,
+                      uint32_t batch, int32_t algo, int32_t precision, uint32_t iterations, float* out_amp_dev,
+                      float* out_phase_dev, void* stream) {
+    clear_error();
+    Context* c = static_cast<Context*>(ctx);
+    if (!c || !packed_sources_dev || !foci_dev || !amps_dev || !offsets_host || !out_amp_dev || !out_phase_dev ||
+        n_packed % AFH_SOURCE_PAD != 0 || n_packed < n_sources || n_packed > 0xFFFFFFFFull ||
+        (source_type != kSphere && source_type != kT4010A1) || algo < kAlgoGSPAT || algo > kAlgoNaive ||
+        precision < AFH_HOLO_FP32 || precision > AFH_HOLO_MIXED_SPLIT ||
+        (algo != kAlgoGSPAT && precision != AFH_HOLO_FP32) || offsets_host[0] != 0) {
+        set_error("AFH_HoloSolve: bad argument");
+        return AFH_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> g(c->mu);
+    HoloBatch hb;
+    hb.sources = packed_sources_dev;
+    hb.n = (uint32_t)n_sources;
+    hb.n_packed = (uint32_t)n_packed;
+    hb.source_type = source_type;
+    hb.foci = foci_dev;
+    hb.amps = amps_dev;
+    hb.offsets_host = offsets_host;
+    hb.batch = batch;
+    hb.algo = algo;
+    hb.precision = precision;
+    hb.iterations = iterations;
+    hb.out_amp = out_amp_dev;
+    hb.out_phase = out_phase_dev;
+    return holo_solve(c, hb, stream ? (hipStream_t)stream : c->stream);
+}
+
+}  // extern "C"

@@ -6,8 +6,14 @@ random phases, amplitude 1) observed on a 256 x 256 x 128 grid at 1 mm
 One step = one evaluation of the whole volume (3.342e10 source-point terms)
 with sources and output resident in HBM and the grid generated on device.
 
+A second section times batched GS-PAT: 1024 instances on the same 3984-source
+array, 2..64 foci each (uniform), foci in [0,170]^2 x [100,250] mm, targets in
+[0.5, 1], 100 iterations, per precision mode; it reports solves/s, the batch
+latency and the focal-amplitude error of sampled instances against the FP64
+oracle, beside the unbatched plain FP32 path (AFO_GSPAT, one instance per call).
+
 Prints one JSON line (metric, value in Gsrc*pt/s, roofline of the field
-kernel, CPU baseline of the reference's FP32 path restated in C).
+kernel, CPU baseline of the reference's FP32 path restated in C, GS-PAT).
 """
 import argparse
 import json
@@ -67,6 +73,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--precision", choices=["fast", "accurate"], default="fast")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-gspat", action="store_true")
+    ap.add_argument("--gspat-batch", type=int, default=1024)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -148,10 +156,90 @@ def main():
                      "flop_per_term": FLOP_PER_TERM, "kernel_ms": round(kern_ms, 4)},
         "accuracy": {"rel_l2_vs_fp64": err, "sample": "512 random points of the timed output"},
     }
+    if not args.no_gspat:
+        result["gspat"] = bench_gspat(torch, afc, D, O, src, k, a, args.gspat_batch)
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline(src, k, a, area["order"], cnt, o, area["res"])
     print(json.dumps(result), flush=True)
+
+
+def bench_gspat(torch, afc, D, O, src, k, a, batch):
+    """Batched GS-PAT (gs_pat.rs:38-81) on device-resident inputs."""
+    src = src.copy()
+    src[:, 7] = 0.0  # GS-PAT zeroes the phases (gs_pat.rs:39-41)
+    n = src.shape[0]
+    foci, amps, offs = O.gspat_batch(batch, m_range=(2, 64), seed=3)
+    dev = torch.device("cuda:0")
+    packed = D.pack_sources(1, src, k, a)
+    src_d = torch.from_numpy(packed).to(dev)
+    foci_d = torch.from_numpy(foci).to(dev)
+    amps_d = torch.from_numpy(amps).to(dev)
+    out_a = torch.empty((batch, n), dtype=torch.float32, device=dev)
+    out_p = torch.empty((batch, n), dtype=torch.float32, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    ctx = D.Context()
+    rng = np.random.default_rng(1)
+    sample = rng.choice(batch, 6, replace=False)
+    ref = {}
+    for b in sample:
+        f, am = foci[offs[b]:offs[b + 1]], amps[offs[b]:offs[b + 1]]
+        a64, p64 = O.gspat(1, src, k, a, f, am, "f64")
+        s2 = src.copy()
+        s2[:, 6], s2[:, 7] = a64, p64
+        ref[b] = np.abs(O.field(1, s2, k, a, f, "f64"))
+    modes = {"fp32": D.HOLO_FP32, "mixed": D.HOLO_MIXED, "mixed_split": D.HOLO_MIXED_SPLIT,
+             "mixed_closing": D.HOLO_MIXED_CLOSING}
+    out = {"workload": f"GS-PAT B={batch}, N={n}, M~U[2,64] (sum M={int(offs[-1])}), 100 iterations"}
+    for name, prec in modes.items():
+        def run():
+            D.holo_solve(ctx, src_d.data_ptr(), packed.shape[0], n, 1, foci_d.data_ptr(), amps_d.data_ptr(), offs,
+                         D.ALGO_GSPAT, prec, 100, out_a.data_ptr(), out_p.data_ptr(), stream.cuda_stream)
+        run()
+        torch.cuda.synchronize()
+        reps = 5
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+        for e0, e1 in ev:
+            e0.record(stream)
+            run()
+            e1.record(stream)
+        torch.cuda.synchronize()
+        ms = float(np.median([e0.elapsed_time(e1) for e0, e1 in ev]))
+        am = out_a.cpu().numpy()
+        ph = out_p.cpu().numpy()
+        errs = []
+        for b in sample:
+            f = foci[offs[b]:offs[b + 1]]
+            s2 = src.copy()
+            s2[:, 6], s2[:, 7] = am[b], ph[b]
+            got = np.abs(O.field(1, s2, k, a, f, "f64"))
+            errs.append(float(np.max(np.abs(got - ref[b]) / ref[b])))
+        out[name] = {"solves_per_s": round(batch / (ms * 1e-3), 1), "batch_ms": round(ms, 4),
+                     "focal_amp_rel_err_max": max(errs), "sample_instances": int(len(sample))}
+        log(f"gspat {name}: {ms:.3f} ms per batch of {batch}, {batch / (ms * 1e-3):.0f} solves/s, "
+            f"max focal-amplitude rel err vs FP64 {max(errs):.2e}")
+    ctx.close()
+    # the plain FP32 path: AFO_GSPAT, one instance per call through the C ABI
+    system = afc.UniformSystem(O.TEMP_340)
+    for row in src:
+        system.add_wave_source(afc.T4010A1(row[0:3], row[3:6], row[6], row[7], row[8]))
+    import time as _t
+    reps = 20
+    lat = []
+    for b in range(reps):
+        f, am = foci[offs[b]:offs[b + 1]], amps[offs[b]:offs[b + 1]]
+        srcs = system.source_array()
+        srcs[:, 6] = 1.0
+        t0 = _t.perf_counter()
+        afc.Optimizer.gs_pat(system, f, am)
+        lat.append(_t.perf_counter() - t0)
+    out["plain_fp32_unbatched"] = {"solves_per_s": round(reps / sum(lat), 1),
+                                   "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                                   "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3)}
+    out["speedup_mixed_split_vs_plain"] = round(out["mixed_split"]["solves_per_s"] /
+                                                out["plain_fp32_unbatched"]["solves_per_s"], 1)
+    log(f"gspat plain FP32 unbatched: {reps / sum(lat):.0f} solves/s")
+    return out
 
 
 if __name__ == "__main__":

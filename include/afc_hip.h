@@ -195,6 +195,37 @@ enum { AFH_PREC_FAST = 0, AFH_PREC_ACCURATE = 1 };
 int32_t AFH_FieldEval(const float* packed_sources_dev, uint64_t n_packed, int32_t source_type, const AFH_PointSet* pts,
                       int32_t field_type, int32_t precision, float* out_dev, void* stream);
 
+/* ---- hologram solvers, batched over instances sharing one transducer array */
+
+/* A context owns a HIP stream and a grow-only device workspace on the current
+ * device; calls on one context are serialized (use one context per thread or
+ * stream for concurrency). */
+int32_t AFH_CreateContext(void** out);
+void AFH_DestroyContext(void* ctx);
+
+enum { AFH_ALGO_GSPAT = 0, AFH_ALGO_GS = 1, AFH_ALGO_NAIVE = 2 };
+/* GS-PAT precision: FP32 forms R = G diag(1/d^2) G^H on the VALU; MIXED forms
+ * it from an FP16 copy of G scaled per column by 1/d (|entries| <= 1) on MFMA
+ * with FP32 accumulation; MIXED_CLOSING additionally redoes the last R p
+ * product as G (B p) in FP32 before q = B p; MIXED_SPLIT carries each FP16
+ * entry as a hi + lo pair and forms hi hi^H + hi lo^H + lo hi^H on MFMA
+ * (near-FP32 R). GS and Naive are FP32 only. */
+enum { AFH_HOLO_FP32 = 0, AFH_HOLO_MIXED = 1, AFH_HOLO_MIXED_CLOSING = 2, AFH_HOLO_MIXED_SPLIT = 3 };
+
+/* Solves `batch` instances. packed_sources_dev: AFH_PackSources layout
+ * (phases ignored: the optimizers zero them, gs_pat.rs:39-41); instance b
+ * has foci foci_dev[3*offsets_host[b] ...] (3 floats each) with target
+ * amplitudes amps_dev[offsets_host[b] ...], offsets_host[0] = 0, at most 128
+ * foci per instance. Writes per-source amplitude |q| and phase arg q of
+ * instance b to out_amp_dev / out_phase_dev [b * n_sources ...]
+ * (gs_pat.rs:75-80, gs.rs:71-76, naive.rs:45-50). Asynchronous on `stream`
+ * (NULL: the context's stream). iterations: 100 in the reference
+ * (gs.rs:17, gs_pat.rs:17). */
+int32_t AFH_HoloSolve(void* ctx, const float* packed_sources_dev, uint64_t n_packed, uint64_t n_sources,
+                      int32_t source_type, const float* foci_dev, const float* amps_dev, const uint32_t* offsets_host,
+                      uint32_t batch, int32_t algo, int32_t precision, uint32_t iterations, float* out_amp_dev,
+                      float* out_phase_dev, void* stream);
+
 /* T4010A1 directivity as a function of cos(theta), evaluated on the host with
  * exactly the formulas the kernels use in each precision mode (cell table
  * for AFH_PREC_FAST, reference binning for AFH_PREC_ACCURATE). For tests and

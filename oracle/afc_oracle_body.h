@@ -524,6 +524,42 @@ uint32_t SFX(orc_lm)(int source_type, const float* sources, const float* wavenum
     return it;
 }
 
+/* F(x) and g(x) of the LM objective for given x (nls/macros.rs:17-58), for
+ * gradient checks: returns F, writes g (n + m). */
+R SFX(orc_lm_eval)(int source_type, const float* sources, const float* wavenums, const float* attens, uint64_t n,
+                   const float* foci, const float* amps, uint64_t m, const R* x, R* g_out) {
+    const uint64_t np = n + m;
+    SFX(cplx)* G = SFX(make_G)(source_type, sources, wavenums, attens, n, foci, m);
+    SFX(cplx)* Bm = (SFX(cplx)*)calloc(m * np, sizeof(SFX(cplx)));
+    SFX(cplx)* BhB = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * np * np);
+    SFX(cplx)* T = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * np);
+    SFX(cplx)* tv = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * np);
+    SFX(cplx)* row = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * np);
+    R* A = (R*)malloc(sizeof(R) * np * np);
+    R F;
+    uint64_t i, j, k;
+    for (j = 0; j < n; ++j)
+        for (i = 0; i < m; ++i) Bm[i + j * m] = G[i + j * m];
+    for (i = 0; i < m; ++i) Bm[i + (n + i) * m] = SFX(c_make)(-(R)amps[i], 0);
+    for (j = 0; j < np; ++j) {
+        for (k = 0; k < np; ++k) BhB[k + j * np] = SFX(c_make)(0, 0);
+        for (i = 0; i < m; ++i)
+            for (k = 0; k < np; ++k)
+                BhB[k + j * np] = SFX(c_add)(SFX(c_mul)(Bm[i + j * m], SFX(c_conj)(Bm[i + k * m])), BhB[k + j * np]);
+    }
+    for (k = 0; k < np; ++k) T[k] = SFX(c_make)(RCOS(x[k]), -RSIN(x[k]));
+    SFX(lm_A_g)(BhB, T, np, A, g_out);
+    F = SFX(lm_Fx)(BhB, x, np, tv, row);
+    free(G);
+    free(Bm);
+    free(BhB);
+    free(T);
+    free(tv);
+    free(row);
+    free(A);
+    return F;
+}
+
 /* FocalPoint phases: acoustic-field-optimizer/src/focal_point.rs:28-39 */
 void SFX(orc_focal_point_phases)(int source_type, const float* sources, uint64_t n, R sound_speed, R fx, R fy, R fz,
                                  R* out_phase) {

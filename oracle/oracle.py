@@ -65,6 +65,9 @@ def lib():
             f = getattr(L, f"orc_lm_{sfx}")
             f.restype = c_uint32
             f.argtypes = [c_int, fp, fp, fp, c_uint64, fp, fp, c_uint64, c_uint32, fp, fp]
+            f = getattr(L, f"orc_lm_eval_{sfx}")
+            f.restype = R
+            f.argtypes = [c_int, fp, fp, fp, c_uint64, fp, fp, c_uint64, fp, fp]
             f = getattr(L, f"orc_focal_point_phases_{sfx}")
             f.restype = None
             f.argtypes = [c_int, fp, c_uint64, R, R, R, R, fp]
@@ -183,6 +186,16 @@ def lm(source_type, sources, wavenums, attens, foci, amps, prec="f64", k_max=200
     it = getattr(lib(), f"orc_lm_{sfx}")(source_type, _p(src), _p(k), _p(a), n, _p(f), _p(am), m, k_max, _p(ph),
                                          _p(x))
     return ph, x, int(it)
+
+
+def lm_eval(source_type, sources, wavenums, attens, foci, amps, x):
+    """(F(x), g(x)) of the LM objective in FP64 (nls/macros.rs:17-58)."""
+    src, k, a, f, am = _f32(sources), _f32(wavenums), _f32(attens), _f32(foci).reshape(-1, 3), _f32(amps)
+    n, m = k.shape[0], f.shape[0]
+    xx = np.ascontiguousarray(x, np.float64)
+    g = np.empty(n + m, np.float64)
+    F = lib().orc_lm_eval_f64(source_type, _p(src), _p(k), _p(a), n, _p(f), _p(am), m, _p(xx), _p(g))
+    return F, g
 
 
 def focal_point_phases(source_type, sources, sound_speed_mm_s, focus, prec="f32") -> np.ndarray:

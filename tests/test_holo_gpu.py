@@ -1,0 +1,164 @@
+"""GPU parity of the hologram solvers against the oracle.
+
+Tolerances:
+  * single-instance AFO_GSPAT / AFO_GS / AFO_Naive (FP32 on the GPU) vs the
+    reference's FP32 restatement: focal amplitudes within 1e-3 relative of the
+    FP64 oracle's, source phases within 1e-3 rad where amplitudes are not tiny;
+  * batched GS-PAT, FP32 and MIXED_SPLIT: focal amplitudes within 1e-3
+    relative of the FP64 oracle, instance by instance; MIXED and
+    MIXED_CLOSING (single FP16 copy of G) within 5e-3;
+  * an instance gives bit-identical results alone and inside a batch.
+Reference: acoustic-field-optimizer/src/multiple_foci/linear_synthesis/
+{gs,gs_pat,naive}.rs, capi/src/optimizer.rs:80-127.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def log(*a):
+    print(*a, flush=True)
+
+
+def build_system(afc, src, temperature):
+    s = afc.UniformSystem(temperature)
+    for row in src:
+        s.add_wave_source(afc.T4010A1(row[0:3], row[3:6], row[6], row[7], row[8]))
+    return s
+
+
+def setup_array(oracle, tiles=(1, 1)):
+    pos = oracle.tiled_array(*tiles) if tiles != (1, 1) else oracle.array_249()
+    src = oracle.t4010a1_sources(pos)
+    n = pos.shape[0]
+    k = np.full(n, oracle.WAVENUM_40K, np.float32)
+    a = np.full(n, np.float32(oracle.attenuation_coef(40e3, 30, 1, 1, oracle.TEMP_340)), np.float32)
+    return src, k, a
+
+
+def with_solution(src, amp, phase):
+    s = src.copy()
+    s[:, 6] = amp
+    s[:, 7] = phase
+    return s
+
+
+def focal_amps(oracle, src, k, a, foci, amp, phase):
+    return np.abs(oracle.field(1, with_solution(src, amp, phase), k, a, foci, "f64"))
+
+
+def random_foci(rng, m, lo=(40, 30, 120), hi=(130, 110, 200)):
+    return np.stack([rng.uniform(lo[0], hi[0], m), rng.uniform(lo[1], hi[1], m), rng.uniform(lo[2], hi[2], m)],
+                    1).astype(np.float32)
+
+
+@pytest.mark.parametrize("method,m", [("gs_pat", 2), ("gs_pat", 9), ("gs", 3), ("naive", 4)])
+def test_single_instance_capi(afc, oracle, torch_cuda, method, m):
+    log(f"case: AFO_{method} single instance, M={m}")
+    rng = np.random.default_rng(m)
+    src, k, a = setup_array(oracle)
+    foci = random_foci(rng, m)
+    amps = rng.uniform(0.5, 1.0, m).astype(np.float32)
+    s = build_system(afc, src, oracle.TEMP_340)
+    getattr(afc.Optimizer, method)(s, foci, amps)
+    got = s.source_array().copy()
+    ref = {"gs_pat": oracle.gspat, "gs": oracle.gs, "naive": oracle.naive}[method]
+    a64, p64 = ref(1, src, k, a, foci, amps, "f64")
+    f_gpu = focal_amps(oracle, src, k, a, foci, got[:, 6], got[:, 7])
+    f_ref = focal_amps(oracle, src, k, a, foci, a64, p64)
+    err = np.max(np.abs(f_gpu - f_ref) / f_ref)
+    log(f"  focal amplitude max rel err vs FP64 oracle: {err:.3e}")
+    assert err <= 1e-3
+    # the complex drive q_j, relative to the largest one
+    q_gpu = got[:, 6].astype(np.float64) * np.exp(1j * got[:, 7].astype(np.float64))
+    q_ref = a64 * np.exp(1j * p64)
+    assert np.max(np.abs(q_gpu - q_ref)) <= 1e-3 * np.max(np.abs(q_ref))
+
+
+def run_batch(afc, torch, oracle, src, k, a, foci, amps, offs, algo, precision, iters=100):
+    from afc import device as D
+    dev = torch.device("cuda:0")
+    packed = D.pack_sources(1, src, k, a)
+    src_d = torch.from_numpy(packed).to(dev)
+    foci_d = torch.from_numpy(np.ascontiguousarray(foci, np.float32)).to(dev)
+    amps_d = torch.from_numpy(np.ascontiguousarray(amps, np.float32)).to(dev)
+    B = offs.shape[0] - 1
+    n = src.shape[0]
+    out_a = torch.empty((B, n), dtype=torch.float32, device=dev)
+    out_p = torch.empty((B, n), dtype=torch.float32, device=dev)
+    ctx = D.Context()
+    D.holo_solve(ctx, src_d.data_ptr(), packed.shape[0], n, 1, foci_d.data_ptr(), amps_d.data_ptr(), offs, algo,
+                 precision, iters, out_a.data_ptr(), out_p.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    ctx.close()
+    return out_a.cpu().numpy(), out_p.cpu().numpy()
+
+
+@pytest.fixture(scope="module")
+def batch16(oracle):
+    """16 instances on the 3984-transducer array, M from 2 to 128."""
+    src, k, a = setup_array(oracle, (4, 4))
+    rng = np.random.default_rng(42)
+    ms = np.array([2, 5, 16, 17, 31, 32, 33, 47, 64, 3, 128, 100, 65, 9, 24, 40], np.int64)
+    offs = np.zeros(ms.size + 1, np.uint32)
+    offs[1:] = np.cumsum(ms)
+    foci = random_foci(rng, int(offs[-1]), lo=(0, 0, 100), hi=(730, 570, 250))
+    amps = rng.uniform(0.5, 1.0, int(offs[-1])).astype(np.float32)
+    ref = []
+    for b in range(ms.size):
+        f = foci[offs[b]:offs[b + 1]]
+        am = amps[offs[b]:offs[b + 1]]
+        a64, p64 = oracle.gspat(1, src, k, a, f, am, "f64")
+        ref.append(focal_amps(oracle, src, k, a, f, a64, p64))
+    return dict(src=src, k=k, a=a, foci=foci, amps=amps, offs=offs, ref=ref)
+
+
+@pytest.mark.parametrize("precision,tol", [(0, 1e-3), (3, 1e-3), (2, 5e-3), (1, 5e-3)])
+def test_gspat_batched_vs_fp64(afc, oracle, torch_cuda, batch16, precision, tol):
+    log(f"case: batched GS-PAT precision={precision}, 16 instances, N=3984")
+    d = batch16
+    amp, ph = run_batch(afc, torch_cuda, oracle, d["src"], d["k"], d["a"], d["foci"], d["amps"], d["offs"], 0,
+                        precision)
+    errs = []
+    for b in range(d["offs"].shape[0] - 1):
+        f = d["foci"][d["offs"][b]:d["offs"][b + 1]]
+        got = focal_amps(oracle, d["src"], d["k"], d["a"], f, amp[b], ph[b])
+        errs.append(float(np.max(np.abs(got - d["ref"][b]) / d["ref"][b])))
+    ms = np.diff(d["offs"])
+    log(f"  per-instance max rel focal-amplitude error: max {max(errs):.3e} (M={ms[int(np.argmax(errs))]}), "
+        f"median {np.median(errs):.3e}")
+    assert max(errs) <= tol
+
+
+def test_gspat_instance_isolation(afc, oracle, torch_cuda, batch16):
+    """An instance's result does not depend on its batch-mates."""
+    d = batch16
+    for precision in (0, 3):
+        full_a, full_p = run_batch(afc, torch_cuda, oracle, d["src"], d["k"], d["a"], d["foci"], d["amps"], d["offs"],
+                                   0, precision)
+        for b in (0, 8, 10):
+            lo, hi = d["offs"][b], d["offs"][b + 1]
+            one = np.array([0, hi - lo], np.uint32)
+            a1, p1 = run_batch(afc, torch_cuda, oracle, d["src"], d["k"], d["a"], d["foci"][lo:hi], d["amps"][lo:hi],
+                               one, 0, precision)
+            assert np.array_equal(a1[0], full_a[b]) and np.array_equal(p1[0], full_p[b])
+
+
+def test_gs_and_naive_batched(afc, oracle, torch_cuda):
+    log("case: batched GS / Naive on the 249 array")
+    src, k, a = setup_array(oracle)
+    rng = np.random.default_rng(3)
+    ms = np.array([2, 7, 20], np.int64)
+    offs = np.zeros(4, np.uint32)
+    offs[1:] = np.cumsum(ms)
+    foci = random_foci(rng, int(offs[-1]))
+    amps = rng.uniform(0.5, 1.0, int(offs[-1])).astype(np.float32)
+    for algo, ref in ((1, oracle.gs), (2, oracle.naive)):
+        amp, ph = run_batch(afc, torch_cuda, oracle, src, k, a, foci, amps, offs, algo, 0)
+        for b in range(3):
+            f, am = foci[offs[b]:offs[b + 1]], amps[offs[b]:offs[b + 1]]
+            a64, p64 = ref(1, src, k, a, f, am, "f64")
+            got = focal_amps(oracle, src, k, a, f, amp[b], ph[b])
+            want = focal_amps(oracle, src, k, a, f, a64, p64)
+            assert np.max(np.abs(got - want) / want) <= 1e-3, (algo, b)

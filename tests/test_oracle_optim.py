@@ -1,0 +1,115 @@
+"""Pins the oracle's optimizers (oracle/afc_oracle_body.h) with closed forms
+and identities rather than restated formulas.
+
+Reference: acoustic-field-optimizer/src/multiple_foci/linear_synthesis/
+{gs,gs_pat,naive}.rs, nls/{lm,macros}.rs, multiple_foci/macros.rs."""
+import numpy as np
+import pytest
+
+
+def small_problem(oracle, n_tiles=1, m=1, seed=0):
+    rng = np.random.default_rng(seed)
+    pos = oracle.array_249()
+    src = oracle.t4010a1_sources(pos)
+    n = pos.shape[0]
+    k = np.full(n, oracle.WAVENUM_40K, np.float32)
+    a = np.full(n, np.float32(oracle.attenuation_coef(40e3, 30, 1, 1, oracle.TEMP_340)), np.float32)
+    foci = np.stack([rng.uniform(40, 130, m), rng.uniform(30, 110, m), rng.uniform(120, 200, m)], 1).astype(np.float32)
+    amps = rng.uniform(0.5, 1.0, m).astype(np.float32)
+    return src, k, a, foci, amps
+
+
+def with_solution(src, amp, phase):
+    s = src.copy()
+    s[:, 6] = amp
+    s[:, 7] = phase
+    return s
+
+
+def focal(oracle, src, k, a, foci):
+    return oracle.field(1, src, k, a, foci, "f64")
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_gspat_single_focus_closed_form(oracle, prec):
+    """M = 1: R = N, the final p is p0 / N and q_j = conj(G_j) / |G_j|^2 p0 / N,
+    so amp_j = p0 / (N |G_j|), phase_j = -arg G_j and the focal field is p0."""
+    src, k, a, foci, amps = small_problem(oracle, m=1)
+    amp, ph = oracle.gspat(1, src, k, a, foci, amps, prec)
+    G = oracle.propagation_matrix(1, with_solution(src, 1.0, 0.0), k, a, foci, "f64")[0]
+    n = src.shape[0]
+    tol = 2e-5 if prec == "f32" else 1e-12
+    assert np.allclose(amp, amps[0] / (n * np.abs(G)), rtol=tol)
+    assert np.allclose(np.angle(np.exp(1j * (ph + np.angle(G)))), 0, atol=tol * 10)
+    p = focal(oracle, with_solution(src, amp, ph), k, a, foci)[0]
+    assert abs(p) == pytest.approx(amps[0], rel=max(tol * 10, 1e-6))  # solution stored as f32
+
+
+def test_gs_keeps_source_amplitudes(oracle):
+    src, k, a, foci, amps = small_problem(oracle, m=5, seed=1)
+    amp, _ = oracle.gs(1, src, k, a, foci, amps, "f64")
+    assert np.allclose(amp, 1.0, rtol=1e-12)
+
+
+def test_naive_single_focus(oracle):
+    src, k, a, foci, amps = small_problem(oracle, m=1, seed=2)
+    amp, ph = oracle.naive(1, src, k, a, foci, amps, "f64")
+    G = oracle.propagation_matrix(1, with_solution(src, 1.0, 0.0), k, a, foci, "f64")[0]
+    q = np.conj(G) * amps[0]
+    assert np.allclose(amp, np.abs(q), rtol=1e-12) and np.allclose(ph, np.angle(q), atol=1e-12)
+
+
+def test_gspat_f32_close_to_f64(oracle):
+    src, k, a, foci, amps = small_problem(oracle, m=8, seed=3)
+    a32, p32 = oracle.gspat(1, src, k, a, foci, amps, "f32")
+    a64, p64 = oracle.gspat(1, src, k, a, foci, amps, "f64")
+    f32 = np.abs(focal(oracle, with_solution(src, a32, p32), k, a, foci))
+    f64 = np.abs(focal(oracle, with_solution(src, a64, p64), k, a, foci))
+    assert np.max(np.abs(f32 - f64) / f64) < 1e-4
+
+
+def test_gspat_reaches_target_ratios(oracle):
+    """GS-PAT's closing normalisation (gamma / |gamma|^2 p0^2) puts the focal
+    amplitudes close to the targets themselves (GS keeps |q| = q0 and cannot)."""
+    src, k, a, foci, amps = small_problem(oracle, m=4, seed=4)
+    am, ph = oracle.gspat(1, src, k, a, foci, amps, "f64")
+    f = np.abs(focal(oracle, with_solution(src, am, ph), k, a, foci))
+    ratio = f / amps
+    assert np.all(np.abs(ratio - 1) < 0.1), ratio
+
+
+def test_lm_gradient_is_half_dF(oracle):
+    """g of calc_JtJ_Jtf is dF/dx / 2 with F = Re(t^H BhB t) (pins the sign of T = e^{-ix})."""
+    src, k, a, foci, amps = small_problem(oracle, m=3, seed=5)
+    src = src[:40]
+    k, a = k[:40], a[:40]
+    rng = np.random.default_rng(9)
+    x = rng.uniform(-np.pi, np.pi, 43)
+    F0, g = oracle.lm_eval(1, src, k, a, foci, amps, x)
+    h = 1e-6
+    for idx in (0, 7, 39, 40, 42):
+        e = np.zeros_like(x)
+        e[idx] = h
+        Fp, _ = oracle.lm_eval(1, src, k, a, foci, amps, x + e)
+        Fm, _ = oracle.lm_eval(1, src, k, a, foci, amps, x - e)
+        assert (Fp - Fm) / (2 * h) == pytest.approx(2 * g[idx], rel=1e-5, abs=1e-9)
+
+
+def test_lm_single_focus_reaches_target(oracle):
+    """M = 1: F = |sum_j G_j e^{i x_j} - a e^{i x_N}|^2 has minimum 0 whenever
+    sum_j |G_j| >= a, so LM ends with |p(f)| = a."""
+    src, k, a, foci, amps = small_problem(oracle, m=1, seed=6)
+    amps = np.ones(1, np.float32)
+    ph, x, it = oracle.lm(1, src, k, a, foci, amps, "f64", k_max=200)
+    F, _ = oracle.lm_eval(1, src, k, a, foci, amps, x)
+    f = np.abs(focal(oracle, with_solution(src, 1.0, ph), k, a, foci))
+    assert it < 200 and F < 1e-9 and np.allclose(f, 1.0, rtol=1e-5), (f, it, F)
+
+
+def test_lm_decreases_objective(oracle):
+    src, k, a, foci, amps = small_problem(oracle, m=2, seed=6)
+    amps = np.ones(2, np.float32)
+    F0, _ = oracle.lm_eval(1, src, k, a, foci, amps, np.zeros(src.shape[0] + 2))
+    _, x, _ = oracle.lm(1, src, k, a, foci, amps, "f64", k_max=50)
+    F, _ = oracle.lm_eval(1, src, k, a, foci, amps, x)
+    assert F < 0.2 * F0
