@@ -75,7 +75,18 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-gspat", action="store_true")
     ap.add_argument("--gspat-batch", type=int, default=1024)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 256 MiB device-to-device copy (reference for HBM byte counters)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        x = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": x.numel() * 4}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
               f"the field path shards by points with no communication", file=sys.stderr)
