@@ -155,6 +155,8 @@ def _declare(d):
     _sig(d, "AFH_HoloSolve", [c_void_p, c_void_p, c_uint64, c_uint64, ctypes.c_int32, c_void_p, c_void_p, c_void_p,
                               c_uint32, ctypes.c_int32, ctypes.c_int32, c_uint32, c_void_p, c_void_p, c_void_p],
          ctypes.c_int32)
+    _sig(d, "AFH_LMSolve", [c_void_p, c_void_p, c_uint64, c_uint64, ctypes.c_int32, c_void_p, c_void_p, c_void_p,
+                            c_uint32, c_uint32, c_void_p, c_void_p, c_void_p], ctypes.c_int32)
     _sig(d, "AFH_DirectivityT4010A1", [c_void_p, c_uint64, ctypes.c_int32, c_void_p], ctypes.c_int32)
     _sig(d, "AFH_PackSources", [c_void_p, c_void_p, c_void_p, c_uint64, ctypes.c_int32, c_void_p], ctypes.c_int32)
     _sig(d, "AFH_FieldEval", [c_void_p, c_uint64, ctypes.c_int32, POINTER(AFH_PointSet), ctypes.c_int32,

@@ -69,6 +69,9 @@ Context& default_context();
 // Solves one instance on the device and returns per-source amplitude and
 // phase (host arrays of n floats). false on error (message set).
 bool holo_solve_single(const HoloProblem& p, int algo, float* amp_out, float* phase_out);
+int32_t lm_solve_batch(Context* ctx, const float* sources, uint32_t n, uint32_t n_packed, int source_type,
+                       const float* foci, const float* amps, const uint32_t* offsets_host, uint32_t batch,
+                       uint32_t k_max, float* out_phase, uint32_t* out_iters, hipStream_t st);
 // LM (lm.rs:40-105): phases of the n sources.
 bool lm_solve_single(const HoloProblem& p, float* phase_out);
 

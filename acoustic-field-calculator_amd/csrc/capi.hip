@@ -87,7 +87,7 @@ void pack_sources(const float* src, const float* wavenums, const float* attens, 
         o[8] = kh;
         o[9] = (float)(kr - (double)kh);
         o[10] = (float)((double)attens[j] * log2e);
-        o[11] = 0.0f;
+        o[11] = attens[j];  // the f32 attenuation itself, for FP64 consumers
     }
 }
 

@@ -837,7 +837,7 @@ int32_t AFH_HoloSolve(void* ctx, const float* packed_sources_dev, uint64_t n_pac
     hb.iterations = iterations;
     hb.out_amp = out_amp_dev;
     hb.out_phase = out_phase_dev;
-    return holo_solve(c, hb, stream ? (hipStream_t)stream : c->stream);
+    return holo_solve(c, hb, (hipStream_t)stream);
 }
 
 }  // extern "C"

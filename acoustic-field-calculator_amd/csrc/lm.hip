@@ -1,12 +1,472 @@
-// Levenberg-Marquardt hologram solver (work in progress).
+// Levenberg-Marquardt hologram solver on the GPU, batched, FP64.
+//
+// Reference: acoustic-field-optimizer/src/multiple_foci/nls/lm.rs:40-105 with
+// make_BhB / make_T / calc_JtJ_Jtf / calc_Fx (nls/macros.rs:17-58):
+//   B = [G | -diag(amps)] (M x n, n = N + M), BhB = B^H B,
+//   T = e^{-ix}, A = Re(BhB o T T^H), g_k = sum_l Im(BhB o T T^H)_kl,
+//   F(x) = Re(t^H BhB t), t = e^{ix}; mu0 = 1e-3 max diag A;
+//   h = -(A + mu I)^{-1} g; stop if |h| <= 1e-8 (|x| + 1e-8) or max_k g_k <= 1e-8;
+//   rho = (F - F_new) / (h.(mu h - g) / 2); F = F_new (also on rejection);
+//   accept if rho > 0: mu *= max(1/3, 1 - (2 rho - 1)^3), nu = 2; else mu *= nu, nu *= 2.
+//
+// Same quantities, low-rank form. With C = B diag(e^{ix}) (M x n) and
+// W = [Re C; Im C] (2M x n):
+//   A = Re(C^H C) = W^T W,     s = C 1 = B e^{ix} (the focal field),
+//   g = Im(C^H s) = W^T v with v = [Im s; -Re s],     F = |s|^2,
+// so the damped step is h = -(mu I + W^T W)^{-1} W^T v = -W^T (mu I + W W^T)^{-1} v
+// (push-through identity): an FP64 Cholesky of the 2M x 2M matrix
+// K = mu I + W W^T replaces the n x n solve, and an iteration costs O(M^2 n).
+// Everything is FP64, G included: the reference's acceptance rule makes the
+// iterate path sensitive to perturbations of a few FP32 ulp.
+//
+// One workgroup per instance; K, v, y in LDS; G, W, x, g, h, e^{ix} in a
+// per-instance global scratch. Every reduction has a fixed order.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
 #include "afh_holo.hpp"
 #include "afh_internal.hpp"
 
 namespace afh {
 
-bool lm_solve_single(const HoloProblem&, float*) {
-    set_error("AFO_LM: not built yet");
-    return false;
+constexpr int kLB = 256;
+constexpr int kLmMaxM = 32;
+constexpr int kLmMaxD = 2 * kLmMaxM;
+
+__constant__ DirEntry c_lm_dir[11] = {AFH_DIR_ENTRIES(AFH_DIR_ENTRY)};
+
+struct LmArgs {
+    const float* sources;  // packed (AFH_SOURCE_STRIDE floats each)
+    int source_type;
+    uint32_t n;  // sources N
+    const float* foci;
+    const float* amps;
+    const uint32_t* offsets;
+    uint32_t k_max;
+    char* scratch;  // per instance, scratch_stride bytes
+    uint64_t scratch_stride;
+    float* out_phase;     // batch x N
+    uint32_t* out_iters;  // batch (may be null)
+};
+
+// Scratch layout of one instance (n = N + M): G (M x N, double2) | W (2M x n) |
+// x | xn | g | h (n each) | E | En (n, double2).
+struct LmScratch {
+    double2* G;
+    double* W;
+    double *x, *xn, *g, *h;
+    double2 *E, *En;
+};
+
+__host__ __device__ inline uint64_t lm_scratch_bytes(uint64_t N, uint64_t M) {
+    const uint64_t n = N + M;
+    auto al = [](uint64_t b) { return (b + 255) & ~(uint64_t)255; };
+    return al(M * N * 16) + al(2 * M * n * 8) + 4 * al(n * 8) + 2 * al(n * 16);
+}
+
+__device__ inline LmScratch lm_scratch(char* base, uint64_t N, uint64_t M) {
+    const uint64_t n = N + M;
+    auto al = [](uint64_t b) { return (b + 255) & ~(uint64_t)255; };
+    LmScratch s;
+    char* p = base;
+    s.G = reinterpret_cast<double2*>(p);
+    p += al(M * N * 16);
+    s.W = reinterpret_cast<double*>(p);
+    p += al(2 * M * n * 8);
+    s.x = reinterpret_cast<double*>(p);
+    p += al(n * 8);
+    s.xn = reinterpret_cast<double*>(p);
+    p += al(n * 8);
+    s.g = reinterpret_cast<double*>(p);
+    p += al(n * 8);
+    s.h = reinterpret_cast<double*>(p);
+    p += al(n * 8);
+    s.E = reinterpret_cast<double2*>(p);
+    p += al(n * 16);
+    s.En = reinterpret_cast<double2*>(p);
+    return s;
+}
+
+// Reference directivity in FP64: theta binned by ceil(theta / (pi/18)), the
+// f32 coefficients of t4010a1.rs:18-69 promoted.
+__device__ inline double dir_f64(double theta) {
+    const double ad = 3.14159265358979323846 / 18.0;
+    const double q = ceil(theta / ad);
+    const int i = q > 0 ? (int)fmin(q, 10.0) : 0;
+    if (i == 0) return 1.0;
+    if (i > 9) return 0.0;
+    const DirEntry e = c_lm_dir[i];
+    const double x = theta - (i - 1.0) * ad;
+    return (double)e.a + ((double)e.b + ((double)e.c + (double)e.d * x) * x) * x;
+}
+
+// G_ij in FP64 (uniform_medium.rs:111-120 with phase 0), from the packed source:
+// the f32 wavenumber is recovered exactly by rounding (k/2pi hi + lo) 2 pi to
+// f32, and slot 11 carries the f32 attenuation, so G uses the same inputs as
+// the reference.
+__device__ inline double2 term_f64(const float* src, int source_type, double tx, double ty, double tz) {
+    const double dx = tx - (double)src[0], dy = ty - (double)src[1], dz = tz - (double)src[2];
+    const double dist = sqrt((dx * dx + dy * dy) + dz * dz);
+    double d = 1.0;
+    if (source_type == kT4010A1) {
+        const double c = (((double)src[3] * dx + (double)src[4] * dy) + (double)src[5] * dz) / dist;
+        d = dir_f64(acos(c));
+    }
+    const double k = (double)(float)(((double)src[8] + (double)src[9]) * 6.283185307179586476925);
+    const double alpha = (double)src[11];
+    const double r = (double)src[6] * d * exp(-dist * alpha) / dist;
+    double sn, cs;
+    sincos(k * dist, &sn, &cs);
+    return make_double2(r * cs, r * sn);
+}
+
+__global__ __launch_bounds__(kLB) void lm_gen_g(LmArgs a) {
+    const uint32_t b = blockIdx.y;
+    const uint32_t j = blockIdx.x * kLB + threadIdx.x;
+    const uint32_t base = a.offsets[b], M = a.offsets[b + 1] - base;
+    if (j >= a.n) return;
+    LmScratch s = lm_scratch(a.scratch + b * a.scratch_stride, a.n, M);
+    const float* src = a.sources + AFH_SOURCE_STRIDE * (uint64_t)j;
+    for (uint32_t i = 0; i < M; ++i)
+        s.G[(uint64_t)i * a.n + j] = term_f64(src, a.source_type, a.foci[3 * (uint64_t)(base + i)],
+                                              a.foci[3 * (uint64_t)(base + i) + 1], a.foci[3 * (uint64_t)(base + i) + 2]);
+}
+
+// Block reductions in a fixed order: lanes by xor shuffles, then the 4 waves.
+__device__ inline double block_sum(double v, double* red) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[w] = v;
+    __syncthreads();
+    return ((red[0] + red[1]) + red[2]) + red[3];
+}
+
+__device__ inline double block_max(double v, double* red) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[w] = v;
+    __syncthreads();
+    return fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+}
+
+// s_i = sum_l B_il E_l (E = e^{ix}); a wave per row, lanes over columns.
+__device__ inline void field_at_foci(const LmScratch& s, const double2* E, uint32_t N, uint32_t M, const float* amps,
+                                     double2* s_out) {
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    for (uint32_t i = w; i < M; i += 4) {
+        double re = 0.0, im = 0.0;
+        for (uint32_t j = l; j < N; j += 64) {
+            const double2 gv = s.G[(uint64_t)i * N + j], e = E[j];
+            re += gv.x * e.x - gv.y * e.y;
+            im += gv.x * e.y + gv.y * e.x;
+        }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            re += __shfl_xor(re, off);
+            im += __shfl_xor(im, off);
+        }
+        if (l == 0) {
+            // the -diag(amps) block of B: column N + i
+            const double2 e = E[N + i];
+            const double am = (double)amps[i];
+            s_out[i] = make_double2(re - am * e.x, im - am * e.y);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kLB) void lm_solve(LmArgs a) {
+    __shared__ double s_K[kLmMaxD * kLmMaxD];
+    __shared__ double s_v[kLmMaxD];
+    __shared__ double s_gd[kLmMaxD];  // Gram diagonal (the factor overwrites s_K's)
+    __shared__ double s_y[kLmMaxD];
+    __shared__ double2 s_s[kLmMaxM];
+    __shared__ double s_red[4];
+    const int t = threadIdx.x;
+    const uint32_t b = blockIdx.x;
+    const uint32_t base = a.offsets[b], M = a.offsets[b + 1] - base;
+    const uint32_t N = a.n, n = N + M, D = 2 * M;
+    const float* amps = a.amps + base;
+    LmScratch s = lm_scratch(a.scratch + b * a.scratch_stride, N, M);
+    const double EPS_1 = 1e-8, EPS_2 = 1e-8, TAU = 1e-3;
+
+    // W = [Re C; Im C] with C = B diag(E), v = [Im s; -Re s], g = W^T v,
+    // the Gram matrix W W^T into s_K (lower triangle and diagonal), F = |s|^2.
+    // Returns (F, max_l g_l, max_l diag(A)_l).
+    auto linearize = [&](double& F, double& gmax, double& amax) {
+        for (uint32_t l = t; l < n; l += kLB) {
+            const double2 e = s.E[l];
+            double colsq = 0.0;
+            for (uint32_t i = 0; i < M; ++i) {
+                double2 bv;
+                if (l < N) {
+                    bv = s.G[(uint64_t)i * N + l];
+                } else {
+                    bv = make_double2(l - N == i ? -(double)amps[i] : 0.0, 0.0);
+                }
+                const double cr = bv.x * e.x - bv.y * e.y, ci = bv.x * e.y + bv.y * e.x;
+                s.W[(uint64_t)i * n + l] = cr;
+                s.W[(uint64_t)(M + i) * n + l] = ci;
+                colsq += cr * cr + ci * ci;
+            }
+            s.h[l] = colsq;  // diag(A)_l, reused as scratch here
+        }
+        __syncthreads();
+        field_at_foci(s, s.E, N, M, amps, s_s);
+        __syncthreads();
+        if (t < (int)M) {
+            s_v[t] = s_s[t].y;
+            s_v[M + t] = -s_s[t].x;
+        }
+        double fl = 0.0;
+        for (uint32_t i = 0; i < M; ++i) fl += s_s[i].x * s_s[i].x + s_s[i].y * s_s[i].y;
+        F = fl;
+        __syncthreads();
+        double gm = -INFINITY, am = -INFINITY;
+        for (uint32_t l = t; l < n; l += kLB) {
+            double gl = 0.0;
+            for (uint32_t q = 0; q < D; ++q) gl += s.W[(uint64_t)q * n + l] * s_v[q];
+            s.g[l] = gl;
+            gm = fmax(gm, gl);
+            am = fmax(am, s.h[l]);
+        }
+        gmax = block_max(gm, s_red);
+        amax = block_max(am, s_red);
+        // Gram matrix entries (p >= q), each summed over l in order
+        const uint32_t ne = D * (D + 1) / 2;
+        for (uint32_t e = t; e < ne; e += kLB) {
+            // e -> (p, q) with p >= q
+            uint32_t p = (uint32_t)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
+            while (p * (p + 1) / 2 > e) --p;
+            while ((p + 1) * (p + 2) / 2 <= e) ++p;
+            const uint32_t q = e - p * (p + 1) / 2;
+            const double* wp = s.W + (uint64_t)p * n;
+            const double* wq = s.W + (uint64_t)q * n;
+            double acc = 0.0;
+            for (uint32_t l = 0; l < n; ++l) acc += wp[l] * wq[l];
+            s_K[p * kLmMaxD + q] = acc;
+            if (p == q) s_gd[p] = acc;
+        }
+        __syncthreads();
+    };
+
+    for (uint32_t l = t; l < n; l += kLB) {
+        s.x[l] = 0.0;
+        s.E[l] = make_double2(1.0, 0.0);
+    }
+    __syncthreads();
+    double F, gmax, amax;
+    linearize(F, gmax, amax);
+    // the Gram matrix must survive rejected steps: keep a copy of its lower
+    // triangle in the upper triangle of s_K (mirrored), the factor goes low
+    for (uint32_t e = t; e < D * D; e += kLB) {
+        const uint32_t p = e / D, q = e % D;
+        if (q > p) s_K[p * kLmMaxD + q] = s_K[q * kLmMaxD + p];
+    }
+    __syncthreads();
+    double mu = TAU * amax, nu = 2.0;
+    bool found = gmax <= EPS_1;
+    uint32_t it = 0;
+    for (; it < a.k_max; ++it) {
+        if (found) break;
+        // K = Gram + mu I into the lower triangle (from the mirrored copy)
+        for (uint32_t e = t; e < D * D; e += kLB) {
+            const uint32_t p = e / D, q = e % D;
+            if (q < p) s_K[p * kLmMaxD + q] = s_K[q * kLmMaxD + p];
+            if (q == p) s_K[p * kLmMaxD + p] = s_gd[p] + mu;
+        }
+        __syncthreads();
+        // Cholesky K = L L^T (lower), right-looking, then L z = v, L^T y = z
+        bool ok = true;
+        for (uint32_t k = 0; k < D; ++k) {
+            const double dkk = s_K[k * kLmMaxD + k];
+            ok = ok && dkk > 0.0;
+            const double lkk = sqrt(dkk);
+            __syncthreads();
+            for (uint32_t i = k + 1 + t; i < D; i += kLB) s_K[i * kLmMaxD + k] /= lkk;
+            if (t == 0) s_K[k * kLmMaxD + k] = lkk;
+            __syncthreads();
+            const uint32_t rem = D - k - 1;
+            for (uint32_t e = t; e < rem * rem; e += kLB) {
+                const uint32_t i = k + 1 + e / rem, j = k + 1 + e % rem;
+                if (j <= i) s_K[i * kLmMaxD + j] -= s_K[i * kLmMaxD + k] * s_K[j * kLmMaxD + k];
+            }
+            __syncthreads();
+        }
+        if (!ok) break;  // the reference stops when its solve fails (lm.rs:72-77)
+        if (t == 0) {
+            for (uint32_t i = 0; i < D; ++i) {
+                double acc = s_v[i];
+                for (uint32_t j = 0; j < i; ++j) acc -= s_K[i * kLmMaxD + j] * s_y[j];
+                s_y[i] = acc / s_K[i * kLmMaxD + i];
+            }
+            for (uint32_t i = D; i-- > 0;) {
+                double acc = s_y[i];
+                for (uint32_t j = i + 1; j < D; ++j) acc -= s_K[j * kLmMaxD + i] * s_y[j];
+                s_y[i] = acc / s_K[i * kLmMaxD + i];
+            }
+        }
+        __syncthreads();
+        // h = -W^T y; |h|, |x|, L = h.(mu h - g) / 2, x_new, e^{i x_new}
+        double hh = 0.0, xx = 0.0, lp = 0.0;
+        for (uint32_t l = t; l < n; l += kLB) {
+            double acc = 0.0;
+            for (uint32_t q = 0; q < D; ++q) acc += s.W[(uint64_t)q * n + l] * s_y[q];
+            const double hl = -acc;
+            s.h[l] = hl;
+            const double xl = s.x[l];
+            hh += hl * hl;
+            xx += xl * xl;
+            lp += hl * (mu * hl - s.g[l]);
+            const double xn = xl + hl;
+            s.xn[l] = xn;
+            double sn, cs;
+            sincos(xn, &sn, &cs);
+            s.En[l] = make_double2(cs, sn);
+        }
+        const double hnorm = sqrt(block_sum(hh, s_red));
+        const double xnorm = sqrt(block_sum(xx, s_red));
+        const double L0 = 0.5 * block_sum(lp, s_red);
+        if (hnorm <= EPS_2 * (xnorm + EPS_2)) {
+            found = true;
+            continue;
+        }
+        field_at_foci(s, s.En, N, M, amps, s_s);
+        __syncthreads();
+        double Fn = 0.0;
+        for (uint32_t i = 0; i < M; ++i) Fn += s_s[i].x * s_s[i].x + s_s[i].y * s_s[i].y;
+        const double rho = (F - Fn) / L0;
+        F = Fn;  // assigned on rejection too (lm.rs:85)
+        if (rho > 0.0) {
+            __syncthreads();
+            for (uint32_t l = t; l < n; l += kLB) {
+                s.x[l] = s.xn[l];
+                s.E[l] = s.En[l];
+            }
+            __syncthreads();
+            double Fdummy, am2;
+            linearize(Fdummy, gmax, am2);
+            for (uint32_t e = t; e < D * D; e += kLB) {
+                const uint32_t p = e / D, q = e % D;
+                if (q > p) s_K[p * kLmMaxD + q] = s_K[q * kLmMaxD + p];
+            }
+            __syncthreads();
+            found = gmax <= EPS_1;
+            const double c = 2.0 * rho - 1.0;
+            mu *= fmax(1.0 / 3.0, 1.0 - c * c * c);
+            nu = 2.0;
+        } else {
+            mu *= nu;
+            nu *= 2.0;
+        }
+    }
+    for (uint32_t j = t; j < N; j += kLB) a.out_phase[(uint64_t)b * N + j] = (float)s.x[j];
+    if (t == 0 && a.out_iters) a.out_iters[b] = it;
+}
+
+int32_t lm_solve_batch(Context* ctx, const float* sources, uint32_t n, uint32_t n_packed, int source_type,
+                       const float* foci, const float* amps, const uint32_t* offsets_host, uint32_t batch,
+                       uint32_t k_max, float* out_phase, uint32_t* out_iters, hipStream_t st) {
+    (void)n_packed;
+    if (batch == 0) return AFH_OK;
+    uint32_t maxm = 0;
+    for (uint32_t b = 0; b < batch; ++b) {
+        const uint32_t m = offsets_host[b + 1] - offsets_host[b];
+        if (m == 0) {
+            set_error("LM: an instance has no foci");
+            return AFH_ERR_ARG;
+        }
+        maxm = std::max(maxm, m);
+    }
+    if (maxm > (uint32_t)kLmMaxM) {
+        set_error("LM: more than 32 foci in one instance is not supported");
+        return AFH_ERR_UNSUPPORTED;
+    }
+    const uint64_t stride = lm_scratch_bytes(n, maxm);
+    const size_t off_bytes = (batch + 1) * sizeof(uint32_t);
+    const size_t o_scr = (off_bytes + 255) & ~(size_t)255;
+    if (!ctx->work.reserve(o_scr + stride * batch)) return AFH_ERR_HIP;
+    if (!ctx->reserve_staging(off_bytes)) return AFH_ERR_HIP;
+    memcpy(ctx->staging.ptr, offsets_host, off_bytes);
+    char* w = static_cast<char*>(ctx->work.ptr);
+    if (!hip_ok(hipMemcpyAsync(w, ctx->staging.ptr, off_bytes, hipMemcpyHostToDevice, st), "H2D offsets"))
+        return AFH_ERR_HIP;
+    if (!ctx->upload_done && !hip_ok(hipEventCreateWithFlags(&ctx->upload_done, hipEventDisableTiming), "event"))
+        return AFH_ERR_HIP;
+    if (!hip_ok(hipEventRecord(ctx->upload_done, st), "hipEventRecord")) return AFH_ERR_HIP;
+    LmArgs a;
+    memset(&a, 0, sizeof(a));
+    a.sources = sources;
+    a.source_type = source_type;
+    a.n = n;
+    a.foci = foci;
+    a.amps = amps;
+    a.offsets = reinterpret_cast<const uint32_t*>(w);
+    a.k_max = k_max;
+    a.scratch = w + o_scr;
+    a.scratch_stride = stride;
+    a.out_phase = out_phase;
+    a.out_iters = out_iters;
+    hipLaunchKernelGGL(lm_gen_g, dim3((n + kLB - 1) / kLB, batch), dim3(kLB), 0, st, a);
+    hipLaunchKernelGGL(lm_solve, dim3(batch), dim3(kLB), 0, st, a);
+    if (!hip_ok(hipGetLastError(), "LM kernel launch")) return AFH_ERR_HIP;
+    return AFH_OK;
+}
+
+bool lm_solve_single(const HoloProblem& p, float* phase_out) {
+    if (!device().ensure()) return false;
+    Context& ctx = default_context();
+    std::lock_guard<std::mutex> g(ctx.mu);
+    if (!ctx.ensure_stream()) return false;
+    const uint64_t n = p.n, np = packed_count(n), m = p.m;
+    const size_t src_b = np * AFH_SOURCE_STRIDE * sizeof(float), foci_b = m * 3 * sizeof(float);
+    const size_t amps_b = m * sizeof(float), out_b = n * sizeof(float);
+    auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t o_foci = align(src_b), o_amps = o_foci + align(foci_b), o_ph = o_amps + align(amps_b);
+    const size_t total = o_ph + align(out_b);
+    if (!ctx.io.reserve(total) || !ctx.io_host.reserve(total)) return false;
+    char* h = static_cast<char*>(ctx.io_host.ptr);
+    char* d = static_cast<char*>(ctx.io.ptr);
+    pack_sources(p.sources, p.wavenums, p.attens, n, p.source_type, reinterpret_cast<float*>(h));
+    memcpy(h + o_foci, p.foci, foci_b);
+    memcpy(h + o_amps, p.amps, amps_b);
+    if (!hip_ok(hipMemcpyAsync(d, h, o_ph, hipMemcpyHostToDevice, ctx.stream), "H2D")) return false;
+    const uint32_t offs[2] = {0, (uint32_t)m};
+    // K_MAX = 200 (lm.rs:20)
+    if (lm_solve_batch(&ctx, reinterpret_cast<const float*>(d), (uint32_t)n, (uint32_t)np, p.source_type,
+                       reinterpret_cast<const float*>(d + o_foci), reinterpret_cast<const float*>(d + o_amps), offs,
+                       1, 200, reinterpret_cast<float*>(d + o_ph), nullptr, ctx.stream) != AFH_OK)
+        return false;
+    if (!hip_ok(hipMemcpyAsync(h + o_ph, d + o_ph, out_b, hipMemcpyDeviceToHost, ctx.stream), "D2H")) return false;
+    if (!hip_ok(hipStreamSynchronize(ctx.stream), "hipStreamSynchronize")) return false;
+    memcpy(phase_out, h + o_ph, out_b);
+    return true;
 }
 
 }  // namespace afh
+
+using namespace afh;
+
+extern "C" int32_t AFH_LMSolve(void* ctx, const float* packed_sources_dev, uint64_t n_packed, uint64_t n_sources,
+                               int32_t source_type, const float* foci_dev, const float* amps_dev,
+                               const uint32_t* offsets_host, uint32_t batch, uint32_t k_max, float* out_phase_dev,
+                               uint32_t* out_iters_dev, void* stream) {
+    clear_error();
+    Context* c = static_cast<Context*>(ctx);
+    if (!c || !packed_sources_dev || !foci_dev || !amps_dev || !offsets_host || !out_phase_dev ||
+        n_packed % AFH_SOURCE_PAD != 0 || n_packed < n_sources || n_sources > 0xFFFFFFFFull ||
+        (source_type != kSphere && source_type != kT4010A1) || offsets_host[0] != 0) {
+        set_error("AFH_LMSolve: bad argument");
+        return AFH_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> g(c->mu);
+    return lm_solve_batch(c, packed_sources_dev, (uint32_t)n_sources, (uint32_t)n_packed, source_type, foci_dev,
+                          amps_dev, offsets_host, batch, k_max, out_phase_dev, out_iters_dev,
+                          (hipStream_t)stream);
+}

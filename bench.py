@@ -75,6 +75,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-gspat", action="store_true")
     ap.add_argument("--gspat-batch", type=int, default=1024)
+    ap.add_argument("--no-lm", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device-to-device copy (reference for HBM byte counters)")
     args = ap.parse_args()
@@ -169,10 +170,74 @@ def main():
     }
     if not args.no_gspat:
         result["gspat"] = bench_gspat(torch, afc, D, O, src, k, a, args.gspat_batch)
+    if not args.no_lm:
+        result["lm"] = bench_lm(torch, D, O)
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline(src, k, a, area["order"], cnt, o, area["res"])
     print(json.dumps(result), flush=True)
+
+
+def bench_lm(torch, D, O):
+    """Batched FP64 LM (lm.rs:40-105): 64 instances on the 249- and on the
+    996-transducer array, 16 foci each (as the GS-PAT workload), targets 1,
+    50 iterations."""
+    out = {}
+    dev = torch.device("cuda:0")
+    for name, tiles in (("n249", (1, 1)), ("n996", (2, 2))):
+        pos = O.array_249() if tiles == (1, 1) else O.tiled_array(*tiles)
+        src = O.t4010a1_sources(pos)
+        n = pos.shape[0]
+        k = np.full(n, O.WAVENUM_40K, np.float32)
+        a = np.full(n, np.float32(O.attenuation_coef(40e3, 30, 1, 1, O.TEMP_340)), np.float32)
+        foci, amps, offs = O.gspat_batch(64, m_range=(16, 16), seed=4)
+        amps = np.ones_like(amps)
+        packed = D.pack_sources(1, src, k, a)
+        src_d = torch.from_numpy(packed).to(dev)
+        foci_d = torch.from_numpy(foci).to(dev)
+        amps_d = torch.from_numpy(amps).to(dev)
+        ph_d = torch.empty((64, n), dtype=torch.float32, device=dev)
+        it_d = torch.zeros(64, dtype=torch.int32, device=dev)
+        stream = torch.cuda.current_stream(dev)
+        ctx = D.Context()
+
+        def run():
+            D.lm_solve(ctx, src_d.data_ptr(), packed.shape[0], n, 1, foci_d.data_ptr(), amps_d.data_ptr(), offs, 50,
+                       ph_d.data_ptr(), it_d.data_ptr(), stream.cuda_stream)
+        run()
+        torch.cuda.synchronize()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(3)]
+        for e0, e1 in ev:
+            e0.record(stream)
+            run()
+            e1.record(stream)
+        torch.cuda.synchronize()
+        ms = float(np.median([e0.elapsed_time(e1) for e0, e1 in ev]))
+        entry = {"solves_per_s": round(64 / (ms * 1e-3), 1), "batch_ms": round(ms, 3), "iterations": 50,
+                 "n_params": n + 16}
+        if name == "n249":
+            # the reference iteration is chaotic beyond ~40 steps (a one-ulp input
+            # change reaches O(1) phase differences by 200): agreement with the
+            # FP64 oracle is checked on 20-iteration trajectories
+            sub = np.array([0, 16, 32], np.uint32)
+            D.lm_solve(ctx, src_d.data_ptr(), packed.shape[0], n, 1, foci_d.data_ptr(), amps_d.data_ptr(), sub, 20,
+                       ph_d.data_ptr(), it_d.data_ptr(), stream.cuda_stream)
+            torch.cuda.synchronize()
+            ph = ph_d.cpu().numpy()
+            errs = []
+            for b in range(2):
+                f = foci[offs[b]:offs[b + 1]]
+                ref, _, _ = O.lm(1, src, k, a, f, amps[offs[b]:offs[b + 1]], "f64", k_max=20)
+                s1, s2 = src.copy(), src.copy()
+                s1[:, 7], s2[:, 7] = ph[b], ref
+                g1 = np.abs(O.field(1, s1, k, a, f, "f64"))
+                g2 = np.abs(O.field(1, s2, k, a, f, "f64"))
+                errs.append(float(np.max(np.abs(g1 - g2) / g2)))
+            entry["focal_amp_rel_err_vs_fp64_20it_max"] = max(errs)
+        ctx.close()
+        out[name] = entry
+        log(f"lm {name}: {ms:.3f} ms per batch of 64, {64 / (ms * 1e-3):.0f} solves/s")
+    return out
 
 
 def bench_gspat(torch, afc, D, O, src, k, a, batch):

@@ -167,7 +167,7 @@ uint64_t AFH_GetSourceConstants(void* system_handle, float* wavenums, float* att
 /* Packs n reference structs (AFC_SphereWaveSource or AFC_T4010A1 by
  * source_type) with their wavenumbers [1/mm] and attenuations [Np/mm] into
  * the device layout: per source {x, y, z, nx, ny, nz, amp, phase/(2pi) mod 1,
- * k/(2pi) hi, k/(2pi) lo, alpha*log2(e), 0}. Writes AFH_PackedCount(n)
+ * k/(2pi) hi, k/(2pi) lo, alpha*log2(e), alpha}. Writes AFH_PackedCount(n)
  * entries (the tail is zero-amplitude padding) to out (host memory). */
 uint64_t AFH_PackedCount(uint64_t n);
 int32_t AFH_PackSources(const void* sources, const float* wavenums, const float* attens, uint64_t n,
@@ -219,12 +219,22 @@ enum { AFH_HOLO_FP32 = 0, AFH_HOLO_MIXED = 1, AFH_HOLO_MIXED_CLOSING = 2, AFH_HO
  * foci per instance. Writes per-source amplitude |q| and phase arg q of
  * instance b to out_amp_dev / out_phase_dev [b * n_sources ...]
  * (gs_pat.rs:75-80, gs.rs:71-76, naive.rs:45-50). Asynchronous on `stream`
- * (NULL: the context's stream). iterations: 100 in the reference
- * (gs.rs:17, gs_pat.rs:17). */
+ * (a hipStream_t; NULL is the default stream). iterations: 100 in the
+ * reference (gs.rs:17, gs_pat.rs:17). */
 int32_t AFH_HoloSolve(void* ctx, const float* packed_sources_dev, uint64_t n_packed, uint64_t n_sources,
                       int32_t source_type, const float* foci_dev, const float* amps_dev, const uint32_t* offsets_host,
                       uint32_t batch, int32_t algo, int32_t precision, uint32_t iterations, float* out_amp_dev,
                       float* out_phase_dev, void* stream);
+
+/* Levenberg-Marquardt (nls/lm.rs:40-105) for `batch` instances, FP64: writes
+ * the source phases x[0..N) of instance b to out_phase_dev[b * n_sources ...]
+ * (amplitudes are left to the caller, as the reference leaves them) and, if
+ * out_iters_dev is not NULL, the iteration count. At most 32 foci per
+ * instance; k_max = 200 in the reference (lm.rs:20). Same argument
+ * conventions as AFH_HoloSolve. */
+int32_t AFH_LMSolve(void* ctx, const float* packed_sources_dev, uint64_t n_packed, uint64_t n_sources,
+                    int32_t source_type, const float* foci_dev, const float* amps_dev, const uint32_t* offsets_host,
+                    uint32_t batch, uint32_t k_max, float* out_phase_dev, uint32_t* out_iters_dev, void* stream);
 
 /* T4010A1 directivity as a function of cos(theta), evaluated on the host with
  * exactly the formulas the kernels use in each precision mode (cell table

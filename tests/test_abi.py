@@ -108,6 +108,9 @@ def test_packing_layout(afc, oracle):
     kr = (p[:n, 8].astype(np.float64) + p[:n, 9].astype(np.float64)) * 2 * math.pi
     assert np.allclose(kr, k.astype(np.float64), rtol=1e-15)
     assert np.allclose(p[:n, 10], a.astype(np.float64) * math.log2(math.e), rtol=1e-7)
+    assert np.array_equal(p[:n, 11], a)
+    # the f32 wavenumber is recoverable exactly from the hi/lo pair
+    assert np.array_equal(kr.astype(np.float32), k)
     assert np.all(p[n:, 6] == 0)  # padding has zero amplitude
 
 
