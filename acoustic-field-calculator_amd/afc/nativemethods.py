@@ -79,6 +79,11 @@ class T4010A1(Structure):
         return 1
 
 
+class AFS_Config(Structure):
+    _fields_ = [("hbm_budget_bytes", c_uint64), ("pinned_budget_bytes", c_uint64), ("n_streams", c_uint32),
+                ("max_gspat_batch", c_uint32), ("gspat_precision", ctypes.c_int32), ("gspat_iterations", c_uint32)]
+
+
 class AFH_PointSet(Structure):
     _fields_ = [("kind", ctypes.c_int32), ("order", ctypes.c_int32 * 3), ("count", c_uint32 * 3),
                 ("origin", c_float * 3), ("resolution", c_float), ("points", c_void_p), ("n_points", c_uint64)]
@@ -157,6 +162,17 @@ def _declare(d):
          ctypes.c_int32)
     _sig(d, "AFH_LMSolve", [c_void_p, c_void_p, c_uint64, c_uint64, ctypes.c_int32, c_void_p, c_void_p, c_void_p,
                             c_uint32, c_uint32, c_void_p, c_void_p, c_void_p], ctypes.c_int32)
+    _sig(d, "AFS_Create", [POINTER(AFS_Config), POINTER(c_void_p)], ctypes.c_int32)
+    _sig(d, "AFS_Destroy", [c_void_p])
+    _sig(d, "AFS_AddArray", [c_void_p, c_void_p, c_void_p, c_void_p, c_uint64, ctypes.c_int32, POINTER(c_uint32)],
+         ctypes.c_int32)
+    _sig(d, "AFS_SubmitField", [c_void_p, c_uint32, POINTER(AFH_PointSet), ctypes.c_int32, c_void_p,
+                                POINTER(c_uint64)], ctypes.c_int32)
+    _sig(d, "AFS_SubmitGSPAT", [c_void_p, c_uint32, c_void_p, c_void_p, c_uint32, c_void_p, c_void_p,
+                                POINTER(c_uint64)], ctypes.c_int32)
+    _sig(d, "AFS_Wait", [c_void_p, c_uint64, POINTER(ctypes.c_double)], ctypes.c_int32)
+    _sig(d, "AFS_Stats", [c_void_p, POINTER(ctypes.c_double), POINTER(ctypes.c_double), POINTER(c_uint64),
+                          ctypes.c_int32], ctypes.c_int32)
     _sig(d, "AFH_DirectivityT4010A1", [c_void_p, c_uint64, ctypes.c_int32, c_void_p], ctypes.c_int32)
     _sig(d, "AFH_PackSources", [c_void_p, c_void_p, c_void_p, c_uint64, ctypes.c_int32, c_void_p], ctypes.c_int32)
     _sig(d, "AFH_FieldEval", [c_void_p, c_uint64, ctypes.c_int32, POINTER(AFH_PointSet), ctypes.c_int32,

@@ -1,0 +1,83 @@
+"""Python handle of the request server (AFS_* in include/afc_hip.h).
+
+Requests carry host arrays; the library keeps arrays resident on the GPU,
+admits requests by HBM budget and batches them continuously. The numpy
+buffers handed to a submit call must stay alive until its wait returns.
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import byref, c_double, c_uint32, c_uint64, c_void_p
+
+import numpy as np
+
+from . import AFCError, dll
+from .nativemethods import AFH_PointSet, AFS_Config
+
+OK, ERR_BUDGET = 0, 5
+
+
+def _err(what, rc):
+    if rc != 0:
+        raise AFCError(f"{what} failed ({rc}): {dll().AFC_GetLastError().decode()}")
+
+
+class Server:
+    def __init__(self, hbm_budget=1 << 30, pinned_budget=256 << 20, n_streams=4, max_gspat_batch=256,
+                 gspat_precision=3, gspat_iterations=100):
+        cfg = AFS_Config(hbm_budget, pinned_budget, n_streams, max_gspat_batch, gspat_precision, gspat_iterations)
+        self.handle = c_void_p()
+        _err("AFS_Create", dll().AFS_Create(byref(cfg), byref(self.handle)))
+        self._keep = {}
+
+    def close(self):
+        if self.handle:
+            dll().AFS_Destroy(self.handle)
+            self.handle = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_array(self, source_type, sources, wavenums, attens) -> int:
+        src = np.ascontiguousarray(sources, np.float32)
+        k = np.ascontiguousarray(wavenums, np.float32)
+        a = np.ascontiguousarray(attens, np.float32)
+        aid = c_uint32()
+        _err("AFS_AddArray", dll().AFS_AddArray(self.handle, src.ctypes.data_as(c_void_p), k.ctypes.data_as(c_void_p),
+                                               a.ctypes.data_as(c_void_p), k.shape[0], source_type, byref(aid)))
+        return aid.value
+
+    def submit_field(self, array_id: int, pointset: AFH_PointSet, field_type: int, out: np.ndarray) -> int:
+        t = c_uint64()
+        rc = dll().AFS_SubmitField(self.handle, array_id, byref(pointset), field_type, out.ctypes.data_as(c_void_p),
+                                   byref(t))
+        _err("AFS_SubmitField", rc)
+        self._keep[t.value] = (out, pointset)
+        return t.value
+
+    def submit_gspat(self, array_id: int, foci: np.ndarray, amps: np.ndarray, out_amp: np.ndarray,
+                     out_phase: np.ndarray) -> int:
+        f = np.ascontiguousarray(foci, np.float32)
+        a = np.ascontiguousarray(amps, np.float32)
+        t = c_uint64()
+        rc = dll().AFS_SubmitGSPAT(self.handle, array_id, f.ctypes.data_as(c_void_p), a.ctypes.data_as(c_void_p),
+                                   a.shape[0], out_amp.ctypes.data_as(c_void_p), out_phase.ctypes.data_as(c_void_p),
+                                   byref(t))
+        _err("AFS_SubmitGSPAT", rc)
+        self._keep[t.value] = (out_amp, out_phase)
+        return t.value
+
+    def wait(self, ticket: int) -> float:
+        lat = c_double()
+        rc = dll().AFS_Wait(self.handle, ticket, byref(lat))
+        self._keep.pop(ticket, None)
+        _err("AFS_Wait", rc)
+        return lat.value
+
+    def stats(self, reset=False):
+        p50, p99, n = c_double(), c_double(), c_uint64()
+        _err("AFS_Stats", dll().AFS_Stats(self.handle, byref(p50), byref(p99), byref(n), 1 if reset else 0))
+        return p50.value, p99.value, n.value

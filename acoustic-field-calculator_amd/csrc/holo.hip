@@ -655,6 +655,9 @@ int32_t holo_solve(Context* ctx, const HoloBatch& hb, hipStream_t st) {
     const size_t o_off = 0, o_list = align(off_bytes), o_r = o_list + align(list_bytes);
     const size_t o_p = o_r + align(r_elems * sizeof(float2)), o_g = o_p + align(p_elems * sizeof(float2));
     const size_t o_q = o_g + align(g_elems * sizeof(float2)), total = o_q + align(q_elems * sizeof(float2));
+    // growing the workspace frees the old block: work queued on this stream
+    // may still use it
+    if (total > ctx->work.cap && !hip_ok(hipStreamSynchronize(st), "hipStreamSynchronize")) return AFH_ERR_HIP;
     if (!ctx->work.reserve(total)) return AFH_ERR_HIP;
     if (!ctx->reserve_staging(off_bytes + list_bytes)) return AFH_ERR_HIP;
     uint32_t* stage = static_cast<uint32_t*>(ctx->staging.ptr);

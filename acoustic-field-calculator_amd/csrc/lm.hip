@@ -391,6 +391,8 @@ int32_t lm_solve_batch(Context* ctx, const float* sources, uint32_t n, uint32_t 
     const uint64_t stride = lm_scratch_bytes(n, maxm);
     const size_t off_bytes = (batch + 1) * sizeof(uint32_t);
     const size_t o_scr = (off_bytes + 255) & ~(size_t)255;
+    if (o_scr + stride * batch > ctx->work.cap && !hip_ok(hipStreamSynchronize(st), "hipStreamSynchronize"))
+        return AFH_ERR_HIP;
     if (!ctx->work.reserve(o_scr + stride * batch)) return AFH_ERR_HIP;
     if (!ctx->reserve_staging(off_bytes)) return AFH_ERR_HIP;
     memcpy(ctx->staging.ptr, offsets_host, off_bytes);

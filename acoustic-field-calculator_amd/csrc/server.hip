@@ -1,0 +1,577 @@
+// Request server: field evaluations and GS-PAT solves served continuously on
+// one GPU. No reference counterpart (the reference evaluates one call at a
+// time on the caller's thread, capi/src/calculator.rs:53-117).
+//
+//   * Transducer arrays are registered once and stay resident (packed sources).
+//   * Admission by HBM budget: every request (or GS-PAT batch) takes its inputs
+//     and outputs from a device arena of a fixed size, allocated once; a request
+//     that does not fit waits (FIFO) until completions free space. Results are
+//     staged through a pinned host arena the same way.
+//   * Continuous batching: one dispatcher thread takes everything that has
+//     arrived each cycle; GS-PAT requests on the same array become one batched
+//     launch, field requests are spread over K streams so that small grids run
+//     side by side; completions are polled through events, so new requests are
+//     admitted while earlier ones are still on the GPU.
+//   * Each request's result is produced by the same kernels, with the same
+//     per-request reduction order, as when it runs alone.
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "afh_holo.hpp"
+#include "afh_host.hpp"
+#include "afh_internal.hpp"
+
+namespace afh {
+
+static double now_ms() {
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+// First-fit allocator over one block, 256-byte granularity.
+struct Arena {
+    size_t size = 0;
+    std::map<size_t, size_t> free_;  // offset -> bytes
+    void init(size_t bytes) {
+        size = bytes;
+        free_.clear();
+        free_[0] = bytes;
+    }
+    static size_t round(size_t b) { return (std::max<size_t>(b, 1) + 255) & ~(size_t)255; }
+    bool alloc(size_t bytes, size_t* off) {
+        bytes = round(bytes);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= bytes) {
+                *off = it->first;
+                const size_t rest = it->second - bytes, o = it->first;
+                free_.erase(it);
+                if (rest) free_[o + bytes] = rest;
+                return true;
+            }
+        }
+        return false;
+    }
+    void release(size_t off, size_t bytes) {
+        bytes = round(bytes);
+        auto it = free_.emplace(off, bytes).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+};
+
+struct SrvArray {
+    int source_type;
+    uint32_t n, np;
+    float* packed = nullptr;  // device
+};
+
+enum ReqKind { kReqField = 0, kReqGSPAT = 1 };
+enum ReqState { kPending = 0, kRunning = 1, kDone = 2 };
+
+struct Request {
+    uint64_t ticket = 0;
+    int kind = 0;
+    uint32_t array_id = 0;
+    // field
+    GridArea grid;
+    bool scatter = false;
+    std::vector<float> points;
+    int field_type = 0;
+    uint64_t n_points = 0;
+    // GS-PAT
+    std::vector<float> foci, amps;
+    uint32_t m = 0;
+    // outputs (caller memory)
+    float* out = nullptr;
+    float* out2 = nullptr;
+    size_t out_bytes = 0;   // bytes copied to `out` (and again to `out2` for GS-PAT)
+    size_t pin_off = 0;     // this request's results in the pinned arena
+    size_t pin_off2 = 0;    // GS-PAT: its phases
+    double t_submit = 0, t_done = 0;
+    int state = kPending;
+    int32_t status = AFH_OK;
+};
+
+// One launch in flight: a field request or a GS-PAT batch.
+struct Flight {
+    std::vector<Request*> reqs;
+    hipEvent_t ev = nullptr;
+    size_t dev_off = 0, dev_bytes = 0, pin_off = 0, pin_bytes = 0;
+};
+
+struct Server {
+    AFS_Config cfg;
+    char* dev_base = nullptr;
+    char* pin_base = nullptr;
+    Arena dev, pin;
+    std::vector<hipStream_t> streams;
+    std::vector<std::unique_ptr<Context>> ctxs;
+    std::vector<SrvArray> arrays;
+    std::mutex mu;
+    std::condition_variable cv_submit, cv_done;
+    std::deque<Request*> pending;
+    std::unordered_map<uint64_t, Request*> by_ticket;
+    std::deque<Flight> flights;
+    std::vector<double> latencies;
+    uint64_t next_ticket = 1, completed = 0;
+    bool stop = false;
+    std::thread worker;
+    int next_stream = 0;
+    int device = 0;
+
+    ~Server() {
+        {
+            std::lock_guard<std::mutex> g(mu);
+            stop = true;
+        }
+        cv_submit.notify_all();
+        if (worker.joinable()) worker.join();
+        for (auto s : streams) (void)hipStreamSynchronize(s);
+        ctxs.clear();
+        for (auto s : streams) (void)hipStreamDestroy(s);
+        for (auto& a : arrays)
+            if (a.packed) (void)hipFree(a.packed);
+        if (dev_base) (void)hipFree(dev_base);
+        if (pin_base) (void)hipHostFree(pin_base);
+        for (auto& kv : by_ticket) delete kv.second;  // the worker drained `pending` first
+    }
+
+    hipStream_t take_stream() {
+        const int s = next_stream;
+        next_stream = (next_stream + 1) % (int)streams.size();
+        return streams[s];
+    }
+    int stream_index(hipStream_t s) const {
+        for (size_t i = 0; i < streams.size(); ++i)
+            if (streams[i] == s) return (int)i;
+        return 0;
+    }
+
+    // Launches what fits, in arrival order: field requests one launch each (a
+    // field request that does not fit holds back the later ones, so large
+    // requests are not starved); GS-PAT requests grouped per array into batched
+    // launches of at most max_gspat_batch instances.
+    void dispatch(const std::vector<Request*>& todo, std::vector<Request*>& admitted) {
+        std::map<uint32_t, std::vector<Request*>> groups;
+        for (Request* r : todo) {
+            if (r->kind == kReqGSPAT) {
+                groups[r->array_id].push_back(r);
+                continue;
+            }
+            if (!launch_field(r)) break;
+            admitted.push_back(r);
+        }
+        for (auto& kv : groups) {
+            std::vector<Request*>& rs = kv.second;
+            for (size_t start = 0; start < rs.size();) {
+                // the largest batch that fits the arena now (one request always
+                // fits an empty arena: checked at submission)
+                size_t cnt = std::min<size_t>(rs.size() - start, cfg.max_gspat_batch);
+                while (cnt > 0 && !launch_gspat(kv.first, &rs[start], cnt)) cnt /= 2;
+                if (cnt == 0) break;
+                admitted.insert(admitted.end(), rs.begin() + start, rs.begin() + start + cnt);
+                start += cnt;
+            }
+        }
+    }
+
+    bool launch_field(Request* r) {
+        const SrvArray& A = arrays[r->array_id];
+        const size_t per = r->field_type == AFC_FIELD_COMPLEX ? 2 : 1;
+        const size_t out_b = r->n_points * per * sizeof(float);
+        const size_t pts_b = r->scatter ? r->n_points * 3 * sizeof(float) : 0;
+        Flight f;
+        f.dev_bytes = Arena::round(out_b) + Arena::round(pts_b);  // out | points
+        f.pin_bytes = out_b + pts_b;
+        if (!dev.alloc(f.dev_bytes, &f.dev_off)) return false;
+        if (!pin.alloc(f.pin_bytes, &f.pin_off)) {
+            dev.release(f.dev_off, f.dev_bytes);
+            return false;
+        }
+        hipStream_t st = take_stream();
+        char* d = dev_base + f.dev_off;
+        char* h = pin_base + f.pin_off;
+        const float* pts_dev = nullptr;
+        if (r->scatter) {
+            memcpy(h + out_b, r->points.data(), pts_b);
+            (void)hipMemcpyAsync(d + Arena::round(out_b), h + out_b, pts_b, hipMemcpyHostToDevice, st);
+            pts_dev = reinterpret_cast<const float*>(d + Arena::round(out_b));
+        }
+        FieldArgs a = field_args_for(r->scatter ? nullptr : &r->grid, pts_dev, r->n_points);
+        a.sources = A.packed;
+        a.n_sources = A.np;
+        a.out = reinterpret_cast<float*>(d);
+        hipError_t e = afh::launch_field(a, A.source_type, r->scatter ? kScatter : kGrid, r->field_type, kFast, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(h, d, out_b, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&f.ev, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventRecord(f.ev, st);
+        r->out_bytes = out_b;
+        r->pin_off = f.pin_off;
+        f.reqs.push_back(r);
+        if (e != hipSuccess) {
+            r->status = AFH_ERR_HIP;
+            (void)hipStreamSynchronize(st);
+        }
+        r->state = kRunning;
+        flights.push_back(f);
+        return true;
+    }
+
+    bool launch_gspat(uint32_t array_id, Request** rs, size_t cnt) {
+        const SrvArray& A = arrays[array_id];
+        size_t sum_m = 0;
+        for (size_t i = 0; i < cnt; ++i) sum_m += rs[i]->m;
+        const size_t foci_b = sum_m * 3 * sizeof(float), amps_b = sum_m * sizeof(float);
+        const size_t out_b = cnt * A.n * sizeof(float);
+        const size_t o_amps = Arena::round(foci_b), o_oa = o_amps + Arena::round(amps_b);
+        const size_t o_op = o_oa + Arena::round(out_b);
+        Flight f;
+        f.dev_bytes = o_op + out_b;
+        f.pin_bytes = f.dev_bytes;
+        if (!dev.alloc(f.dev_bytes, &f.dev_off)) return false;
+        if (!pin.alloc(f.pin_bytes, &f.pin_off)) {
+            dev.release(f.dev_off, f.dev_bytes);
+            return false;
+        }
+        hipStream_t st = take_stream();
+        Context* ctx = ctxs[stream_index(st)].get();
+        char* d = dev_base + f.dev_off;
+        char* h = pin_base + f.pin_off;
+        std::vector<uint32_t> offs(cnt + 1, 0);
+        size_t pos = 0;
+        for (size_t i = 0; i < cnt; ++i) {
+            memcpy(h + pos * 3 * sizeof(float), rs[i]->foci.data(), rs[i]->m * 3 * sizeof(float));
+            memcpy(h + o_amps + pos * sizeof(float), rs[i]->amps.data(), rs[i]->m * sizeof(float));
+            pos += rs[i]->m;
+            offs[i + 1] = (uint32_t)pos;
+        }
+        hipError_t e = hipMemcpyAsync(d, h, o_oa, hipMemcpyHostToDevice, st);
+        HoloBatch hb;
+        hb.sources = A.packed;
+        hb.n = A.n;
+        hb.n_packed = A.np;
+        hb.source_type = A.source_type;
+        hb.foci = reinterpret_cast<const float*>(d);
+        hb.amps = reinterpret_cast<const float*>(d + o_amps);
+        hb.offsets_host = offs.data();
+        hb.batch = (uint32_t)cnt;
+        hb.algo = kAlgoGSPAT;
+        hb.precision = cfg.gspat_precision;
+        hb.iterations = cfg.gspat_iterations;
+        hb.out_amp = reinterpret_cast<float*>(d + o_oa);
+        hb.out_phase = reinterpret_cast<float*>(d + o_op);
+        int32_t rc = AFH_OK;
+        if (e == hipSuccess) {
+            std::lock_guard<std::mutex> g(ctx->mu);
+            rc = holo_solve(ctx, hb, st);
+        }
+        if (e == hipSuccess && rc == AFH_OK)
+            e = hipMemcpyAsync(h + o_oa, d + o_oa, o_op + out_b - o_oa, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&f.ev, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventRecord(f.ev, st);
+        for (size_t i = 0; i < cnt; ++i) {
+            Request* r = rs[i];
+            r->out_bytes = A.n * sizeof(float);
+            r->pin_off = f.pin_off + o_oa + i * A.n * sizeof(float);
+            r->pin_off2 = f.pin_off + o_op + i * A.n * sizeof(float);
+            r->status = (e != hipSuccess || rc != AFH_OK) ? (rc != AFH_OK ? rc : AFH_ERR_HIP) : AFH_OK;
+            r->state = kRunning;
+            f.reqs.push_back(r);
+        }
+        if (e != hipSuccess || rc != AFH_OK) (void)hipStreamSynchronize(st);
+        flights.push_back(f);
+        return true;
+    }
+
+    // Completes finished flights (caller holds no lock); returns how many.
+    int reap(std::vector<Request*>& done_out) {
+        int n = 0;
+        for (auto it = flights.begin(); it != flights.end();) {
+            const hipError_t q = it->ev ? hipEventQuery(it->ev) : hipSuccess;
+            if (q == hipErrorNotReady) {
+                ++it;
+                continue;
+            }
+            const double t = now_ms();
+            for (Request* r : it->reqs) {
+                if (q != hipSuccess && r->status == AFH_OK) r->status = AFH_ERR_HIP;
+                if (r->status == AFH_OK) {
+                    memcpy(r->out, pin_base + r->pin_off, r->out_bytes);
+                    if (r->kind == kReqGSPAT) memcpy(r->out2, pin_base + r->pin_off2, r->out_bytes);
+                }
+                r->t_done = t;
+                done_out.push_back(r);
+                ++n;
+            }
+            if (it->ev) (void)hipEventDestroy(it->ev);
+            dev.release(it->dev_off, it->dev_bytes);
+            pin.release(it->pin_off, it->pin_bytes);
+            it = flights.erase(it);
+        }
+        return n;
+    }
+
+    void run() {
+        (void)hipSetDevice(device);
+        std::vector<Request*> todo;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> g(mu);
+                if (pending.empty() && flights.empty()) {
+                    cv_submit.wait_for(g, std::chrono::milliseconds(2), [&] { return stop || !pending.empty(); });
+                }
+                if (stop && pending.empty() && flights.empty()) return;
+                todo.assign(pending.begin(), pending.end());
+            }
+            std::vector<Request*> admitted;
+            if (!todo.empty()) dispatch(todo, admitted);
+            if (!admitted.empty()) {
+                std::lock_guard<std::mutex> g(mu);
+                for (Request* r : admitted) {
+                    auto it = std::find(pending.begin(), pending.end(), r);
+                    if (it != pending.end()) pending.erase(it);
+                }
+            }
+            std::vector<Request*> done;
+            reap(done);
+            if (!done.empty()) {
+                std::lock_guard<std::mutex> g(mu);
+                for (Request* r : done) {
+                    r->state = kDone;
+                    latencies.push_back(r->t_done - r->t_submit);
+                    ++completed;
+                }
+                cv_done.notify_all();
+            } else if (admitted.empty()) {
+                std::this_thread::sleep_for(std::chrono::microseconds(20));
+            }
+        }
+    }
+};
+
+}  // namespace afh
+
+using namespace afh;
+
+extern "C" {
+
+int32_t AFS_Create(const AFS_Config* cfg, void** out) {
+    clear_error();
+    if (!cfg || !out || cfg->n_streams == 0 || cfg->n_streams > 16 || cfg->max_gspat_batch == 0 ||
+        cfg->hbm_budget_bytes < 4096 || cfg->pinned_budget_bytes < 4096 ||
+        cfg->gspat_precision < AFH_HOLO_FP32 || cfg->gspat_precision > AFH_HOLO_MIXED_SPLIT) {
+        set_error("AFS_Create: bad configuration");
+        return AFH_ERR_ARG;
+    }
+    *out = nullptr;
+    if (!device().ensure()) return AFH_ERR_NO_DEVICE;
+    std::unique_ptr<Server> s(new Server());
+    s->cfg = *cfg;
+    (void)hipGetDevice(&s->device);
+    if (!hip_ok(hipMalloc(&s->dev_base, cfg->hbm_budget_bytes), "AFS_Create: hipMalloc (HBM budget)")) return AFH_ERR_HIP;
+    if (!hip_ok(hipHostMalloc(&s->pin_base, cfg->pinned_budget_bytes, hipHostMallocDefault),
+                "AFS_Create: hipHostMalloc"))
+        return AFH_ERR_HIP;
+    s->dev.init(cfg->hbm_budget_bytes);
+    s->pin.init(cfg->pinned_budget_bytes);
+    for (uint32_t i = 0; i < cfg->n_streams; ++i) {
+        hipStream_t st;
+        if (!hip_ok(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "AFS_Create: stream")) return AFH_ERR_HIP;
+        s->streams.push_back(st);
+        s->ctxs.emplace_back(new Context());
+    }
+    if (!dir_cells_device()) {
+        set_error("AFS_Create: directivity table");
+        return AFH_ERR_HIP;
+    }
+    Server* raw = s.get();
+    s->worker = std::thread([raw] { raw->run(); });
+    *out = s.release();
+    return AFH_OK;
+}
+
+void AFS_Destroy(void* srv) { delete static_cast<Server*>(srv); }
+
+int32_t AFS_AddArray(void* srv, const void* sources, const float* wavenums, const float* attens, uint64_t n,
+                     int32_t source_type, uint32_t* array_id) {
+    clear_error();
+    Server* s = static_cast<Server*>(srv);
+    if (!s || !sources || !wavenums || !attens || !array_id || n == 0 || n > 0xFFFFFFF ||
+        (source_type != kSphere && source_type != kT4010A1)) {
+        set_error("AFS_AddArray: bad argument");
+        return AFH_ERR_ARG;
+    }
+    SrvArray A;
+    A.source_type = source_type;
+    A.n = (uint32_t)n;
+    A.np = (uint32_t)packed_count(n);
+    std::vector<float> host((size_t)A.np * AFH_SOURCE_STRIDE);
+    pack_sources(static_cast<const float*>(sources), wavenums, attens, n, source_type, host.data());
+    if (!hip_ok(hipMalloc(&A.packed, host.size() * sizeof(float)), "AFS_AddArray: hipMalloc")) return AFH_ERR_HIP;
+    if (!hip_ok(hipMemcpy(A.packed, host.data(), host.size() * sizeof(float), hipMemcpyHostToDevice),
+                "AFS_AddArray: upload"))
+        return AFH_ERR_HIP;
+    std::lock_guard<std::mutex> g(s->mu);
+    *array_id = (uint32_t)s->arrays.size();
+    s->arrays.push_back(A);
+    return AFH_OK;
+}
+
+static int32_t enqueue(Server* s, Request* r, uint64_t* ticket) {
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        r->ticket = s->next_ticket++;
+        r->t_submit = now_ms();
+        s->by_ticket[r->ticket] = r;
+        s->pending.push_back(r);
+    }
+    s->cv_submit.notify_one();
+    *ticket = r->ticket;
+    return AFH_OK;
+}
+
+int32_t AFS_SubmitField(void* srv, uint32_t array_id, const AFH_PointSet* pts, int32_t field_type, float* out_host,
+                        uint64_t* ticket) {
+    clear_error();
+    Server* s = static_cast<Server*>(srv);
+    if (!s || !pts || !out_host || !ticket || field_type < 0 || field_type > 2) {
+        set_error("AFS_SubmitField: bad argument");
+        return AFH_ERR_ARG;
+    }
+    std::unique_ptr<Request> r(new Request());
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        if (array_id >= s->arrays.size()) {
+            set_error("AFS_SubmitField: unknown array");
+            return AFH_ERR_ARG;
+        }
+    }
+    r->kind = kReqField;
+    r->array_id = array_id;
+    r->field_type = field_type;
+    r->out = out_host;
+    if (pts->kind == 0) {
+        GridArea& g = r->grid;
+        for (int k = 0; k < 3; ++k) {
+            if (pts->order[k] < 0 || pts->order[k] > 2) {
+                set_error("AFS_SubmitField: bad grid order");
+                return AFH_ERR_ARG;
+            }
+            g.order[k] = pts->order[k];
+            g.bounds[pts->order[k]] = pts->count[k];
+            g.origin[k] = pts->origin[k];
+        }
+        if (g.order[0] == g.order[1] || g.order[1] == g.order[2] || g.order[0] == g.order[2]) {
+            set_error("AFS_SubmitField: grid order must be a permutation");
+            return AFH_ERR_ARG;
+        }
+        g.res = pts->resolution;
+        r->n_points = (uint64_t)pts->count[0] * pts->count[1] * pts->count[2];
+    } else if (pts->kind == 1 && (pts->points || pts->n_points == 0)) {
+        r->scatter = true;
+        r->n_points = pts->n_points;
+        r->points.assign(pts->points, pts->points + 3 * pts->n_points);  // host points
+    } else {
+        set_error("AFS_SubmitField: bad point set");
+        return AFH_ERR_ARG;
+    }
+    const size_t per = field_type == AFC_FIELD_COMPLEX ? 2 : 1;
+    const uint64_t need = r->n_points * per * sizeof(float) + (r->scatter ? r->n_points * 12 : 0) + 512;
+    if (r->n_points == 0 || r->n_points > kMaxPointsPerCall || need > s->cfg.hbm_budget_bytes ||
+        need > s->cfg.pinned_budget_bytes) {
+        set_error("AFS_SubmitField: request does not fit the HBM / pinned budget (or has no points)");
+        return AFH_ERR_BUDGET;
+    }
+    return enqueue(s, r.release(), ticket);
+}
+
+int32_t AFS_SubmitGSPAT(void* srv, uint32_t array_id, const float* foci, const float* amps, uint32_t m,
+                        float* out_amp, float* out_phase, uint64_t* ticket) {
+    clear_error();
+    Server* s = static_cast<Server*>(srv);
+    if (!s || !foci || !amps || !out_amp || !out_phase || !ticket || m == 0 || m > 128) {
+        set_error("AFS_SubmitGSPAT: bad argument (1 to 128 foci)");
+        return AFH_ERR_ARG;
+    }
+    std::unique_ptr<Request> r(new Request());
+    uint32_t n;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        if (array_id >= s->arrays.size()) {
+            set_error("AFS_SubmitGSPAT: unknown array");
+            return AFH_ERR_ARG;
+        }
+        n = s->arrays[array_id].n;
+    }
+    const uint64_t need = (uint64_t)m * 16 + 2ull * n * 4 + 1024;
+    if (need > s->cfg.hbm_budget_bytes || need > s->cfg.pinned_budget_bytes) {
+        set_error("AFS_SubmitGSPAT: request does not fit the budget");
+        return AFH_ERR_BUDGET;
+    }
+    r->kind = kReqGSPAT;
+    r->array_id = array_id;
+    r->m = m;
+    r->foci.assign(foci, foci + 3 * (size_t)m);
+    r->amps.assign(amps, amps + m);
+    r->out = out_amp;
+    r->out2 = out_phase;
+    return enqueue(s, r.release(), ticket);
+}
+
+int32_t AFS_Wait(void* srv, uint64_t ticket, double* latency_ms) {
+    clear_error();
+    Server* s = static_cast<Server*>(srv);
+    if (!s) return AFH_ERR_ARG;
+    std::unique_lock<std::mutex> g(s->mu);
+    auto it = s->by_ticket.find(ticket);
+    if (it == s->by_ticket.end()) {
+        set_error("AFS_Wait: unknown ticket");
+        return AFH_ERR_ARG;
+    }
+    Request* r = it->second;
+    s->cv_done.wait(g, [&] { return r->state == kDone; });
+    if (latency_ms) *latency_ms = r->t_done - r->t_submit;
+    const int32_t st = r->status;
+    s->by_ticket.erase(it);
+    delete r;
+    return st;
+}
+
+int32_t AFS_Stats(void* srv, double* p50_ms, double* p99_ms, uint64_t* completed, int32_t reset) {
+    Server* s = static_cast<Server*>(srv);
+    if (!s) return AFH_ERR_ARG;
+    std::lock_guard<std::mutex> g(s->mu);
+    std::vector<double> v = s->latencies;
+    std::sort(v.begin(), v.end());
+    auto pct = [&](double q) { return v.empty() ? 0.0 : v[std::min(v.size() - 1, (size_t)(q * (v.size() - 1) + 0.5))]; };
+    if (p50_ms) *p50_ms = pct(0.50);
+    if (p99_ms) *p99_ms = pct(0.99);
+    if (completed) *completed = s->completed;
+    if (reset) {
+        s->latencies.clear();
+        s->completed = 0;
+    }
+    return AFH_OK;
+}
+
+}  // extern "C"

@@ -76,6 +76,9 @@ def main():
     ap.add_argument("--no-gspat", action="store_true")
     ap.add_argument("--gspat-batch", type=int, default=1024)
     ap.add_argument("--no-lm", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-qps", default="500,1000,2000,3000")
+    ap.add_argument("--serve-seconds", type=float, default=1.5)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device-to-device copy (reference for HBM byte counters)")
     args = ap.parse_args()
@@ -172,10 +175,100 @@ def main():
         result["gspat"] = bench_gspat(torch, afc, D, O, src, k, a, args.gspat_batch)
     if not args.no_lm:
         result["lm"] = bench_lm(torch, D, O)
+    if not args.no_serve:
+        result["serve"] = bench_serve(D, O, [float(q) for q in args.serve_qps.split(",")], args.serve_seconds)
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline(src, k, a, area["order"], cnt, o, area["res"])
     print(json.dumps(result), flush=True)
+
+
+def bench_serve(D, O, qps_list, seconds):
+    """Served mixed stream: Poisson arrivals at a fixed QPS, 70 % field
+    evaluations (grids of 1e4..1e6 points, log-uniform, 1 mm, complex output)
+    and 30 % GS-PAT solves (2..128 foci), each on one of the 249-, 996- or
+    3984-transducer arrays chosen uniformly; random seeded phases and targets."""
+    import queue
+    import threading
+    from afc.server import Server
+
+    arrs = []
+    for tiles in ((1, 1), (2, 2), (4, 4)):
+        pos = O.array_249() if tiles == (1, 1) else O.tiled_array(*tiles)
+        rng = np.random.default_rng(len(pos))
+        src = O.t4010a1_sources(pos, phase=rng.uniform(0, 2 * np.pi, len(pos)).astype(np.float32))
+        n = len(pos)
+        k = np.full(n, O.WAVENUM_40K, np.float32)
+        a = np.full(n, np.float32(O.attenuation_coef(40e3, 30, 1, 1, O.TEMP_340)), np.float32)
+        ext = pos.max(0)
+        arrs.append((src, k, a, ext))
+    out = {"workload": "Poisson arrivals; 70% field (1e4..1e6 pts log-uniform, complex) / 30% GS-PAT (M~U[2,128], "
+                       "MIXED_SPLIT); arrays 249/996/3984 uniform", "runs": []}
+    for qps in qps_list:
+        srv = Server(hbm_budget=4 << 30, pinned_budget=1 << 30, n_streams=4, max_gspat_batch=256,
+                     gspat_precision=D.HOLO_MIXED_SPLIT)
+        ids = [srv.add_array(1, src, k, a) for src, k, a, _ in arrs]
+        rng = np.random.default_rng(int(qps))
+        nreq = max(1, int(qps * seconds))
+        arrivals = np.cumsum(rng.exponential(1.0 / qps, nreq))
+        specs = []
+        for _ in range(nreq):
+            aid = int(rng.integers(0, 3))
+            ext = arrs[aid][3]
+            if rng.random() < 0.7:
+                P = float(np.exp(rng.uniform(np.log(1e4), np.log(1e6))))
+                nz = int(rng.integers(1, 17))
+                nxy = max(1, int(round((P / nz) ** 0.5)))
+                origin = (float(rng.uniform(0, ext[0])), float(rng.uniform(0, ext[1])), float(rng.uniform(50, 200)))
+                specs.append(("f", aid, (nxy, nxy, nz), origin))
+            else:
+                m = int(rng.integers(2, 129))
+                foci = np.stack([rng.uniform(0, ext[0], m), rng.uniform(0, ext[1], m), rng.uniform(100, 250, m)],
+                                1).astype(np.float32)
+                specs.append(("g", aid, foci, rng.uniform(0.5, 1.0, m).astype(np.float32)))
+        q = queue.Queue()
+        lat = []
+
+        def waiter():
+            while True:
+                item = q.get()
+                if item is None:
+                    return
+                lat.append(srv.wait(item))
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        terms = 0.0
+        solves = 0
+        t0 = time.perf_counter()
+        for i, sp in enumerate(specs):
+            delay = t0 + arrivals[i] - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            if sp[0] == "f":
+                _, aid, cnt, origin = sp
+                ps = D.grid_pointset((0, 1, 2), cnt, origin, 1.0)
+                buf = np.empty(cnt[0] * cnt[1] * cnt[2] * 2, np.float32)
+                q.put(srv.submit_field(ids[aid], ps, 2, buf))
+                terms += cnt[0] * cnt[1] * cnt[2] * arrs[aid][0].shape[0]
+            else:
+                _, aid, foci, amps = sp
+                n = arrs[aid][0].shape[0]
+                q.put(srv.submit_gspat(ids[aid], foci, amps, np.empty(n, np.float32), np.empty(n, np.float32)))
+                solves += 1
+        q.put(None)
+        th.join()
+        elapsed = time.perf_counter() - t0
+        p50, p99, done = srv.stats()
+        srv.close()
+        run = {"qps_offered": qps, "requests": nreq, "achieved_req_per_s": round(nreq / elapsed, 1),
+               "p50_ms": round(p50, 3), "p99_ms": round(p99, 3),
+               "field_gsrc_pt_per_s": round(terms / elapsed / 1e9, 2), "gspat_solves_per_s": round(solves / elapsed, 1)}
+        out["runs"].append(run)
+        log(f"serve qps {qps}: {run}")
+    ok = [r for r in out["runs"] if r["p99_ms"] <= 5.0]
+    out["max_qps_p99_le_5ms"] = max((r["qps_offered"] for r in ok), default=None)
+    return out
 
 
 def bench_lm(torch, D, O):

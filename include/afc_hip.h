@@ -4,7 +4,8 @@
  *
  * Part 1 is the drop-in surface: the same symbols, argument order, value
  * structs, type tags and ownership rules as the reference's cdylib `afccapi`
- * (capi/src/*.rs; ctypes argtypes in python/afc/nativemethods.py:113-234).
+ * (the .rs files of capi/src; ctypes argtypes in
+ * python/afc/nativemethods.py:113-234).
  * Each declaration cites the reference export it replaces.
  *
  * Part 2 adds entry points the reference has no counterpart for: device-
@@ -235,6 +236,39 @@ int32_t AFH_HoloSolve(void* ctx, const float* packed_sources_dev, uint64_t n_pac
 int32_t AFH_LMSolve(void* ctx, const float* packed_sources_dev, uint64_t n_packed, uint64_t n_sources,
                     int32_t source_type, const float* foci_dev, const float* amps_dev, const uint32_t* offsets_host,
                     uint32_t batch, uint32_t k_max, float* out_phase_dev, uint32_t* out_iters_dev, void* stream);
+
+/* ---- request server: field evaluations and GS-PAT solves, continuously
+ * batched on one GPU (see the design notes in server.hip) */
+typedef struct {
+    uint64_t hbm_budget_bytes;     /* device arena for request inputs / outputs */
+    uint64_t pinned_budget_bytes;  /* pinned host arena for result staging */
+    uint32_t n_streams;            /* 1..16 */
+    uint32_t max_gspat_batch;      /* instances per batched GS-PAT launch */
+    int32_t gspat_precision;       /* AFH_HOLO_* */
+    uint32_t gspat_iterations;     /* 100 in the reference */
+} AFS_Config;
+
+int32_t AFS_Create(const AFS_Config* cfg, void** out);
+void AFS_Destroy(void* srv);
+/* Registers a transducer array (reference structs + k + alpha, as
+ * AFH_PackSources takes them); it stays resident on the device. */
+int32_t AFS_AddArray(void* srv, const void* sources, const float* wavenums, const float* attens, uint64_t n,
+                     int32_t source_type, uint32_t* array_id);
+/* Field of a registered array at a point set (grid, or scatter with HOST
+ * points), FAST precision, written to out_host when the request completes
+ * (1 or 2 floats per point as AFC_FIELD_*). */
+int32_t AFS_SubmitField(void* srv, uint32_t array_id, const AFH_PointSet* pts, int32_t field_type, float* out_host,
+                        uint64_t* ticket);
+/* GS-PAT on a registered array: m (1..128) foci (host, 3 floats each) and
+ * target amplitudes; per-source amplitude / phase written to out_amp /
+ * out_phase (n floats each) when the request completes. */
+int32_t AFS_SubmitGSPAT(void* srv, uint32_t array_id, const float* foci, const float* amps, uint32_t m,
+                        float* out_amp, float* out_phase, uint64_t* ticket);
+/* Blocks until the request completes; returns its status and latency
+ * (submit to result in the caller's buffer). Each ticket is waited once. */
+int32_t AFS_Wait(void* srv, uint64_t ticket, double* latency_ms);
+/* Latency percentiles over the requests completed since the last reset. */
+int32_t AFS_Stats(void* srv, double* p50_ms, double* p99_ms, uint64_t* completed, int32_t reset);
 
 /* T4010A1 directivity as a function of cos(theta), evaluated on the host with
  * exactly the formulas the kernels use in each precision mode (cell table

@@ -1,0 +1,129 @@
+"""Request server: continuous batching, HBM-budget admission and isolation.
+
+Each request's result must be bit-identical when it runs alone and when it
+runs in a mix of concurrent field and GS-PAT requests on several arrays (the
+kernels keep a fixed per-request reduction order). A server with a small HBM
+budget must still complete a stream larger than the budget (admission waits
+for memory) and must refuse a request that can never fit."""
+import threading
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def log(*a):
+    print(*a, flush=True)
+
+
+def arrays(oracle):
+    out = []
+    for tiles in ((1, 1), (2, 2)):
+        pos = oracle.array_249() if tiles == (1, 1) else oracle.tiled_array(*tiles)
+        rng = np.random.default_rng(len(pos))
+        src = oracle.t4010a1_sources(pos, phase=rng.uniform(0, 6.28, len(pos)).astype(np.float32))
+        n = len(pos)
+        k = np.full(n, oracle.WAVENUM_40K, np.float32)
+        a = np.full(n, np.float32(oracle.attenuation_coef(40e3, 30, 1, 1, oracle.TEMP_340)), np.float32)
+        out.append((src, k, a))
+    return out
+
+
+def make_requests(rng, n_arrays, count):
+    reqs = []
+    for i in range(count):
+        aid = int(rng.integers(0, n_arrays))
+        if rng.random() < 0.6:
+            nx, ny, nz = (int(v) for v in rng.integers(8, 64, 3))
+            origin = (float(rng.uniform(0, 100)), float(rng.uniform(0, 80)), float(rng.uniform(60, 200)))
+            reqs.append(("field", aid, (nx, ny, nz), origin, int(rng.integers(0, 3))))
+        else:
+            m = int(rng.integers(2, 129))
+            foci = np.stack([rng.uniform(0, 170, m), rng.uniform(0, 130, m), rng.uniform(100, 250, m)],
+                            1).astype(np.float32)
+            amps = rng.uniform(0.5, 1.0, m).astype(np.float32)
+            reqs.append(("gspat", aid, foci, amps))
+    return reqs
+
+
+def submit(srv, ids, arr_n, req):
+    from afc import device as D
+    if req[0] == "field":
+        _, aid, cnt, origin, ft = req
+        ps = D.grid_pointset((0, 1, 2), cnt, origin, 1.0)
+        out = np.empty(cnt[0] * cnt[1] * cnt[2] * (2 if ft == 2 else 1), np.float32)
+        return srv.submit_field(ids[aid], ps, ft, out), (out,)
+    _, aid, foci, amps = req
+    oa, op = np.empty(arr_n[aid], np.float32), np.empty(arr_n[aid], np.float32)
+    return srv.submit_gspat(ids[aid], foci, amps, oa, op), (oa, op)
+
+
+def test_mix_equals_alone(afc, oracle, torch_cuda):
+    from afc.server import Server
+    arrs = arrays(oracle)
+    rng = np.random.default_rng(0)
+    reqs = make_requests(rng, len(arrs), 48)
+    srv = Server(hbm_budget=256 << 20, pinned_budget=128 << 20, n_streams=4, max_gspat_batch=64)
+    ids = [srv.add_array(1, *a) for a in arrs]
+    arr_n = [a[0].shape[0] for a in arrs]
+    # alone: one request in the server at a time
+    alone = []
+    for r in reqs:
+        t, bufs = submit(srv, ids, arr_n, r)
+        srv.wait(t)
+        alone.append(bufs)
+    # mixed: three submitting threads, everything in flight together
+    mixed = [None] * len(reqs)
+    tickets = [None] * len(reqs)
+
+    def worker(idx):
+        for i in idx:
+            tickets[i], mixed[i] = submit(srv, ids, arr_n, reqs[i])
+
+    th = [threading.Thread(target=worker, args=(list(range(j, len(reqs), 3)),)) for j in range(3)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    for i in range(len(reqs)):
+        srv.wait(tickets[i])
+    for i, r in enumerate(reqs):
+        for x, y in zip(alone[i], mixed[i]):
+            assert np.array_equal(x, y), (i, r[0])
+    p50, p99, n = srv.stats()
+    log(f"  {n} requests completed; p50 {p50:.3f} ms p99 {p99:.3f} ms")
+    # field results match the FP64 oracle (random phases: an incoherent field,
+    # where the FP32 CPU path itself is ~1e-5 away from FP64)
+    k = next(i for i, r in enumerate(reqs) if r[0] == "field" and r[4] == 2)
+    _, aid, cnt, origin, _ = reqs[k]
+    src, kk, aa = arrs[aid]
+    pts, _ = oracle.grid_points((0, 1, 2), [(origin[a], origin[a] + cnt[a] - 1) for a in range(3)], 1.0)
+    want = oracle.field(1, src, kk, aa, pts, "f64")
+    got = alone[k][0].view(np.complex64).astype(np.complex128)
+    assert np.linalg.norm(got - want) / np.linalg.norm(want) <= 5e-6
+    srv.close()
+
+
+def test_small_budget_queues_and_rejects(afc, oracle, torch_cuda):
+    from afc import AFCError
+    from afc import device as D
+    from afc.server import Server
+    arrs = arrays(oracle)
+    srv = Server(hbm_budget=4 << 20, pinned_budget=4 << 20, n_streams=2, max_gspat_batch=16)
+    aid = srv.add_array(1, *arrs[0])
+    # 16 requests of 1.6 MB each: 25.6 MB through a 4 MB budget
+    outs, ts = [], []
+    for i in range(16):
+        ps = D.grid_pointset((0, 1, 2), (100, 100, 20), (30.0, 20.0, 100.0 + i), 1.0)
+        out = np.empty(100 * 100 * 20 * 2, np.float32)
+        ts.append(srv.submit_field(aid, ps, 2, out))
+        outs.append(out)
+    for t in ts:
+        srv.wait(t)
+    assert all(np.all(np.isfinite(o)) for o in outs)
+    # 8 MB of output can never fit a 4 MB arena
+    ps = D.grid_pointset((0, 1, 2), (100, 100, 100), (30.0, 20.0, 100.0), 1.0)
+    with pytest.raises(AFCError, match="budget"):
+        srv.submit_field(aid, ps, 2, np.empty(100 * 100 * 100 * 2, np.float32))
+    srv.close()
