@@ -24,8 +24,9 @@ def _err(what, rc):
 
 class Server:
     def __init__(self, hbm_budget=1 << 30, pinned_budget=256 << 20, n_streams=4, max_gspat_batch=256,
-                 gspat_precision=3, gspat_iterations=100):
-        cfg = AFS_Config(hbm_budget, pinned_budget, n_streams, max_gspat_batch, gspat_precision, gspat_iterations)
+                 gspat_precision=3, gspat_iterations=100, gspat_cus=0):
+        cfg = AFS_Config(hbm_budget, pinned_budget, n_streams, max_gspat_batch, gspat_precision, gspat_iterations,
+                         gspat_cus)
         self.handle = c_void_p()
         _err("AFS_Create", dll().AFS_Create(byref(cfg), byref(self.handle)))
         self._keep = {}
@@ -77,7 +78,8 @@ class Server:
         _err("AFS_Wait", rc)
         return lat.value
 
-    def stats(self, reset=False):
+    def stats(self, kind=-1, reset=False):
+        """(p50 ms, p99 ms, completed) of field (0), GS-PAT (1) or all (-1) requests."""
         p50, p99, n = c_double(), c_double(), c_uint64()
-        _err("AFS_Stats", dll().AFS_Stats(self.handle, byref(p50), byref(p99), byref(n), 1 if reset else 0))
+        _err("AFS_Stats", dll().AFS_Stats(self.handle, kind, byref(p50), byref(p99), byref(n), 1 if reset else 0))
         return p50.value, p99.value, n.value

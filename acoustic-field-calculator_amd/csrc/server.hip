@@ -117,6 +117,8 @@ struct Flight {
     std::vector<Request*> reqs;
     hipEvent_t ev = nullptr;
     size_t dev_off = 0, dev_bytes = 0, pin_off = 0, pin_bytes = 0;
+    int stream = 0;
+    double work = 0;  // source-point terms, for the stream choice
 };
 
 struct Server {
@@ -125,6 +127,7 @@ struct Server {
     char* pin_base = nullptr;
     Arena dev, pin;
     std::vector<hipStream_t> streams;
+    std::vector<double> outstanding;  // queued terms per stream
     std::vector<std::unique_ptr<Context>> ctxs;
     std::vector<SrvArray> arrays;
     std::mutex mu;
@@ -132,7 +135,7 @@ struct Server {
     std::deque<Request*> pending;
     std::unordered_map<uint64_t, Request*> by_ticket;
     std::deque<Flight> flights;
-    std::vector<double> latencies;
+    std::vector<double> latencies[2];  // by request kind
     uint64_t next_ticket = 1, completed = 0;
     bool stop = false;
     std::thread worker;
@@ -156,10 +159,17 @@ struct Server {
         for (auto& kv : by_ticket) delete kv.second;  // the worker drained `pending` first
     }
 
-    hipStream_t take_stream() {
-        const int s = next_stream;
-        next_stream = (next_stream + 1) % (int)streams.size();
-        return streams[s];
+    // Stream 0 carries the GS-PAT batches (on its own CU partition when the
+    // configuration reserves one), so a solve never waits behind a large field
+    // evaluation; a field request goes to the field stream with the least
+    // queued work (all streams when there is only one).
+    hipStream_t gspat_stream() { return streams[0]; }
+    int field_stream_index() {
+        if (streams.size() == 1) return 0;
+        int best = 1;
+        for (int i = 2; i < (int)streams.size(); ++i)
+            if (outstanding[i] < outstanding[best]) best = i;
+        return best;
     }
     int stream_index(hipStream_t s) const {
         for (size_t i = 0; i < streams.size(); ++i)
@@ -208,7 +218,10 @@ struct Server {
             dev.release(f.dev_off, f.dev_bytes);
             return false;
         }
-        hipStream_t st = take_stream();
+        f.stream = field_stream_index();
+        f.work = (double)r->n_points * A.n;
+        outstanding[f.stream] += f.work;
+        hipStream_t st = streams[f.stream];
         char* d = dev_base + f.dev_off;
         char* h = pin_base + f.pin_off;
         const float* pts_dev = nullptr;
@@ -253,7 +266,7 @@ struct Server {
             dev.release(f.dev_off, f.dev_bytes);
             return false;
         }
-        hipStream_t st = take_stream();
+        hipStream_t st = gspat_stream();
         Context* ctx = ctxs[stream_index(st)].get();
         char* d = dev_base + f.dev_off;
         char* h = pin_base + f.pin_off;
@@ -324,6 +337,7 @@ struct Server {
                 ++n;
             }
             if (it->ev) (void)hipEventDestroy(it->ev);
+            outstanding[it->stream] -= it->work;
             dev.release(it->dev_off, it->dev_bytes);
             pin.release(it->pin_off, it->pin_bytes);
             it = flights.erase(it);
@@ -346,11 +360,12 @@ struct Server {
             std::vector<Request*> admitted;
             if (!todo.empty()) dispatch(todo, admitted);
             if (!admitted.empty()) {
+                // admitted requests are no longer kPending; keep the rest in order
                 std::lock_guard<std::mutex> g(mu);
-                for (Request* r : admitted) {
-                    auto it = std::find(pending.begin(), pending.end(), r);
-                    if (it != pending.end()) pending.erase(it);
-                }
+                std::deque<Request*> keep;
+                for (Request* r : pending)
+                    if (r->state == kPending) keep.push_back(r);
+                pending.swap(keep);
             }
             std::vector<Request*> done;
             reap(done);
@@ -358,7 +373,7 @@ struct Server {
                 std::lock_guard<std::mutex> g(mu);
                 for (Request* r : done) {
                     r->state = kDone;
-                    latencies.push_back(r->t_done - r->t_submit);
+                    latencies[r->kind].push_back(r->t_done - r->t_submit);
                     ++completed;
                 }
                 cv_done.notify_all();
@@ -394,11 +409,37 @@ int32_t AFS_Create(const AFS_Config* cfg, void** out) {
         return AFH_ERR_HIP;
     s->dev.init(cfg->hbm_budget_bytes);
     s->pin.init(cfg->pinned_budget_bytes);
+    // CU partition: the GS-PAT stream (0) on the first gspat_cus CUs, the field
+    // streams on the others; plain streams when no partition is asked for or
+    // the runtime refuses the masks
+    int n_cu = 0;
+    (void)hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, s->device);
+    const bool part = cfg->gspat_cus > 0 && cfg->n_streams > 1 && n_cu > (int)cfg->gspat_cus && n_cu <= 1024;
     for (uint32_t i = 0; i < cfg->n_streams; ++i) {
-        hipStream_t st;
-        if (!hip_ok(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "AFS_Create: stream")) return AFH_ERR_HIP;
+        hipStream_t st = nullptr;
+        bool made = false;
+        if (part) {
+            std::vector<uint32_t> mask((n_cu + 31) / 32, 0u);
+            for (int c = 0; c < n_cu; ++c) {
+                const bool mine = (i == 0) ? c < (int)cfg->gspat_cus : c >= (int)cfg->gspat_cus;
+                if (mine) mask[c / 32] |= 1u << (c % 32);
+            }
+            made = hipExtStreamCreateWithCUMask(&st, (uint32_t)mask.size(), mask.data()) == hipSuccess;
+            if (!made) (void)hipGetLastError();
+        }
+        if (!made && !hip_ok(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "AFS_Create: stream"))
+            return AFH_ERR_HIP;
         s->streams.push_back(st);
+        s->outstanding.push_back(0.0);
         s->ctxs.emplace_back(new Context());
+    }
+    // the GS-PAT workspace (R and p for a full batch of 128-focus instances)
+    // is reserved now rather than grown while serving
+    {
+        const size_t mb = cfg->max_gspat_batch;
+        if (!s->ctxs[0]->work.reserve(mb * (128 * 128 + 128) * sizeof(float2) + (2 * mb + 2) * 4 + 4096) ||
+            !s->ctxs[0]->staging.reserve((2 * mb + 2) * 4))
+            return AFH_ERR_HIP;
     }
     if (!dir_cells_device()) {
         set_error("AFS_Create: directivity table");
@@ -557,18 +598,21 @@ int32_t AFS_Wait(void* srv, uint64_t ticket, double* latency_ms) {
     return st;
 }
 
-int32_t AFS_Stats(void* srv, double* p50_ms, double* p99_ms, uint64_t* completed, int32_t reset) {
+int32_t AFS_Stats(void* srv, int32_t kind, double* p50_ms, double* p99_ms, uint64_t* completed, int32_t reset) {
     Server* s = static_cast<Server*>(srv);
-    if (!s) return AFH_ERR_ARG;
+    if (!s || kind < -1 || kind > 1) return AFH_ERR_ARG;
     std::lock_guard<std::mutex> g(s->mu);
-    std::vector<double> v = s->latencies;
+    std::vector<double> v;
+    for (int k = 0; k < 2; ++k)
+        if (kind < 0 || kind == k) v.insert(v.end(), s->latencies[k].begin(), s->latencies[k].end());
     std::sort(v.begin(), v.end());
     auto pct = [&](double q) { return v.empty() ? 0.0 : v[std::min(v.size() - 1, (size_t)(q * (v.size() - 1) + 0.5))]; };
     if (p50_ms) *p50_ms = pct(0.50);
     if (p99_ms) *p99_ms = pct(0.99);
-    if (completed) *completed = s->completed;
+    if (completed) *completed = v.size();
     if (reset) {
-        s->latencies.clear();
+        s->latencies[0].clear();
+        s->latencies[1].clear();
         s->completed = 0;
     }
     return AFH_OK;

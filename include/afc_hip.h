@@ -246,6 +246,7 @@ typedef struct {
     uint32_t max_gspat_batch;      /* instances per batched GS-PAT launch */
     int32_t gspat_precision;       /* AFH_HOLO_* */
     uint32_t gspat_iterations;     /* 100 in the reference */
+    uint32_t gspat_cus;            /* CUs reserved for GS-PAT batches (0: shared) */
 } AFS_Config;
 
 int32_t AFS_Create(const AFS_Config* cfg, void** out);
@@ -267,8 +268,9 @@ int32_t AFS_SubmitGSPAT(void* srv, uint32_t array_id, const float* foci, const f
 /* Blocks until the request completes; returns its status and latency
  * (submit to result in the caller's buffer). Each ticket is waited once. */
 int32_t AFS_Wait(void* srv, uint64_t ticket, double* latency_ms);
-/* Latency percentiles over the requests completed since the last reset. */
-int32_t AFS_Stats(void* srv, double* p50_ms, double* p99_ms, uint64_t* completed, int32_t reset);
+/* Latency percentiles over the requests completed since the last reset, of
+ * one kind (0 field, 1 GS-PAT) or of both (-1). */
+int32_t AFS_Stats(void* srv, int32_t kind, double* p50_ms, double* p99_ms, uint64_t* completed, int32_t reset);
 
 /* T4010A1 directivity as a function of cos(theta), evaluated on the host with
  * exactly the formulas the kernels use in each precision mode (cell table

@@ -64,7 +64,8 @@ def test_mix_equals_alone(afc, oracle, torch_cuda):
     arrs = arrays(oracle)
     rng = np.random.default_rng(0)
     reqs = make_requests(rng, len(arrs), 48)
-    srv = Server(hbm_budget=256 << 20, pinned_budget=128 << 20, n_streams=4, max_gspat_batch=64)
+    # GS-PAT batches on a reserved CU partition, field requests on the rest
+    srv = Server(hbm_budget=256 << 20, pinned_budget=128 << 20, n_streams=4, max_gspat_batch=64, gspat_cus=32)
     ids = [srv.add_array(1, *a) for a in arrs]
     arr_n = [a[0].shape[0] for a in arrs]
     # alone: one request in the server at a time
@@ -92,6 +93,8 @@ def test_mix_equals_alone(afc, oracle, torch_cuda):
         for x, y in zip(alone[i], mixed[i]):
             assert np.array_equal(x, y), (i, r[0])
     p50, p99, n = srv.stats()
+    gp50, gp99, gn = srv.stats(kind=1)
+    assert gn > 0 and gn < n
     log(f"  {n} requests completed; p50 {p50:.3f} ms p99 {p99:.3f} ms")
     # field results match the FP64 oracle (random phases: an incoherent field,
     # where the FP32 CPU path itself is ~1e-5 away from FP64)
