@@ -50,17 +50,20 @@ struct DirEntry {
 };
 
 // Directivity as a function of c = cos(theta), for the FAST mode. Cells of
-// width exactly 1/512 in c: cell j (1..kDirCells-2) covers
-// [(j - 2) / 512, (j - 1) / 512); cell 0 holds c < -1/512 (theta > 90 deg,
-// D = 0) and the last cell c >= 482/512 (theta < 20 deg, D = 1). Inside a
-// cell, with t = 512 c - floor(512 c) in [0, 1), D is a quadratic in t; a
-// cell that contains a segment boundary of the reference's piecewise cubic
-// (theta = 20, 30, ..., 90 deg) holds two quadratics split at t = ts, so the
-// jumps of the table are kept. Max deviation from the reference formula
-// evaluated in FP64: below 3e-8.
+// width exactly 1/512 in c, indexed by u = 512 c (exact): cell j
+// (1..kDirCells-2) covers u in [j - 2, j - 1); cell 0 holds u < -1
+// (theta > 90 deg, D = 0) and the last cell u >= 482 (theta < 20 deg, D = 1).
+// Inside a cell, with t = u - floor(u) in [0, 1), D = P(t) + step(t) Q(t)
+// with quadratics P, Q and step(t) = clamp(t s + k, 0, 1): a cell that
+// contains a segment boundary of the reference's piecewise cubic (theta = 20,
+// 30, ..., 90 deg) at t = ts has s = 2^24 and k = 1 - 2^24 ts (ts rounded to a
+// multiple of 2^-24), so step is 0 below ts and 1 from ts on and the jumps of
+// the table are kept; every other cell has Q = 0 and k = -2^25. The form
+// needs no compare or select (no VCC traffic in the kernels' inner loops).
+// Max deviation from the reference formula evaluated in FP64: 1.2e-7.
 struct alignas(16) DirCell {
-    float a0, a1, a2, ts;  // piece for t < ts
-    float b0, b1, b2, pad; // piece for t >= ts
+    float p0, p1, p2, k;  // P(t) = p0 + p1 t + p2 t^2, step offset
+    float q0, q1, q2, s;  // Q(t) = q0 + q1 t + q2 t^2 (piece after ts minus P), step slope
 };
 constexpr int kDirCells = 485;
 constexpr float kDirScale = 512.0f;
@@ -144,16 +147,25 @@ __host__ __device__ inline float directivity_band(float c, const DirEntry* tab) 
     return d;
 }
 
-__host__ __device__ inline float directivity_table(float c, const DirCell* cells) {
-    const float cs = c * kDirScale;  // exact
-    const float fl = floorf(cs);
-    const float t = cs - fl;  // exact
-    int j = (int)fl + 2;
+// D from u = 512 c (see DirCell). fract(u) is v_fract_f32 on the device
+// (u - floor(u), kept below 1) and the same formula on the host.
+__host__ __device__ inline float directivity_cells(float u, const DirCell* cells) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const float t = __builtin_amdgcn_fractf(u);
+#else
+    const float t = fminf(u - floorf(u), 0x1.fffffep-1f);
+#endif
+    int j = (int)floorf(u) + 2;
     j = j < 0 ? 0 : (j > kDirCells - 1 ? kDirCells - 1 : j);
     const DirCell& e = cells[j];
-    const float pa = fmaf(fmaf(e.a2, t, e.a1), t, e.a0);
-    const float pb = fmaf(fmaf(e.b2, t, e.b1), t, e.b0);
-    return t >= e.ts ? pb : pa;
+    const float p = fmaf(fmaf(e.p2, t, e.p1), t, e.p0);
+    const float q = fmaf(fmaf(e.q2, t, e.q1), t, e.q0);
+    const float step = fminf(fmaxf(fmaf(t, e.s, e.k), 0.0f), 1.0f);
+    return fmaf(step, q, p);
+}
+
+__host__ __device__ inline float directivity_table(float c, const DirCell* cells) {
+    return directivity_cells(c * kDirScale, cells);  // exact scaling
 }
 
 #if defined(__HIPCC__)
@@ -187,6 +199,136 @@ __device__ __forceinline__ void term_value(const Src& s, float px, float py, flo
     } else {
         sincos_rev_poly(u, &sn, &cs);
     }
+}
+
+// FAST-mode source operands as the field kernels stage them in LDS (derived
+// once per source from Src):
+//   la  = log2|amp| (-inf for amp = 0), the amplitude enters the exponent;
+//   n   = 512 * direction, so n.diff / r is the cell coordinate u = 512 c;
+//   ph  = phase / 2 pi in [0, 1), plus 1/2 (mod 1) when amp < 0;
+//   nal = -alpha log2 e, kh2 = kh / 2.
+struct SrcF {
+    float x, y, z, la;
+    float nx, ny, nz, ph;
+    float kh, kl, nal, kh2;
+};
+
+__device__ __forceinline__ SrcF stage_fast(const Src& s) {
+    SrcF f;
+    f.x = s.x;
+    f.y = s.y;
+    f.z = s.z;
+    f.la = log2f(fabsf(s.amp));
+    f.nx = s.nx * kDirScale;
+    f.ny = s.ny * kDirScale;
+    f.nz = s.nz * kDirScale;
+    f.ph = s.amp < 0.0f ? __builtin_amdgcn_fractf(s.ph + 0.5f) : s.ph;
+    f.kh = s.kh;
+    f.kl = s.kl;
+    f.nal = -s.al;
+    f.kh2 = 0.5f * s.kh;
+    return f;
+}
+
+// One staged source's term at one point: (re, im) = a (cos phi, sin phi).
+// Same quantities as term_value's FAST mode, arranged so that every VALU
+// operand is a VGPR or an inline constant:
+//   a   = D(u) / r * 2^(la - alpha log2e r);
+//   phi = 2 pi (fract(r kh) + [err(r kh) + r kl + ph + r_lo kh]), with
+//         r_lo kh = (d2 - r^2) (kh / 2) / r; sin / cos take revolutions
+//         (argument in [0, 2)).
+template <int SRC>
+__device__ __forceinline__ void term_staged(const SrcF& s, float px, float py, float pz, const DirCell* cells,
+                                            float& a, float& cs, float& sn) {
+    const float dx = px - s.x;
+    const float dy = py - s.y;
+    const float dz = pz - s.z;
+    const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    const float rinv = __builtin_amdgcn_rsqf(d2);
+    const float r = d2 * rinv;
+    float w = rinv;
+    if (SRC == kT4010A1) w *= directivity_cells(fmaf(s.nx, dx, fmaf(s.ny, dy, s.nz * dz)) * rinv, cells);
+    a = w * __builtin_amdgcn_exp2f(fmaf(s.nal, r, s.la));
+    const float e = fmaf(-r, r, d2);
+    const float t_hi = r * s.kh;
+    const float err = fmaf(r, s.kh, -t_hi);
+    float lo = fmaf(r, s.kl, s.ph);
+    lo = fmaf(e, rinv * s.kh2, lo);
+    const float u = __builtin_amdgcn_fractf(t_hi) + (lo + err);
+    sn = __builtin_amdgcn_sinf(u);
+    cs = __builtin_amdgcn_cosf(u);
+}
+
+// floor(u) as an integer in one instruction (v_cvt_flr_i32_f32; NaN gives 0).
+__device__ __forceinline__ int floor_to_int(float u) {
+    int j;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(j) : "v"(u));
+    return j;
+}
+
+// The terms of one staged source at NP points (coordinates px[k], py[k],
+// pz[k]), added to (re[k], im[k]). Same arithmetic as term_staged (up to the
+// order of the two products that form a), arranged in stages over the
+// points: distances and cell lookups for every point first, so the LDS reads
+// of the cell table are in flight while the exponentials, phases and sin /
+// cos are computed. The table is read as two float4 arrays (cp = p0 p1 p2 k,
+// cq = q0 q1 q2 s of each DirCell), 16 bytes per read.
+template <int SRC, int NP>
+__device__ __forceinline__ void add_terms_staged(const SrcF& s, const float* px, const float* py, const float* pz,
+                                                 const float4* cp, const float4* cq, float* re, float* im) {
+    float d2[NP], rinv[NP], t[NP];
+    float4 P[NP], Q[NP];
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        const float dx = px[k] - s.x;
+        const float dy = py[k] - s.y;
+        const float dz = pz[k] - s.z;
+        d2[k] = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+        rinv[k] = __builtin_amdgcn_rsqf(d2[k]);
+        if (SRC == kT4010A1) {
+            const float u = fmaf(s.nx, dx, fmaf(s.ny, dy, s.nz * dz)) * rinv[k];
+            t[k] = __builtin_amdgcn_fractf(u);
+            int j = floor_to_int(u) + 2;
+            j = j < 0 ? 0 : (j > kDirCells - 1 ? kDirCells - 1 : j);
+            P[k] = cp[j];
+            Q[k] = cq[j];
+        }
+    }
+    float w[NP], sn[NP], cs[NP];
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        const float r = d2[k] * rinv[k];
+        w[k] = rinv[k] * __builtin_amdgcn_exp2f(fmaf(s.nal, r, s.la));
+        const float e = fmaf(-r, r, d2[k]);
+        const float t_hi = r * s.kh;
+        const float err = fmaf(r, s.kh, -t_hi);
+        float lo = fmaf(r, s.kl, s.ph);
+        lo = fmaf(e, rinv[k] * s.kh2, lo);
+        const float u = __builtin_amdgcn_fractf(t_hi) + (lo + err);
+        sn[k] = __builtin_amdgcn_sinf(u);
+        cs[k] = __builtin_amdgcn_cosf(u);
+    }
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        float a = w[k];
+        if (SRC == kT4010A1) {
+            const float p = fmaf(fmaf(P[k].z, t[k], P[k].y), t[k], P[k].x);
+            const float q = fmaf(fmaf(Q[k].z, t[k], Q[k].y), t[k], Q[k].x);
+            const float step = fminf(fmaxf(fmaf(t[k], Q[k].w, P[k].w), 0.0f), 1.0f);
+            a *= fmaf(step, q, p);
+        }
+        re[k] = fmaf(a, cs[k], re[k]);
+        im[k] = fmaf(a, sn[k], im[k]);
+    }
+}
+
+template <int SRC>
+__device__ __forceinline__ void add_term_staged(const SrcF& s, float px, float py, float pz, const DirCell* cells,
+                                                float& re, float& im) {
+    float a, cs, sn;
+    term_staged<SRC>(s, px, py, pz, cells, a, cs, sn);
+    re = fmaf(a, cs, re);
+    im = fmaf(a, sn, im);
 }
 
 // Adds one source's term at one point to (re, im).

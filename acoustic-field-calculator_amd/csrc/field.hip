@@ -7,7 +7,8 @@
 //
 // Layout and schedule (gfx950):
 //   * workgroup = 256 lanes (4 wave64); each lane owns PPL observation points
-//     strided by 256 so loads of scatter points and all output stores coalesce;
+//     (PPL = 1) strided by 256 so loads of scatter points and all output
+//     stores coalesce;
 //   * grid points are generated in registers from (origin, resolution,
 //     counts), with the reference's own f32 rounding (origin + n * res, no
 //     contraction), so no point array is read from HBM;
@@ -27,7 +28,7 @@
 namespace afh {
 
 constexpr int kBlock = 256;
-constexpr int kTile = 128;
+constexpr int kTile = 64;
 constexpr int kChunk = 32;
 static_assert(kTile % kChunk == 0, "tile must hold whole chunks");
 static_assert(AFH_SOURCE_PAD % kChunk == 0, "packed sources must be whole chunks");
@@ -82,18 +83,16 @@ static int segment_of(double c) {
 }
 
 void build_dir_cells(DirCell* cells) {
-    const float inf = __builtin_inff();
+    const float two24 = 16777216.0f;
     for (int j = 0; j < kDirCells; ++j) {
         DirCell& e = cells[j];
-        e.ts = inf;
-        e.pad = 0.f;
-        e.a1 = e.a2 = e.b1 = e.b2 = 0.f;
-        if (j == 0) {
-            e.a0 = e.b0 = 0.f;
-            continue;
-        }
+        e.p0 = e.p1 = e.p2 = 0.f;
+        e.q0 = e.q1 = e.q2 = 0.f;
+        e.k = -2.0f * two24;  // step = 0 everywhere in the cell
+        e.s = two24;
+        if (j == 0) continue;  // D = 0
         if (j == kDirCells - 1) {
-            e.a0 = e.b0 = 1.f;
+            e.p0 = 1.f;
             continue;
         }
         const double lo = (double)(j - 2);  // in units of 1/512
@@ -104,15 +103,18 @@ void build_dir_cells(DirCell* cells) {
             if (cb > 0.0 && cb < 1.0) split = cb;
         }
         if (split < 0) {
-            const int seg = segment_of((lo + 0.5) / 512.0);
-            fit_piece(lo, 0.0, 1.0, seg, &e.a0);
-            e.b0 = e.a0;
-            e.b1 = e.a1;
-            e.b2 = e.a2;
+            fit_piece(lo, 0.0, 1.0, segment_of((lo + 0.5) / 512.0), &e.p0);
         } else {
-            fit_piece(lo, 0.0, split, segment_of((lo + 0.5 * split) / 512.0), &e.a0);
-            fit_piece(lo, split, 1.0, segment_of((lo + 0.5 * (1.0 + split)) / 512.0), &e.b0);
-            e.ts = (float)split;
+            // ts on the 2^-24 grid, so that step(t) = clamp(2^24 (t - ts) + 1)
+            // is exactly 0 or 1 for t on that grid
+            const double ts = nearbyint(split * 16777216.0) / 16777216.0;
+            float b[3];
+            fit_piece(lo, 0.0, ts, segment_of((lo + 0.5 * ts) / 512.0), &e.p0);
+            fit_piece(lo, ts, 1.0, segment_of((lo + 0.5 * (1.0 + ts)) / 512.0), b);
+            e.q0 = (float)((double)b[0] - (double)e.p0);
+            e.q1 = (float)((double)b[1] - (double)e.p1);
+            e.q2 = (float)((double)b[2] - (double)e.p2);
+            e.k = (float)(1.0 - ts * 16777216.0);  // integer below 2^24: exact
         }
     }
 }
@@ -167,17 +169,43 @@ __device__ __forceinline__ void point_at(const FieldArgs& a, uint32_t idx, float
     }
 }
 
+// Copies the FAST-mode cell table into LDS as two float4 arrays (the first
+// and second half of each DirCell).
+__device__ __forceinline__ void stage_cells(const DirCell* __restrict__ cells, float4* cp, float4* cq) {
+    const float4* g = reinterpret_cast<const float4*>(cells);
+    for (int j = threadIdx.x; j < kDirCells; j += kBlock) {
+        cp[j] = g[2 * j];
+        cq[j] = g[2 * j + 1];
+    }
+}
+
+// Copies cnt packed sources (3 float4 each) into an LDS tile: unchanged for
+// the ACCURATE mode, as SrcF (the FAST mode's derived operands) otherwise.
+template <int PREC>
+__device__ __forceinline__ void stage_tile(const float4* __restrict__ g, uint32_t cnt, float4* tile) {
+    if (PREC == kFast) {
+        for (uint32_t j = threadIdx.x; j < cnt; j += kBlock) {
+            const float4 q0 = g[3 * j], q1 = g[3 * j + 1], q2 = g[3 * j + 2];
+            const SrcF f = stage_fast({q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w});
+            tile[3 * j] = make_float4(f.x, f.y, f.z, f.la);
+            tile[3 * j + 1] = make_float4(f.nx, f.ny, f.nz, f.ph);
+            tile[3 * j + 2] = make_float4(f.kh, f.kl, f.nal, f.kh2);
+        }
+    } else {
+        for (uint32_t v = threadIdx.x; v < cnt * 3; v += kBlock) tile[v] = g[v];
+    }
+}
+
 template <int SRC, int PTS, int OUT, int PREC, int PPL>
 __global__ __launch_bounds__(kBlock) void field_kernel(FieldArgs a) {
+    constexpr int kCells = (SRC == kT4010A1 && PREC == kFast) ? kDirCells : 1;
     __shared__ float4 s_src[kTile * 3];
     __shared__ DirEntry s_dir[11];
-    __shared__ DirCell s_cells[(SRC == kT4010A1 && PREC == kFast) ? kDirCells : 1];
+    __shared__ float4 s_cp[kCells], s_cq[kCells];
+    const DirCell* s_cells = nullptr;  // the ACCURATE mode reads s_dir instead
     const int tid = threadIdx.x;
     if (SRC == kT4010A1 && PREC == kAccurate && tid < 11) s_dir[tid] = c_dir_tab[tid];
-    if (SRC == kT4010A1 && PREC == kFast) {
-        for (int v = tid; v < kDirCells * 2; v += kBlock)
-            reinterpret_cast<float4*>(s_cells)[v] = reinterpret_cast<const float4*>(a.cells)[v];
-    }
+    if (SRC == kT4010A1 && PREC == kFast) stage_cells(a.cells, s_cp, s_cq);
 
     const uint32_t base = blockIdx.x * (uint32_t)(kBlock * PPL) + tid;
     float px[PPL], py[PPL], pz[PPL];
@@ -194,20 +222,26 @@ __global__ __launch_bounds__(kBlock) void field_kernel(FieldArgs a) {
     for (uint32_t t0 = 0; t0 < a.n_sources; t0 += kTile) {
         const uint32_t cnt = min((uint32_t)kTile, a.n_sources - t0);
         __syncthreads();
-        for (uint32_t v = tid; v < cnt * 3; v += kBlock) s_src[v] = gsrc[(uint64_t)t0 * 3 + v];
+        stage_tile<PREC>(gsrc + (uint64_t)t0 * 3, cnt, s_src);
         __syncthreads();
         for (uint32_t j0 = 0; j0 < cnt; j0 += kChunk) {
             float tr[PPL], ti[PPL];
 #pragma unroll
             for (int k = 0; k < PPL; ++k) tr[k] = ti[k] = 0.0f;
-#pragma unroll 2
+#pragma unroll 4
             for (int j = 0; j < kChunk; ++j) {
                 const float4 q0 = s_src[(j0 + j) * 3];
                 const float4 q1 = s_src[(j0 + j) * 3 + 1];
                 const float4 q2 = s_src[(j0 + j) * 3 + 2];
-                const Src s = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
+                if (PREC == kFast) {
+                    const SrcF s = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
+                    add_terms_staged<SRC, PPL>(s, px, py, pz, s_cp, s_cq, tr, ti);
+                } else {
+                    const Src s = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
 #pragma unroll
-                for (int k = 0; k < PPL; ++k) add_term<SRC, PREC>(s, px[k], py[k], pz[k], s_dir, s_cells, tr[k], ti[k]);
+                    for (int k = 0; k < PPL; ++k)
+                        add_term<SRC, PREC>(s, px[k], py[k], pz[k], s_dir, s_cells, tr[k], ti[k]);
+                }
             }
 #pragma unroll
             for (int k = 0; k < PPL; ++k) {
@@ -241,6 +275,7 @@ __global__ __launch_bounds__(kBlock) void field_kernel(FieldArgs a) {
 // over many waves instead of leaving most CUs idle, and the result of a point
 // set depends only on the point set itself.
 constexpr int kTileS = 256;  // sources per LDS tile (4 per lane)
+constexpr int kSplitPoints = 4;  // points per wave
 
 __device__ __forceinline__ float wave_sum_f(float v) {
 #pragma unroll
@@ -250,15 +285,14 @@ __device__ __forceinline__ float wave_sum_f(float v) {
 
 template <int SRC, int PTS, int OUT, int PREC, int PPW>
 __global__ __launch_bounds__(kBlock) void field_split_kernel(FieldArgs a) {
+    constexpr int kCells = (SRC == kT4010A1 && PREC == kFast) ? kDirCells : 1;
     __shared__ float4 s_src[kTileS * 3];
     __shared__ DirEntry s_dir[11];
-    __shared__ DirCell s_cells[(SRC == kT4010A1 && PREC == kFast) ? kDirCells : 1];
+    __shared__ float4 s_cp[kCells], s_cq[kCells];
+    const DirCell* s_cells = nullptr;  // the ACCURATE mode reads s_dir instead
     const int tid = threadIdx.x, lane = tid & 63;
     if (SRC == kT4010A1 && PREC == kAccurate && tid < 11) s_dir[tid] = c_dir_tab[tid];
-    if (SRC == kT4010A1 && PREC == kFast) {
-        for (int v = tid; v < kDirCells * 2; v += kBlock)
-            reinterpret_cast<float4*>(s_cells)[v] = reinterpret_cast<const float4*>(a.cells)[v];
-    }
+    if (SRC == kT4010A1 && PREC == kFast) stage_cells(a.cells, s_cp, s_cq);
     const uint32_t wave = blockIdx.x * (kBlock / 64) + (tid >> 6);
     const uint32_t base = wave * PPW;
     float px[PPW], py[PPW], pz[PPW];
@@ -274,7 +308,7 @@ __global__ __launch_bounds__(kBlock) void field_split_kernel(FieldArgs a) {
     for (uint32_t t0 = 0; t0 < a.n_sources; t0 += kTileS) {
         const uint32_t cnt = min((uint32_t)kTileS, a.n_sources - t0);
         __syncthreads();
-        for (uint32_t v = tid; v < cnt * 3; v += kBlock) s_src[v] = gsrc[(uint64_t)t0 * 3 + v];
+        stage_tile<PREC>(gsrc + (uint64_t)t0 * 3, cnt, s_src);
         __syncthreads();
         float tr[PPW], ti[PPW];
 #pragma unroll
@@ -282,9 +316,17 @@ __global__ __launch_bounds__(kBlock) void field_split_kernel(FieldArgs a) {
         // cnt is a multiple of 32 (packed padding): lanes past it idle
         for (uint32_t j = lane; j < cnt; j += 64) {
             const float4 q0 = s_src[j * 3], q1 = s_src[j * 3 + 1], q2 = s_src[j * 3 + 2];
-            const Src src = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
+            if (PREC == kFast) {
+                const SrcF src = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
 #pragma unroll
-            for (int k = 0; k < PPW; ++k) add_term<SRC, PREC>(src, px[k], py[k], pz[k], s_dir, s_cells, tr[k], ti[k]);
+                for (int k = 0; k < PPW; k += 2)
+                    add_terms_staged<SRC, 2>(src, px + k, py + k, pz + k, s_cp, s_cq, tr + k, ti + k);
+            } else {
+                const Src src = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
+#pragma unroll
+                for (int k = 0; k < PPW; ++k)
+                    add_term<SRC, PREC>(src, px[k], py[k], pz[k], s_dir, s_cells, tr[k], ti[k]);
+            }
         }
 #pragma unroll
         for (int k = 0; k < PPW; ++k) {
@@ -314,9 +356,10 @@ __global__ __launch_bounds__(kBlock) void field_split_kernel(FieldArgs a) {
 static uint32_t split_threshold() {
     static uint32_t thr = [] {
         const char* e = getenv("AFH_FIELD_SPLIT_BELOW");
-        // below ~1.5M points the point-parallel grid leaves CUs idle (measured
-        // on MI355X with 3984 sources: 16x slower at 1e4 points, even at 2e6)
-        return e ? (uint32_t)strtoul(e, nullptr, 10) : (uint32_t)1572864;
+        // below ~2e5 points the point-parallel grid leaves CUs idle (measured
+        // on MI355X with 3984 sources: 6.5x slower at 1e4 points, even at
+        // 2.3e5, 1.05x faster at 3e5)
+        return e ? (uint32_t)strtoul(e, nullptr, 10) : (uint32_t)200000;
     }();
     return thr;
 }
@@ -329,25 +372,17 @@ static hipError_t launch_ppl(const FieldArgs& a, hipStream_t stream) {
     return hipGetLastError();
 }
 
-static int points_per_lane() {
-    static int ppl = [] {
-        const char* e = getenv("AFH_FIELD_PPL");
-        return (e && atoi(e) == 8) ? 8 : 4;
-    }();
-    return ppl;
-}
-
 template <int SRC, int PTS, int OUT, int PREC>
 static hipError_t launch4(const FieldArgs& a, hipStream_t stream) {
     if (a.n_points < split_threshold()) {
-        constexpr int PPW = 8;
+        constexpr int PPW = kSplitPoints;
         const uint32_t per_block = (kBlock / 64) * PPW;
         const uint32_t blocks = (uint32_t)(((uint64_t)a.n_points + per_block - 1) / per_block);
         hipLaunchKernelGGL((field_split_kernel<SRC, PTS, OUT, PREC, PPW>), dim3(blocks), dim3(kBlock), 0, stream, a);
         return hipGetLastError();
     }
-    return points_per_lane() == 8 ? launch_ppl<SRC, PTS, OUT, PREC, 8>(a, stream)
-                                  : launch_ppl<SRC, PTS, OUT, PREC, 4>(a, stream);
+    // one point per lane: few VGPRs, 8 waves per SIMD
+    return launch_ppl<SRC, PTS, OUT, PREC, 1>(a, stream);
 }
 
 template <int SRC, int PTS, int OUT>
