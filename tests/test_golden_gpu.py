@@ -1,0 +1,117 @@
+"""The HIP path against the golden fixtures (tests/golden/*.npz, FP64 oracle
+vectors), through the device C ABI with the fixtures' own k and alpha.
+
+Tolerances: field within 1e-5 relative L2 of the FP32 CPU path and 5e-6 of
+FP64; GS-PAT focal amplitudes within 1e-3 relative of the fixture's
+(FP32 and MIXED_SPLIT); LM: single focus at its target within 1e-5 after 200
+iterations, and the 20-iteration phases within 1e-5 rad of the fixture.
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load(name):
+    with np.load(os.path.join(GOLDEN, name), allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def log(*a):
+    print(*a, flush=True)
+
+
+def rel_l2(a, b):
+    a, b = np.asarray(a, np.complex128), np.asarray(b, np.complex128)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("name,st", [("field_t4010a1.npz", 1), ("field_sphere.npz", 0)])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_field_golden(afc, torch_cuda, name, st, prec):
+    log(f"case: golden {name}, precision {prec}")
+    import torch
+    from afc import device as D
+    g = load(name)
+    dev = torch.device("cuda:0")
+    packed = D.pack_sources(st, g["src"], g["k"], g["a"])
+    src_d = torch.from_numpy(packed).to(dev)
+    pts = torch.from_numpy(np.ascontiguousarray(g["points"], np.float32)).to(dev)
+    n = g["points"].shape[0]
+    out = torch.empty((n, 2), dtype=torch.float32, device=dev)
+    D.field_eval(src_d.data_ptr(), packed.shape[0], st, D.scatter_pointset(pts.data_ptr(), n), D.FIELD_COMPLEX,
+                 prec, out.data_ptr())
+    torch.cuda.synchronize()
+    got = out.cpu().numpy().view(np.complex64).reshape(-1)
+    e32, e64 = rel_l2(got, g["field_f32"]), rel_l2(got, g["field_f64"])
+    log(f"  vs FP32 CPU path {e32:.2e}, vs FP64 {e64:.2e}")
+    assert e32 <= 1e-5 and e64 <= 5e-6
+
+
+def focal(oracle, g, amp, phase):
+    s = g["src"].copy()
+    s[:, 6] = amp
+    s[:, 7] = phase
+    return np.abs(oracle.np_field_f64(1, s, g["k"], g["a"], g["foci"]))
+
+
+@pytest.mark.parametrize("name", ["gspat_m1.npz", "gspat_m5.npz"])
+@pytest.mark.parametrize("precision", [0, 3])
+def test_gspat_golden(afc, oracle, torch_cuda, name, precision):
+    log(f"case: golden {name}, precision {precision}")
+    import torch
+    from afc import device as D
+    g = load(name)
+    dev = torch.device("cuda:0")
+    n = g["src"].shape[0]
+    packed = D.pack_sources(1, g["src"], g["k"], g["a"])
+    src_d = torch.from_numpy(packed).to(dev)
+    foci = torch.from_numpy(np.ascontiguousarray(g["foci"], np.float32)).to(dev)
+    amps = torch.from_numpy(np.ascontiguousarray(g["amps"], np.float32)).to(dev)
+    oa = torch.empty((1, n), dtype=torch.float32, device=dev)
+    op = torch.empty((1, n), dtype=torch.float32, device=dev)
+    ctx = D.Context()
+    D.holo_solve(ctx, src_d.data_ptr(), packed.shape[0], n, 1, foci.data_ptr(), amps.data_ptr(),
+                 np.array([0, g["foci"].shape[0]], np.uint32), D.ALGO_GSPAT, precision, 100, oa.data_ptr(),
+                 op.data_ptr())
+    torch.cuda.synchronize()
+    ctx.close()
+    got = focal(oracle, g, oa.cpu().numpy()[0], op.cpu().numpy()[0])
+    want = focal(oracle, g, g["out_amp"], g["out_phase"])
+    err = float(np.max(np.abs(got - want) / want))
+    log(f"  focal amplitude max rel err vs fixture {err:.2e}")
+    assert err <= 1e-3
+
+
+def test_lm_golden(afc, oracle, torch_cuda):
+    log("case: golden lm_m1 (200 iterations) and lm_m4_20it")
+    import torch
+    from afc import device as D
+    dev = torch.device("cuda:0")
+    for name, check in (("lm_m1.npz", "target"), ("lm_m4_20it.npz", "phases")):
+        g = load(name)
+        n = g["src"].shape[0]
+        packed = D.pack_sources(1, g["src"], g["k"], g["a"])
+        src_d = torch.from_numpy(packed).to(dev)
+        foci = torch.from_numpy(np.ascontiguousarray(g["foci"], np.float32)).to(dev)
+        amps = torch.from_numpy(np.ascontiguousarray(g["amps"], np.float32)).to(dev)
+        out = torch.empty((1, n), dtype=torch.float32, device=dev)
+        iters = torch.zeros(1, dtype=torch.int32, device=dev)
+        ctx = D.Context()
+        D.lm_solve(ctx, src_d.data_ptr(), packed.shape[0], n, 1, foci.data_ptr(), amps.data_ptr(),
+                   np.array([0, g["foci"].shape[0]], np.uint32), int(g["k_max"]), out.data_ptr(), iters.data_ptr())
+        torch.cuda.synchronize()
+        ctx.close()
+        ph = out.cpu().numpy()[0].astype(np.float64)
+        if check == "target":
+            p = focal(oracle, g, g["src"][:, 6], ph)
+            log(f"  {name}: |p(f)| = {p[0]:.7f}, target {float(g['amps'][0]):.7f}")
+            assert p[0] == pytest.approx(float(g["amps"][0]), rel=1e-5)
+        else:
+            d = np.abs(np.angle(np.exp(1j * (ph - g["out_phase"]))))
+            log(f"  {name}: iterations {int(iters.cpu()[0])} vs {int(g['iterations'])}, max phase diff {d.max():.2e}")
+            assert int(iters.cpu()[0]) == int(g["iterations"])
+            assert d.max() <= 1e-5

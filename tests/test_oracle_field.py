@@ -77,6 +77,26 @@ def test_attenuation_f32_matches_f64_and_scales(oracle):
     assert oracle.attenuation_coef(80e3, 30, 1, 1, 300.0, "f64") > a64 > 0
 
 
+# ISO 9613-2 Table 2: atmospheric attenuation coefficient (dB/km) at 70 %
+# relative humidity, octave midband frequencies 63 Hz .. 8 kHz (the exact
+# frequencies 1000 * 10^(3b/10) of ISO 9613-1). core/attenuation.rs:31-48 is
+# the ISO 9613-1 (Bass et al.) formula, so it must reproduce the table.
+ISO9613_70RH = {
+    293.15: [0.1, 0.3, 1.1, 2.8, 5.0, 9.0, 22.9, 76.6],
+    283.15: [0.1, 0.4, 1.0, 1.9, 3.7, 9.7, 32.8, 117.0],
+    303.15: [0.1, 0.3, 1.0, 3.1, 7.4, 12.7, 23.1, 59.3],
+}
+
+
+@pytest.mark.parametrize("temp", sorted(ISO9613_70RH))
+def test_attenuation_matches_iso9613_table(oracle, temp):
+    freqs = [1000.0 * 10 ** (3 * b / 10) for b in range(-4, 4)]
+    for f, want in zip(freqs, ISO9613_70RH[temp]):
+        db_per_km = oracle.attenuation_coef(f, 70, 1, 1, temp, "f64") * 1e6 * 20 / math.log(10)
+        # table values are rounded to 0.1 dB/km
+        assert abs(db_per_km - want) <= 0.05 + 0.005 * want, (temp, f, db_per_km, want)
+
+
 # ---------------------------------------------------------------- one source
 def _sphere(pos, amp=1.0, phase=0.0, freq=40e3):
     return np.array([[*pos, amp, phase, freq]], np.float32)
