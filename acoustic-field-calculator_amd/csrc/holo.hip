@@ -30,6 +30,7 @@
 
 #include <algorithm>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "afh_holo.hpp"
@@ -52,6 +53,7 @@ struct HoloArgs {
     const float* amps;     // target amplitude per focus
     const uint32_t* offsets;  // batch + 1
     const uint32_t* inst;     // instance ids of this bucket
+    const float4* staged;     // SrcF of every packed source (3 float4 each), phases zeroed
     const DirCell* cells;
     float2* R;   // bucket: MP * MP per instance (row-major)
     float2* p;   // bucket: MP per instance
@@ -61,22 +63,50 @@ struct HoloArgs {
     float* out_phase;  // batch x n
 };
 
-__device__ __forceinline__ Src load_src(const float* sources, uint32_t j) {
+// Source j in the FAST-mode operand form (SrcF), from the staged copy.
+__device__ __forceinline__ SrcF load_staged(const float4* staged, uint32_t j) {
+    const float4 q0 = staged[3 * (uint64_t)j], q1 = staged[3 * (uint64_t)j + 1], q2 = staged[3 * (uint64_t)j + 2];
+    return {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
+}
+
+// Stages every packed source once per solve: the optimizers zero every phase
+// before building G (gs_pat.rs:39-41, gs.rs:37-39, naive.rs:34-36), so the
+// staged phase is 0 (1/2 turn for a negative amplitude, see stage_fast).
+__global__ __launch_bounds__(kHB) void stage_holo_sources(const float* __restrict__ sources, uint32_t np,
+                                                          float4* __restrict__ staged) {
+    const uint32_t j = blockIdx.x * kHB + threadIdx.x;
+    if (j >= np) return;
     const float4* p = reinterpret_cast<const float4*>(sources) + 3 * (uint64_t)j;
     const float4 q0 = p[0], q1 = p[1], q2 = p[2];
     Src s = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
-    s.ph = 0.0f;  // the optimizers zero every phase before building G
-    return s;
+    s.ph = 0.0f;
+    const SrcF f = stage_fast(s);
+    staged[3 * (uint64_t)j] = make_float4(f.x, f.y, f.z, f.la);
+    staged[3 * (uint64_t)j + 1] = make_float4(f.nx, f.ny, f.nz, f.ph);
+    staged[3 * (uint64_t)j + 2] = make_float4(f.kh, f.kl, f.nal, f.kh2);
+}
+
+// The FAST-mode cell table in LDS as two float4 arrays (see DirCell).
+struct CellsLds {
+    float4 p[kDirCells], q[kDirCells];
+};
+
+__device__ __forceinline__ void stage_cells_lds(const DirCell* __restrict__ cells, CellsLds* l) {
+    const float4* g = reinterpret_cast<const float4*>(cells);
+    for (int j = threadIdx.x; j < kDirCells; j += kHB) {
+        l->p[j] = g[2 * j];
+        l->q[j] = g[2 * j + 1];
+    }
 }
 
 // Per-instance foci in LDS; rows >= M are padding.
 template <int MP>
 struct Foci {
-    float x[MP], y[MP], z[MP], a[MP];
+    float x[MP], y[MP], z[MP], a[MP], v[MP];  // v: 1 for a focus, 0 for padding
 };
 
 template <int SRC, int MP>
-__device__ __forceinline__ void load_instance(const HoloArgs& a, uint32_t b, Foci<MP>& f, DirCell* s_cells,
+__device__ __forceinline__ void load_instance(const HoloArgs& a, uint32_t b, Foci<MP>& f, CellsLds* cells,
                                               uint32_t& M, uint32_t& base) {
     base = a.offsets[b];
     M = a.offsets[b + 1] - base;
@@ -86,28 +116,29 @@ __device__ __forceinline__ void load_instance(const HoloArgs& a, uint32_t b, Foc
         f.y[i] = v ? a.foci[3 * (uint64_t)(base + i) + 1] : 0.f;
         f.z[i] = v ? a.foci[3 * (uint64_t)(base + i) + 2] : 1.f;
         f.a[i] = v ? a.amps[base + i] : 0.f;
+        f.v[i] = v ? 1.f : 0.f;
     }
-    if (SRC == kT4010A1)
-        for (int v = threadIdx.x; v < kDirCells * 2; v += kHB)
-            reinterpret_cast<float4*>(s_cells)[v] = reinterpret_cast<const float4*>(a.cells)[v];
+    if (SRC == kT4010A1) stage_cells_lds(a.cells, cells);
 }
 
 // G for column j0 + (t & 31) at rows (t >> 5) + 8 r, r < MP / 8 (zero for
-// padding rows and padding columns).
+// padding rows and padding columns), two rows at a time through the staged
+// term.
 template <int SRC, int MP>
-__device__ __forceinline__ void g_tile(const HoloArgs& a, uint32_t j0, const Foci<MP>& f, uint32_t M,
-                                       const DirCell* cells, float2 (&g)[MP / 8]) {
+__device__ __forceinline__ void g_tile(const HoloArgs& a, uint32_t j0, const Foci<MP>& f, const CellsLds* cells,
+                                       float2 (&g)[MP / 8]) {
     const int t = threadIdx.x;
-    const uint32_t j = j0 + (t & 31);
     const int grp = t >> 5;
-    const Src s = load_src(a.sources, j);
+    const SrcF s = load_staged(a.staged, j0 + (t & 31));
 #pragma unroll
-    for (int r = 0; r < MP / 8; ++r) {
-        const int i = grp + 8 * r;
-        float amp, cs, sn;
-        term_value<SRC, kFast>(s, f.x[i], f.y[i], f.z[i], nullptr, cells, amp, cs, sn);
-        const bool v = i < (int)M;
-        g[r] = make_float2(v ? amp * cs : 0.f, v ? amp * sn : 0.f);
+    for (int r = 0; r < MP / 8; r += 2) {
+        const int i0 = grp + 8 * r, i1 = i0 + 8;
+        const float px[2] = {f.x[i0], f.x[i1]}, py[2] = {f.y[i0], f.y[i1]}, pz[2] = {f.z[i0], f.z[i1]};
+        const float w[2] = {f.v[i0], f.v[i1]};
+        float re[2] = {0.f, 0.f}, im[2] = {0.f, 0.f};
+        add_terms_staged<SRC, 2>(s, px, py, pz, cells->p, cells->q, re, im, w);
+        g[r] = make_float2(re[0], im[0]);
+        g[r + 1] = make_float2(re[1], im[1]);
     }
 }
 
@@ -135,7 +166,8 @@ __device__ __forceinline__ void column_scale(const float2 (&g)[MP / 8], uint32_t
 // ---------------------------------------------------------------- kernel A (FP32)
 template <int SRC, int MP>
 __global__ __launch_bounds__(kHB) void gspat_r_fp32(HoloArgs a) {
-    __shared__ DirCell s_cells[SRC == kT4010A1 ? kDirCells : 1];
+    __shared__ std::conditional_t<SRC == kT4010A1, CellsLds, float> s_cells_mem;
+    CellsLds* s_cells = reinterpret_cast<CellsLds*>(&s_cells_mem);
     __shared__ Foci<MP> f;
     __shared__ float s_part[8][kKC];
     __shared__ float s_inv[kKC];
@@ -155,7 +187,7 @@ __global__ __launch_bounds__(kHB) void gspat_r_fp32(HoloArgs a) {
 
     for (uint32_t j0 = 0; j0 < a.n_packed; j0 += kKC) {
         float2 g[MP / 8];
-        g_tile<SRC, MP>(a, j0, f, M, s_cells, g);
+        g_tile<SRC, MP>(a, j0, f, s_cells, g);
         column_scale<MP>(g, j0, a.n, false, s_part, s_inv);
         const float inv = s_inv[t & 31];
 #pragma unroll
@@ -200,7 +232,8 @@ __global__ __launch_bounds__(kHB) void gspat_r_mfma(HoloArgs a) {
     constexpr int TPW = T * T / 4;  // tiles per wave
     constexpr int HS = kKC + 8;     // row stride in halves (80 bytes)
     constexpr int NH = SPLIT ? 2 : 1;
-    __shared__ DirCell s_cells[SRC == kT4010A1 ? kDirCells : 1];
+    __shared__ std::conditional_t<SRC == kT4010A1, CellsLds, float> s_cells_mem;
+    CellsLds* s_cells = reinterpret_cast<CellsLds*>(&s_cells_mem);
     __shared__ Foci<MP> f;
     __shared__ float s_part[8][kKC];
     __shared__ float s_inv[kKC];
@@ -223,7 +256,7 @@ __global__ __launch_bounds__(kHB) void gspat_r_mfma(HoloArgs a) {
     }
     for (uint32_t j0 = 0; j0 < a.n_packed; j0 += kKC) {
         float2 g[MP / 8];
-        g_tile<SRC, MP>(a, j0, f, M, s_cells, g);
+        g_tile<SRC, MP>(a, j0, f, s_cells, g);
         column_scale<MP>(g, j0, a.n, false, s_part, s_inv);
         const float inv = s_inv[t & 31] * scale;
 #pragma unroll
@@ -347,7 +380,8 @@ __global__ __launch_bounds__(kHB) void gspat_iter(HoloArgs a, uint32_t iters, in
 // ---------------------------------------------------------------- kernel C
 template <int SRC, int MP>
 __global__ __launch_bounds__(kHB) void gspat_q(HoloArgs a, int closing) {
-    __shared__ DirCell s_cells[SRC == kT4010A1 ? kDirCells : 1];
+    __shared__ std::conditional_t<SRC == kT4010A1, CellsLds, float> s_cells_mem;
+    CellsLds* s_cells = reinterpret_cast<CellsLds*>(&s_cells_mem);
     __shared__ Foci<MP> f;
     __shared__ float s_part[8][kKC];
     __shared__ float s_inv[kKC];
@@ -392,7 +426,7 @@ __global__ __launch_bounds__(kHB) void gspat_q(HoloArgs a, int closing) {
         for (int r = 0; r < MP / 8; ++r) gam[r] = make_float2(0.f, 0.f);
         for (uint32_t j0 = 0; j0 < a.n_packed; j0 += kKC) {
             float2 g[MP / 8];
-            g_tile<SRC, MP>(a, j0, f, M, s_cells, g);
+            g_tile<SRC, MP>(a, j0, f, s_cells, g);
             column_scale<MP>(g, j0, a.n, true, s_part, s_inv);
             chunk_q(g);
             const float2 qj = s_q[jj];
@@ -425,7 +459,7 @@ __global__ __launch_bounds__(kHB) void gspat_q(HoloArgs a, int closing) {
     }
     for (uint32_t j0 = 0; j0 < a.n_packed; j0 += kKC) {
         float2 g[MP / 8];
-        g_tile<SRC, MP>(a, j0, f, M, s_cells, g);
+        g_tile<SRC, MP>(a, j0, f, s_cells, g);
         column_scale<MP>(g, j0, a.n, true, s_part, s_inv);
         chunk_q(g);
         if (t < kKC && j0 + t < a.n) {
@@ -441,22 +475,21 @@ __global__ __launch_bounds__(kHB) void gspat_q(HoloArgs a, int closing) {
 // G (FP32, M x n_packed row-major) of every instance of the bucket.
 template <int SRC>
 __global__ __launch_bounds__(kHB) void gen_g(HoloArgs a, uint32_t g_stride_inst) {
-    __shared__ DirCell s_cells[SRC == kT4010A1 ? kDirCells : 1];
-    if (SRC == kT4010A1)
-        for (int v = threadIdx.x; v < kDirCells * 2; v += kHB)
-            reinterpret_cast<float4*>(s_cells)[v] = reinterpret_cast<const float4*>(a.cells)[v];
+    __shared__ std::conditional_t<SRC == kT4010A1, CellsLds, float> s_cells_mem;
+    CellsLds* s_cells = reinterpret_cast<CellsLds*>(&s_cells_mem);
+    if (SRC == kT4010A1) stage_cells_lds(a.cells, s_cells);
     __syncthreads();
     const uint32_t k = blockIdx.y, b = a.inst[k];
     const uint32_t j = blockIdx.x * kHB + threadIdx.x;
     if (j >= a.n_packed) return;
     const uint32_t base = a.offsets[b], M = a.offsets[b + 1] - base;
-    const Src s = load_src(a.sources, j);
+    const SrcF s = load_staged(a.staged, j);
     float2* G = a.G + (uint64_t)k * g_stride_inst;
     for (uint32_t i = 0; i < M; ++i) {
-        float amp, cs, sn;
-        term_value<SRC, kFast>(s, a.foci[3 * (uint64_t)(base + i)], a.foci[3 * (uint64_t)(base + i) + 1],
-                               a.foci[3 * (uint64_t)(base + i) + 2], nullptr, s_cells, amp, cs, sn);
-        G[(uint64_t)i * a.n_packed + j] = make_float2(amp * cs, amp * sn);
+        const float* fp = a.foci + 3 * (uint64_t)(base + i);
+        float re = 0.f, im = 0.f;
+        add_terms_staged<SRC, 1>(s, fp, fp + 1, fp + 2, s_cells->p, s_cells->q, &re, &im);
+        G[(uint64_t)i * a.n_packed + j] = make_float2(re, im);
     }
 }
 
@@ -526,25 +559,24 @@ __global__ __launch_bounds__(kHB) void gs_iter(HoloArgs a, uint32_t iters, uint3
 // ---------------------------------------------------------------- naive
 template <int SRC>
 __global__ __launch_bounds__(kHB) void naive_q(HoloArgs a) {
-    __shared__ DirCell s_cells[SRC == kT4010A1 ? kDirCells : 1];
-    if (SRC == kT4010A1)
-        for (int v = threadIdx.x; v < kDirCells * 2; v += kHB)
-            reinterpret_cast<float4*>(s_cells)[v] = reinterpret_cast<const float4*>(a.cells)[v];
+    __shared__ std::conditional_t<SRC == kT4010A1, CellsLds, float> s_cells_mem;
+    CellsLds* s_cells = reinterpret_cast<CellsLds*>(&s_cells_mem);
+    if (SRC == kT4010A1) stage_cells_lds(a.cells, s_cells);
     __syncthreads();
     const uint32_t b = blockIdx.y;
     const uint32_t j = blockIdx.x * kHB + threadIdx.x;
     if (j >= a.n) return;
     const uint32_t base = a.offsets[b], M = a.offsets[b + 1] - base;
-    const Src s = load_src(a.sources, j);
+    const SrcF s = load_staged(a.staged, j);
     float2 q = make_float2(0.f, 0.f);
     for (uint32_t i = 0; i < M; ++i) {
-        float amp, cs, sn;
-        term_value<SRC, kFast>(s, a.foci[3 * (uint64_t)(base + i)], a.foci[3 * (uint64_t)(base + i) + 1],
-                               a.foci[3 * (uint64_t)(base + i) + 2], nullptr, s_cells, amp, cs, sn);
+        const float* fp = a.foci + 3 * (uint64_t)(base + i);
+        float re = 0.f, im = 0.f;
+        add_terms_staged<SRC, 1>(s, fp, fp + 1, fp + 2, s_cells->p, s_cells->q, &re, &im);
         const float p = a.amps[base + i];
         // q += conj(G_ij) p_i
-        q.x = fmaf(amp * cs, p, q.x);
-        q.y = fmaf(-amp * sn, p, q.y);
+        q.x = fmaf(re, p, q.x);
+        q.y = fmaf(-im, p, q.y);
     }
     a.out_amp[(uint64_t)b * a.n + j] = hypotf(q.x, q.y);
     a.out_phase[(uint64_t)b * a.n + j] = atan2f(q.y, q.x);
@@ -637,7 +669,7 @@ int32_t holo_solve(Context* ctx, const HoloBatch& hb, hipStream_t st) {
             if (kinds[q] == mp) lists[q].push_back(b);
     }
     const uint32_t np = hb.n_packed;
-    // workspace layout: offsets | instance lists | R | p | G | q
+    // workspace layout: offsets | instance lists | staged sources | R | p | G | q
     size_t off_bytes = (B + 1) * sizeof(uint32_t);
     size_t list_bytes = B * sizeof(uint32_t);
     size_t r_elems = 0, p_elems = 0, g_elems = 0, q_elems = 0;
@@ -652,7 +684,8 @@ int32_t holo_solve(Context* ctx, const HoloBatch& hb, hipStream_t st) {
         }
     }
     auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    const size_t o_off = 0, o_list = align(off_bytes), o_r = o_list + align(list_bytes);
+    const size_t o_off = 0, o_list = align(off_bytes), o_stg = o_list + align(list_bytes);
+    const size_t o_r = o_stg + align((size_t)np * AFH_SOURCE_STRIDE * sizeof(float));
     const size_t o_p = o_r + align(r_elems * sizeof(float2)), o_g = o_p + align(p_elems * sizeof(float2));
     const size_t o_q = o_g + align(g_elems * sizeof(float2)), total = o_q + align(q_elems * sizeof(float2));
     // growing the workspace frees the old block: work queued on this stream
@@ -683,6 +716,7 @@ int32_t holo_solve(Context* ctx, const HoloBatch& hb, hipStream_t st) {
     a.foci = hb.foci;
     a.amps = hb.amps;
     a.offsets = reinterpret_cast<const uint32_t*>(wbase + o_off);
+    a.staged = reinterpret_cast<const float4*>(wbase + o_stg);
     a.cells = dir_cells_device();
     a.out_amp = hb.out_amp;
     a.out_phase = hb.out_phase;
@@ -690,6 +724,9 @@ int32_t holo_solve(Context* ctx, const HoloBatch& hb, hipStream_t st) {
         set_error("holo: directivity table allocation failed");
         return AFH_ERR_HIP;
     }
+    hipLaunchKernelGGL(stage_holo_sources, dim3((np + kHB - 1) / kHB), dim3(kHB), 0, st, hb.sources, np,
+                       reinterpret_cast<float4*>(wbase + o_stg));
+    if (!hip_ok(hipGetLastError(), "holo staging launch")) return AFH_ERR_HIP;
     size_t list_pos = 0, r_pos = 0, p_pos = 0, g_pos = 0, q_pos = 0;
     for (int q = 0; q < 4; ++q) {
         const uint32_t nb = (uint32_t)lists[q].size();
