@@ -27,6 +27,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "acoustic-field-calculator_amd"))
 
 FLOP_PER_TERM = 38  # T4010A1 term, counted from uniform_medium.rs:111-119 and t4010a1.rs:107-120
+# HBM bytes of one field launch at 3984 sources x 256x256x128 points, measured
+# with rocprofv3 (FETCH_SIZE scaled by 2.0 and WRITE_SIZE by 1.0, the factors
+# a 256 MiB device copy reads)
+FIELD_TRAFFIC_BYTES = 68.8e6
 PEAK_FP32 = 157.3  # TFLOP/s, MI355X vector FP32
 
 
@@ -181,7 +185,10 @@ def main():
         "config": {"workload": "field 3984 T4010A1 x 256x256x128 grid, complex output", "sources": n,
                    "points": n_pts, "precision": args.precision},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32, "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK_FP32, 4), "traffic": None,
+                     "frac": round(achieved / PEAK_FP32, 4), "traffic": FIELD_TRAFFIC_BYTES,
+                     "traffic_source": "rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 per dispatch "
+                                       "(profiles/field_kernel_3984src_256x256x128.md)",
+                     "algorithmic_bytes": n_pts * 8 + packed.nbytes,
                      "flop_per_term": FLOP_PER_TERM, "kernel_ms": round(kern_ms, 4)},
         "accuracy": {"rel_l2_vs_fp64": err, "sample": "512 random points of the timed output"},
         "pcie_inclusive": {"ms": round(with_copy_ms, 4), "gsrc_pt_per_s": round(terms / with_copy_ms / 1e6, 1),
