@@ -356,16 +356,19 @@ def bench_lm(torch, D, O):
                        ph_d.data_ptr(), it_d.data_ptr(), stream.cuda_stream)
             torch.cuda.synchronize()
             ph = ph_d.cpu().numpy()
-            errs = []
+            errs, dph = [], []
             for b in range(2):
                 f = foci[offs[b]:offs[b + 1]]
                 ref, _, _ = O.lm(1, src, k, a, f, amps[offs[b]:offs[b + 1]], "f64", k_max=20)
+                dph.append(float(np.max(np.abs(np.angle(np.exp(1j * (ph[b].astype(np.float64) - ref)))))))
                 s1, s2 = src.copy(), src.copy()
                 s1[:, 7], s2[:, 7] = ph[b], ref
                 g1 = np.abs(O.field(1, s1, k, a, f, "f64"))
                 g2 = np.abs(O.field(1, s2, k, a, f, "f64"))
                 errs.append(float(np.max(np.abs(g1 - g2) / g2)))
+            # phases are compared after the FP32 rounding the output format applies
             entry["focal_amp_rel_err_vs_fp64_20it_max"] = max(errs)
+            entry["phase_max_abs_diff_vs_fp64_20it_rad"] = max(dph)
         ctx.close()
         out[name] = entry
         log(f"lm {name}: {ms:.3f} ms per batch of 64, {64 / (ms * 1e-3):.0f} solves/s")

@@ -284,3 +284,31 @@ def test_split_and_point_parallel_kernels_agree(afc, oracle, torch_cuda):
         f"{rel_l2(big, want):.2e}")
     assert rel_l2(small, big) <= 2e-6
     assert rel_l2(small, want) <= 5e-5 and rel_l2(big, want) <= 5e-5
+
+
+@pytest.mark.parametrize("n", [1, 63, 255, 256, 257, 199999, 200000, 200001])
+def test_ragged_point_counts(afc, oracle, torch_cuda, n):
+    """Point counts around the workgroup size (256 points) and the source-split
+    threshold (200 000): every point is written, none beyond, and sampled
+    points match the FP32 CPU path to 1e-5."""
+    log(f"case: ragged scatter set of {n} points")
+    import torch
+    from afc import device as D
+    src, k, a, _ = oracle.config1()
+    packed = D.pack_sources(1, src, k, a)
+    dev = torch.device("cuda:0")
+    src_d = torch.from_numpy(packed).to(dev)
+    rng = np.random.default_rng(n)
+    pts = np.stack([rng.uniform(0, 180, n), rng.uniform(0, 140, n), rng.uniform(30, 300, n)], 1).astype(np.float32)
+    pts_d = torch.from_numpy(pts).to(dev)
+    out = torch.full((n + 64, 2), 7.0, dtype=torch.float32, device=dev)  # 64 guard points after the end
+    D.field_eval(src_d.data_ptr(), packed.shape[0], 1, D.scatter_pointset(pts_d.data_ptr(), n), D.FIELD_COMPLEX,
+                 D.PREC_FAST, out.data_ptr())
+    torch.cuda.synchronize()
+    host = out.cpu().numpy()
+    assert np.all(host[n:] == 7.0)
+    got = host[:n].view(np.complex64).reshape(-1)
+    sel = np.unique(np.concatenate([[0, n - 1], rng.choice(n, min(n, 300), replace=False)]))
+    want = oracle.field(1, src, k, a, pts[sel], "f32")
+    assert np.all(np.isfinite(got))
+    assert rel_l2(got[sel], want) <= 1e-5
