@@ -181,6 +181,178 @@ __device__ inline void field_at_foci(const LmScratch& s, const double2* E, uint3
     }
 }
 
+// Gram matrix W W^T of the D x n matrix W (row-major, D <= 16 BS): the columns
+// are staged through LDS kGramChunk at a time and thread t accumulates the
+// BS x BS block at rows (t / 16) BS.., columns (t % 16) BS.., each entry summed
+// over l in increasing order. Writes the lower triangle (p >= q) into sK (row
+// stride kLmMaxD) and the diagonal into sgd.
+constexpr int kGramChunk = 32;
+
+template <int BS>
+__device__ void gram(const double* __restrict__ W, uint32_t n, uint32_t D, double* tile, double* sK, double* sgd) {
+    constexpr int R = 16 * BS;  // rows held in the tile
+    constexpr int TS = kGramChunk + 1;
+    const int t = threadIdx.x;
+    const int p0 = (t >> 4) * BS, q0 = (t & 15) * BS;
+    double acc[BS][BS];
+#pragma unroll
+    for (int x = 0; x < BS; ++x)
+#pragma unroll
+        for (int y = 0; y < BS; ++y) acc[x][y] = 0.0;
+    for (uint32_t l0 = 0; l0 < n; l0 += kGramChunk) {
+        for (int e = t; e < R * kGramChunk; e += kLB) {
+            const int r = e / kGramChunk, c = e % kGramChunk;
+            tile[r * TS + c] = ((uint32_t)r < D && l0 + c < n) ? W[(uint64_t)r * n + l0 + c] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int c = 0; c < kGramChunk; ++c) {
+            double u[BS], v[BS];
+#pragma unroll
+            for (int x = 0; x < BS; ++x) u[x] = tile[(p0 + x) * TS + c];
+#pragma unroll
+            for (int y = 0; y < BS; ++y) v[y] = tile[(q0 + y) * TS + c];
+#pragma unroll
+            for (int x = 0; x < BS; ++x)
+#pragma unroll
+                for (int y = 0; y < BS; ++y) acc[x][y] = fma(u[x], v[y], acc[x][y]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int x = 0; x < BS; ++x)
+#pragma unroll
+        for (int y = 0; y < BS; ++y) {
+            const uint32_t p = p0 + x, q = q0 + y;
+            if (p < D && q <= p) {
+                sK[p * kLmMaxD + q] = acc[x][y];
+                if (p == q) sgd[p] = acc[x][y];
+            }
+        }
+}
+
+// One wave (lanes 0..63) factors K = L L^T (D x D, lower triangle of sK with
+// row stride kLmMaxD) and solves L z = v, L^T y = z. Lane i keeps row i of
+// the matrix in registers (DP >= D entries, rows i >= D are identity rows, so
+// the padded system is block-diagonal); a column is passed to all lanes
+// through the LDS array col (a wave's LDS operations complete in order).
+// Right-looking elimination, forward substitution in increasing j, backward
+// in decreasing i, every update an fma. Writes L back to sK (the lower
+// triangle only: the upper triangle holds the Gram copy) and y to sy[0..D).
+// Returns false when a pivot is not positive (or NaN).
+template <int DP>
+__device__ __attribute__((noinline)) bool chol_solve_wave(double* sK, const double* sv, double* sy, double* col, uint32_t D) {
+    const int i = threadIdx.x;  // < 64
+    const bool live = i < DP;
+    const bool real = i < (int)D;
+    double r[DP];
+#pragma unroll
+    for (int j = 0; j < DP; ++j) r[j] = real ? (j <= i ? sK[i * kLmMaxD + j] : 0.0) : (j == i ? 1.0 : 0.0);
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < DP; ++k) {
+        if (live) col[i] = r[k];
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        const double dkk = col[k];
+        ok = ok && dkk > 0.0;
+        const double lkk = sqrt(dkk);
+        if (i > k) r[k] = r[k] / lkk;
+        if (i == k) r[k] = lkk;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        if (live) col[i] = r[k];
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+#pragma unroll
+        for (int j = k + 1; j < DP; ++j) {
+            const double ljk = col[j];
+            if (i >= j) r[j] = fma(-r[k], ljk, r[j]);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+    // L back to LDS for the backward substitution (row i' read across lanes)
+    if (real)
+#pragma unroll
+        for (int j = 0; j < DP; ++j)
+            if (j <= i) sK[i * kLmMaxD + j] = r[j];
+    // L z = v
+    double acc = real ? sv[i] : 0.0, z = 0.0;
+#pragma unroll
+    for (int j = 0; j < DP; ++j) {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        if (i == j) col[j] = acc / r[j];
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        const double zj = col[j];
+        if (i == j) z = zj;
+        if (i > j) acc = fma(-r[j], zj, acc);
+    }
+    // L^T y = z
+    acc = z;
+    double y = 0.0;
+#pragma unroll
+    for (int j = DP - 1; j >= 0; --j) {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        if (i == j) col[j] = acc / r[j];
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        const double yj = col[j];
+        if (i == j) y = yj;
+        // L[j][i] for i < j: row j of L, read by lane i
+        if (i < j && j < (int)D) acc = fma(-sK[j * kLmMaxD + i], yj, acc);
+    }
+    if (real) sy[i] = y;
+    return ok;
+}
+
+// The same factorization and solves for 32 < D <= 64 with the rows kept in
+// LDS (work, row stride kLmMaxD + 1 doubles, so that a column is spread over
+// the banks) instead of registers; identical arithmetic in the same order.
+__device__ __attribute__((noinline)) bool chol_solve_wave_lds(double* sK, const double* sv, double* sy, double* col, double* work,
+                                    uint32_t D) {
+    constexpr int DP = kLmMaxD, WS = kLmMaxD + 1;
+    const int i = threadIdx.x;  // < 64
+    const bool real = i < (int)D;
+    for (int j = 0; j < DP; ++j) work[i * WS + j] = real ? (j <= i ? sK[i * kLmMaxD + j] : 0.0) : (j == i ? 1.0 : 0.0);
+    bool ok = true;
+    for (int k = 0; k < DP; ++k) {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        const double dkk = work[k * WS + k];
+        ok = ok && dkk > 0.0;
+        const double lkk = sqrt(dkk);
+        double rk = work[i * WS + k];
+        if (i > k) rk = rk / lkk;
+        if (i == k) rk = lkk;
+        work[i * WS + k] = rk;
+        col[i] = rk;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        for (int j = k + 1; j < DP; ++j) {
+            const double ljk = col[j];
+            if (i >= j) work[i * WS + j] = fma(-rk, ljk, work[i * WS + j]);
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    if (real)
+        for (int j = 0; j <= i; ++j) sK[i * kLmMaxD + j] = work[i * WS + j];
+    double acc = real ? sv[i] : 0.0, z = 0.0;
+    for (int j = 0; j < DP; ++j) {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        if (i == j) col[j] = acc / work[i * WS + j];
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        const double zj = col[j];
+        if (i == j) z = zj;
+        if (i > j) acc = fma(-work[i * WS + j], zj, acc);
+    }
+    acc = z;
+    double y = 0.0;
+    for (int j = DP - 1; j >= 0; --j) {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        if (i == j) col[j] = acc / work[i * WS + j];
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        const double yj = col[j];
+        if (i == j) y = yj;
+        if (i < j && j < (int)D) acc = fma(-work[j * WS + i], yj, acc);
+    }
+    if (real) sy[i] = y;
+    return ok;
+}
+
 __global__ __launch_bounds__(kLB) void lm_solve(LmArgs a) {
     __shared__ double s_K[kLmMaxD * kLmMaxD];
     __shared__ double s_v[kLmMaxD];
@@ -188,6 +360,10 @@ __global__ __launch_bounds__(kLB) void lm_solve(LmArgs a) {
     __shared__ double s_y[kLmMaxD];
     __shared__ double2 s_s[kLmMaxM];
     __shared__ double s_red[4];
+    __shared__ double s_tile[kLmMaxD * (kGramChunk + 1)];
+    __shared__ double s_col[kLmMaxD];
+    __shared__ double s_work[kLmMaxD * (kLmMaxD + 1)];
+    __shared__ int s_ok;
     const int t = threadIdx.x;
     const uint32_t b = blockIdx.x;
     const uint32_t base = a.offsets[b], M = a.offsets[b + 1] - base;
@@ -238,21 +414,11 @@ __global__ __launch_bounds__(kLB) void lm_solve(LmArgs a) {
         }
         gmax = block_max(gm, s_red);
         amax = block_max(am, s_red);
-        // Gram matrix entries (p >= q), each summed over l in order
-        const uint32_t ne = D * (D + 1) / 2;
-        for (uint32_t e = t; e < ne; e += kLB) {
-            // e -> (p, q) with p >= q
-            uint32_t p = (uint32_t)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
-            while (p * (p + 1) / 2 > e) --p;
-            while ((p + 1) * (p + 2) / 2 <= e) ++p;
-            const uint32_t q = e - p * (p + 1) / 2;
-            const double* wp = s.W + (uint64_t)p * n;
-            const double* wq = s.W + (uint64_t)q * n;
-            double acc = 0.0;
-            for (uint32_t l = 0; l < n; ++l) acc += wp[l] * wq[l];
-            s_K[p * kLmMaxD + q] = acc;
-            if (p == q) s_gd[p] = acc;
-        }
+        // Gram matrix W W^T (p >= q) into s_K and its diagonal into s_gd
+        if (D <= 32)
+            gram<2>(s.W, n, D, s_tile, s_K, s_gd);
+        else
+            gram<4>(s.W, n, D, s_tile, s_K, s_gd);
         __syncthreads();
     };
 
@@ -282,37 +448,19 @@ __global__ __launch_bounds__(kLB) void lm_solve(LmArgs a) {
             if (q == p) s_K[p * kLmMaxD + p] = s_gd[p] + mu;
         }
         __syncthreads();
-        // Cholesky K = L L^T (lower), right-looking, then L z = v, L^T y = z
-        bool ok = true;
-        for (uint32_t k = 0; k < D; ++k) {
-            const double dkk = s_K[k * kLmMaxD + k];
-            ok = ok && dkk > 0.0;
-            const double lkk = sqrt(dkk);
-            __syncthreads();
-            for (uint32_t i = k + 1 + t; i < D; i += kLB) s_K[i * kLmMaxD + k] /= lkk;
-            if (t == 0) s_K[k * kLmMaxD + k] = lkk;
-            __syncthreads();
-            const uint32_t rem = D - k - 1;
-            for (uint32_t e = t; e < rem * rem; e += kLB) {
-                const uint32_t i = k + 1 + e / rem, j = k + 1 + e % rem;
-                if (j <= i) s_K[i * kLmMaxD + j] -= s_K[i * kLmMaxD + k] * s_K[j * kLmMaxD + k];
-            }
-            __syncthreads();
-        }
-        if (!ok) break;  // the reference stops when its solve fails (lm.rs:72-77)
-        if (t == 0) {
-            for (uint32_t i = 0; i < D; ++i) {
-                double acc = s_v[i];
-                for (uint32_t j = 0; j < i; ++j) acc -= s_K[i * kLmMaxD + j] * s_y[j];
-                s_y[i] = acc / s_K[i * kLmMaxD + i];
-            }
-            for (uint32_t i = D; i-- > 0;) {
-                double acc = s_y[i];
-                for (uint32_t j = i + 1; j < D; ++j) acc -= s_K[j * kLmMaxD + i] * s_y[j];
-                s_y[i] = acc / s_K[i * kLmMaxD + i];
-            }
+        // Cholesky K = L L^T and K y = v by wave 0 (the other waves wait)
+        if (t < 64) {
+            bool okw;
+            if (D <= 16)
+                okw = chol_solve_wave<16>(s_K, s_v, s_y, s_col, D);
+            else if (D <= 32)
+                okw = chol_solve_wave<32>(s_K, s_v, s_y, s_col, D);
+            else
+                okw = chol_solve_wave_lds(s_K, s_v, s_y, s_col, s_work, D);
+            if (t == 0) s_ok = okw ? 1 : 0;
         }
         __syncthreads();
+        if (!s_ok) break;  // the reference stops when its solve fails (lm.rs:72-77)
         // h = -W^T y; |h|, |x|, L = h.(mu h - g) / 2, x_new, e^{i x_new}
         double hh = 0.0, xx = 0.0, lp = 0.0;
         for (uint32_t l = t; l < n; l += kLB) {
