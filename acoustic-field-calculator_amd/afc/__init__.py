@@ -404,3 +404,10 @@ class Optimizer:
     @staticmethod
     def lm(system: UniformSystem, foci, amps):
         Optimizer._run("AFO_LM", system, foci, amps)
+
+    @staticmethod
+    def greedy(system: UniformSystem, foci, amps, phase_div: int = 16):
+        size, f, a = _pack_foci(foci, amps)
+        dll().AFO_Greedy(system.handle, f.ctypes.data_as(c_void_p), a.ctypes.data_as(POINTER(c_float)),
+                         c_ulong(size), c_ulong(phase_div), system._source_type)
+        _check("AFO_Greedy")

@@ -126,3 +126,14 @@ def lm_solve(ctx: Context, packed_dev_ptr: int, n_packed: int, n_sources: int, s
                            offs.shape[0] - 1, k_max, c_void_p(out_phase_dev_ptr), c_void_p(out_iters_dev_ptr or None),
                            c_void_p(stream_ptr))
     _err("AFH_LMSolve", rc)
+
+
+def greedy_solve(ctx: Context, packed_dev_ptr: int, n_packed: int, n_sources: int, source_type: int, foci_dev_ptr: int,
+                 amps_dev_ptr: int, offsets: np.ndarray, phase_div: int, out_phase_dev_ptr: int,
+                 stream_ptr: int = 0) -> None:
+    """Batched greedy phase search (phases only) on device arrays."""
+    offs = np.ascontiguousarray(offsets, np.uint32)
+    rc = dll().AFH_GreedySolve(ctx.handle, c_void_p(packed_dev_ptr), n_packed, n_sources, source_type,
+                               c_void_p(foci_dev_ptr), c_void_p(amps_dev_ptr), offs.ctypes.data_as(c_void_p),
+                               offs.shape[0] - 1, phase_div, c_void_p(out_phase_dev_ptr), c_void_p(stream_ptr))
+    _err("AFH_GreedySolve", rc)

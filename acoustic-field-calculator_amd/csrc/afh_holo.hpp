@@ -10,7 +10,11 @@
 
 namespace afh {
 
-enum HoloAlgo { kAlgoGSPAT = 0, kAlgoGS = 1, kAlgoNaive = 2, kAlgoLM = 3 };
+enum HoloAlgo { kAlgoGSPAT = 0, kAlgoGS = 1, kAlgoNaive = 2, kAlgoLM = 3, kAlgoGreedy = 4 };
+
+// Greedy limits: foci per instance and phase candidates.
+constexpr uint32_t kGreedyMaxM = 128;
+constexpr uint32_t kGreedyMaxDiv = 4096;
 
 // One problem in the reference's host layout: sources as repr(C) structs,
 // foci as 3 floats each.
@@ -23,6 +27,7 @@ struct HoloProblem {
     const float* foci;
     const float* amps;
     uint64_t m;
+    uint32_t phase_div = 0;  // Greedy only
 };
 
 // A batch of instances on the device (all pointers device memory except
@@ -37,6 +42,7 @@ struct HoloBatch {
     uint32_t batch;
     int algo, precision;
     uint32_t iterations;
+    uint32_t phase_div = 0;  // Greedy only
     float* out_amp;
     float* out_phase;
 };

@@ -10,6 +10,8 @@
 //   linear_synthesis/gs.rs:36-77
 //       100 x {gamma = G q; p = gamma / |gamma| p0; xi = G^H p; q = xi / |xi| q0}
 //   linear_synthesis/naive.rs:32-51   q = G^H p0
+//   combination/greedy.rs:23-83       per source in order, the phase candidate
+//       minimising sum_j |a_j - |G_ji c + cache_j|| (first minimum), cache += G_:i c
 //
 // GS-PAT formulation here: H_ij = G_ij / d_j (|H_ij| <= 1, the per-column
 // scaling), so R = H H^H. Three kernels per bucket of instances with the same
@@ -582,6 +584,93 @@ __global__ __launch_bounds__(kHB) void naive_q(HoloArgs a) {
     a.out_phase[(uint64_t)b * a.n + j] = atan2f(q.y, q.x);
 }
 
+// ---------------------------------------------------------------- greedy
+// One workgroup per instance, sources in index order (greedy.rs:60-82): the M
+// terms of source i in parallel, one lane per candidate summing
+// |a_j - |G_ji c + cache_j|| over j in increasing order, the first minimum
+// over the candidates (strict <: ties keep the lower index, as the
+// reference's scan does) by a fixed-order wave reduction, then the cache
+// update and the phase arg(c) from the host's table. cand / cand_phase: div
+// candidates e^{i 2 pi idx / div} and their arguments.
+template <int SRC>
+__global__ __launch_bounds__(kHB) void greedy_kernel(HoloArgs a, const float2* __restrict__ cand,
+                                                     const float* __restrict__ cand_phase, uint32_t div) {
+    __shared__ std::conditional_t<SRC == kT4010A1, CellsLds, float> s_cells_mem;
+    CellsLds* s_cells = reinterpret_cast<CellsLds*>(&s_cells_mem);
+    __shared__ float s_fx[kGreedyMaxM], s_fy[kGreedyMaxM], s_fz[kGreedyMaxM], s_a[kGreedyMaxM];
+    __shared__ float2 s_g[kGreedyMaxM], s_cache[kGreedyMaxM];
+    __shared__ float2 s_cand[kGreedyMaxDiv];
+    __shared__ float s_v[kGreedyMaxDiv];
+    __shared__ int s_best;
+    const int t = threadIdx.x;
+    const uint32_t b = blockIdx.x;
+    const uint32_t base = a.offsets[b], M = a.offsets[b + 1] - base;
+    if (SRC == kT4010A1) stage_cells_lds(a.cells, s_cells);
+    for (uint32_t j = t; j < M; j += kHB) {
+        s_fx[j] = a.foci[3 * (uint64_t)(base + j)];
+        s_fy[j] = a.foci[3 * (uint64_t)(base + j) + 1];
+        s_fz[j] = a.foci[3 * (uint64_t)(base + j) + 2];
+        s_a[j] = a.amps[base + j];
+        s_cache[j] = make_float2(0.f, 0.f);
+    }
+    for (uint32_t c = t; c < div; c += kHB) s_cand[c] = cand[c];
+    __syncthreads();
+    for (uint32_t i = 0; i < a.n; ++i) {
+        const SrcF s = load_staged(a.staged, i);
+        for (uint32_t j = t; j < M; j += kHB) {
+            float re = 0.f, im = 0.f;
+            add_terms_staged<SRC, 1>(s, &s_fx[j], &s_fy[j], &s_fz[j], s_cells->p, s_cells->q, &re, &im);
+            s_g[j] = make_float2(re, im);
+        }
+        __syncthreads();
+        for (uint32_t c = t; c < div; c += kHB) {
+            const float2 cc = s_cand[c];
+            float v = 0.f;
+            for (uint32_t j = 0; j < M; ++j) {
+                const float2 g = s_g[j], ca = s_cache[j];
+                const float re = g.x * cc.x - g.y * cc.y + ca.x;
+                const float im = g.x * cc.y + g.y * cc.x + ca.y;
+                v += fabsf(s_a[j] - hypotf(re, im));
+            }
+            s_v[c] = v;
+        }
+        __syncthreads();
+        if (t < 64) {
+            // first minimum: scan this lane's candidates in increasing order,
+            // then combine lanes preferring the smaller value, then the lower index
+            float bv = INFINITY;
+            int bi = (int)div;
+            for (uint32_t c = t; c < div; c += 64) {
+                const float v = s_v[c];
+                if (v < bv) {
+                    bv = v;
+                    bi = (int)c;
+                }
+            }
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) {
+                const float ov = __shfl_xor(bv, off);
+                const int oi = __shfl_xor(bi, off);
+                if (ov < bv || (ov == bv && oi < bi)) {
+                    bv = ov;
+                    bi = oi;
+                }
+            }
+            if (t == 0) s_best = bi < (int)div ? bi : 0;  // every v NaN or infinite: index 0
+        }
+        __syncthreads();
+        const int best = s_best;
+        const float2 cb = s_cand[best];
+        for (uint32_t j = t; j < M; j += kHB) {
+            const float2 g = s_g[j];
+            s_cache[j].x += g.x * cb.x - g.y * cb.y;
+            s_cache[j].y += g.x * cb.y + g.y * cb.x;
+        }
+        if (t == 0) a.out_phase[(uint64_t)b * a.n + i] = cand_phase[best];
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------- host side
 static int padded_m(uint32_t m, bool mfma) {
     if (m <= 16) return mfma ? 32 : 16;
@@ -665,6 +754,7 @@ int32_t holo_solve(Context* ctx, const HoloBatch& hb, hipStream_t st) {
             set_error("holo: more than 128 foci in one instance is not supported");
             return AFH_ERR_UNSUPPORTED;
         }
+        if (hb.algo == kAlgoGreedy) continue;  // no buckets
         for (int q = 0; q < 4; ++q)
             if (kinds[q] == mp) lists[q].push_back(b);
     }
@@ -688,11 +778,20 @@ int32_t holo_solve(Context* ctx, const HoloBatch& hb, hipStream_t st) {
     const size_t o_r = o_stg + align((size_t)np * AFH_SOURCE_STRIDE * sizeof(float));
     const size_t o_p = o_r + align(r_elems * sizeof(float2)), o_g = o_p + align(p_elems * sizeof(float2));
     const size_t o_q = o_g + align(g_elems * sizeof(float2)), total = o_q + align(q_elems * sizeof(float2));
+    // Greedy: the candidate table (div float2) and their arguments (div floats)
+    const uint32_t div = hb.algo == kAlgoGreedy ? hb.phase_div : 0;
+    if (hb.algo == kAlgoGreedy && (div == 0 || div > kGreedyMaxDiv)) {
+        set_error("greedy: phase_div must be in 1..4096");
+        return AFH_ERR_ARG;
+    }
+    const size_t cand_bytes = (size_t)div * (sizeof(float2) + sizeof(float));
+    const size_t o_cand = total;
+    const size_t total_all = total + (div ? ((cand_bytes + 255) & ~(size_t)255) : 0);
     // growing the workspace frees the old block: work queued on this stream
     // may still use it
-    if (total > ctx->work.cap && !hip_ok(hipStreamSynchronize(st), "hipStreamSynchronize")) return AFH_ERR_HIP;
-    if (!ctx->work.reserve(total)) return AFH_ERR_HIP;
-    if (!ctx->reserve_staging(off_bytes + list_bytes)) return AFH_ERR_HIP;
+    if (total_all > ctx->work.cap && !hip_ok(hipStreamSynchronize(st), "hipStreamSynchronize")) return AFH_ERR_HIP;
+    if (!ctx->work.reserve(total_all)) return AFH_ERR_HIP;
+    if (!ctx->reserve_staging(off_bytes + list_bytes + cand_bytes)) return AFH_ERR_HIP;
     uint32_t* stage = static_cast<uint32_t*>(ctx->staging.ptr);
     memcpy(stage, hb.offsets_host, off_bytes);
     {
@@ -704,6 +803,19 @@ int32_t holo_solve(Context* ctx, const HoloBatch& hb, hipStream_t st) {
     if (!hip_ok(hipMemcpyAsync(wbase + o_off, stage, off_bytes, hipMemcpyHostToDevice, st), "H2D offsets") ||
         !hip_ok(hipMemcpyAsync(wbase + o_list, stage + (B + 1), list_bytes, hipMemcpyHostToDevice, st), "H2D lists"))
         return AFH_ERR_HIP;
+    if (div) {
+        // greedy.rs:23-29: Complex::new(0, 2 PI i / div).exp(), in f32, and arg()
+        char* cs = reinterpret_cast<char*>(stage) + off_bytes + list_bytes;
+        float2* c2 = reinterpret_cast<float2*>(cs);
+        float* cph = reinterpret_cast<float*>(cs + (size_t)div * sizeof(float2));
+        for (uint32_t q = 0; q < div; ++q) {
+            const float th = 2.0f * 3.14159265358979323846f * (float)q / (float)div;
+            c2[q] = make_float2(expf(0.0f) * cosf(th), expf(0.0f) * sinf(th));
+            cph[q] = atan2f(c2[q].y, c2[q].x);
+        }
+        if (!hip_ok(hipMemcpyAsync(wbase + o_cand, cs, cand_bytes, hipMemcpyHostToDevice, st), "H2D candidates"))
+            return AFH_ERR_HIP;
+    }
     if (!ctx->upload_done && !hip_ok(hipEventCreateWithFlags(&ctx->upload_done, hipEventDisableTiming), "event"))
         return AFH_ERR_HIP;
     if (!hip_ok(hipEventRecord(ctx->upload_done, st), "hipEventRecord")) return AFH_ERR_HIP;
@@ -761,6 +873,20 @@ int32_t holo_solve(Context* ctx, const HoloBatch& hb, hipStream_t st) {
         if (!hip_ok(e, "holo kernel launch")) return AFH_ERR_HIP;
         list_pos += nb;
     }
+    if (hb.algo == kAlgoGreedy) {
+        const float2* cand = reinterpret_cast<const float2*>(wbase + o_cand);
+        const float* cph = reinterpret_cast<const float*>(wbase + o_cand + (size_t)div * sizeof(float2));
+        for (uint32_t b = 0; b < B; ++b)
+            if (hb.offsets_host[b + 1] - hb.offsets_host[b] > kGreedyMaxM) {
+                set_error("greedy: more than 128 foci in one instance is not supported");
+                return AFH_ERR_UNSUPPORTED;
+            }
+        if (hb.source_type == kT4010A1)
+            hipLaunchKernelGGL((greedy_kernel<kT4010A1>), dim3(B), dim3(kHB), 0, st, a, cand, cph, div);
+        else
+            hipLaunchKernelGGL((greedy_kernel<kSphere>), dim3(B), dim3(kHB), 0, st, a, cand, cph, div);
+        if (!hip_ok(hipGetLastError(), "greedy kernel launch")) return AFH_ERR_HIP;
+    }
     if (hb.algo == kAlgoNaive) {
         const dim3 grid((hb.n + kHB - 1) / kHB, B);
         if (hb.source_type == kT4010A1)
@@ -810,6 +936,7 @@ bool holo_solve_single(const HoloProblem& p, int algo, float* amp_out, float* ph
     hb.algo = algo;
     hb.precision = AFH_HOLO_FP32;
     hb.iterations = 100;  // REPEAT in gs.rs:17 and gs_pat.rs:17
+    hb.phase_div = p.phase_div;
     hb.out_amp = reinterpret_cast<float*>(d + o_amp);
     hb.out_phase = reinterpret_cast<float*>(d + o_ph);
     if (holo_solve(&ctx, hb, ctx.stream) != AFH_OK) return false;
@@ -878,6 +1005,41 @@ int32_t AFH_HoloSolve(void* ctx, const float* packed_sources_dev, uint64_t n_pac
     hb.out_amp = out_amp_dev;
     hb.out_phase = out_phase_dev;
     return holo_solve(c, hb, (hipStream_t)stream);
+}
+
+int32_t AFH_GreedySolve(void* ctx_handle, const float* packed, uint64_t n_packed, uint64_t n, int32_t source_type,
+                        const float* foci, const float* amps, const uint32_t* offsets, uint32_t batch,
+                        uint32_t phase_div, float* out_phase, void* stream) {
+    clear_error();
+    Context* ctx = static_cast<Context*>(ctx_handle);
+    if (!ctx || !packed || !foci || !amps || !offsets || !out_phase || n == 0 || n_packed % AFH_SOURCE_PAD != 0 ||
+        n_packed < n || n_packed > 0xFFFFFFFFull || (source_type != kSphere && source_type != kT4010A1) ||
+        offsets[0] != 0) {
+        set_error("AFH_GreedySolve: bad arguments");
+        return AFH_ERR_ARG;
+    }
+    for (uint32_t b = 0; b < batch; ++b)
+        if (offsets[b + 1] <= offsets[b]) {
+            set_error("AFH_GreedySolve: an instance has no foci");
+            return AFH_ERR_ARG;
+        }
+    HoloBatch hb;
+    hb.sources = packed;
+    hb.n = (uint32_t)n;
+    hb.n_packed = (uint32_t)n_packed;
+    hb.source_type = source_type;
+    hb.foci = foci;
+    hb.amps = amps;
+    hb.offsets_host = offsets;
+    hb.batch = batch;
+    hb.algo = kAlgoGreedy;
+    hb.precision = AFH_HOLO_FP32;
+    hb.iterations = 0;
+    hb.phase_div = phase_div;
+    hb.out_amp = nullptr;
+    hb.out_phase = out_phase;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    return holo_solve(ctx, hb, static_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"

@@ -37,7 +37,7 @@ void unsupported(const char* who) {
 // system's sources the way the reference does (gs.rs:71-76, gs_pat.rs:75-80,
 // naive.rs:45-50, lm.rs:102-104).
 void run_single(void* handle, const void* foci, const float* amps, uint64_t size, int32_t source_type, int algo,
-                const char* who) {
+                const char* who, uint32_t phase_div = 0) {
     clear_error();
     System* s = as_system(handle, source_type, who);
     if (!s) return;
@@ -57,12 +57,13 @@ void run_single(void* handle, const void* foci, const float* amps, uint64_t size
     p.foci = static_cast<const float*>(foci);
     p.amps = amps;
     p.m = size;
+    p.phase_div = phase_div;
     if (!holo_solve_single(p, algo, amp_out.data(), phase_out.data())) return;
     const size_t st = s->stride();
     const size_t ia = source_type == kT4010A1 ? 6 : 3;
     for (uint64_t j = 0; j < n; ++j) {
         float* src = s->sources.data() + st * j;
-        if (algo != kAlgoLM) src[ia] = amp_out[j];
+        if (algo != kAlgoLM && algo != kAlgoGreedy) src[ia] = amp_out[j];  // LM and Greedy set phases only
         src[ia + 1] = phase_out[j];
     }
 }
@@ -112,6 +113,15 @@ void AFO_GaussNewton(void*, const void*, const float*, uint64_t, int32_t) { unsu
 void AFO_GradientDescent(void*, const void*, const float*, uint64_t, int32_t) {
     unsupported("AFO_GradientDescent");
 }
-void AFO_Greedy(void*, const void*, const float*, uint64_t, uint64_t, int32_t) { unsupported("AFO_Greedy"); }
+// combination/greedy.rs:23-83 (capi/src/optimizer.rs:227-241)
+void AFO_Greedy(void* handle, const void* foci, const float* amps, uint64_t size, uint64_t phase_div,
+                int32_t source_type) {
+    if (phase_div == 0 || phase_div > kGreedyMaxDiv) {
+        clear_error();
+        set_error("AFO_Greedy: phase_div must be in 1..4096");
+        return;
+    }
+    run_single(handle, foci, amps, size, source_type, kAlgoGreedy, "AFO_Greedy", (uint32_t)phase_div);
+}
 
 }  // extern "C"

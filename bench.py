@@ -80,6 +80,7 @@ def main():
     ap.add_argument("--no-gspat", action="store_true")
     ap.add_argument("--gspat-batch", type=int, default=1024)
     ap.add_argument("--no-lm", action="store_true")
+    ap.add_argument("--no-greedy", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-qps", default="500,1000,2000,3000")
     ap.add_argument("--serve-seconds", type=float, default=1.5)
@@ -198,6 +199,8 @@ def main():
         result["gspat"] = bench_gspat(torch, afc, D, O, src, k, a, args.gspat_batch)
     if not args.no_lm:
         result["lm"] = bench_lm(torch, D, O)
+    if not args.no_greedy:
+        result["greedy"] = bench_greedy(torch, D, O)
     if not args.no_serve:
         result["serve"] = bench_serve(D, O, [float(q) for q in args.serve_qps.split(",")], args.serve_seconds,
                                       args.serve_gspat_cus)
@@ -308,6 +311,53 @@ def bench_serve(D, O, qps_list, seconds, gspat_cus=32):
     okg = [r for r in out["runs"] if r["gspat_p99_ms"] <= 5.0]
     out["max_qps_gspat_p99_le_5ms"] = max((r["qps_offered"] for r in okg), default=None)
     return out
+
+
+def bench_greedy(torch, D, O):
+    """Batched greedy phase search (combination/greedy.rs:23-83): 256
+    instances on the 996-transducer array, M ~ U[2,16] foci drawn as in the
+    GS-PAT workload, 16 phase candidates (the reference's default)."""
+    dev = torch.device("cuda:0")
+    pos = O.tiled_array(2, 2)
+    src = O.t4010a1_sources(pos)
+    n = pos.shape[0]
+    k = np.full(n, O.WAVENUM_40K, np.float32)
+    a = np.full(n, np.float32(O.attenuation_coef(40e3, 30, 1, 1, O.TEMP_340)), np.float32)
+    foci, amps, offs = O.gspat_batch(256, m_range=(2, 16), seed=8)
+    packed = D.pack_sources(1, src, k, a)
+    src_d = torch.from_numpy(packed).to(dev)
+    foci_d = torch.from_numpy(foci).to(dev)
+    amps_d = torch.from_numpy(amps).to(dev)
+    out = torch.empty((256, n), dtype=torch.float32, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    ctx = D.Context()
+
+    def run():
+        D.greedy_solve(ctx, src_d.data_ptr(), packed.shape[0], n, 1, foci_d.data_ptr(), amps_d.data_ptr(), offs, 16,
+                       out.data_ptr(), stream.cuda_stream)
+    run()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(3)]
+    for e0, e1 in ev:
+        e0.record(stream)
+        run()
+        e1.record(stream)
+    torch.cuda.synchronize()
+    ms = float(np.median([e0.elapsed_time(e1) for e0, e1 in ev]))
+    # objective of instance 0 against the FP64 oracle's run
+    ph = out.cpu().numpy()[0].astype(np.float64)
+    f = foci[offs[0]:offs[1]]
+    am = amps[offs[0]:offs[1]]
+    ref = O.greedy(1, src, k, a, f, am, 16, "f64")
+
+    def objective(phase):
+        s1 = src.copy()
+        s1[:, 7] = phase
+        return float(np.sum(np.abs(am - np.abs(O.field(1, s1, k, a, f, "f64")))))
+    ctx.close()
+    log(f"greedy: {ms:.3f} ms per batch of 256, {256 / (ms * 1e-3):.0f} solves/s")
+    return {"workload": "Greedy B=256, N=996, M~U[2,16], 16 phase candidates", "solves_per_s": round(256 / (ms * 1e-3), 1),
+            "batch_ms": round(ms, 3), "objective_instance0": objective(ph), "objective_instance0_fp64_oracle": objective(ref)}
 
 
 def bench_lm(torch, D, O):

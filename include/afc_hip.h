@@ -128,8 +128,12 @@ void AFO_GS(void* handle, const void* foci, const float* amps, uint64_t size, in
 void AFO_Naive(void* handle, const void* foci, const float* amps, uint64_t size, int32_t source_type);
 /* capi/src/optimizer.rs:213-224 */
 void AFO_LM(void* handle, const void* foci, const float* amps, uint64_t size, int32_t source_type);
+/* capi/src/optimizer.rs:227-241, combination/greedy.rs:23-83: sets the
+ * phases (amplitudes unchanged); phase_div in 1..4096, at most 128 foci */
+void AFO_Greedy(void* handle, const void* foci, const float* amps, uint64_t size, uint64_t phase_div,
+                int32_t source_type);
 /* Exported for binding compatibility only; each records "unsupported".
- * capi/src/optimizer.rs:49, :61, :130, :144, :158, :177, :195, :227 */
+ * capi/src/optimizer.rs:49, :61, :130, :144, :158, :177, :195 */
 void AFO_BesselBeam(void* handle, AFC_Vector3 point, AFC_Vector3 dir, float theta, int32_t source_type);
 void AFO_IFFT(void* handle, const char* path, AFC_Vector3 bottom_left, AFC_Vector3 top_left,
               AFC_Vector3 bottom_right, float spacing, float z, int32_t source_type);
@@ -138,8 +142,6 @@ void AFO_Long(void* handle, const void* foci, const float* amps, uint64_t size, 
 void AFO_APO(void* handle, const void* foci, const float* amps, uint64_t size, float lambda, int32_t source_type);
 void AFO_GaussNewton(void* handle, const void* foci, const float* amps, uint64_t size, int32_t source_type);
 void AFO_GradientDescent(void* handle, const void* foci, const float* amps, uint64_t size, int32_t source_type);
-void AFO_Greedy(void* handle, const void* foci, const float* amps, uint64_t size, uint64_t phase_div,
-                int32_t source_type);
 
 /* ============================================== part 2: no reference counterpart */
 
@@ -226,6 +228,16 @@ int32_t AFH_HoloSolve(void* ctx, const float* packed_sources_dev, uint64_t n_pac
                       int32_t source_type, const float* foci_dev, const float* amps_dev, const uint32_t* offsets_host,
                       uint32_t batch, int32_t algo, int32_t precision, uint32_t iterations, float* out_amp_dev,
                       float* out_phase_dev, void* stream);
+
+/* Greedy phase search (combination/greedy.rs:23-83) for `batch` instances:
+ * candidates e^{i 2 pi idx / phase_div}, sources in index order; writes the
+ * phase of every source of instance b to out_phase_dev[b * n_sources ...].
+ * phase_div in 1..4096, at most 128 foci per instance. Same argument
+ * conventions as AFH_HoloSolve. */
+int32_t AFH_GreedySolve(void* ctx, const float* packed_sources_dev, uint64_t n_packed, uint64_t n_sources,
+                        int32_t source_type, const float* foci_dev, const float* amps_dev,
+                        const uint32_t* offsets_host, uint32_t batch, uint32_t phase_div, float* out_phase_dev,
+                        void* stream);
 
 /* Levenberg-Marquardt (nls/lm.rs:40-105) for `batch` instances, FP64: writes
  * the source phases x[0..N) of instance b to out_phase_dev[b * n_sources ...]

@@ -20,6 +20,7 @@
  *   orc_propagation_matrix   acoustic-field-optimizer/src/multiple_foci/macros.rs:17-53
  *   orc_gs / orc_gspat       multiple_foci/linear_synthesis/gs.rs:36-77, gs_pat.rs:38-81
  *   orc_naive                multiple_foci/linear_synthesis/naive.rs:32-51
+ *   orc_greedy               multiple_foci/combination/greedy.rs:23-83
  *   orc_lm                   multiple_foci/nls/lm.rs:40-105, nls/macros.rs:17-58
  *   orc_focal_point_phases   acoustic-field-optimizer/src/focal_point.rs:28-39
  *   orc_grid_points          observe_area/grid/builder.rs:144-209, grid_area.rs:268-438
@@ -86,6 +87,7 @@ static inline size_t orc_stride(int source_type) { return source_type == 1 ? 9 :
 #define RATAN2 atan2f
 #define RFMOD fmodf
 #define RPOW powf
+#define RABS fabsf
 #include "afc_oracle_body.h"
 #undef R
 #undef SFX
@@ -100,6 +102,7 @@ static inline size_t orc_stride(int source_type) { return source_type == 1 ? 9 :
 #undef RATAN2
 #undef RFMOD
 #undef RPOW
+#undef RABS
 
 /* ---- FP64 instantiation ---- */
 #define R double
@@ -115,6 +118,7 @@ static inline size_t orc_stride(int source_type) { return source_type == 1 ? 9 :
 #define RATAN2 atan2
 #define RFMOD fmod
 #define RPOW pow
+#define RABS fabs
 #include "afc_oracle_body.h"
 
 /* ---------------------------------------------------------------------------

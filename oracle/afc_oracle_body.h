@@ -338,6 +338,45 @@ void SFX(orc_naive)(int source_type, const float* sources, const float* wavenums
     free(G);
 }
 
+/* Greedy: combination/greedy.rs:23-83. Candidates e^{i 2 pi idx / div}
+ * (Complex::exp of (0, 2 pi idx / div), evaluated left to right); for each
+ * source in index order the first candidate minimising
+ * sum_j |amps_j - |G_ji c + cache_j|| is kept (strict <, so ties keep the
+ * earliest), cache += G_:i c, and the source's phase becomes arg(c). The
+ * amplitudes are not changed. */
+void SFX(orc_greedy)(int source_type, const float* sources, const float* wavenums, const float* attens, uint64_t n,
+                     const float* foci, const float* amps, uint64_t m, uint64_t phase_div, R* out_phase) {
+    SFX(cplx)* G = SFX(make_G)(source_type, sources, wavenums, attens, n, foci, m);
+    SFX(cplx)* cand = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * phase_div);
+    SFX(cplx)* cache = (SFX(cplx)*)calloc(m, sizeof(SFX(cplx)));
+    const R pi = SFX(ORC_PI);
+    uint64_t i, j, idx;
+    for (idx = 0; idx < phase_div; ++idx) {
+        const R th = (R)2.0 * pi * (R)idx / (R)phase_div;
+        cand[idx] = SFX(c_make)(REXP((R)0) * RCOS(th), REXP((R)0) * RSIN(th));
+    }
+    for (i = 0; i < n; ++i) {
+        uint64_t min_idx = 0;
+        R min_v = (R)INFINITY;
+        for (idx = 0; idx < phase_div; ++idx) {
+            R v = 0;
+            for (j = 0; j < m; ++j) {
+                const SFX(cplx) tmp = SFX(c_mul)(G[j + i * m], cand[idx]);
+                v += RABS((R)amps[j] - SFX(c_abs)(SFX(c_add)(tmp, cache[j])));
+            }
+            if (v < min_v) {
+                min_v = v;
+                min_idx = idx;
+            }
+        }
+        for (j = 0; j < m; ++j) cache[j] = SFX(c_add)(cache[j], SFX(c_mul)(G[j + i * m], cand[min_idx]));
+        out_phase[i] = SFX(c_arg)(cand[min_idx]);
+    }
+    free(G);
+    free(cand);
+    free(cache);
+}
+
 /* ---------------------------------------------------------------------------
  * Levenberg-Marquardt: acoustic-field-optimizer/src/multiple_foci/nls/lm.rs:40-105
  * with make_BhB / make_T / calc_JtJ_Jtf / calc_Fx from nls/macros.rs:17-58.

@@ -62,6 +62,9 @@ def lib():
             f = getattr(L, f"orc_naive_{sfx}")
             f.restype = None
             f.argtypes = [c_int, fp, fp, fp, c_uint64, fp, fp, c_uint64, fp, fp]
+            f = getattr(L, f"orc_greedy_{sfx}")
+            f.restype = None
+            f.argtypes = [c_int, fp, fp, fp, c_uint64, fp, fp, c_uint64, c_uint64, fp]
             f = getattr(L, f"orc_lm_{sfx}")
             f.restype = c_uint32
             f.argtypes = [c_int, fp, fp, fp, c_uint64, fp, fp, c_uint64, c_uint32, fp, fp]
@@ -175,6 +178,17 @@ def gspat(source_type, sources, wavenums, attens, foci, amps, prec="f32", repeat
 def naive(source_type, sources, wavenums, attens, foci, amps, prec="f32"):
     """Naive linear synthesis (naive.rs:32-51). Returns (amp, phase) per source."""
     return _solver("orc_naive", source_type, sources, wavenums, attens, foci, amps, prec)
+
+
+def greedy(source_type, sources, wavenums, attens, foci, amps, phase_div, prec="f32"):
+    """Greedy phase search (combination/greedy.rs:23-83). Returns the phase per source."""
+    dt, sfx = _dt(prec)
+    src, k, a, f, am = _f32(sources), _f32(wavenums), _f32(attens), _f32(foci).reshape(-1, 3), _f32(amps)
+    n, m = k.shape[0], f.shape[0]
+    ph = np.empty(n, dt)
+    getattr(lib(), f"orc_greedy_{sfx}")(source_type, _p(src), _p(k), _p(a), n, _p(f), _p(am), m, int(phase_div),
+                                        _p(ph))
+    return ph
 
 
 def lm(source_type, sources, wavenums, attens, foci, amps, prec="f64", k_max=200):

@@ -113,3 +113,41 @@ def test_lm_decreases_objective(oracle):
     _, x, _ = oracle.lm(1, src, k, a, foci, amps, "f64", k_max=50)
     F, _ = oracle.lm_eval(1, src, k, a, foci, amps, x)
     assert F < 0.2 * F0
+
+
+def test_greedy_single_unreachable_focus_aligns_all_sources(oracle):
+    """M = 1 with a target no phase choice reaches: each step maximises |cache|,
+    so every term lands within pi / div of the running sum's direction and the
+    focal amplitude is sum_j |G_j| cos(pi / div) at least (combination/greedy.rs)."""
+    src, k, a, _, _ = small_problem(oracle, m=1)
+    foci = np.array([[90, 70, 150]], np.float32)
+    G = oracle.propagation_matrix(1, src, k, a, foci, "f64")
+    total = np.abs(G).sum()
+    for div in (16, 4096):
+        ph = oracle.greedy(1, src, k, a, foci, np.array([1e3], np.float32), div, "f64")
+        p = np.abs(focal(oracle, with_solution(src, src[:, 6], ph), k, a, foci))[0]
+        assert total * np.cos(np.pi / div) - 1e-9 <= p <= total * (1 + 1e-12)
+
+
+def test_greedy_phases_are_candidates(oracle):
+    """Every phase is arg(e^{i 2 pi idx / div}) for some idx, and f32 agrees
+    with f64 on which candidate each source takes, relative to the first
+    source's choice (a tie on the empty cache, so a global rotation)."""
+    src, k, a, foci, amps = small_problem(oracle, m=3, seed=5)
+    div = 8
+    ph32 = oracle.greedy(1, src, k, a, foci, amps, div, "f32")
+    ph64 = oracle.greedy(1, src, k, a, foci, amps, div, "f64")
+    idx32 = np.round(np.mod(ph32, 2 * np.pi) / (2 * np.pi / div)).astype(int) % div
+    idx64 = np.round(np.mod(ph64, 2 * np.pi) / (2 * np.pi / div)).astype(int) % div
+    assert np.allclose(np.angle(np.exp(1j * (ph64 - 2 * np.pi * idx64 / div))), 0, atol=1e-12)
+    assert np.mean((idx32 - idx32[0]) % div == (idx64 - idx64[0]) % div) > 0.95
+
+
+def test_greedy_lowers_objective(oracle):
+    """The greedy objective sum_j |a_j - |p_j|| at the end is below that of
+    the all-zero phases."""
+    src, k, a, foci, amps = small_problem(oracle, m=4, seed=6)
+    ph = oracle.greedy(1, src, k, a, foci, amps, 16, "f64")
+    p1 = np.abs(focal(oracle, with_solution(src, src[:, 6], ph), k, a, foci))
+    p0 = np.abs(focal(oracle, with_solution(src, src[:, 6], np.zeros(len(src))), k, a, foci))
+    assert np.sum(np.abs(amps - p1)) < 0.5 * np.sum(np.abs(amps - p0))
