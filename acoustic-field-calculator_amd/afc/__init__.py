@@ -406,6 +406,14 @@ class Optimizer:
         Optimizer._run("AFO_LM", system, foci, amps)
 
     @staticmethod
+    def gauss_newton(system: UniformSystem, foci, amps):
+        Optimizer._run("AFO_GaussNewton", system, foci, amps)
+
+    @staticmethod
+    def gradient_descent(system: UniformSystem, foci, amps):
+        Optimizer._run("AFO_GradientDescent", system, foci, amps)
+
+    @staticmethod
     def greedy(system: UniformSystem, foci, amps, phase_div: int = 16):
         size, f, a = _pack_foci(foci, amps)
         dll().AFO_Greedy(system.handle, f.ctypes.data_as(c_void_p), a.ctypes.data_as(POINTER(c_float)),

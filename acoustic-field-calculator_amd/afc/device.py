@@ -116,6 +116,21 @@ def holo_solve(ctx: Context, packed_dev_ptr: int, n_packed: int, n_sources: int,
     _err("AFH_HoloSolve", rc)
 
 
+NLS_LM, NLS_GAUSS_NEWTON, NLS_GRADIENT_DESCENT = 0, 1, 2
+
+
+def nl_solve(ctx: Context, packed_dev_ptr: int, n_packed: int, n_sources: int, source_type: int, foci_dev_ptr: int,
+             amps_dev_ptr: int, offsets: np.ndarray, algo: int, k_max: int, out_phase_dev_ptr: int,
+             out_iters_dev_ptr: int = 0, stream_ptr: int = 0) -> None:
+    """Batched FP64 LM / Gauss-Newton / gradient descent (phases only) on device arrays."""
+    offs = np.ascontiguousarray(offsets, np.uint32)
+    rc = dll().AFH_NLSolve(ctx.handle, c_void_p(packed_dev_ptr), n_packed, n_sources, source_type,
+                           c_void_p(foci_dev_ptr), c_void_p(amps_dev_ptr), offs.ctypes.data_as(c_void_p),
+                           offs.shape[0] - 1, algo, k_max, c_void_p(out_phase_dev_ptr),
+                           c_void_p(out_iters_dev_ptr or None), c_void_p(stream_ptr))
+    _err("AFH_NLSolve", rc)
+
+
 def lm_solve(ctx: Context, packed_dev_ptr: int, n_packed: int, n_sources: int, source_type: int, foci_dev_ptr: int,
              amps_dev_ptr: int, offsets: np.ndarray, k_max: int, out_phase_dev_ptr: int, out_iters_dev_ptr: int = 0,
              stream_ptr: int = 0) -> None:

@@ -10,7 +10,10 @@
 
 namespace afh {
 
-enum HoloAlgo { kAlgoGSPAT = 0, kAlgoGS = 1, kAlgoNaive = 2, kAlgoLM = 3, kAlgoGreedy = 4 };
+enum HoloAlgo { kAlgoGSPAT = 0, kAlgoGS = 1, kAlgoNaive = 2, kAlgoLM = 3, kAlgoGreedy = 4, kAlgoGN = 5, kAlgoGD = 6 };
+
+// Nonlinear least-squares solvers of lm.hip (= AFH_NLS_* of the C ABI).
+enum NlsAlgo { kNlsLM = 0, kNlsGN = 1, kNlsGD = 2 };
 
 // Greedy limits: foci per instance and phase candidates.
 constexpr uint32_t kGreedyMaxM = 128;
@@ -77,8 +80,9 @@ Context& default_context();
 bool holo_solve_single(const HoloProblem& p, int algo, float* amp_out, float* phase_out);
 int32_t lm_solve_batch(Context* ctx, const float* sources, uint32_t n, uint32_t n_packed, int source_type,
                        const float* foci, const float* amps, const uint32_t* offsets_host, uint32_t batch,
-                       uint32_t k_max, float* out_phase, uint32_t* out_iters, hipStream_t st);
-// LM (lm.rs:40-105): phases of the n sources.
-bool lm_solve_single(const HoloProblem& p, float* phase_out);
+                       uint32_t k_max, float* out_phase, uint32_t* out_iters, hipStream_t st, int algo = kNlsLM);
+// LM (lm.rs:40-105), Gauss-Newton (gauss_newton.rs:35-89) or gradient descent
+// (gradient_descent.rs:34-61) with the reference's K_MAX: phases of the n sources.
+bool lm_solve_single(const HoloProblem& p, float* phase_out, int algo = kNlsLM);
 
 }  // namespace afh

@@ -905,7 +905,9 @@ Context& default_context() {
 }
 
 bool holo_solve_single(const HoloProblem& p, int algo, float* amp_out, float* phase_out) {
-    if (algo == kAlgoLM) return lm_solve_single(p, phase_out);
+    if (algo == kAlgoLM) return lm_solve_single(p, phase_out, kNlsLM);
+    if (algo == kAlgoGN) return lm_solve_single(p, phase_out, kNlsGN);
+    if (algo == kAlgoGD) return lm_solve_single(p, phase_out, kNlsGD);
     if (!device().ensure()) return false;
     Context& ctx = default_context();
     std::lock_guard<std::mutex> g(ctx.mu);

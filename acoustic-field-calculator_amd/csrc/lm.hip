@@ -19,6 +19,14 @@
 // Everything is FP64, G included: the reference's acceptance rule makes the
 // iterate path sensitive to perturbations of a few FP32 ulp.
 //
+// The same kernel runs the other two optimizers of nls/ on this machinery:
+//   Gauss-Newton (gauss_newton.rs:35-89): h = -pinv(A, 1e-3) g; with A = W^T W
+//     the pseudo-inverse keeps the eigenvalues of K = W W^T above 1e-3 (A and K
+//     share their nonzero eigenvalues), so h = -W^T U diag(1/lambda) U^T v over
+//     those, U and lambda from a Jacobi eigen-decomposition of K;
+//   gradient descent (gradient_descent.rs:34-61): x -= 0.1 g until max g <= 1e-6,
+//     g_l = sum_i Im(conj(C_il) s_i) without forming W.
+//
 // One workgroup per instance; K, v, y in LDS; G, W, x, g, h, e^{ix} in a
 // per-instance global scratch. Every reduction has a fixed order.
 #include <math.h>
@@ -46,6 +54,7 @@ struct LmArgs {
     const float* amps;
     const uint32_t* offsets;
     uint32_t k_max;
+    int algo;  // kNlsLM, kNlsGN, kNlsGD
     char* scratch;  // per instance, scratch_stride bytes
     uint64_t scratch_stride;
     float* out_phase;     // batch x N
@@ -353,6 +362,80 @@ __device__ __attribute__((noinline)) bool chol_solve_wave_lds(double* sK, const 
     return ok;
 }
 
+// y = pinv(K, eps) v for the Gram matrix K (lower triangle of sK with row
+// stride kLmMaxD, diagonal in sgd), D <= 64, by one wave: cyclic Jacobi
+// rotations on a copy of K (A, row stride kLmMaxD + 1) with the eigenvectors
+// accumulated in V, sweeps until the off-diagonal mass is below 1e-30 of the
+// diagonal's; eigenvalues |lambda| <= eps are dropped (as the reference's
+// SVD pseudo-inverse drops singular values <= eps). Lane k owns row / column k.
+__device__ __attribute__((noinline)) void jacobi_pinv_wave(const double* sK, const double* sgd, const double* sv,
+                                                           double* sy, double* A, double* V, double* col,
+                                                           uint32_t D, double eps) {
+    constexpr int WS = kLmMaxD + 1;
+    const int k = threadIdx.x;  // < 64
+    const bool live = k < (int)D;
+    if (live)
+        for (uint32_t j = 0; j < D; ++j) {
+            A[k * WS + j] = (j == (uint32_t)k) ? sgd[k] : ((j < (uint32_t)k) ? sK[k * kLmMaxD + j] : sK[j * kLmMaxD + k]);
+            V[k * WS + j] = (j == (uint32_t)k) ? 1.0 : 0.0;
+        }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0, dg = 0.0;
+        if (live)
+            for (uint32_t j = 0; j < D; ++j) {
+                const double v = A[k * WS + j];
+                if (j == (uint32_t)k)
+                    dg += v * v;
+                else
+                    off += v * v;
+            }
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+            off += __shfl_xor(off, o);
+            dg += __shfl_xor(dg, o);
+        }
+        if (off <= dg * 1e-30 || off == 0.0) break;
+        for (uint32_t p = 0; p + 1 < D; ++p)
+            for (uint32_t q = p + 1; q < D; ++q) {
+                const double apq = A[p * WS + q];
+                if (apq == 0.0) continue;
+                const double theta = (A[q * WS + q] - A[p * WS + p]) / (2.0 * apq);
+                const double tt = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                const double c = 1.0 / sqrt(tt * tt + 1.0), sn = tt * c;
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                if (live) {
+                    const double akp = A[k * WS + p], akq = A[k * WS + q];
+                    A[k * WS + p] = c * akp - sn * akq;
+                    A[k * WS + q] = sn * akp + c * akq;
+                }
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                if (live) {
+                    const double apk = A[p * WS + k], aqk = A[q * WS + k];
+                    A[p * WS + k] = c * apk - sn * aqk;
+                    A[q * WS + k] = sn * apk + c * aqk;
+                    const double vkp = V[k * WS + p], vkq = V[k * WS + q];
+                    V[k * WS + p] = c * vkp - sn * vkq;
+                    V[k * WS + q] = sn * vkp + c * vkq;
+                }
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            }
+    }
+    // w_k = (V_:k . v) / lambda_k over the kept eigenvalues, y = V w
+    if (live) {
+        double d = 0.0;
+        for (uint32_t j = 0; j < D; ++j) d += V[j * WS + k] * sv[j];
+        const double lam = A[k * WS + k];
+        col[k] = fabs(lam) > eps ? d / lam : 0.0;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    if (live) {
+        double y = 0.0;
+        for (uint32_t j = 0; j < D; ++j) y += V[k * WS + j] * col[j];
+        sy[k] = y;
+    }
+}
+
 __global__ __launch_bounds__(kLB) void lm_solve(LmArgs a) {
     __shared__ double s_K[kLmMaxD * kLmMaxD];
     __shared__ double s_v[kLmMaxD];
@@ -363,6 +446,7 @@ __global__ __launch_bounds__(kLB) void lm_solve(LmArgs a) {
     __shared__ double s_tile[kLmMaxD * (kGramChunk + 1)];
     __shared__ double s_col[kLmMaxD];
     __shared__ double s_work[kLmMaxD * (kLmMaxD + 1)];
+    __shared__ double s_vec[kLmMaxD * (kLmMaxD + 1)];
     __shared__ int s_ok;
     const int t = threadIdx.x;
     const uint32_t b = blockIdx.x;
@@ -427,8 +511,84 @@ __global__ __launch_bounds__(kLB) void lm_solve(LmArgs a) {
         s.E[l] = make_double2(1.0, 0.0);
     }
     __syncthreads();
+    if (a.algo == kNlsGD) {
+        // gradient_descent.rs:49-56
+        uint32_t it = 0;
+        for (; it < a.k_max; ++it) {
+            field_at_foci(s, s.E, N, M, amps, s_s);
+            __syncthreads();
+            double gm = -INFINITY;
+            for (uint32_t l = t; l < n; l += kLB) {
+                const double2 e = s.E[l];
+                double gl = 0.0;
+                for (uint32_t i = 0; i < M; ++i) {
+                    double2 bv;
+                    if (l < N)
+                        bv = s.G[(uint64_t)i * N + l];
+                    else
+                        bv = make_double2(l - N == i ? -(double)amps[i] : 0.0, 0.0);
+                    const double cr = bv.x * e.x - bv.y * e.y, ci = bv.x * e.y + bv.y * e.x;
+                    gl += cr * s_s[i].y - ci * s_s[i].x;  // Im(conj(C_il) s_i)
+                }
+                s.g[l] = gl;
+                gm = fmax(gm, gl);
+            }
+            const double gmax = block_max(gm, s_red);
+            if (gmax <= 1e-6) break;
+            for (uint32_t l = t; l < n; l += kLB) {
+                const double xl = s.x[l] - 0.1 * s.g[l];
+                s.x[l] = xl;
+                double sn, cs;
+                sincos(xl, &sn, &cs);
+                s.E[l] = make_double2(cs, sn);
+            }
+            __syncthreads();
+        }
+        for (uint32_t j = t; j < N; j += kLB) a.out_phase[(uint64_t)b * N + j] = (float)s.x[j];
+        if (t == 0 && a.out_iters) a.out_iters[b] = it;
+        return;
+    }
     double F, gmax, amax;
     linearize(F, gmax, amax);
+    if (a.algo == kNlsGN) {
+        // gauss_newton.rs:49-84
+        bool found = gmax <= EPS_1;
+        uint32_t it = 0;
+        for (; it < a.k_max; ++it) {
+            if (found) break;
+            if (t < 64) jacobi_pinv_wave(s_K, s_gd, s_v, s_y, s_work, s_vec, s_col, D, 1e-3);
+            __syncthreads();
+            double hh = 0.0, xx = 0.0;
+            for (uint32_t l = t; l < n; l += kLB) {
+                double acc = 0.0;
+                for (uint32_t q = 0; q < D; ++q) acc += s.W[(uint64_t)q * n + l] * s_y[q];
+                const double hl = -acc;
+                const double xl = s.x[l];
+                hh += hl * hl;
+                xx += xl * xl;
+                s.xn[l] = xl + hl;
+            }
+            const double hnorm = sqrt(block_sum(hh, s_red));
+            const double xnorm = sqrt(block_sum(xx, s_red));
+            if (hnorm <= EPS_2 * (xnorm + EPS_2)) {
+                found = true;
+                continue;
+            }
+            for (uint32_t l = t; l < n; l += kLB) {
+                const double xl = s.xn[l];
+                s.x[l] = xl;
+                double sn, cs;
+                sincos(xl, &sn, &cs);
+                s.E[l] = make_double2(cs, sn);
+            }
+            __syncthreads();
+            linearize(F, gmax, amax);
+            found = gmax <= EPS_1;
+        }
+        for (uint32_t j = t; j < N; j += kLB) a.out_phase[(uint64_t)b * N + j] = (float)s.x[j];
+        if (t == 0 && a.out_iters) a.out_iters[b] = it;
+        return;
+    }
     // the Gram matrix must survive rejected steps: keep a copy of its lower
     // triangle in the upper triangle of s_K (mirrored), the factor goes low
     for (uint32_t e = t; e < D * D; e += kLB) {
@@ -520,7 +680,7 @@ __global__ __launch_bounds__(kLB) void lm_solve(LmArgs a) {
 
 int32_t lm_solve_batch(Context* ctx, const float* sources, uint32_t n, uint32_t n_packed, int source_type,
                        const float* foci, const float* amps, const uint32_t* offsets_host, uint32_t batch,
-                       uint32_t k_max, float* out_phase, uint32_t* out_iters, hipStream_t st) {
+                       uint32_t k_max, float* out_phase, uint32_t* out_iters, hipStream_t st, int algo) {
     (void)n_packed;
     if (batch == 0) return AFH_OK;
     uint32_t maxm = 0;
@@ -559,6 +719,7 @@ int32_t lm_solve_batch(Context* ctx, const float* sources, uint32_t n, uint32_t 
     a.amps = amps;
     a.offsets = reinterpret_cast<const uint32_t*>(w);
     a.k_max = k_max;
+    a.algo = algo;
     a.scratch = w + o_scr;
     a.scratch_stride = stride;
     a.out_phase = out_phase;
@@ -569,7 +730,7 @@ int32_t lm_solve_batch(Context* ctx, const float* sources, uint32_t n, uint32_t 
     return AFH_OK;
 }
 
-bool lm_solve_single(const HoloProblem& p, float* phase_out) {
+bool lm_solve_single(const HoloProblem& p, float* phase_out, int algo) {
     if (!device().ensure()) return false;
     Context& ctx = default_context();
     std::lock_guard<std::mutex> g(ctx.mu);
@@ -588,10 +749,11 @@ bool lm_solve_single(const HoloProblem& p, float* phase_out) {
     memcpy(h + o_amps, p.amps, amps_b);
     if (!hip_ok(hipMemcpyAsync(d, h, o_ph, hipMemcpyHostToDevice, ctx.stream), "H2D")) return false;
     const uint32_t offs[2] = {0, (uint32_t)m};
-    // K_MAX = 200 (lm.rs:20)
+    // K_MAX = 200 (lm.rs:20, gauss_newton.rs:19), 10 000 (gradient_descent.rs:18)
+    const uint32_t k_max = algo == kNlsGD ? 10000 : 200;
     if (lm_solve_batch(&ctx, reinterpret_cast<const float*>(d), (uint32_t)n, (uint32_t)np, p.source_type,
                        reinterpret_cast<const float*>(d + o_foci), reinterpret_cast<const float*>(d + o_amps), offs,
-                       1, 200, reinterpret_cast<float*>(d + o_ph), nullptr, ctx.stream) != AFH_OK)
+                       1, k_max, reinterpret_cast<float*>(d + o_ph), nullptr, ctx.stream, algo) != AFH_OK)
         return false;
     if (!hip_ok(hipMemcpyAsync(h + o_ph, d + o_ph, out_b, hipMemcpyDeviceToHost, ctx.stream), "D2H")) return false;
     if (!hip_ok(hipStreamSynchronize(ctx.stream), "hipStreamSynchronize")) return false;
@@ -603,20 +765,37 @@ bool lm_solve_single(const HoloProblem& p, float* phase_out) {
 
 using namespace afh;
 
-extern "C" int32_t AFH_LMSolve(void* ctx, const float* packed_sources_dev, uint64_t n_packed, uint64_t n_sources,
-                               int32_t source_type, const float* foci_dev, const float* amps_dev,
-                               const uint32_t* offsets_host, uint32_t batch, uint32_t k_max, float* out_phase_dev,
-                               uint32_t* out_iters_dev, void* stream) {
+static int32_t nls_entry(const char* who, void* ctx, const float* packed_sources_dev, uint64_t n_packed,
+                         uint64_t n_sources, int32_t source_type, const float* foci_dev, const float* amps_dev,
+                         const uint32_t* offsets_host, uint32_t batch, int32_t algo, uint32_t k_max,
+                         float* out_phase_dev, uint32_t* out_iters_dev, void* stream) {
     clear_error();
     Context* c = static_cast<Context*>(ctx);
     if (!c || !packed_sources_dev || !foci_dev || !amps_dev || !offsets_host || !out_phase_dev ||
         n_packed % AFH_SOURCE_PAD != 0 || n_packed < n_sources || n_sources > 0xFFFFFFFFull ||
-        (source_type != kSphere && source_type != kT4010A1) || offsets_host[0] != 0) {
-        set_error("AFH_LMSolve: bad argument");
+        (source_type != kSphere && source_type != kT4010A1) || offsets_host[0] != 0 || algo < kNlsLM ||
+        algo > kNlsGD) {
+        set_error(std::string(who) + ": bad argument");
         return AFH_ERR_ARG;
     }
     std::lock_guard<std::mutex> g(c->mu);
     return lm_solve_batch(c, packed_sources_dev, (uint32_t)n_sources, (uint32_t)n_packed, source_type, foci_dev,
-                          amps_dev, offsets_host, batch, k_max, out_phase_dev, out_iters_dev,
-                          (hipStream_t)stream);
+                          amps_dev, offsets_host, batch, k_max, out_phase_dev, out_iters_dev, (hipStream_t)stream,
+                          algo);
+}
+
+extern "C" int32_t AFH_LMSolve(void* ctx, const float* packed_sources_dev, uint64_t n_packed, uint64_t n_sources,
+                               int32_t source_type, const float* foci_dev, const float* amps_dev,
+                               const uint32_t* offsets_host, uint32_t batch, uint32_t k_max, float* out_phase_dev,
+                               uint32_t* out_iters_dev, void* stream) {
+    return nls_entry("AFH_LMSolve", ctx, packed_sources_dev, n_packed, n_sources, source_type, foci_dev, amps_dev,
+                     offsets_host, batch, kNlsLM, k_max, out_phase_dev, out_iters_dev, stream);
+}
+
+extern "C" int32_t AFH_NLSolve(void* ctx, const float* packed_sources_dev, uint64_t n_packed, uint64_t n_sources,
+                               int32_t source_type, const float* foci_dev, const float* amps_dev,
+                               const uint32_t* offsets_host, uint32_t batch, int32_t algo, uint32_t k_max,
+                               float* out_phase_dev, uint32_t* out_iters_dev, void* stream) {
+    return nls_entry("AFH_NLSolve", ctx, packed_sources_dev, n_packed, n_sources, source_type, foci_dev, amps_dev,
+                     offsets_host, batch, algo, k_max, out_phase_dev, out_iters_dev, stream);
 }

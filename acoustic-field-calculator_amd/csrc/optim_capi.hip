@@ -63,7 +63,8 @@ void run_single(void* handle, const void* foci, const float* amps, uint64_t size
     const size_t ia = source_type == kT4010A1 ? 6 : 3;
     for (uint64_t j = 0; j < n; ++j) {
         float* src = s->sources.data() + st * j;
-        if (algo != kAlgoLM && algo != kAlgoGreedy) src[ia] = amp_out[j];  // LM and Greedy set phases only
+        // LM, Gauss-Newton, gradient descent and Greedy set phases only
+        if (algo != kAlgoLM && algo != kAlgoGN && algo != kAlgoGD && algo != kAlgoGreedy) src[ia] = amp_out[j];
         src[ia + 1] = phase_out[j];
     }
 }
@@ -109,9 +110,13 @@ void AFO_IFFT(void*, const char*, AFC_Vector3, AFC_Vector3, AFC_Vector3, float, 
 void AFO_Horn(void*, const void*, const float*, uint64_t, int32_t) { unsupported("AFO_Horn"); }
 void AFO_Long(void*, const void*, const float*, uint64_t, int32_t) { unsupported("AFO_Long"); }
 void AFO_APO(void*, const void*, const float*, uint64_t, float, int32_t) { unsupported("AFO_APO"); }
-void AFO_GaussNewton(void*, const void*, const float*, uint64_t, int32_t) { unsupported("AFO_GaussNewton"); }
-void AFO_GradientDescent(void*, const void*, const float*, uint64_t, int32_t) {
-    unsupported("AFO_GradientDescent");
+// nls/gauss_newton.rs:35-89 (capi/src/optimizer.rs:177-192)
+void AFO_GaussNewton(void* handle, const void* foci, const float* amps, uint64_t size, int32_t source_type) {
+    run_single(handle, foci, amps, size, source_type, kAlgoGN, "AFO_GaussNewton");
+}
+// nls/gradient_descent.rs:34-61 (capi/src/optimizer.rs:195-210)
+void AFO_GradientDescent(void* handle, const void* foci, const float* amps, uint64_t size, int32_t source_type) {
+    run_single(handle, foci, amps, size, source_type, kAlgoGD, "AFO_GradientDescent");
 }
 // combination/greedy.rs:23-83 (capi/src/optimizer.rs:227-241)
 void AFO_Greedy(void* handle, const void* foci, const float* amps, uint64_t size, uint64_t phase_div,

@@ -419,6 +419,25 @@ def bench_lm(torch, D, O):
             # phases are compared after the FP32 rounding the output format applies
             entry["focal_amp_rel_err_vs_fp64_20it_max"] = max(errs)
             entry["phase_max_abs_diff_vs_fp64_20it_rad"] = max(dph)
+        if name == "n249":
+            # the other two solvers of nls/ on the same machinery and workload
+            for algo, key, kmax in ((D.NLS_GAUSS_NEWTON, "gauss_newton_n249", 200),
+                                    (D.NLS_GRADIENT_DESCENT, "gradient_descent_n249", 10000)):
+                def run_nls():
+                    D.nl_solve(ctx, src_d.data_ptr(), packed.shape[0], n, 1, foci_d.data_ptr(), amps_d.data_ptr(),
+                               offs, algo, kmax, ph_d.data_ptr(), it_d.data_ptr(), stream.cuda_stream)
+                run_nls()
+                torch.cuda.synchronize()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                run_nls()
+                e1.record(stream)
+                torch.cuda.synchronize()
+                nms = e0.elapsed_time(e1)
+                its = it_d.cpu().numpy()
+                out[key] = {"solves_per_s": round(64 / (nms * 1e-3), 1), "batch_ms": round(nms, 3),
+                            "k_max": kmax, "iterations_mean": float(its.mean())}
+                log(f"{key}: {nms:.3f} ms per batch of 64")
         ctx.close()
         out[name] = entry
         log(f"lm {name}: {ms:.3f} ms per batch of 64, {64 / (ms * 1e-3):.0f} solves/s")

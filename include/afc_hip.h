@@ -132,16 +132,20 @@ void AFO_LM(void* handle, const void* foci, const float* amps, uint64_t size, in
  * phases (amplitudes unchanged); phase_div in 1..4096, at most 128 foci */
 void AFO_Greedy(void* handle, const void* foci, const float* amps, uint64_t size, uint64_t phase_div,
                 int32_t source_type);
+/* capi/src/optimizer.rs:177-192, nls/gauss_newton.rs:35-89 (phases only,
+ * FP64, at most 32 foci) */
+void AFO_GaussNewton(void* handle, const void* foci, const float* amps, uint64_t size, int32_t source_type);
+/* capi/src/optimizer.rs:195-210, nls/gradient_descent.rs:34-61 (phases only,
+ * FP64, at most 32 foci) */
+void AFO_GradientDescent(void* handle, const void* foci, const float* amps, uint64_t size, int32_t source_type);
 /* Exported for binding compatibility only; each records "unsupported".
- * capi/src/optimizer.rs:49, :61, :130, :144, :158, :177, :195 */
+ * capi/src/optimizer.rs:49, :61, :130, :144, :158 */
 void AFO_BesselBeam(void* handle, AFC_Vector3 point, AFC_Vector3 dir, float theta, int32_t source_type);
 void AFO_IFFT(void* handle, const char* path, AFC_Vector3 bottom_left, AFC_Vector3 top_left,
               AFC_Vector3 bottom_right, float spacing, float z, int32_t source_type);
 void AFO_Horn(void* handle, const void* foci, const float* amps, uint64_t size, int32_t source_type);
 void AFO_Long(void* handle, const void* foci, const float* amps, uint64_t size, int32_t source_type);
 void AFO_APO(void* handle, const void* foci, const float* amps, uint64_t size, float lambda, int32_t source_type);
-void AFO_GaussNewton(void* handle, const void* foci, const float* amps, uint64_t size, int32_t source_type);
-void AFO_GradientDescent(void* handle, const void* foci, const float* amps, uint64_t size, int32_t source_type);
 
 /* ============================================== part 2: no reference counterpart */
 
@@ -248,6 +252,16 @@ int32_t AFH_GreedySolve(void* ctx, const float* packed_sources_dev, uint64_t n_p
 int32_t AFH_LMSolve(void* ctx, const float* packed_sources_dev, uint64_t n_packed, uint64_t n_sources,
                     int32_t source_type, const float* foci_dev, const float* amps_dev, const uint32_t* offsets_host,
                     uint32_t batch, uint32_t k_max, float* out_phase_dev, uint32_t* out_iters_dev, void* stream);
+
+/* The nonlinear least-squares solvers of nls/ on the LM machinery, batched:
+ * algo AFH_NLS_LM (as AFH_LMSolve), AFH_NLS_GAUSS_NEWTON (gauss_newton.rs:35-89,
+ * K_MAX 200) or AFH_NLS_GRADIENT_DESCENT (gradient_descent.rs:34-61, K_MAX
+ * 10 000). Same arguments and outputs as AFH_LMSolve. */
+enum { AFH_NLS_LM = 0, AFH_NLS_GAUSS_NEWTON = 1, AFH_NLS_GRADIENT_DESCENT = 2 };
+int32_t AFH_NLSolve(void* ctx, const float* packed_sources_dev, uint64_t n_packed, uint64_t n_sources,
+                    int32_t source_type, const float* foci_dev, const float* amps_dev, const uint32_t* offsets_host,
+                    uint32_t batch, int32_t algo, uint32_t k_max, float* out_phase_dev, uint32_t* out_iters_dev,
+                    void* stream);
 
 /* ---- request server: field evaluations and GS-PAT solves, continuously
  * batched on one GPU (see the design notes in server.hip) */

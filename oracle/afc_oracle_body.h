@@ -563,6 +563,180 @@ uint32_t SFX(orc_lm)(int source_type, const float* sources, const float* wavenum
     return it;
 }
 
+/* BhB = B^H B with B = [G | -diag(amps)] (nls/macros.rs:17-31), column-major
+ * np x np, caller frees. */
+static SFX(cplx) * SFX(nls_BhB)(int source_type, const float* sources, const float* wavenums, const float* attens,
+                                 uint64_t n, const float* foci, const float* amps, uint64_t m) {
+    const uint64_t np = n + m;
+    SFX(cplx)* G = SFX(make_G)(source_type, sources, wavenums, attens, n, foci, m);
+    SFX(cplx)* Bm = (SFX(cplx)*)calloc(m * np, sizeof(SFX(cplx)));
+    SFX(cplx)* BhB = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * np * np);
+    uint64_t i, j, k;
+    for (j = 0; j < n; ++j)
+        for (i = 0; i < m; ++i) Bm[i + j * m] = G[i + j * m];
+    for (i = 0; i < m; ++i) Bm[i + (n + i) * m] = SFX(c_make)(-(R)amps[i], 0);
+    for (j = 0; j < np; ++j) {
+        for (k = 0; k < np; ++k) BhB[k + j * np] = SFX(c_make)(0, 0);
+        for (i = 0; i < m; ++i) {
+            SFX(cplx) bil = Bm[i + j * m];
+            for (k = 0; k < np; ++k)
+                BhB[k + j * np] = SFX(c_add)(SFX(c_mul)(bil, SFX(c_conj)(Bm[i + k * m])), BhB[k + j * np]);
+        }
+    }
+    free(G);
+    free(Bm);
+    return BhB;
+}
+
+/* Eigen-decomposition of a symmetric matrix (column-major n x n, destroyed)
+ * by cyclic Jacobi rotations: evals[k], eigenvectors in the columns of V. */
+static void SFX(sym_eig)(R* A, uint64_t n, R* evals, R* V) {
+    uint64_t p, q, k, sweep;
+    for (p = 0; p < n; ++p)
+        for (q = 0; q < n; ++q) V[p + q * n] = (p == q) ? (R)1 : (R)0;
+    for (sweep = 0; sweep < 100; ++sweep) {
+        R off = 0, diag = 0;
+        for (q = 0; q < n; ++q)
+            for (p = 0; p < n; ++p) {
+                if (p != q) off += A[p + q * n] * A[p + q * n];
+                else diag += A[p + p * n] * A[p + p * n];
+            }
+        if (off <= diag * (R)1e-30 || off == 0) break;
+        for (p = 0; p + 1 < n; ++p)
+            for (q = p + 1; q < n; ++q) {
+                const R apq = A[p + q * n];
+                R theta, t, c, sn;
+                if (apq == 0) continue;
+                theta = (A[q + q * n] - A[p + p * n]) / ((R)2 * apq);
+                t = (theta >= 0 ? (R)1 : (R)-1) / (RABS(theta) + RSQRT(theta * theta + (R)1));
+                c = (R)1 / RSQRT(t * t + (R)1);
+                sn = t * c;
+                for (k = 0; k < n; ++k) {
+                    const R akp = A[k + p * n], akq = A[k + q * n];
+                    A[k + p * n] = c * akp - sn * akq;
+                    A[k + q * n] = sn * akp + c * akq;
+                }
+                for (k = 0; k < n; ++k) {
+                    const R apk = A[p + k * n], aqk = A[q + k * n];
+                    A[p + k * n] = c * apk - sn * aqk;
+                    A[q + k * n] = sn * apk + c * aqk;
+                }
+                for (k = 0; k < n; ++k) {
+                    const R vkp = V[k + p * n], vkq = V[k + q * n];
+                    V[k + p * n] = c * vkp - sn * vkq;
+                    V[k + q * n] = sn * vkp + c * vkq;
+                }
+            }
+    }
+    for (k = 0; k < n; ++k) evals[k] = A[k + k * n];
+}
+
+/* Gauss-Newton: nls/gauss_newton.rs:35-89. h = -pinv(A, 1e-3) g, the
+ * pseudo-inverse keeping singular values above 1e-3 (A is symmetric positive
+ * semi-definite, so its singular values are |eigenvalues|). Returns the
+ * iterations taken. */
+uint32_t SFX(orc_gauss_newton)(int source_type, const float* sources, const float* wavenums, const float* attens,
+                               uint64_t n, const float* foci, const float* amps, uint64_t m, uint32_t k_max,
+                               R* out_phase, R* out_x) {
+    const R EPS_1 = (R)1e-8, EPS_2 = (R)1e-8, PINV_EPS = (R)1e-3;
+    const uint64_t np = n + m;
+    SFX(cplx)* BhB = SFX(nls_BhB)(source_type, sources, wavenums, attens, n, foci, amps, m);
+    SFX(cplx)* T = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * np);
+    R* A = (R*)malloc(sizeof(R) * np * np);
+    R* V = (R*)malloc(sizeof(R) * np * np);
+    R* ev = (R*)malloc(sizeof(R) * np);
+    R* w = (R*)malloc(sizeof(R) * np);
+    R* g = (R*)malloc(sizeof(R) * np);
+    R* h = (R*)malloc(sizeof(R) * np);
+    R* x = (R*)calloc(np, sizeof(R));
+    uint64_t k, l;
+    uint32_t it;
+    int found;
+    R gmax;
+    for (k = 0; k < np; ++k) T[k] = SFX(c_make)(RCOS(x[k]), -RSIN(x[k]));
+    SFX(lm_A_g)(BhB, T, np, A, g);
+    gmax = g[0];
+    for (k = 1; k < np; ++k)
+        if (g[k] > gmax) gmax = g[k];
+    found = gmax <= EPS_1;
+    for (it = 0; it < k_max; ++it) {
+        R hn = 0, xn = 0;
+        if (found) break;
+        SFX(sym_eig)(A, np, ev, V);
+        for (k = 0; k < np; ++k) {
+            R d = 0;
+            for (l = 0; l < np; ++l) d += V[l + k * np] * g[l];
+            w[k] = RABS(ev[k]) > PINV_EPS ? d / ev[k] : (R)0;
+        }
+        for (l = 0; l < np; ++l) {
+            R acc = 0;
+            for (k = 0; k < np; ++k) acc += V[l + k * np] * w[k];
+            h[l] = -acc;
+        }
+        for (k = 0; k < np; ++k) hn += h[k] * h[k];
+        for (k = 0; k < np; ++k) xn += x[k] * x[k];
+        if (RSQRT(hn) <= EPS_2 * (RSQRT(xn) + EPS_2)) {
+            found = 1;
+        } else {
+            for (k = 0; k < np; ++k) x[k] += h[k];
+            for (k = 0; k < np; ++k) T[k] = SFX(c_make)(RCOS(x[k]), -RSIN(x[k]));
+            SFX(lm_A_g)(BhB, T, np, A, g);
+            gmax = g[0];
+            for (k = 1; k < np; ++k)
+                if (g[k] > gmax) gmax = g[k];
+            found = gmax <= EPS_1;
+        }
+    }
+    for (k = 0; k < n; ++k) out_phase[k] = x[k];
+    if (out_x)
+        for (k = 0; k < np; ++k) out_x[k] = x[k];
+    free(BhB);
+    free(T);
+    free(A);
+    free(V);
+    free(ev);
+    free(w);
+    free(g);
+    free(h);
+    free(x);
+    return it;
+}
+
+/* Gradient descent: nls/gradient_descent.rs:34-61. x -= 0.1 g until
+ * max(g) <= 1e-6 (signed max), at most k_max steps. Returns the steps taken. */
+uint32_t SFX(orc_gradient_descent)(int source_type, const float* sources, const float* wavenums, const float* attens,
+                                   uint64_t n, const float* foci, const float* amps, uint64_t m, uint32_t k_max,
+                                   R* out_phase, R* out_x) {
+    const R EPS = (R)1e-6;
+    const uint64_t np = n + m;
+    SFX(cplx)* BhB = SFX(nls_BhB)(source_type, sources, wavenums, attens, n, foci, amps, m);
+    SFX(cplx)* T = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * np);
+    R* A = (R*)malloc(sizeof(R) * np * np);
+    R* g = (R*)malloc(sizeof(R) * np);
+    R* x = (R*)calloc(np, sizeof(R));
+    uint64_t k;
+    uint32_t it;
+    for (it = 0; it < k_max; ++it) {
+        R gmax;
+        for (k = 0; k < np; ++k) T[k] = SFX(c_make)(RCOS(x[k]), -RSIN(x[k]));
+        SFX(lm_A_g)(BhB, T, np, A, g);
+        gmax = g[0];
+        for (k = 1; k < np; ++k)
+            if (g[k] > gmax) gmax = g[k];
+        if (gmax <= EPS) break;
+        for (k = 0; k < np; ++k) x[k] = x[k] - (R)0.1 * g[k];
+    }
+    for (k = 0; k < n; ++k) out_phase[k] = x[k];
+    if (out_x)
+        for (k = 0; k < np; ++k) out_x[k] = x[k];
+    free(BhB);
+    free(T);
+    free(A);
+    free(g);
+    free(x);
+    return it;
+}
+
 /* F(x) and g(x) of the LM objective for given x (nls/macros.rs:17-58), for
  * gradient checks: returns F, writes g (n + m). */
 R SFX(orc_lm_eval)(int source_type, const float* sources, const float* wavenums, const float* attens, uint64_t n,

@@ -62,6 +62,10 @@ def lib():
             f = getattr(L, f"orc_naive_{sfx}")
             f.restype = None
             f.argtypes = [c_int, fp, fp, fp, c_uint64, fp, fp, c_uint64, fp, fp]
+            for nm in ("orc_gauss_newton", "orc_gradient_descent"):
+                f = getattr(L, f"{nm}_{sfx}")
+                f.restype = c_uint32
+                f.argtypes = [c_int, fp, fp, fp, c_uint64, fp, fp, c_uint64, c_uint32, fp, fp]
             f = getattr(L, f"orc_greedy_{sfx}")
             f.restype = None
             f.argtypes = [c_int, fp, fp, fp, c_uint64, fp, fp, c_uint64, c_uint64, fp]
@@ -200,6 +204,25 @@ def lm(source_type, sources, wavenums, attens, foci, amps, prec="f64", k_max=200
     it = getattr(lib(), f"orc_lm_{sfx}")(source_type, _p(src), _p(k), _p(a), n, _p(f), _p(am), m, k_max, _p(ph),
                                          _p(x))
     return ph, x, int(it)
+
+
+def _nls(name, source_type, sources, wavenums, attens, foci, amps, prec, k_max):
+    dt, sfx = _dt(prec)
+    src, k, a, f, am = _f32(sources), _f32(wavenums), _f32(attens), _f32(foci).reshape(-1, 3), _f32(amps)
+    n, m = k.shape[0], f.shape[0]
+    ph, x = np.empty(n, dt), np.empty(n + m, dt)
+    it = getattr(lib(), f"{name}_{sfx}")(source_type, _p(src), _p(k), _p(a), n, _p(f), _p(am), m, k_max, _p(ph), _p(x))
+    return ph, x, int(it)
+
+
+def gauss_newton(source_type, sources, wavenums, attens, foci, amps, prec="f64", k_max=200):
+    """Gauss-Newton (gauss_newton.rs:35-89). Returns (phases[N], x[N+M], iterations)."""
+    return _nls("orc_gauss_newton", source_type, sources, wavenums, attens, foci, amps, prec, k_max)
+
+
+def gradient_descent(source_type, sources, wavenums, attens, foci, amps, prec="f64", k_max=10000):
+    """Gradient descent (gradient_descent.rs:34-61). Returns (phases[N], x[N+M], steps)."""
+    return _nls("orc_gradient_descent", source_type, sources, wavenums, attens, foci, amps, prec, k_max)
 
 
 def lm_eval(source_type, sources, wavenums, attens, foci, amps, x):

@@ -151,3 +151,58 @@ def test_greedy_lowers_objective(oracle):
     p1 = np.abs(focal(oracle, with_solution(src, src[:, 6], ph), k, a, foci))
     p0 = np.abs(focal(oracle, with_solution(src, src[:, 6], np.zeros(len(src))), k, a, foci))
     assert np.sum(np.abs(amps - p1)) < 0.5 * np.sum(np.abs(amps - p0))
+
+
+def _bhb_numpy(oracle, src, k, a, foci, amps):
+    """B^H B with B = [G | -diag(amps)] (nls/macros.rs:17-31) from the pinned G."""
+    G = oracle.propagation_matrix(1, src, k, a, foci, "f64")  # M x N
+    B = np.concatenate([G, -np.diag(amps.astype(np.float64))], 1)
+    return B.conj().T @ B
+
+
+def test_gauss_newton_step_is_truncated_pseudo_inverse(oracle):
+    """One Gauss-Newton step from x = 0 equals -pinv(A, 1e-3) g with A =
+    Re(B^H B) and the pseudo-inverse from LAPACK's eigh (independent of the
+    oracle's Jacobi solver), g from the pinned gradient."""
+    src, k, a, foci, amps = small_problem(oracle, m=3, seed=9)
+    _, x1, it = oracle.gauss_newton(1, src, k, a, foci, amps, "f64", k_max=1)
+    assert it == 1
+    A = _bhb_numpy(oracle, src, k, a, foci, amps).real
+    _, g = oracle.lm_eval(1, src, k, a, foci, amps, np.zeros(len(src) + 3))
+    lam, V = np.linalg.eigh(A)
+    keep = np.abs(lam) > 1e-3
+    h = -(V[:, keep] @ ((V[:, keep].T @ g) / lam[keep]))
+    assert np.allclose(x1, h, rtol=1e-8, atol=1e-10 * np.max(np.abs(h)))
+
+
+def test_gradient_descent_first_step_is_scaled_gradient(oracle):
+    """x_1 = -0.1 g(0) = -0.05 dF/dx(0), dF/dx by central differences of F."""
+    src, k, a, foci, amps = small_problem(oracle, m=2, seed=10)
+    n = len(src) + 2
+    _, x1, it = oracle.gradient_descent(1, src, k, a, foci, amps, "f64", k_max=1)
+    assert it == 1
+    x0 = np.zeros(n)
+    dF = np.empty(n)
+    for i in range(0, n, 17):
+        e = np.zeros(n)
+        e[i] = 1e-6
+        dF[i] = (oracle.lm_eval(1, src, k, a, foci, amps, x0 + e)[0] - oracle.lm_eval(1, src, k, a, foci, amps,
+                                                                                      x0 - e)[0]) / 2e-6
+    idx = np.arange(0, n, 17)
+    assert np.allclose(x1[idx], -0.05 * dF[idx], rtol=1e-5, atol=1e-9)
+
+
+def test_gradient_descent_and_gauss_newton_lower_F(oracle):
+    src, k, a, foci, amps = small_problem(oracle, m=3, seed=11)
+    n = len(src) + 3
+    F0 = oracle.lm_eval(1, src, k, a, foci, amps, np.zeros(n))[0]
+    prev = F0
+    for kk in (10, 100, 1000):
+        _, x, _ = oracle.gradient_descent(1, src, k, a, foci, amps, "f64", k_max=kk)
+        F = oracle.lm_eval(1, src, k, a, foci, amps, x)[0]
+        assert F < prev
+        prev = F
+    # Gauss-Newton stops early once the pinv cut-off (1e-3) removes the
+    # remaining descent directions (F / F0 = 0.026 here)
+    _, x, _ = oracle.gauss_newton(1, src, k, a, foci, amps, "f64")
+    assert oracle.lm_eval(1, src, k, a, foci, amps, x)[0] < 0.1 * F0
