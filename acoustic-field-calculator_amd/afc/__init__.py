@@ -414,6 +414,13 @@ class Optimizer:
         Optimizer._run("AFO_GradientDescent", system, foci, amps)
 
     @staticmethod
+    def apo(system: UniformSystem, foci, amps, lambda_reg: float):
+        size, f, a = _pack_foci(foci, amps)
+        dll().AFO_APO(system.handle, f.ctypes.data_as(c_void_p), a.ctypes.data_as(POINTER(c_float)), c_ulong(size),
+                      c_float(lambda_reg), system._source_type)
+        _check("AFO_APO")
+
+    @staticmethod
     def greedy(system: UniformSystem, foci, amps, phase_div: int = 16):
         size, f, a = _pack_foci(foci, amps)
         dll().AFO_Greedy(system.handle, f.ctypes.data_as(c_void_p), a.ctypes.data_as(POINTER(c_float)),

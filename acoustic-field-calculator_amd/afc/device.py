@@ -152,3 +152,15 @@ def greedy_solve(ctx: Context, packed_dev_ptr: int, n_packed: int, n_sources: in
                                c_void_p(foci_dev_ptr), c_void_p(amps_dev_ptr), offs.ctypes.data_as(c_void_p),
                                offs.shape[0] - 1, phase_div, c_void_p(out_phase_dev_ptr), c_void_p(stream_ptr))
     _err("AFH_GreedySolve", rc)
+
+
+def apo_solve(ctx: Context, packed_dev_ptr: int, n_packed: int, n_sources: int, source_type: int, foci_dev_ptr: int,
+              amps_dev_ptr: int, offsets: np.ndarray, lam: float, k_max: int, out_amp_dev_ptr: int,
+              out_phase_dev_ptr: int, out_iters_dev_ptr: int = 0, stream_ptr: int = 0) -> None:
+    """Batched FP64 APO (amplitudes and phases) on device arrays."""
+    offs = np.ascontiguousarray(offsets, np.uint32)
+    rc = dll().AFH_APOSolve(ctx.handle, c_void_p(packed_dev_ptr), n_packed, n_sources, source_type,
+                            c_void_p(foci_dev_ptr), c_void_p(amps_dev_ptr), offs.ctypes.data_as(c_void_p),
+                            offs.shape[0] - 1, lam, k_max, c_void_p(out_amp_dev_ptr), c_void_p(out_phase_dev_ptr),
+                            c_void_p(out_iters_dev_ptr or None), c_void_p(stream_ptr))
+    _err("AFH_APOSolve", rc)

@@ -84,5 +84,10 @@ int32_t lm_solve_batch(Context* ctx, const float* sources, uint32_t n, uint32_t 
 // LM (lm.rs:40-105), Gauss-Newton (gauss_newton.rs:35-89) or gradient descent
 // (gradient_descent.rs:34-61) with the reference's K_MAX: phases of the n sources.
 bool lm_solve_single(const HoloProblem& p, float* phase_out, int algo = kNlsLM);
+// APO (apo.rs:100-158), K_MAX 200: amplitudes and phases of the n sources.
+int32_t apo_solve_batch(Context* ctx, const float* sources, uint32_t n, int source_type, const float* foci,
+                        const float* amps, const uint32_t* offsets_host, uint32_t batch, double lambda,
+                        uint32_t k_max, float* out_amp, float* out_phase, uint32_t* out_iters, hipStream_t st);
+bool apo_solve_single(const HoloProblem& p, float lambda, float* amp_out, float* phase_out);
 
 }  // namespace afh

@@ -27,6 +27,11 @@
 //   gradient descent (gradient_descent.rs:34-61): x -= 0.1 g until max g <= 1e-6,
 //     g_l = sum_i Im(conj(C_il) s_i) without forming W.
 //
+// APO (apo.rs:100-158) runs in its own kernel on the same pieces: q0 from the
+// push-through form G^H (G G^H + lambda I)^{-1} p (the M x M complex system
+// real-ified and solved by the one-wave Cholesky), then the reference's
+// quasi-Newton iteration with a dense N x N H.
+//
 // One workgroup per instance; K, v, y in LDS; G, W, x, g, h, e^{ix} in a
 // per-instance global scratch. Every reduction has a fixed order.
 #include <math.h>
@@ -43,6 +48,7 @@ namespace afh {
 constexpr int kLB = 256;
 constexpr int kLmMaxM = 32;
 constexpr int kLmMaxD = 2 * kLmMaxM;
+constexpr int kApoLineSearch = 100;  // LINE_SEARCH_MAX (apo.rs:19)
 
 __constant__ DirEntry c_lm_dir[11] = {AFH_DIR_ENTRIES(AFH_DIR_ENTRY)};
 
@@ -54,7 +60,9 @@ struct LmArgs {
     const float* amps;
     const uint32_t* offsets;
     uint32_t k_max;
-    int algo;  // kNlsLM, kNlsGN, kNlsGD
+    int algo;       // kNlsLM, kNlsGN, kNlsGD
+    double lambda;  // APO's regularisation
+    float* out_amp; // APO: batch x N
     char* scratch;  // per instance, scratch_stride bytes
     uint64_t scratch_stride;
     float* out_phase;     // batch x N
@@ -678,6 +686,314 @@ __global__ __launch_bounds__(kLB) void lm_solve(LmArgs a) {
     if (t == 0 && a.out_iters) a.out_iters[b] = it;
 }
 
+// ---------------------------------------------------------------- APO
+// Scratch of one APO instance: G (M x N) first, as lm_gen_g writes it, then
+// H (N x N, column-major) and the N-vectors q, q_new, d, grad, grad_new, s, u.
+__host__ __device__ inline uint64_t apo_scratch_bytes(uint64_t N, uint64_t M) {
+    auto al = [](uint64_t b) { return (b + 255) & ~(uint64_t)255; };
+    return al(M * N * 16) + al(N * N * 16) + 7 * al(N * 16);
+}
+
+struct ApoScratch {
+    double2 *G, *H, *q, *qn, *d, *gr, *grn, *sv, *u;
+};
+
+__device__ inline ApoScratch apo_scratch(char* base, uint64_t N, uint64_t M) {
+    auto al = [](uint64_t b) { return (b + 255) & ~(uint64_t)255; };
+    ApoScratch s;
+    char* p = base;
+    s.G = reinterpret_cast<double2*>(p);
+    p += al(M * N * 16);
+    s.H = reinterpret_cast<double2*>(p);
+    p += al(N * N * 16);
+    double2** v[7] = {&s.q, &s.qn, &s.d, &s.gr, &s.grn, &s.sv, &s.u};
+    for (int k = 0; k < 7; ++k) {
+        *v[k] = reinterpret_cast<double2*>(p);
+        p += al(N * 16);
+    }
+    return s;
+}
+
+__device__ inline double2 cmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+
+__device__ inline double2 block_sum2(double2 v, double* red) {
+    const double re = block_sum(v.x, red);
+    const double im = block_sum(v.y, red);
+    return make_double2(re, im);
+}
+
+// gq_i = g_i q for every focus (a wave per row, lanes over sources).
+__device__ inline void rows_times(const double2* G, const double2* q, uint32_t N, uint32_t M, double2* out) {
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    for (uint32_t i = w; i < M; i += kLB / 64) {
+        double re = 0.0, im = 0.0;
+        for (uint32_t j = l; j < N; j += 64) {
+            const double2 g = G[(uint64_t)i * N + j], v = q[j];
+            re += g.x * v.x - g.y * v.y;
+            im += g.x * v.y + g.y * v.x;
+        }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            re += __shfl_xor(re, off);
+            im += __shfl_xor(im, off);
+        }
+        if (l == 0) out[i] = make_double2(re, im);
+    }
+}
+
+__global__ __launch_bounds__(kLB) void apo_solve(LmArgs a) {
+    __shared__ double s_K[kLmMaxD * kLmMaxD];
+    __shared__ double s_v[kLmMaxD], s_y[kLmMaxD], s_col[kLmMaxD];
+    __shared__ double s_work[kLmMaxD * (kLmMaxD + 1)];
+    __shared__ double2 s_gq[kLmMaxM], s_gd[kLmMaxM];
+    __shared__ double s_p2[kLmMaxM];
+    __shared__ double s_J[kApoLineSearch];
+    __shared__ double s_red[4];
+    __shared__ double s_alpha;
+    const int t = threadIdx.x;
+    const uint32_t b = blockIdx.x;
+    const uint32_t base = a.offsets[b], M = a.offsets[b + 1] - base;
+    const uint32_t N = a.n, D = 2 * M;
+    const double lam = a.lambda;
+    const float* amps = a.amps + base;
+    ApoScratch s = apo_scratch(a.scratch + b * a.scratch_stride, N, M);
+    for (uint32_t i = t; i < M; i += kLB) s_p2[i] = (double)amps[i] * (double)amps[i];
+
+    // q0 = G^H (G G^H + lambda I)^{-1} p: the Hermitian M x M system as the
+    // real symmetric 2M x 2M system [[Re K, -Im K], [Im K, Re K]] (lower
+    // triangle into s_K)
+    for (uint32_t e = t; e < M * M; e += kLB) {
+        const uint32_t i = e / M, k = e % M;
+        if (k > i) continue;
+        double re = 0.0, im = 0.0;
+        for (uint32_t l = 0; l < N; ++l) {
+            const double2 gi = s.G[(uint64_t)i * N + l], gk = s.G[(uint64_t)k * N + l];
+            re += gi.x * gk.x + gi.y * gk.y;  // G_il conj(G_kl)
+            im += gi.y * gk.x - gi.x * gk.y;
+        }
+        if (i == k) re += lam;
+        s_K[i * kLmMaxD + k] = re;
+        s_K[(i + M) * kLmMaxD + (k + M)] = re;
+        s_K[(i + M) * kLmMaxD + k] = im;
+        if (i != k) s_K[(k + M) * kLmMaxD + i] = -im;
+    }
+    for (uint32_t i = t; i < D; i += kLB) s_v[i] = i < M ? (double)amps[i] : 0.0;
+    __syncthreads();
+    if (t < 64) {
+        if (D <= 16)
+            (void)chol_solve_wave<16>(s_K, s_v, s_y, s_col, D);
+        else if (D <= 32)
+            (void)chol_solve_wave<32>(s_K, s_v, s_y, s_col, D);
+        else
+            (void)chol_solve_wave_lds(s_K, s_v, s_y, s_col, s_work, D);
+    }
+    __syncthreads();
+    for (uint32_t j = t; j < N; j += kLB) {
+        double re = 0.0, im = 0.0;
+        for (uint32_t i = 0; i < M; ++i) {
+            const double2 g = s.G[(uint64_t)i * N + j];
+            const double zr = s_y[i], zi = s_y[M + i];
+            re += g.x * zr + g.y * zi;  // conj(G_ij) z_i
+            im += g.x * zi - g.y * zr;
+        }
+        s.q[j] = make_double2(re, im);
+    }
+    for (uint64_t e = t; e < (uint64_t)N * N; e += kLB)
+        s.H[e] = make_double2(e % ((uint64_t)N + 1) == 0 ? 1.0 : 0.0, 0.0);
+    __syncthreads();
+
+    // grad J (apo.rs:56-70) at v into out
+    auto grad = [&](const double2* v, double2* out) {
+        rows_times(s.G, v, N, M, s_gq);
+        __syncthreads();
+        for (uint32_t j = t; j < N; j += kLB) {
+            double re = 0.0, im = 0.0;
+            for (uint32_t i = 0; i < M; ++i) {
+                const double2 g = s.G[(uint64_t)i * N + j], gq = s_gq[i];
+                const double w = s_p2[i] - (gq.x * gq.x + gq.y * gq.y);
+                re += (g.x * gq.x + g.y * gq.y) * w;  // conj(G_ij) gq_i w_i
+                im += (g.x * gq.y - g.y * gq.x) * w;
+            }
+            out[j] = make_double2(re + lam * v[j].x, im + lam * v[j].y);
+        }
+        __syncthreads();
+    };
+    grad(s.q, s.gr);
+    uint32_t it = 0;
+    for (; it < a.k_max; ++it) {
+        // d = -H grad
+        for (uint32_t j = t; j < N; j += kLB) {
+            double re = 0.0, im = 0.0;
+            for (uint32_t k = 0; k < N; ++k) {
+                const double2 h = s.H[(uint64_t)k * N + j], g = s.gr[k];
+                re += h.x * g.x - h.y * g.y;
+                im += h.x * g.y + h.y * g.x;
+            }
+            s.d[j] = make_double2(-re, -im);
+        }
+        __syncthreads();
+        // line search (apo.rs:76-97): J(q + a d), a = l / 100, first minimum
+        rows_times(s.G, s.q, N, M, s_gq);
+        rows_times(s.G, s.d, N, M, s_gd);
+        double2 qq = make_double2(0.0, 0.0), qd = qq, dd = qq;
+        for (uint32_t j = t; j < N; j += kLB) {
+            const double2 qv = s.q[j], dv = s.d[j];
+            const double2 a1 = cmul(qv, qv), a2 = cmul(qv, dv), a3 = cmul(dv, dv);
+            qq.x += a1.x;
+            qq.y += a1.y;
+            qd.x += a2.x;
+            qd.y += a2.y;
+            dd.x += a3.x;
+            dd.y += a3.y;
+        }
+        qq = block_sum2(qq, s_red);
+        qd = block_sum2(qd, s_red);
+        dd = block_sum2(dd, s_red);
+        for (uint32_t l = t; l < kApoLineSearch; l += kLB) {
+            const double al = (double)l / (double)kApoLineSearch;
+            double v = 0.0;
+            for (uint32_t i = 0; i < M; ++i) {
+                const double re = s_gq[i].x + al * s_gd[i].x, im = s_gq[i].y + al * s_gd[i].y;
+                const double sdiff = re * re + im * im - s_p2[i];
+                v += sdiff * sdiff;
+            }
+            // (q + a d) . (q + a d) without conjugation
+            const double dr = qq.x + 2.0 * al * qd.x + al * al * dd.x, di = qq.y + 2.0 * al * qd.y + al * al * dd.y;
+            s_J[l] = v + sqrt(dr * dr + di * di) * lam;
+        }
+        __syncthreads();
+        if (t < 64) {
+            double bv = INFINITY;
+            int bi = kApoLineSearch;
+            for (int l = t; l < kApoLineSearch; l += 64)
+                if (s_J[l] < bv) {
+                    bv = s_J[l];
+                    bi = l;
+                }
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) {
+                const double ov = __shfl_xor(bv, off);
+                const int oi = __shfl_xor(bi, off);
+                if (ov < bv || (ov == bv && oi < bi)) {
+                    bv = ov;
+                    bi = oi;
+                }
+            }
+            if (t == 0) s_alpha = bi < kApoLineSearch ? (double)bi / (double)kApoLineSearch : 0.0;
+        }
+        __syncthreads();
+        const double alpha = s_alpha;
+        double dn = 0.0;
+        for (uint32_t j = t; j < N; j += kLB) {
+            const double2 dv = make_double2(s.d[j].x * alpha, s.d[j].y * alpha);
+            s.d[j] = dv;
+            dn += dv.x * dv.x + dv.y * dv.y;
+            s.qn[j] = make_double2(s.q[j].x + dv.x, s.q[j].y + dv.y);
+        }
+        dn = block_sum(dn, s_red);
+        if (sqrt(dn) < 1e-8) break;
+        grad(s.qn, s.grn);
+        for (uint32_t j = t; j < N; j += kLB) s.sv[j] = make_double2(s.grn[j].x - s.gr[j].x, s.grn[j].y - s.gr[j].y);
+        __syncthreads();
+        // u = H s, y . s, s . u (no conjugation)
+        double2 ys = make_double2(0.0, 0.0), su = ys;
+        for (uint32_t j = t; j < N; j += kLB) {
+            double re = 0.0, im = 0.0;
+            for (uint32_t k = 0; k < N; ++k) {
+                const double2 h = s.H[(uint64_t)k * N + j], v = s.sv[k];
+                re += h.x * v.x - h.y * v.y;
+                im += h.x * v.y + h.y * v.x;
+            }
+            const double2 uj = make_double2(re, im);
+            s.u[j] = uj;
+            const double2 a1 = cmul(s.d[j], s.sv[j]), a2 = cmul(s.sv[j], uj);
+            ys.x += a1.x;
+            ys.y += a1.y;
+            su.x += a2.x;
+            su.y += a2.y;
+        }
+        ys = block_sum2(ys, s_red);
+        su = block_sum2(su, s_red);
+        const double ny = ys.x * ys.x + ys.y * ys.y, nu = su.x * su.x + su.y * su.y;
+        const double2 iy = make_double2(ys.x / ny, -ys.y / ny), iu = make_double2(su.x / nu, -su.y / nu);
+        // H += y y^T / (y . s) - u u^T / (s . u)
+        for (uint64_t e = t; e < (uint64_t)N * N; e += kLB) {
+            const uint32_t j = (uint32_t)(e % N), k = (uint32_t)(e / N);
+            const double2 a1 = cmul(cmul(s.d[j], s.d[k]), iy), a2 = cmul(cmul(s.u[j], s.u[k]), iu);
+            const double2 h = s.H[e];
+            s.H[e] = make_double2(h.x + a1.x - a2.x, h.y + a1.y - a2.y);
+        }
+        for (uint32_t j = t; j < N; j += kLB) {
+            s.q[j] = s.qn[j];
+            s.gr[j] = s.grn[j];
+        }
+        __syncthreads();
+    }
+    // camax (nalgebra): max_j |Re q_j| + |Im q_j|
+    double cm = 0.0;
+    for (uint32_t j = t; j < N; j += kLB) cm = fmax(cm, fabs(s.q[j].x) + fabs(s.q[j].y));
+    cm = block_max(cm, s_red);
+    for (uint32_t j = t; j < N; j += kLB) {
+        const double2 qv = s.q[j];
+        const double amp = (double)a.sources[AFH_SOURCE_STRIDE * (uint64_t)j + 6];
+        a.out_amp[(uint64_t)b * N + j] = (float)(amp * sqrt(qv.x * qv.x + qv.y * qv.y) / cm);
+        a.out_phase[(uint64_t)b * N + j] = (float)atan2(qv.y, qv.x);
+    }
+    if (t == 0 && a.out_iters) a.out_iters[b] = it;
+}
+
+int32_t apo_solve_batch(Context* ctx, const float* sources, uint32_t n, int source_type, const float* foci,
+                        const float* amps, const uint32_t* offsets_host, uint32_t batch, double lambda,
+                        uint32_t k_max, float* out_amp, float* out_phase, uint32_t* out_iters, hipStream_t st) {
+    if (batch == 0) return AFH_OK;
+    uint32_t maxm = 0;
+    for (uint32_t b = 0; b < batch; ++b) {
+        const uint32_t m = offsets_host[b + 1] - offsets_host[b];
+        if (m == 0) {
+            set_error("APO: an instance has no foci");
+            return AFH_ERR_ARG;
+        }
+        maxm = std::max(maxm, m);
+    }
+    if (maxm > (uint32_t)kLmMaxM) {
+        set_error("APO: more than 32 foci in one instance is not supported");
+        return AFH_ERR_UNSUPPORTED;
+    }
+    const uint64_t stride = std::max(apo_scratch_bytes(n, maxm), lm_scratch_bytes(n, maxm));
+    const size_t off_bytes = (batch + 1) * sizeof(uint32_t);
+    const size_t o_scr = (off_bytes + 255) & ~(size_t)255;
+    if (o_scr + stride * batch > ctx->work.cap && !hip_ok(hipStreamSynchronize(st), "hipStreamSynchronize"))
+        return AFH_ERR_HIP;
+    if (!ctx->work.reserve(o_scr + stride * batch)) return AFH_ERR_HIP;
+    if (!ctx->reserve_staging(off_bytes)) return AFH_ERR_HIP;
+    memcpy(ctx->staging.ptr, offsets_host, off_bytes);
+    char* w = static_cast<char*>(ctx->work.ptr);
+    if (!hip_ok(hipMemcpyAsync(w, ctx->staging.ptr, off_bytes, hipMemcpyHostToDevice, st), "H2D offsets"))
+        return AFH_ERR_HIP;
+    if (!ctx->upload_done && !hip_ok(hipEventCreateWithFlags(&ctx->upload_done, hipEventDisableTiming), "event"))
+        return AFH_ERR_HIP;
+    if (!hip_ok(hipEventRecord(ctx->upload_done, st), "hipEventRecord")) return AFH_ERR_HIP;
+    LmArgs a;
+    memset(&a, 0, sizeof(a));
+    a.sources = sources;
+    a.source_type = source_type;
+    a.n = n;
+    a.foci = foci;
+    a.amps = amps;
+    a.offsets = reinterpret_cast<const uint32_t*>(w);
+    a.k_max = k_max;
+    a.lambda = lambda;
+    a.scratch = w + o_scr;
+    a.scratch_stride = stride;
+    a.out_amp = out_amp;
+    a.out_phase = out_phase;
+    a.out_iters = out_iters;
+    hipLaunchKernelGGL(lm_gen_g, dim3((n + kLB - 1) / kLB, batch), dim3(kLB), 0, st, a);
+    hipLaunchKernelGGL(apo_solve, dim3(batch), dim3(kLB), 0, st, a);
+    if (!hip_ok(hipGetLastError(), "APO kernel launch")) return AFH_ERR_HIP;
+    return AFH_OK;
+}
+
 int32_t lm_solve_batch(Context* ctx, const float* sources, uint32_t n, uint32_t n_packed, int source_type,
                        const float* foci, const float* amps, const uint32_t* offsets_host, uint32_t batch,
                        uint32_t k_max, float* out_phase, uint32_t* out_iters, hipStream_t st, int algo) {
@@ -761,6 +1077,39 @@ bool lm_solve_single(const HoloProblem& p, float* phase_out, int algo) {
     return true;
 }
 
+bool apo_solve_single(const HoloProblem& p, float lambda, float* amp_out, float* phase_out) {
+    if (!device().ensure()) return false;
+    Context& ctx = default_context();
+    std::lock_guard<std::mutex> g(ctx.mu);
+    if (!ctx.ensure_stream()) return false;
+    const uint64_t n = p.n, np = packed_count(n), m = p.m;
+    const size_t src_b = np * AFH_SOURCE_STRIDE * sizeof(float), foci_b = m * 3 * sizeof(float);
+    const size_t amps_b = m * sizeof(float), out_b = n * sizeof(float);
+    auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t o_foci = align(src_b), o_amps = o_foci + align(foci_b), o_amp = o_amps + align(amps_b);
+    const size_t o_ph = o_amp + align(out_b), total = o_ph + align(out_b);
+    if (!ctx.io.reserve(total) || !ctx.io_host.reserve(total)) return false;
+    char* h = static_cast<char*>(ctx.io_host.ptr);
+    char* d = static_cast<char*>(ctx.io.ptr);
+    pack_sources(p.sources, p.wavenums, p.attens, n, p.source_type, reinterpret_cast<float*>(h));
+    memcpy(h + o_foci, p.foci, foci_b);
+    memcpy(h + o_amps, p.amps, amps_b);
+    if (!hip_ok(hipMemcpyAsync(d, h, o_amp, hipMemcpyHostToDevice, ctx.stream), "H2D")) return false;
+    const uint32_t offs[2] = {0, (uint32_t)m};
+    // K_MAX = 200 (apo.rs:18)
+    if (apo_solve_batch(&ctx, reinterpret_cast<const float*>(d), (uint32_t)n, p.source_type,
+                        reinterpret_cast<const float*>(d + o_foci), reinterpret_cast<const float*>(d + o_amps), offs,
+                        1, (double)lambda, 200, reinterpret_cast<float*>(d + o_amp), reinterpret_cast<float*>(d + o_ph),
+                        nullptr, ctx.stream) != AFH_OK)
+        return false;
+    if (!hip_ok(hipMemcpyAsync(h + o_amp, d + o_amp, o_ph + out_b - o_amp, hipMemcpyDeviceToHost, ctx.stream), "D2H"))
+        return false;
+    if (!hip_ok(hipStreamSynchronize(ctx.stream), "hipStreamSynchronize")) return false;
+    memcpy(amp_out, h + o_amp, out_b);
+    memcpy(phase_out, h + o_ph, out_b);
+    return true;
+}
+
 }  // namespace afh
 
 using namespace afh;
@@ -798,4 +1147,22 @@ extern "C" int32_t AFH_NLSolve(void* ctx, const float* packed_sources_dev, uint6
                                float* out_phase_dev, uint32_t* out_iters_dev, void* stream) {
     return nls_entry("AFH_NLSolve", ctx, packed_sources_dev, n_packed, n_sources, source_type, foci_dev, amps_dev,
                      offsets_host, batch, algo, k_max, out_phase_dev, out_iters_dev, stream);
+}
+
+extern "C" int32_t AFH_APOSolve(void* ctx, const float* packed_sources_dev, uint64_t n_packed, uint64_t n_sources,
+                                int32_t source_type, const float* foci_dev, const float* amps_dev,
+                                const uint32_t* offsets_host, uint32_t batch, float lambda, uint32_t k_max,
+                                float* out_amp_dev, float* out_phase_dev, uint32_t* out_iters_dev, void* stream) {
+    clear_error();
+    Context* c = static_cast<Context*>(ctx);
+    if (!c || !packed_sources_dev || !foci_dev || !amps_dev || !offsets_host || !out_amp_dev || !out_phase_dev ||
+        n_packed % AFH_SOURCE_PAD != 0 || n_packed < n_sources || n_sources > 0xFFFFFFFFull ||
+        (source_type != kSphere && source_type != kT4010A1) || offsets_host[0] != 0) {
+        set_error("AFH_APOSolve: bad argument");
+        return AFH_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> g(c->mu);
+    return apo_solve_batch(c, packed_sources_dev, (uint32_t)n_sources, source_type, foci_dev, amps_dev, offsets_host,
+                           batch, (double)lambda, k_max, out_amp_dev, out_phase_dev, out_iters_dev,
+                           (hipStream_t)stream);
 }

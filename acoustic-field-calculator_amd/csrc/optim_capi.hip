@@ -109,7 +109,36 @@ void AFO_IFFT(void*, const char*, AFC_Vector3, AFC_Vector3, AFC_Vector3, float, 
 }
 void AFO_Horn(void*, const void*, const float*, uint64_t, int32_t) { unsupported("AFO_Horn"); }
 void AFO_Long(void*, const void*, const float*, uint64_t, int32_t) { unsupported("AFO_Long"); }
-void AFO_APO(void*, const void*, const float*, uint64_t, float, int32_t) { unsupported("AFO_APO"); }
+// nls/apo.rs:100-158 (capi/src/optimizer.rs:158-174): sets amplitudes and phases
+void AFO_APO(void* handle, const void* foci, const float* amps, uint64_t size, float lambda, int32_t source_type) {
+    clear_error();
+    System* s = as_system(handle, source_type, "AFO_APO");
+    if (!s) return;
+    if (size == 0 || !foci || !amps) {
+        set_error("AFO_APO: no foci");
+        return;
+    }
+    const uint64_t n = s->count();
+    if (n == 0) return;
+    std::vector<float> amp_out(n), phase_out(n);
+    HoloProblem p;
+    p.source_type = source_type;
+    p.sources = s->sources.data();
+    p.wavenums = s->wavenums.data();
+    p.attens = s->attens.data();
+    p.n = n;
+    p.foci = static_cast<const float*>(foci);
+    p.amps = amps;
+    p.m = size;
+    if (!apo_solve_single(p, lambda, amp_out.data(), phase_out.data())) return;
+    const size_t st = s->stride();
+    const size_t ia = source_type == kT4010A1 ? 6 : 3;
+    for (uint64_t j = 0; j < n; ++j) {
+        float* src = s->sources.data() + st * j;
+        src[ia] = amp_out[j];
+        src[ia + 1] = phase_out[j];
+    }
+}
 // nls/gauss_newton.rs:35-89 (capi/src/optimizer.rs:177-192)
 void AFO_GaussNewton(void* handle, const void* foci, const float* amps, uint64_t size, int32_t source_type) {
     run_single(handle, foci, amps, size, source_type, kAlgoGN, "AFO_GaussNewton");

@@ -438,6 +438,24 @@ def bench_lm(torch, D, O):
                 out[key] = {"solves_per_s": round(64 / (nms * 1e-3), 1), "batch_ms": round(nms, 3),
                             "k_max": kmax, "iterations_mean": float(its.mean())}
                 log(f"{key}: {nms:.3f} ms per batch of 64")
+            # APO (apo.rs:100-158), lambda 0.1, targets 0.2 (within reach of the 249 array)
+            oa_d = torch.empty((64, n), dtype=torch.float32, device=dev)
+            amps_apo = torch.full_like(amps_d, 0.2)
+
+            def run_apo():
+                D.apo_solve(ctx, src_d.data_ptr(), packed.shape[0], n, 1, foci_d.data_ptr(), amps_apo.data_ptr(), offs,
+                            0.1, 200, oa_d.data_ptr(), ph_d.data_ptr(), it_d.data_ptr(), stream.cuda_stream)
+            run_apo()
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            run_apo()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ams = e0.elapsed_time(e1)
+            out["apo_n249"] = {"solves_per_s": round(64 / (ams * 1e-3), 1), "batch_ms": round(ams, 3), "k_max": 200,
+                               "iterations_mean": float(it_d.cpu().numpy().mean()), "lambda": 0.1}
+            log(f"apo_n249: {ams:.3f} ms per batch of 64")
         ctx.close()
         out[name] = entry
         log(f"lm {name}: {ms:.3f} ms per batch of 64, {64 / (ms * 1e-3):.0f} solves/s")

@@ -138,14 +138,16 @@ void AFO_GaussNewton(void* handle, const void* foci, const float* amps, uint64_t
 /* capi/src/optimizer.rs:195-210, nls/gradient_descent.rs:34-61 (phases only,
  * FP64, at most 32 foci) */
 void AFO_GradientDescent(void* handle, const void* foci, const float* amps, uint64_t size, int32_t source_type);
+/* capi/src/optimizer.rs:158-174, nls/apo.rs:100-158: sets amplitudes
+ * (amp |q_j| / camax q) and phases (FP64, at most 32 foci) */
+void AFO_APO(void* handle, const void* foci, const float* amps, uint64_t size, float lambda, int32_t source_type);
 /* Exported for binding compatibility only; each records "unsupported".
- * capi/src/optimizer.rs:49, :61, :130, :144, :158 */
+ * capi/src/optimizer.rs:49, :61, :130, :144 */
 void AFO_BesselBeam(void* handle, AFC_Vector3 point, AFC_Vector3 dir, float theta, int32_t source_type);
 void AFO_IFFT(void* handle, const char* path, AFC_Vector3 bottom_left, AFC_Vector3 top_left,
               AFC_Vector3 bottom_right, float spacing, float z, int32_t source_type);
 void AFO_Horn(void* handle, const void* foci, const float* amps, uint64_t size, int32_t source_type);
 void AFO_Long(void* handle, const void* foci, const float* amps, uint64_t size, int32_t source_type);
-void AFO_APO(void* handle, const void* foci, const float* amps, uint64_t size, float lambda, int32_t source_type);
 
 /* ============================================== part 2: no reference counterpart */
 
@@ -262,6 +264,17 @@ int32_t AFH_NLSolve(void* ctx, const float* packed_sources_dev, uint64_t n_packe
                     int32_t source_type, const float* foci_dev, const float* amps_dev, const uint32_t* offsets_host,
                     uint32_t batch, int32_t algo, uint32_t k_max, float* out_phase_dev, uint32_t* out_iters_dev,
                     void* stream);
+
+/* APO (nls/apo.rs:100-158) for `batch` instances, FP64, K_MAX 200 in the
+ * reference: writes the amplitude amp_j |q_j| / camax(q) and phase arg q_j
+ * of every source of instance b to out_amp_dev / out_phase_dev
+ * [b * n_sources ...], and the iteration count if out_iters_dev is not NULL.
+ * At most 32 foci per instance; the N x N quasi-Newton matrix takes 16 N^2
+ * bytes of workspace per instance. Same argument conventions as AFH_LMSolve. */
+int32_t AFH_APOSolve(void* ctx, const float* packed_sources_dev, uint64_t n_packed, uint64_t n_sources,
+                     int32_t source_type, const float* foci_dev, const float* amps_dev, const uint32_t* offsets_host,
+                     uint32_t batch, float lambda, uint32_t k_max, float* out_amp_dev, float* out_phase_dev,
+                     uint32_t* out_iters_dev, void* stream);
 
 /* ---- request server: field evaluations and GS-PAT solves, continuously
  * batched on one GPU (see the design notes in server.hip) */

@@ -24,6 +24,7 @@
  *   orc_lm                   multiple_foci/nls/lm.rs:40-105, nls/macros.rs:17-58
  *   orc_gauss_newton         multiple_foci/nls/gauss_newton.rs:35-89
  *   orc_gradient_descent     multiple_foci/nls/gradient_descent.rs:34-61
+ *   orc_apo                  multiple_foci/nls/apo.rs:36-158
  *   orc_focal_point_phases   acoustic-field-optimizer/src/focal_point.rs:28-39
  *   orc_grid_points          observe_area/grid/builder.rs:144-209, grid_area.rs:268-438
  */

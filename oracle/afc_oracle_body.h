@@ -737,6 +737,210 @@ uint32_t SFX(orc_gradient_descent)(int source_type, const float* sources, const 
     return it;
 }
 
+/* Complex dense solve A x = b (column-major n x n, destroyed; b overwritten)
+ * by Gaussian elimination with partial pivoting on |.|. */
+static int SFX(c_solve)(SFX(cplx) * A, uint64_t n, SFX(cplx) * b) {
+    uint64_t k, i, j;
+    for (k = 0; k < n; ++k) {
+        uint64_t piv = k;
+        R best = SFX(c_abs)(A[k + k * n]);
+        SFX(cplx) inv, t;
+        for (i = k + 1; i < n; ++i)
+            if (SFX(c_abs)(A[i + k * n]) > best) {
+                best = SFX(c_abs)(A[i + k * n]);
+                piv = i;
+            }
+        if (best == 0) return 0;
+        if (piv != k) {
+            for (j = 0; j < n; ++j) {
+                t = A[k + j * n];
+                A[k + j * n] = A[piv + j * n];
+                A[piv + j * n] = t;
+            }
+            t = b[k];
+            b[k] = b[piv];
+            b[piv] = t;
+        }
+        {
+            const R d = A[k + k * n].re * A[k + k * n].re + A[k + k * n].im * A[k + k * n].im;
+            inv = SFX(c_make)(A[k + k * n].re / d, -A[k + k * n].im / d);
+        }
+        for (i = k + 1; i < n; ++i) {
+            const SFX(cplx) f = SFX(c_mul)(A[i + k * n], inv);
+            if (f.re == 0 && f.im == 0) continue;
+            for (j = k; j < n; ++j) {
+                const SFX(cplx) pr = SFX(c_mul)(f, A[k + j * n]);
+                A[i + j * n] = SFX(c_make)(A[i + j * n].re - pr.re, A[i + j * n].im - pr.im);
+            }
+            {
+                const SFX(cplx) pr = SFX(c_mul)(f, b[k]);
+                b[i] = SFX(c_make)(b[i].re - pr.re, b[i].im - pr.im);
+            }
+        }
+    }
+    for (k = n; k-- > 0;) {
+        SFX(cplx) acc = b[k];
+        R d;
+        for (j = k + 1; j < n; ++j) {
+            const SFX(cplx) pr = SFX(c_mul)(A[k + j * n], b[j]);
+            acc = SFX(c_make)(acc.re - pr.re, acc.im - pr.im);
+        }
+        d = A[k + k * n].re * A[k + k * n].re + A[k + k * n].im * A[k + k * n].im;
+        b[k] = SFX(c_make)((acc.re * A[k + k * n].re + acc.im * A[k + k * n].im) / d,
+                           (acc.im * A[k + k * n].re - acc.re * A[k + k * n].im) / d);
+    }
+    return 1;
+}
+
+/* J(q) of apo.rs:45-53: sum_i (|g_i q|^2 - p_i^2)^2 + lambda |q . q| (the
+ * non-conjugated dot of nalgebra's dot()); g_i = row i of G (M x N). */
+static R SFX(apo_J)(const SFX(cplx) * G, uint64_t m, uint64_t n, const R* p2, const SFX(cplx) * q, R lambda) {
+    uint64_t i, j;
+    R acc = 0;
+    SFX(cplx) qq = SFX(c_make)(0, 0);
+    for (i = 0; i < m; ++i) {
+        SFX(cplx) gq = SFX(c_make)(0, 0);
+        R sdiff;
+        for (j = 0; j < n; ++j) gq = SFX(c_add)(gq, SFX(c_mul)(G[i + j * m], q[j]));
+        sdiff = gq.re * gq.re + gq.im * gq.im - p2[i];
+        acc += sdiff * sdiff;
+    }
+    for (j = 0; j < n; ++j) qq = SFX(c_add)(qq, SFX(c_mul)(q[j], q[j]));
+    return acc + SFX(c_abs)(qq) * lambda;
+}
+
+/* grad of apo.rs:56-70: sum_i (R_i q) (p_i^2 - |q^H R_i q|) + lambda q, with
+ * R_i = G^H D_i G, so R_i q = conj(g_i)^T (g_i q) and q^H R_i q = |g_i q|^2. */
+static void SFX(apo_grad)(const SFX(cplx) * G, uint64_t m, uint64_t n, const R* p2, const SFX(cplx) * q, R lambda,
+                          SFX(cplx) * out) {
+    uint64_t i, j;
+    for (j = 0; j < n; ++j) out[j] = SFX(c_make)(lambda * q[j].re, lambda * q[j].im);
+    for (i = 0; i < m; ++i) {
+        SFX(cplx) gq = SFX(c_make)(0, 0);
+        R w;
+        for (j = 0; j < n; ++j) gq = SFX(c_add)(gq, SFX(c_mul)(G[i + j * m], q[j]));
+        w = p2[i] - (gq.re * gq.re + gq.im * gq.im);
+        for (j = 0; j < n; ++j) {
+            const SFX(cplx) r = SFX(c_mul)(SFX(c_conj)(G[i + j * m]), gq);
+            out[j] = SFX(c_make)(out[j].re + r.re * w, out[j].im + r.im * w);
+        }
+    }
+}
+
+/* APO: nls/apo.rs:100-158 (Hasegawa et al. 2020). q0 = (G^H G + lambda I)^{-1}
+ * G^H p; quasi-Newton with H = I initially, d = -H grad, alpha the first
+ * minimiser of J(q + a d) over a = 0, 0.01, ..., 0.99, stop when |alpha d| <
+ * 1e-8; H += y y^T / (y . s) - (H s)(H s)^T / (s^T H s) with y = alpha d,
+ * s = grad_new - grad (transposes and dots without conjugation, as the
+ * reference writes them). Output amp_j <- amp_j |q_j| / camax(q), phase
+ * arg q_j, camax = max_j (|Re q_j| + |Im q_j|) (nalgebra's camax). Returns
+ * the iterations taken. */
+uint32_t SFX(orc_apo)(int source_type, const float* sources, const float* wavenums, const float* attens, uint64_t n,
+                      const float* foci, const float* amps, uint64_t m, R lambda, uint32_t k_max, R* out_amp,
+                      R* out_phase) {
+    const R EPS = (R)1e-8;
+    const uint64_t LS = 100;
+    const size_t st = orc_stride(source_type);
+    SFX(cplx)* G = SFX(make_G)(source_type, sources, wavenums, attens, n, foci, m);
+    SFX(cplx)* A = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * n * n);
+    SFX(cplx)* H = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * n * n);
+    SFX(cplx)* q = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * n);
+    SFX(cplx)* qn = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * n);
+    SFX(cplx)* d = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * n);
+    SFX(cplx)* gr = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * n);
+    SFX(cplx)* grn = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * n);
+    SFX(cplx)* sv = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * n);
+    SFX(cplx)* u = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * n);
+    R* p2 = (R*)malloc(sizeof(R) * m);
+    uint64_t i, j, k;
+    uint32_t it;
+    R cmax = 0;
+    for (i = 0; i < m; ++i) p2[i] = (R)amps[i] * (R)amps[i];
+    /* (G^H G + lambda I) q0 = G^H p */
+    for (j = 0; j < n; ++j) {
+        SFX(cplx) b = SFX(c_make)(0, 0);
+        for (k = 0; k < n; ++k) {
+            SFX(cplx) acc = SFX(c_make)(0, 0);
+            for (i = 0; i < m; ++i) acc = SFX(c_add)(acc, SFX(c_mul)(SFX(c_conj)(G[i + k * m]), G[i + j * m]));
+            if (k == j) acc.re += lambda;
+            A[k + j * n] = acc;
+        }
+        for (i = 0; i < m; ++i) b = SFX(c_add)(b, SFX(c_mul)(SFX(c_conj)(G[i + j * m]), SFX(c_make)((R)amps[i], 0)));
+        q[j] = b;
+    }
+    SFX(c_solve)(A, n, q);
+    for (j = 0; j < n; ++j)
+        for (k = 0; k < n; ++k) H[j + k * n] = SFX(c_make)(j == k ? (R)1 : (R)0, 0);
+    SFX(apo_grad)(G, m, n, p2, q, lambda, gr);
+    for (it = 0; it < k_max; ++it) {
+        R alpha = 0, best = (R)INFINITY, dn = 0;
+        uint64_t l;
+        SFX(cplx) ys = SFX(c_make)(0, 0), su = SFX(c_make)(0, 0);
+        for (j = 0; j < n; ++j) {
+            SFX(cplx) acc = SFX(c_make)(0, 0);
+            for (k = 0; k < n; ++k) acc = SFX(c_add)(acc, SFX(c_mul)(H[j + k * n], gr[k]));
+            d[j] = SFX(c_make)(-acc.re, -acc.im);
+        }
+        for (l = 0; l < LS; ++l) {
+            const R a = (R)l / (R)LS;
+            R v;
+            for (j = 0; j < n; ++j) qn[j] = SFX(c_make)(q[j].re + d[j].re * a, q[j].im + d[j].im * a);
+            v = SFX(apo_J)(G, m, n, p2, qn, lambda);
+            if (v < best) {
+                alpha = a;
+                best = v;
+            }
+        }
+        for (j = 0; j < n; ++j) d[j] = SFX(c_make)(d[j].re * alpha, d[j].im * alpha);
+        for (j = 0; j < n; ++j) dn += d[j].re * d[j].re + d[j].im * d[j].im;
+        if (RSQRT(dn) < EPS) break;
+        for (j = 0; j < n; ++j) qn[j] = SFX(c_add)(q[j], d[j]);
+        SFX(apo_grad)(G, m, n, p2, qn, lambda, grn);
+        for (j = 0; j < n; ++j) sv[j] = SFX(c_make)(grn[j].re - gr[j].re, grn[j].im - gr[j].im);
+        /* u = H s, y . s, s^T H s = s . u */
+        for (j = 0; j < n; ++j) {
+            SFX(cplx) acc = SFX(c_make)(0, 0);
+            for (k = 0; k < n; ++k) acc = SFX(c_add)(acc, SFX(c_mul)(H[j + k * n], sv[k]));
+            u[j] = acc;
+        }
+        for (j = 0; j < n; ++j) ys = SFX(c_add)(ys, SFX(c_mul)(d[j], sv[j]));
+        for (j = 0; j < n; ++j) su = SFX(c_add)(su, SFX(c_mul)(sv[j], u[j]));
+        {
+            const R dy = ys.re * ys.re + ys.im * ys.im, du = su.re * su.re + su.im * su.im;
+            const SFX(cplx) iy = SFX(c_make)(ys.re / dy, -ys.im / dy), iu = SFX(c_make)(su.re / du, -su.im / du);
+            for (k = 0; k < n; ++k)
+                for (j = 0; j < n; ++j) {
+                    const SFX(cplx) a1 = SFX(c_mul)(SFX(c_mul)(d[j], d[k]), iy);
+                    const SFX(cplx) a2 = SFX(c_mul)(SFX(c_mul)(u[j], u[k]), iu);
+                    H[j + k * n] = SFX(c_make)(H[j + k * n].re + a1.re - a2.re, H[j + k * n].im + a1.im - a2.im);
+                }
+        }
+        memcpy(q, qn, sizeof(SFX(cplx)) * n);
+        memcpy(gr, grn, sizeof(SFX(cplx)) * n);
+    }
+    for (j = 0; j < n; ++j) {
+        const R c1 = RABS(q[j].re) + RABS(q[j].im);
+        if (c1 > cmax) cmax = c1;
+    }
+    for (j = 0; j < n; ++j) {
+        const R amp = (source_type == 1) ? (R)sources[st * j + 6] : (R)sources[st * j + 3];
+        out_amp[j] = amp * SFX(c_abs)(q[j]) / cmax;
+        out_phase[j] = SFX(c_arg)(q[j]);
+    }
+    free(G);
+    free(A);
+    free(H);
+    free(q);
+    free(qn);
+    free(d);
+    free(gr);
+    free(grn);
+    free(sv);
+    free(u);
+    free(p2);
+    return it;
+}
+
 /* F(x) and g(x) of the LM objective for given x (nls/macros.rs:17-58), for
  * gradient checks: returns F, writes g (n + m). */
 R SFX(orc_lm_eval)(int source_type, const float* sources, const float* wavenums, const float* attens, uint64_t n,

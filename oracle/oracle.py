@@ -66,6 +66,9 @@ def lib():
                 f = getattr(L, f"{nm}_{sfx}")
                 f.restype = c_uint32
                 f.argtypes = [c_int, fp, fp, fp, c_uint64, fp, fp, c_uint64, c_uint32, fp, fp]
+            f = getattr(L, f"orc_apo_{sfx}")
+            f.restype = c_uint32
+            f.argtypes = [c_int, fp, fp, fp, c_uint64, fp, fp, c_uint64, R, c_uint32, fp, fp]
             f = getattr(L, f"orc_greedy_{sfx}")
             f.restype = None
             f.argtypes = [c_int, fp, fp, fp, c_uint64, fp, fp, c_uint64, c_uint64, fp]
@@ -223,6 +226,17 @@ def gauss_newton(source_type, sources, wavenums, attens, foci, amps, prec="f64",
 def gradient_descent(source_type, sources, wavenums, attens, foci, amps, prec="f64", k_max=10000):
     """Gradient descent (gradient_descent.rs:34-61). Returns (phases[N], x[N+M], steps)."""
     return _nls("orc_gradient_descent", source_type, sources, wavenums, attens, foci, amps, prec, k_max)
+
+
+def apo(source_type, sources, wavenums, attens, foci, amps, lam, prec="f64", k_max=200):
+    """APO (apo.rs:100-158). Returns (amp[N], phase[N], iterations)."""
+    dt, sfx = _dt(prec)
+    src, k, a, f, am = _f32(sources), _f32(wavenums), _f32(attens), _f32(foci).reshape(-1, 3), _f32(amps)
+    n, m = k.shape[0], f.shape[0]
+    amp, ph = np.empty(n, dt), np.empty(n, dt)
+    it = getattr(lib(), f"orc_apo_{sfx}")(source_type, _p(src), _p(k), _p(a), n, _p(f), _p(am), m, lam, k_max,
+                                          _p(amp), _p(ph))
+    return amp, ph, int(it)
 
 
 def lm_eval(source_type, sources, wavenums, attens, foci, amps, x):

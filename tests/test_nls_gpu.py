@@ -130,3 +130,75 @@ def test_nls_batch_equals_single(afc, oracle, torch_cuda):
             one, it1 = run_nls(torch, src, k, a, foci[offs[b]:offs[b + 1]], amps[offs[b]:offs[b + 1]],
                                np.array([0, ms[b]], np.uint32), algo, kmax)
             assert np.array_equal(one[0], ph[b]) and it1[0] == it[b], (algo, b)
+
+
+def run_apo(torch, src, k, a, foci, amps, offs, lam, k_max=200):
+    from afc import device as D
+    dev = torch.device("cuda:0")
+    packed = D.pack_sources(1, src, k, a)
+    src_d = torch.from_numpy(packed).to(dev)
+    foci_d = torch.from_numpy(np.ascontiguousarray(foci, np.float32)).to(dev)
+    amps_d = torch.from_numpy(np.ascontiguousarray(amps, np.float32)).to(dev)
+    B = offs.shape[0] - 1
+    n = src.shape[0]
+    oa = torch.empty((B, n), dtype=torch.float32, device=dev)
+    op = torch.empty((B, n), dtype=torch.float32, device=dev)
+    iters = torch.zeros(B, dtype=torch.int32, device=dev)
+    ctx = D.Context()
+    D.apo_solve(ctx, src_d.data_ptr(), packed.shape[0], n, 1, foci_d.data_ptr(), amps_d.data_ptr(), offs, lam, k_max,
+                oa.data_ptr(), op.data_ptr(), iters.data_ptr())
+    torch.cuda.synchronize()
+    ctx.close()
+    return oa.cpu().numpy(), op.cpu().numpy(), iters.cpu().numpy()
+
+
+@pytest.mark.parametrize("m,lam,seed", [(1, 0.01, 11), (3, 0.05, 12), (8, 1.0, 13)])
+def test_apo_vs_oracle(afc, oracle, torch_cuda, m, lam, seed):
+    """APO (nls/apo.rs:100-158) against the FP64 oracle: iteration counts
+    equal, amplitudes within 1e-5 relative, phases within 1e-5 rad where the
+    amplitude is above 1e-3 of the largest (arg of a vanishing q_j is noise)."""
+    log(f"case: APO M={m} lambda={lam}")
+    import torch
+    rng = np.random.default_rng(seed)
+    src, k, a = setup(oracle, 160)
+    foci, amps = problem(rng, m)
+    foci[:, 0] *= 0.6
+    amps *= 0.2
+    oa, op, it = run_apo(torch, src, k, a, foci, amps, np.array([0, m], np.uint32), lam)
+    ra, rp, rit = oracle.apo(1, src, k, a, foci, amps, lam, "f64")
+    big = ra > 1e-3 * ra.max()
+    ea = float(np.max(np.abs(oa[0] - ra) / ra.max()))
+    ep = float(wrapdiff(op[0][big], rp[big]).max())
+    log(f"  iterations gpu {it[0]} oracle {rit}; amp err {ea:.2e}, phase err {ep:.2e}")
+    assert it[0] == rit
+    assert ea <= 1e-5 and ep <= 1e-5
+
+
+def test_apo_capi_and_batch(afc, oracle, torch_cuda):
+    log("case: AFO_APO through the C ABI; batched APO equals one at a time")
+    import torch
+    rng = np.random.default_rng(21)
+    src, k, a = setup(oracle, 120)
+    ms = [2, 1, 4]
+    fs, am = zip(*[problem(rng, m) for m in ms])
+    foci, amps = np.concatenate(fs), np.concatenate(am) * 0.2
+    offs = np.concatenate([[0], np.cumsum(ms)]).astype(np.uint32)
+    oa, op, it = run_apo(torch, src, k, a, foci, amps, offs, 0.1)
+    for b in range(len(ms)):
+        a1, p1, i1 = run_apo(torch, src, k, a, foci[offs[b]:offs[b + 1]], amps[offs[b]:offs[b + 1]],
+                             np.array([0, ms[b]], np.uint32), 0.1)
+        assert np.array_equal(a1[0], oa[b]) and np.array_equal(p1[0], op[b]) and i1[0] == it[b]
+    s = afc.UniformSystem(oracle.TEMP_340)
+    for row in src:
+        s.add_wave_source(afc.T4010A1(row[0:3], row[3:6], row[6], row[7], row[8]))
+    f0, a0 = foci[:2], amps[:2]
+    afc.Optimizer.apo(s, f0, a0, 0.1)
+    got = s.source_array().copy()
+    import ctypes
+    kk = np.zeros(len(src), np.float32)
+    aa = np.zeros(len(src), np.float32)
+    afc.dll().AFH_GetSourceConstants(s.handle, kk.ctypes.data_as(ctypes.c_void_p), aa.ctypes.data_as(ctypes.c_void_p))
+    ra, rp, _ = oracle.apo(1, src, kk, aa, f0, a0, 0.1, "f64")
+    big = ra > 1e-3 * ra.max()
+    assert np.max(np.abs(got[:, 6] - ra) / ra.max()) <= 1e-5
+    assert wrapdiff(got[big, 7], rp[big]).max() <= 1e-5

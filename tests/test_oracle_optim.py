@@ -206,3 +206,30 @@ def test_gradient_descent_and_gauss_newton_lower_F(oracle):
     # remaining descent directions (F / F0 = 0.026 here)
     _, x, _ = oracle.gauss_newton(1, src, k, a, foci, amps, "f64")
     assert oracle.lm_eval(1, src, k, a, foci, amps, x)[0] < 0.1 * F0
+
+
+def test_apo_start_is_regularised_least_squares(oracle):
+    """With no iteration APO returns q0 = (G^H G + lambda I)^{-1} G^H p
+    (apo.rs:116-119), from numpy's LAPACK solve here, scaled by camax =
+    max_j |Re q_j| + |Im q_j| (nalgebra's camax)."""
+    src, k, a, foci, amps = small_problem(oracle, m=3, seed=12)
+    lam = 0.05
+    amp, ph, it = oracle.apo(1, src, k, a, foci, amps, lam, "f64", k_max=0)
+    assert it == 0
+    G = oracle.propagation_matrix(1, src, k, a, foci, "f64")
+    q0 = np.linalg.solve(G.conj().T @ G + lam * np.eye(G.shape[1]), G.conj().T @ amps.astype(np.float64))
+    cm = np.max(np.abs(q0.real) + np.abs(q0.imag))
+    assert np.allclose(amp, src[:, 6] * np.abs(q0) / cm, rtol=1e-7, atol=1e-12)
+    assert np.allclose(np.angle(np.exp(1j * (ph - np.angle(q0)))), 0, atol=1e-7)
+
+
+def test_apo_single_focus_stays_matched(oracle):
+    """M = 1: q0, the gradient and every update stay proportional to conj(g),
+    so the result focuses: phase_j = -arg G_j and amp_j proportional to |G_j|."""
+    src, k, a, _, _ = small_problem(oracle, m=1)
+    foci = np.array([[90, 70, 150]], np.float32)
+    amp, ph, it = oracle.apo(1, src, k, a, foci, np.array([0.3], np.float32), 0.01, "f64")
+    G = oracle.propagation_matrix(1, src, k, a, foci, "f64")[0]
+    assert np.allclose(np.angle(np.exp(1j * (ph + np.angle(G)))), 0, atol=1e-6)
+    r = amp / np.abs(G)
+    assert np.max(r) / np.min(r) - 1 < 1e-6
