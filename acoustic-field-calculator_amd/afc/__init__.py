@@ -414,6 +414,14 @@ class Optimizer:
         Optimizer._run("AFO_GradientDescent", system, foci, amps)
 
     @staticmethod
+    def horn(system: UniformSystem, foci, amps):
+        Optimizer._run("AFO_Horn", system, foci, amps)
+
+    @staticmethod
+    def long(system: UniformSystem, foci, amps):
+        Optimizer._run("AFO_Long", system, foci, amps)
+
+    @staticmethod
     def apo(system: UniformSystem, foci, amps, lambda_reg: float):
         size, f, a = _pack_foci(foci, amps)
         dll().AFO_APO(system.handle, f.ctypes.data_as(c_void_p), a.ctypes.data_as(POINTER(c_float)), c_ulong(size),

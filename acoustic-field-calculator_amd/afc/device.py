@@ -164,3 +164,27 @@ def apo_solve(ctx: Context, packed_dev_ptr: int, n_packed: int, n_sources: int, 
                             offs.shape[0] - 1, lam, k_max, c_void_p(out_amp_dev_ptr), c_void_p(out_phase_dev_ptr),
                             c_void_p(out_iters_dev_ptr or None), c_void_p(stream_ptr))
     _err("AFH_APOSolve", rc)
+
+
+def long_solve(ctx: Context, packed_dev_ptr: int, n_packed: int, n_sources: int, source_type: int, foci_dev_ptr: int,
+               amps_dev_ptr: int, offsets: np.ndarray, out_amp_dev_ptr: int, out_phase_dev_ptr: int,
+               stream_ptr: int = 0) -> None:
+    """Batched FP64 Long (amplitudes and phases) on device arrays."""
+    offs = np.ascontiguousarray(offsets, np.uint32)
+    rc = dll().AFH_LongSolve(ctx.handle, c_void_p(packed_dev_ptr), n_packed, n_sources, source_type,
+                             c_void_p(foci_dev_ptr), c_void_p(amps_dev_ptr), offs.ctypes.data_as(c_void_p),
+                             offs.shape[0] - 1, c_void_p(out_amp_dev_ptr), c_void_p(out_phase_dev_ptr),
+                             c_void_p(stream_ptr))
+    _err("AFH_LongSolve", rc)
+
+
+def horn_solve(ctx: Context, packed_dev_ptr: int, n_packed: int, n_sources: int, source_type: int, foci_dev_ptr: int,
+               amps_dev_ptr: int, offsets: np.ndarray, out_amp_dev_ptr: int, out_phase_dev_ptr: int, seed: int = 0,
+               repeat: int = 100, lam: float = 0.8, tikhonov: float = 1e-5, stream_ptr: int = 0) -> None:
+    """Batched FP64 Horn (amplitudes and phases) on device arrays; instance b uses seed + b."""
+    offs = np.ascontiguousarray(offsets, np.uint32)
+    rc = dll().AFH_HornSolve(ctx.handle, c_void_p(packed_dev_ptr), n_packed, n_sources, source_type,
+                             c_void_p(foci_dev_ptr), c_void_p(amps_dev_ptr), offs.ctypes.data_as(c_void_p),
+                             offs.shape[0] - 1, seed, repeat, lam, tikhonov, c_void_p(out_amp_dev_ptr),
+                             c_void_p(out_phase_dev_ptr), c_void_p(stream_ptr))
+    _err("AFH_HornSolve", rc)

@@ -167,6 +167,11 @@ def _declare(d):
                             c_uint32, ctypes.c_int32, c_uint32, c_void_p, c_void_p, c_void_p], ctypes.c_int32)
     _sig(d, "AFH_APOSolve", [c_void_p, c_void_p, c_uint64, c_uint64, ctypes.c_int32, c_void_p, c_void_p, c_void_p,
                              c_uint32, c_float, c_uint32, c_void_p, c_void_p, c_void_p, c_void_p], ctypes.c_int32)
+    _sig(d, "AFH_LongSolve", [c_void_p, c_void_p, c_uint64, c_uint64, ctypes.c_int32, c_void_p, c_void_p, c_void_p,
+                              c_uint32, c_void_p, c_void_p, c_void_p], ctypes.c_int32)
+    _sig(d, "AFH_HornSolve", [c_void_p, c_void_p, c_uint64, c_uint64, ctypes.c_int32, c_void_p, c_void_p, c_void_p,
+                              c_uint32, c_uint64, c_uint32, c_float, c_float, c_void_p, c_void_p, c_void_p],
+         ctypes.c_int32)
     _sig(d, "AFH_GreedySolve", [c_void_p, c_void_p, c_uint64, c_uint64, ctypes.c_int32, c_void_p, c_void_p, c_void_p,
                                 c_uint32, c_uint32, c_void_p, c_void_p], ctypes.c_int32)
     _sig(d, "AFS_Create", [POINTER(AFS_Config), POINTER(c_void_p)], ctypes.c_int32)

@@ -1,4 +1,5 @@
-// Hologram solvers (GS, GS-PAT, Naive, LM) on the GPU.
+// Hologram solvers (GS, GS-PAT, Naive, Greedy, LM, Gauss-Newton, gradient
+// descent, APO, Long, Horn) on the GPU.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -14,6 +15,9 @@ enum HoloAlgo { kAlgoGSPAT = 0, kAlgoGS = 1, kAlgoNaive = 2, kAlgoLM = 3, kAlgoG
 
 // Nonlinear least-squares solvers of lm.hip (= AFH_NLS_* of the C ABI).
 enum NlsAlgo { kNlsLM = 0, kNlsGN = 1, kNlsGD = 2 };
+
+// Matrix-method solvers of matrix.hip.
+enum MatMethod { kMatLong = 0, kMatHorn = 1 };
 
 // Greedy limits: foci per instance and phase candidates.
 constexpr uint32_t kGreedyMaxM = 128;
@@ -89,5 +93,14 @@ int32_t apo_solve_batch(Context* ctx, const float* sources, uint32_t n, int sour
                         const float* amps, const uint32_t* offsets_host, uint32_t batch, double lambda,
                         uint32_t k_max, float* out_amp, float* out_phase, uint32_t* out_iters, hipStream_t st);
 bool apo_solve_single(const HoloProblem& p, float lambda, float* amp_out, float* phase_out);
+// Long (long.rs:46-106) or Horn (horn.rs:58-129; instance b draws its rows
+// from seed + b): amplitudes and phases of the n sources; M <= 32 per instance.
+int32_t matrix_solve_batch(Context* ctx, const float* sources, uint32_t n, int source_type, const float* foci,
+                           const float* amps, const uint32_t* offsets_host, uint32_t batch, int method,
+                           uint64_t seed, uint32_t repeat, double lambda, double alpha, float* out_amp,
+                           float* out_phase, hipStream_t st);
+// One instance with the reference's constants (REPEAT 100, lambda 0.8,
+// Tikhonov 1e-5) and seed 0.
+bool matrix_solve_single(const HoloProblem& p, int method, float* amp_out, float* phase_out);
 
 }  // namespace afh

@@ -1,8 +1,9 @@
 // Optimizer exports of the C ABI (capi/src/optimizer.rs).
 //
-// GS, GS-PAT, Naive and LM run on the GPU (holo.hip / lm.hip). The other
-// optimizers of the reference are outside this backend: they record
-// "unsupported" and leave the system untouched.
+// GS, GS-PAT, Naive, Greedy, LM, Gauss-Newton, gradient descent, APO, Long
+// and Horn run on the GPU (holo.hip / lm.hip / matrix.hip). BesselBeam and
+// IFFT are outside this backend: they record "unsupported" and leave the
+// system untouched.
 #include <math.h>
 #include <string.h>
 
@@ -69,6 +70,40 @@ void run_single(void* handle, const void* foci, const float* amps, uint64_t size
     }
 }
 
+// Front end of the solvers that set amplitudes and phases and are not part of
+// holo_solve_single (APO, Long, Horn): solve, then write both back.
+template <class Solve>
+void run_amp_phase(void* handle, const void* foci, const float* amps, uint64_t size, int32_t source_type,
+                   const char* who, Solve solve) {
+    clear_error();
+    System* s = as_system(handle, source_type, who);
+    if (!s) return;
+    if (size == 0 || !foci || !amps) {
+        set_error(std::string(who) + ": no foci");
+        return;
+    }
+    const uint64_t n = s->count();
+    if (n == 0) return;
+    std::vector<float> amp_out(n), phase_out(n);
+    HoloProblem p;
+    p.source_type = source_type;
+    p.sources = s->sources.data();
+    p.wavenums = s->wavenums.data();
+    p.attens = s->attens.data();
+    p.n = n;
+    p.foci = static_cast<const float*>(foci);
+    p.amps = amps;
+    p.m = size;
+    if (!solve(p, amp_out.data(), phase_out.data())) return;
+    const size_t st = s->stride();
+    const size_t ia = source_type == kT4010A1 ? 6 : 3;
+    for (uint64_t j = 0; j < n; ++j) {
+        float* src = s->sources.data() + st * j;
+        src[ia] = amp_out[j];
+        src[ia + 1] = phase_out[j];
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -107,37 +142,23 @@ void AFO_BesselBeam(void*, AFC_Vector3, AFC_Vector3, float, int32_t) { unsupport
 void AFO_IFFT(void*, const char*, AFC_Vector3, AFC_Vector3, AFC_Vector3, float, float, int32_t) {
     unsupported("AFO_IFFT");
 }
-void AFO_Horn(void*, const void*, const float*, uint64_t, int32_t) { unsupported("AFO_Horn"); }
-void AFO_Long(void*, const void*, const float*, uint64_t, int32_t) { unsupported("AFO_Long"); }
+// matrix/horn.rs:70-129 (capi/src/optimizer.rs:130-141): the reference's
+// constants; the random rows come from seed 0 (the reference's thread_rng is
+// unseeded)
+void AFO_Horn(void* handle, const void* foci, const float* amps, uint64_t size, int32_t source_type) {
+    run_amp_phase(handle, foci, amps, size, source_type, "AFO_Horn",
+                  [](const HoloProblem& p, float* am, float* ph) { return matrix_solve_single(p, kMatHorn, am, ph); });
+}
+// matrix/long.rs:46-106 (capi/src/optimizer.rs:144-155)
+void AFO_Long(void* handle, const void* foci, const float* amps, uint64_t size, int32_t source_type) {
+    run_amp_phase(handle, foci, amps, size, source_type, "AFO_Long",
+                  [](const HoloProblem& p, float* am, float* ph) { return matrix_solve_single(p, kMatLong, am, ph); });
+}
 // nls/apo.rs:100-158 (capi/src/optimizer.rs:158-174): sets amplitudes and phases
 void AFO_APO(void* handle, const void* foci, const float* amps, uint64_t size, float lambda, int32_t source_type) {
-    clear_error();
-    System* s = as_system(handle, source_type, "AFO_APO");
-    if (!s) return;
-    if (size == 0 || !foci || !amps) {
-        set_error("AFO_APO: no foci");
-        return;
-    }
-    const uint64_t n = s->count();
-    if (n == 0) return;
-    std::vector<float> amp_out(n), phase_out(n);
-    HoloProblem p;
-    p.source_type = source_type;
-    p.sources = s->sources.data();
-    p.wavenums = s->wavenums.data();
-    p.attens = s->attens.data();
-    p.n = n;
-    p.foci = static_cast<const float*>(foci);
-    p.amps = amps;
-    p.m = size;
-    if (!apo_solve_single(p, lambda, amp_out.data(), phase_out.data())) return;
-    const size_t st = s->stride();
-    const size_t ia = source_type == kT4010A1 ? 6 : 3;
-    for (uint64_t j = 0; j < n; ++j) {
-        float* src = s->sources.data() + st * j;
-        src[ia] = amp_out[j];
-        src[ia + 1] = phase_out[j];
-    }
+    run_amp_phase(handle, foci, amps, size, source_type, "AFO_APO", [lambda](const HoloProblem& p, float* am, float* ph) {
+        return apo_solve_single(p, lambda, am, ph);
+    });
 }
 // nls/gauss_newton.rs:35-89 (capi/src/optimizer.rs:177-192)
 void AFO_GaussNewton(void* handle, const void* foci, const float* amps, uint64_t size, int32_t source_type) {

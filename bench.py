@@ -456,6 +456,24 @@ def bench_lm(torch, D, O):
             out["apo_n249"] = {"solves_per_s": round(64 / (ams * 1e-3), 1), "batch_ms": round(ams, 3), "k_max": 200,
                                "iterations_mean": float(it_d.cpu().numpy().mean()), "lambda": 0.1}
             log(f"apo_n249: {ams:.3f} ms per batch of 64")
+            # Long (long.rs:46-106) and Horn (horn.rs:58-129, REPEAT 100,
+            # lambda 0.8, Tikhonov 1e-5) on the same 64 instances
+            for key, fn in (("long_n249", lambda: D.long_solve(
+                    ctx, src_d.data_ptr(), packed.shape[0], n, 1, foci_d.data_ptr(), amps_d.data_ptr(), offs,
+                    oa_d.data_ptr(), ph_d.data_ptr(), stream.cuda_stream)),
+                            ("horn_n249", lambda: D.horn_solve(
+                    ctx, src_d.data_ptr(), packed.shape[0], n, 1, foci_d.data_ptr(), amps_d.data_ptr(), offs,
+                    oa_d.data_ptr(), ph_d.data_ptr(), stream_ptr=stream.cuda_stream))):
+                fn()
+                torch.cuda.synchronize()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                fn()
+                e1.record(stream)
+                torch.cuda.synchronize()
+                xms = e0.elapsed_time(e1)
+                out[key] = {"solves_per_s": round(64 / (xms * 1e-3), 1), "batch_ms": round(xms, 3)}
+                log(f"{key}: {xms:.3f} ms per batch of 64")
         ctx.close()
         out[name] = entry
         log(f"lm {name}: {ms:.3f} ms per batch of 64, {64 / (ms * 1e-3):.0f} solves/s")

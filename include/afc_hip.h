@@ -141,13 +141,21 @@ void AFO_GradientDescent(void* handle, const void* foci, const float* amps, uint
 /* capi/src/optimizer.rs:158-174, nls/apo.rs:100-158: sets amplitudes
  * (amp |q_j| / camax q) and phases (FP64, at most 32 foci) */
 void AFO_APO(void* handle, const void* foci, const float* amps, uint64_t size, float lambda, int32_t source_type);
+/* capi/src/optimizer.rs:130-141, matrix/horn.rs:70-129: sets amplitudes
+ * and phases (FP64, at most 32 foci); REPEAT 100, lambda 0.8, Tikhonov 1e-5
+ * as in the reference, rows drawn from a splitmix64 stream with seed 0 (the
+ * reference's thread_rng is unseeded) */
+void AFO_Horn(void* handle, const void* foci, const float* amps, uint64_t size, int32_t source_type);
+/* capi/src/optimizer.rs:144-155, matrix/long.rs:46-106: sets amplitudes and
+ * phases (FP64, at most 32 foci). The eigenvector matrix whose row imax the
+ * reference reads is taken with ascending eigenvalues and each vector's
+ * largest-modulus entry real positive (nalgebra fixes neither). */
+void AFO_Long(void* handle, const void* foci, const float* amps, uint64_t size, int32_t source_type);
 /* Exported for binding compatibility only; each records "unsupported".
- * capi/src/optimizer.rs:49, :61, :130, :144 */
+ * capi/src/optimizer.rs:49, :61 */
 void AFO_BesselBeam(void* handle, AFC_Vector3 point, AFC_Vector3 dir, float theta, int32_t source_type);
 void AFO_IFFT(void* handle, const char* path, AFC_Vector3 bottom_left, AFC_Vector3 top_left,
               AFC_Vector3 bottom_right, float spacing, float z, int32_t source_type);
-void AFO_Horn(void* handle, const void* foci, const float* amps, uint64_t size, int32_t source_type);
-void AFO_Long(void* handle, const void* foci, const float* amps, uint64_t size, int32_t source_type);
 
 /* ============================================== part 2: no reference counterpart */
 
@@ -275,6 +283,23 @@ int32_t AFH_APOSolve(void* ctx, const float* packed_sources_dev, uint64_t n_pack
                      int32_t source_type, const float* foci_dev, const float* amps_dev, const uint32_t* offsets_host,
                      uint32_t batch, float lambda, uint32_t k_max, float* out_amp_dev, float* out_phase_dev,
                      uint32_t* out_iters_dev, void* stream);
+
+/* Long (matrix/long.rs:46-106) for `batch` instances, FP64: amplitudes and
+ * phases of every source of instance b into out_amp_dev / out_phase_dev
+ * [b * n_sources ...]. At most 32 foci per instance. Same argument
+ * conventions as AFH_LMSolve. */
+int32_t AFH_LongSolve(void* ctx, const float* packed_sources_dev, uint64_t n_packed, uint64_t n_sources,
+                      int32_t source_type, const float* foci_dev, const float* amps_dev, const uint32_t* offsets_host,
+                      uint32_t batch, float* out_amp_dev, float* out_phase_dev, void* stream);
+
+/* Horn (matrix/horn.rs:58-129) for `batch` instances, FP64, with the
+ * reference's parameters as arguments (REPEAT_SDP 100, LAMBDA_SDP 0.8,
+ * TIKHONOV_DEFAULT 1e-5 there): instance b draws its rows from the
+ * splitmix64 stream seeded with seed + b. Outputs as AFH_LongSolve. */
+int32_t AFH_HornSolve(void* ctx, const float* packed_sources_dev, uint64_t n_packed, uint64_t n_sources,
+                      int32_t source_type, const float* foci_dev, const float* amps_dev, const uint32_t* offsets_host,
+                      uint32_t batch, uint64_t seed, uint32_t repeat, float lambda, float tikhonov,
+                      float* out_amp_dev, float* out_phase_dev, void* stream);
 
 /* ---- request server: field evaluations and GS-PAT solves, continuously
  * batched on one GPU (see the design notes in server.hip) */

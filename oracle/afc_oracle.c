@@ -25,6 +25,9 @@
  *   orc_gauss_newton         multiple_foci/nls/gauss_newton.rs:35-89
  *   orc_gradient_descent     multiple_foci/nls/gradient_descent.rs:34-61
  *   orc_apo                  multiple_foci/nls/apo.rs:36-158
+ *   orc_long                 multiple_foci/matrix/long.rs:46-106
+ *   orc_horn                 multiple_foci/matrix/horn.rs:58-129 (seeded row choice)
+ *   orc_herm_eig             the Hermitian eigen-decomposition both use (Jacobi)
  *   orc_focal_point_phases   acoustic-field-optimizer/src/focal_point.rs:28-39
  *   orc_grid_points          observe_area/grid/builder.rs:144-209, grid_area.rs:268-438
  */

@@ -941,6 +941,313 @@ uint32_t SFX(orc_apo)(int source_type, const float* sources, const float* wavenu
     return it;
 }
 
+/* Eigen-decomposition of a Hermitian matrix (column-major n x n, the full
+ * matrix given, destroyed) by cyclic complex Jacobi rotations: for each pair
+ * (p, q) the off-diagonal entry a_pq = |a_pq| e^{i phi} is made real by
+ * scaling column q by e^{-i phi} (row q by e^{i phi}), then a real rotation
+ * zeroes it. evals[k] real, eigenvectors in the columns of V. Sweeps stop when
+ * the off-diagonal mass is below 1e-30 of the diagonal's. Output convention
+ * (nalgebra's SymmetricEigen, which the reference uses, fixes neither order
+ * nor phases in a documented way): eigenvalues ascending, and each
+ * eigenvector's largest-modulus entry real positive. */
+void SFX(orc_herm_eig)(SFX(cplx) * A, uint64_t n, R* evals, SFX(cplx) * V) {
+    uint64_t p, q, k, sweep;
+    for (p = 0; p < n; ++p)
+        for (q = 0; q < n; ++q) V[p + q * n] = SFX(c_make)(p == q ? (R)1 : (R)0, 0);
+    for (sweep = 0; sweep < 100; ++sweep) {
+        R off = 0, dg = 0;
+        for (q = 0; q < n; ++q)
+            for (p = 0; p < n; ++p) {
+                const R m2 = A[p + q * n].re * A[p + q * n].re + A[p + q * n].im * A[p + q * n].im;
+                if (p != q) off += m2;
+                else dg += m2;
+            }
+        if (off <= dg * (R)1e-30 || off == 0) break;
+        for (p = 0; p + 1 < n; ++p)
+            for (q = p + 1; q < n; ++q) {
+                const SFX(cplx) apq = A[p + q * n];
+                const R b = SFX(c_abs)(apq);
+                R theta, t, c, sn;
+                SFX(cplx) ph, phc;
+                if (b == 0) continue;
+                ph = SFX(c_make)(apq.re / b, apq.im / b); /* e^{i phi} */
+                phc = SFX(c_conj)(ph);
+                for (k = 0; k < n; ++k) A[k + q * n] = SFX(c_mul)(A[k + q * n], phc);
+                for (k = 0; k < n; ++k) A[q + k * n] = SFX(c_mul)(A[q + k * n], ph);
+                for (k = 0; k < n; ++k) V[k + q * n] = SFX(c_mul)(V[k + q * n], phc);
+                theta = (A[q + q * n].re - A[p + p * n].re) / ((R)2 * b);
+                t = (theta >= 0 ? (R)1 : (R)-1) / (RABS(theta) + RSQRT(theta * theta + (R)1));
+                c = (R)1 / RSQRT(t * t + (R)1);
+                sn = t * c;
+                for (k = 0; k < n; ++k) {
+                    const SFX(cplx) akp = A[k + p * n], akq = A[k + q * n];
+                    A[k + p * n] = SFX(c_make)(c * akp.re - sn * akq.re, c * akp.im - sn * akq.im);
+                    A[k + q * n] = SFX(c_make)(sn * akp.re + c * akq.re, sn * akp.im + c * akq.im);
+                }
+                for (k = 0; k < n; ++k) {
+                    const SFX(cplx) apk = A[p + k * n], aqk = A[q + k * n];
+                    A[p + k * n] = SFX(c_make)(c * apk.re - sn * aqk.re, c * apk.im - sn * aqk.im);
+                    A[q + k * n] = SFX(c_make)(sn * apk.re + c * aqk.re, sn * apk.im + c * aqk.im);
+                }
+                for (k = 0; k < n; ++k) {
+                    const SFX(cplx) vkp = V[k + p * n], vkq = V[k + q * n];
+                    V[k + p * n] = SFX(c_make)(c * vkp.re - sn * vkq.re, c * vkp.im - sn * vkq.im);
+                    V[k + q * n] = SFX(c_make)(sn * vkp.re + c * vkq.re, sn * vkp.im + c * vkq.im);
+                }
+            }
+    }
+    /* ascending eigenvalues (stable for ties), each eigenvector scaled so
+     * that its largest-modulus entry (the first on ties) is real positive;
+     * A is reused for the permuted vectors */
+    R* dg = (R*)malloc(sizeof(R) * (n ? n : 1));
+    for (k = 0; k < n; ++k) dg[k] = A[k + k * n].re;
+    for (k = 0; k < n; ++k) {
+        const R ek = dg[k];
+        uint64_t rank = 0, j, jm = 0;
+        R best = -1;
+        SFX(cplx) ph;
+        for (j = 0; j < n; ++j) {
+            const R ej = dg[j];
+            if (ej < ek || (ej == ek && j < k)) ++rank;
+        }
+        for (j = 0; j < n; ++j) {
+            const R a2 = V[j + k * n].re * V[j + k * n].re + V[j + k * n].im * V[j + k * n].im;
+            if (a2 > best) {
+                best = a2;
+                jm = j;
+            }
+        }
+        ph = SFX(c_conj)(SFX(c_divr)(V[jm + k * n], RSQRT(best)));
+        evals[rank] = ek;
+        for (j = 0; j < n; ++j) A[j + rank * n] = SFX(c_mul)(V[j + k * n], ph);
+        A[jm + rank * n].im = 0;
+    }
+    memcpy(V, A, sizeof(SFX(cplx)) * n * n);
+    free(dg);
+}
+
+/* Index of the largest entry (the first one on ties), as nalgebra's imax. */
+static uint64_t SFX(imax)(const R* v, uint64_t n) {
+    uint64_t k, best = 0;
+    for (k = 1; k < n; ++k)
+        if (v[k] > v[best]) best = k;
+    return best;
+}
+
+/* amp_j |q_j| / camax(q), arg q_j into the outputs (camax: nalgebra's
+ * max_j |Re q_j| + |Im q_j|). */
+static void SFX(write_amp_phase)(int source_type, const float* sources, uint64_t n, const SFX(cplx) * q, R* out_amp,
+                                 R* out_phase) {
+    const size_t st = orc_stride(source_type);
+    uint64_t j;
+    R cmax = 0;
+    for (j = 0; j < n; ++j) {
+        const R c1 = RABS(q[j].re) + RABS(q[j].im);
+        if (c1 > cmax) cmax = c1;
+    }
+    for (j = 0; j < n; ++j) {
+        const R amp = (source_type == 1) ? (R)sources[st * j + 6] : (R)sources[st * j + 3];
+        out_amp[j] = amp * SFX(c_abs)(q[j]) / cmax;
+        out_phase[j] = SFX(c_arg)(q[j]);
+    }
+}
+
+/* Long: matrix/long.rs:46-106 (Long et al. 2014). denomi = G.column_sum()
+ * (row sums, length M); X = (amps_i conj(G_ij) / denomi_i)^T; R = G X; the
+ * Hermitian matrix read from R's lower triangle (SymmetricEigen reads only
+ * that triangle; the diagonal's real part) is decomposed; e_arg_k =
+ * arg(U[imax][k]) -- row imax of the eigenvector matrix, as written;
+ * sigma_j = sqrt(sum_i |G_ij| amps_i / M); q solves (g^H g) q = g^H f with
+ * g = [G; diag(sigma)], f = [amps e^{i e_arg}; 0] (dense complex solve). */
+void SFX(orc_long)(int source_type, const float* sources, const float* wavenums, const float* attens, uint64_t n,
+                   const float* foci, const float* amps, uint64_t m, R* out_amp, R* out_phase) {
+    SFX(cplx)* G = SFX(make_G)(source_type, sources, wavenums, attens, n, foci, m);
+    SFX(cplx)* den = (SFX(cplx)*)calloc(m, sizeof(SFX(cplx)));
+    SFX(cplx)* Rm = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * m * m);
+    SFX(cplx)* U = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * m * m);
+    R* ev = (R*)malloc(sizeof(R) * m);
+    R* sig2 = (R*)malloc(sizeof(R) * n);
+    SFX(cplx)* A = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * n * n);
+    SFX(cplx)* q = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * n);
+    SFX(cplx)* f = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * m);
+    uint64_t i, j, k, im;
+    for (j = 0; j < n; ++j)
+        for (i = 0; i < m; ++i) den[i] = SFX(c_add)(den[i], G[i + j * m]);
+    for (k = 0; k < m; ++k)
+        for (i = 0; i < m; ++i) {
+            /* R_ik = sum_j G_ij X_jk, X_jk = amps_k conj(G_kj) / denomi_k */
+            SFX(cplx) acc = SFX(c_make)(0, 0);
+            const R dd = den[k].re * den[k].re + den[k].im * den[k].im;
+            const SFX(cplx) inv = SFX(c_make)(den[k].re / dd, -den[k].im / dd);
+            for (j = 0; j < n; ++j) {
+                const SFX(cplx) x = SFX(c_mul)(SFX(c_mul)(SFX(c_make)((R)amps[k], 0), SFX(c_conj)(G[k + j * m])), inv);
+                acc = SFX(c_add)(acc, SFX(c_mul)(G[i + j * m], x));
+            }
+            Rm[i + k * m] = acc;
+        }
+    /* Hermitian matrix from the lower triangle */
+    for (k = 0; k < m; ++k)
+        for (i = 0; i < m; ++i) {
+            if (i > k) U[i + k * m] = Rm[i + k * m];
+            else if (i < k) U[i + k * m] = SFX(c_conj)(Rm[k + i * m]);
+            else U[i + k * m] = SFX(c_make)(Rm[i + i * m].re, 0);
+        }
+    memcpy(Rm, U, sizeof(SFX(cplx)) * m * m);
+    SFX(orc_herm_eig)(Rm, m, ev, U);
+    im = SFX(imax)(ev, m);
+    for (k = 0; k < m; ++k) {
+        const R e = SFX(c_arg)(U[im + k * m]);
+        f[k] = SFX(c_make)((R)amps[k] * RCOS(e), (R)amps[k] * RSIN(e));
+    }
+    for (j = 0; j < n; ++j) {
+        R acc = 0;
+        for (i = 0; i < m; ++i) acc += SFX(c_abs)(G[i + j * m]) * (R)amps[i];
+        sig2[j] = acc / (R)m; /* sigma_j^2 */
+    }
+    for (j = 0; j < n; ++j) {
+        SFX(cplx) b = SFX(c_make)(0, 0);
+        for (k = 0; k < n; ++k) {
+            SFX(cplx) acc = SFX(c_make)(0, 0);
+            for (i = 0; i < m; ++i) acc = SFX(c_add)(acc, SFX(c_mul)(SFX(c_conj)(G[i + k * m]), G[i + j * m]));
+            if (k == j) acc.re += sig2[j];
+            A[k + j * n] = acc;
+        }
+        for (i = 0; i < m; ++i) b = SFX(c_add)(b, SFX(c_mul)(SFX(c_conj)(G[i + j * m]), f[i]));
+        q[j] = b;
+    }
+    SFX(c_solve)(A, n, q);
+    SFX(write_amp_phase)(source_type, sources, n, q, out_amp, out_phase);
+    free(G);
+    free(den);
+    free(Rm);
+    free(U);
+    free(ev);
+    free(sig2);
+    free(A);
+    free(q);
+    free(f);
+}
+
+/* The random row choice of Horn: splitmix64 over (seed, instance), a value
+ * in [0, 1) with 24 bits, ii = (uint)(M * r) in f32 as horn.rs:93 writes it.
+ * (The reference draws from thread_rng, unseeded.) */
+#ifndef ORC_SPLITMIX_DEFINED
+#define ORC_SPLITMIX_DEFINED
+static uint64_t orc_splitmix(uint64_t* s) {
+    uint64_t z = (*s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+#endif
+
+/* Horn: matrix/horn.rs:58-129 (Inoue et al. 2015); the reference's
+ * constants are REPEAT 100, lambda 0.8, Tikhonov alpha 1e-5. With the
+ * thin SVD G = U S V^H, G_pinv = V diag(s / (s^2 + a^2)) U^H equals
+ * G^H K^{-1} with K = G G^H + a^2 I (push-through), and so
+ * I - G G_pinv = a^2 K^{-1} exactly; both are evaluated in that form (in
+ * f32 the reference's I - G G_pinv cancels to rounding noise at a = 1e-5).
+ * MM = P a^2 K^{-1} P; m * repeat random single-row updates of X (the row
+ * drawn by orc_splitmix from the seed, as rand's [0, 1) f32 scaled by m;
+ * the reference draws from thread_rng, unseeded); q = G_pinv P u with u the
+ * eigenvector of X's largest eigenvalue. */
+void SFX(orc_horn)(int source_type, const float* sources, const float* wavenums, const float* attens, uint64_t n,
+                   const float* foci, const float* amps, uint64_t m, uint64_t seed, uint64_t repeat, R lambda,
+                   R alpha, R* out_amp, R* out_phase) {
+    SFX(cplx)* G = SFX(make_G)(source_type, sources, wavenums, attens, n, foci, m);
+    SFX(cplx)* K = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * m * m);
+    SFX(cplx)* Ki = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * m * m);
+    SFX(cplx)* col = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * m);
+    SFX(cplx)* MM = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * m * m);
+    SFX(cplx)* X = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * m * m);
+    SFX(cplx)* U = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * m * m);
+    SFX(cplx)* xb = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * m);
+    SFX(cplx)* z = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * m);
+    SFX(cplx)* q = (SFX(cplx)*)malloc(sizeof(SFX(cplx)) * n);
+    R* ev = (R*)malloc(sizeof(R) * m);
+    uint64_t i, j, k, r, it, im;
+    uint64_t rs = seed;
+    /* K^{-1} column by column (K = G G^H + a^2 I is destroyed by each solve) */
+    for (k = 0; k < m; ++k) {
+        for (j = 0; j < m; ++j)
+            for (i = 0; i < m; ++i) {
+                SFX(cplx) acc = SFX(c_make)(0, 0);
+                uint64_t l;
+                for (l = 0; l < n; ++l) acc = SFX(c_add)(acc, SFX(c_mul)(G[i + l * m], SFX(c_conj)(G[j + l * m])));
+                if (i == j) acc.re += alpha * alpha;
+                K[i + j * m] = acc;
+            }
+        for (i = 0; i < m; ++i) col[i] = SFX(c_make)(i == k ? (R)1 : (R)0, 0);
+        SFX(c_solve)(K, m, col);
+        for (i = 0; i < m; ++i) Ki[i + k * m] = col[i];
+    }
+    /* MM = P (I - G G_pinv) P = a^2 P K^{-1} P */
+    for (k = 0; k < m; ++k)
+        for (i = 0; i < m; ++i) {
+            const R sc = alpha * alpha * (R)amps[i] * (R)amps[k];
+            MM[i + k * m] = SFX(c_make)(Ki[i + k * m].re * sc, Ki[i + k * m].im * sc);
+        }
+    for (k = 0; k < m; ++k)
+        for (i = 0; i < m; ++i) X[i + k * m] = SFX(c_make)(i == k ? (R)1 : (R)0, 0);
+    for (it = 0; it < m * repeat; ++it) {
+        const float rnd = (float)(orc_splitmix(&rs) >> 40) * (1.0f / 16777216.0f);
+        uint64_t ii = (uint64_t)((float)m * rnd);
+        SFX(cplx) gamma = SFX(c_make)(0, 0);
+        if (ii > m - 1) ii = m - 1;
+        /* Xb = Xc MMc over the rows / columns other than ii */
+        for (r = 0; r < m; ++r) {
+            SFX(cplx) acc = SFX(c_make)(0, 0);
+            if (r == ii) continue;
+            for (k = 0; k < m; ++k)
+                if (k != ii) acc = SFX(c_add)(acc, SFX(c_mul)(X[r + k * m], MM[k + ii * m]));
+            xb[r] = acc;
+        }
+        for (r = 0; r < m; ++r)
+            if (r != ii) gamma = SFX(c_add)(gamma, SFX(c_mul)(SFX(c_conj)(xb[r]), MM[r + ii * m]));
+        if (gamma.re > 0) {
+            const R sc = -RSQRT(lambda / gamma.re);
+            for (r = 0; r < m; ++r) {
+                if (r == ii) continue;
+                X[r + ii * m] = SFX(c_make)(xb[r].re * sc, xb[r].im * sc);
+                X[ii + r * m] = SFX(c_conj)(X[r + ii * m]);
+            }
+        } else {
+            for (r = 0; r < m; ++r) {
+                if (r == ii) continue;
+                X[r + ii * m] = SFX(c_make)(0, 0);
+                X[ii + r * m] = SFX(c_make)(0, 0);
+            }
+        }
+    }
+    memcpy(MM, X, sizeof(SFX(cplx)) * m * m);
+    SFX(orc_herm_eig)(MM, m, ev, U);
+    im = SFX(imax)(ev, m);
+    /* z = K^{-1} P u, q = G^H z */
+    for (i = 0; i < m; ++i) {
+        SFX(cplx) acc = SFX(c_make)(0, 0);
+        for (k = 0; k < m; ++k)
+            acc = SFX(c_add)(acc, SFX(c_mul)(Ki[i + k * m], SFX(c_mul)(SFX(c_make)((R)amps[k], 0), U[k + im * m])));
+        z[i] = acc;
+    }
+    for (j = 0; j < n; ++j) {
+        SFX(cplx) acc = SFX(c_make)(0, 0);
+        for (i = 0; i < m; ++i) acc = SFX(c_add)(acc, SFX(c_mul)(SFX(c_conj)(G[i + j * m]), z[i]));
+        q[j] = acc;
+    }
+    SFX(write_amp_phase)(source_type, sources, n, q, out_amp, out_phase);
+    free(G);
+    free(K);
+    free(Ki);
+    free(col);
+    free(MM);
+    free(X);
+    free(U);
+    free(xb);
+    free(z);
+    free(q);
+    free(ev);
+}
+
 /* F(x) and g(x) of the LM objective for given x (nls/macros.rs:17-58), for
  * gradient checks: returns F, writes g (n + m). */
 R SFX(orc_lm_eval)(int source_type, const float* sources, const float* wavenums, const float* attens, uint64_t n,

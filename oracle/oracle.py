@@ -69,6 +69,16 @@ def lib():
             f = getattr(L, f"orc_apo_{sfx}")
             f.restype = c_uint32
             f.argtypes = [c_int, fp, fp, fp, c_uint64, fp, fp, c_uint64, R, c_uint32, fp, fp]
+            f = getattr(L, f"orc_long_{sfx}")
+            f.restype = None
+            f.argtypes = [c_int, fp, fp, fp, c_uint64, fp, fp, c_uint64, fp, fp]
+            f = getattr(L, f"orc_horn_{sfx}")
+            f.restype = None
+            rt = c_float if sfx == "f32" else c_double
+            f.argtypes = [c_int, fp, fp, fp, c_uint64, fp, fp, c_uint64, c_uint64, c_uint64, rt, rt, fp, fp]
+            f = getattr(L, f"orc_herm_eig_{sfx}")
+            f.restype = None
+            f.argtypes = [fp, c_uint64, fp, fp]
             f = getattr(L, f"orc_greedy_{sfx}")
             f.restype = None
             f.argtypes = [c_int, fp, fp, fp, c_uint64, fp, fp, c_uint64, c_uint64, fp]
@@ -226,6 +236,34 @@ def gauss_newton(source_type, sources, wavenums, attens, foci, amps, prec="f64",
 def gradient_descent(source_type, sources, wavenums, attens, foci, amps, prec="f64", k_max=10000):
     """Gradient descent (gradient_descent.rs:34-61). Returns (phases[N], x[N+M], steps)."""
     return _nls("orc_gradient_descent", source_type, sources, wavenums, attens, foci, amps, prec, k_max)
+
+
+def long_(source_type, sources, wavenums, attens, foci, amps, prec="f64"):
+    """Long et al. (matrix/long.rs:46-106). Returns (amp[N], phase[N])."""
+    return _solver("orc_long", source_type, sources, wavenums, attens, foci, amps, prec)
+
+
+def horn(source_type, sources, wavenums, attens, foci, amps, seed=0, repeat=100, lam=0.8, alpha=1e-5, prec="f64"):
+    """Horn / Inoue et al. (matrix/horn.rs:58-129) with the seeded row choice. Returns (amp[N], phase[N])."""
+    dt, sfx = _dt(prec)
+    src, k, a, f, am = _f32(sources), _f32(wavenums), _f32(attens), _f32(foci).reshape(-1, 3), _f32(amps)
+    n, m = k.shape[0], f.shape[0]
+    amp, ph = np.empty(n, dt), np.empty(n, dt)
+    getattr(lib(), f"orc_horn_{sfx}")(source_type, _p(src), _p(k), _p(a), n, _p(f), _p(am), m, int(seed),
+                                      int(repeat), float(lam), float(alpha), _p(amp), _p(ph))
+    return amp, ph
+
+
+def herm_eig(A, prec="f64"):
+    """The oracle's Hermitian Jacobi eigen-decomposition: (eigenvalues, eigenvectors as columns)."""
+    dt, sfx = _dt(prec)
+    cdt = np.complex128 if dt == np.float64 else np.complex64
+    n = A.shape[0]
+    a = np.asfortranarray(np.asarray(A, cdt)).copy(order="F")
+    ev = np.empty(n, dt)
+    V = np.empty((n, n), cdt, order="F")
+    getattr(lib(), f"orc_herm_eig_{sfx}")(a.ctypes.data_as(c_void_p), n, _p(ev), V.ctypes.data_as(c_void_p))
+    return ev, V
 
 
 def apo(source_type, sources, wavenums, attens, foci, amps, lam, prec="f64", k_max=200):

@@ -121,10 +121,7 @@ def test_cpu_calculator_rejected(afc):
 
 def test_unsupported_optimizer_reports(afc):
     s = _system(afc, 300.0, [(0, 0, 0)])
-    foci = np.zeros(3, np.float32)
-    amps = np.ones(1, np.float32)
-    afc.dll().AFO_Horn(s.handle, foci.ctypes.data_as(c_void_p), amps.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
-                       1, 1)
+    afc.dll().AFO_BesselBeam(s.handle, afc.Vector3((0, 0, 100)), afc.Vector3((0, 0, 1)), 0.1, 1)
     assert b"not provided" in afc.dll().AFC_GetLastError()
 
 
