@@ -200,8 +200,11 @@ __device__ void gram(const double* __restrict__ W, uint32_t n, uint32_t D, doubl
 // Right-looking elimination, forward substitution in increasing j, backward
 // in decreasing i, every update an fma. Writes L back to sK (the lower
 // triangle only: the upper triangle holds the Gram copy) and y to sy[0..D).
-// Returns false when a pivot is not positive (or NaN).
-template <int DP>
+// Returns false when a pivot is not positive (or NaN). Site gives each calling
+// kernel its own instance, so that the LDS arrays it passes stay compile-time
+// addresses inside the function (with two callers they become runtime pointers
+// and every unrolled access pays for address arithmetic).
+template <int DP, int Site>
 __device__ __attribute__((noinline)) bool chol_solve_wave(double* sK, const double* sv, double* sy, double* col, uint32_t D) {
     const int i = threadIdx.x;  // < 64
     const bool live = i < DP;
@@ -265,6 +268,7 @@ __device__ __attribute__((noinline)) bool chol_solve_wave(double* sK, const doub
 // The same factorization and solves for 32 < D <= 64 with the rows kept in
 // LDS (work, row stride kLmMaxD + 1 doubles, so that a column is spread over
 // the banks) instead of registers; identical arithmetic in the same order.
+template <int Site>
 __device__ __attribute__((noinline)) bool chol_solve_wave_lds(double* sK, const double* sv, double* sy, double* col, double* work,
                                     uint32_t D) {
     constexpr int DP = kLmMaxD, WS = kLmMaxD + 1;
@@ -564,11 +568,11 @@ __global__ __launch_bounds__(kLB) void lm_solve(LmArgs a) {
         if (t < 64) {
             bool okw;
             if (D <= 16)
-                okw = chol_solve_wave<16>(s_K, s_v, s_y, s_col, D);
+                okw = chol_solve_wave<16, 0>(s_K, s_v, s_y, s_col, D);
             else if (D <= 32)
-                okw = chol_solve_wave<32>(s_K, s_v, s_y, s_col, D);
+                okw = chol_solve_wave<32, 0>(s_K, s_v, s_y, s_col, D);
             else
-                okw = chol_solve_wave_lds(s_K, s_v, s_y, s_col, s_work, D);
+                okw = chol_solve_wave_lds<0>(s_K, s_v, s_y, s_col, s_work, D);
             if (t == 0) s_ok = okw ? 1 : 0;
         }
         __syncthreads();
@@ -717,11 +721,11 @@ __global__ __launch_bounds__(kLB) void apo_solve(LmArgs a) {
     __syncthreads();
     if (t < 64) {
         if (D <= 16)
-            (void)chol_solve_wave<16>(s_K, s_v, s_y, s_col, D);
+            (void)chol_solve_wave<16, 1>(s_K, s_v, s_y, s_col, D);
         else if (D <= 32)
-            (void)chol_solve_wave<32>(s_K, s_v, s_y, s_col, D);
+            (void)chol_solve_wave<32, 1>(s_K, s_v, s_y, s_col, D);
         else
-            (void)chol_solve_wave_lds(s_K, s_v, s_y, s_col, s_work, D);
+            (void)chol_solve_wave_lds<1>(s_K, s_v, s_y, s_col, s_work, D);
     }
     __syncthreads();
     for (uint32_t j = t; j < N; j += kLB) {

@@ -99,6 +99,9 @@ __device__ inline void wave_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "
 // full matrix) by wave 0 (threadIdx.x < 64; lane k owns row / column k).
 // On return A holds the eigenvectors as columns and ev[0..M) the eigenvalues,
 // in the convention above; V and raw are work arrays (V M x M, stride kMS).
+// Site gives each calling kernel its own instance, so that the LDS arrays stay
+// compile-time addresses inside the function.
+template <int Site>
 __device__ __attribute__((noinline)) void herm_eig_wave(double2* A, double2* V, double* ev, double* raw, uint32_t M) {
     const int k = threadIdx.x;
     const bool live = k < (int)M;
@@ -186,6 +189,7 @@ __device__ __attribute__((noinline)) void herm_eig_wave(double2* A, double2* V, 
 // [K | B] (LDS, M rows, ncols <= 64 columns, row stride kAS) by Gauss-Jordan
 // elimination without pivoting, by wave 0 (lane c owns column c): on return
 // columns M..ncols-1 hold K^{-1} B. colbuf holds >= M entries.
+template <int Site>
 __device__ __attribute__((noinline)) void gj_solve_wave(double2* A, uint32_t M, uint32_t ncols, double2* colbuf) {
     const int c = threadIdx.x;
     for (uint32_t p = 0; p < M; ++p) {
@@ -314,7 +318,7 @@ __global__ __launch_bounds__(kLB) void long_solve(MatArgs a) {
         }
     }
     __syncthreads();
-    if (t < 64) herm_eig_wave(s_A, s_V, s_ev, s_raw, M);
+    if (t < 64) herm_eig_wave<0>(s_A, s_V, s_ev, s_raw, M);
     __syncthreads();
     // f_k = amps_k e^{i arg U[imax][k]}
     if (t < (int)M) {
@@ -328,7 +332,7 @@ __global__ __launch_bounds__(kLB) void long_solve(MatArgs a) {
     gram_herm(s.G, s.s2, N, M, 1.0, s_A, kAS);
     for (uint32_t i = t; i < M; i += kLB) s_A[i * kAS + M] = s_col[i];
     __syncthreads();
-    if (t < 64) gj_solve_wave(s_A, M, M + 1, s_col);
+    if (t < 64) gj_solve_wave<0>(s_A, M, M + 1, s_col);
     __syncthreads();
     // q = D^{-1} G^H y
     for (uint32_t j = t; j < N; j += kLB) {
@@ -372,7 +376,7 @@ __global__ __launch_bounds__(kLB) void horn_solve(MatArgs a) {
         s_A[i * kAS + M + k] = make_double2(i == k ? 1.0 : 0.0, 0.0);
     }
     __syncthreads();
-    if (t < 64) gj_solve_wave(s_A, M, 2 * M, s_col);
+    if (t < 64) gj_solve_wave<1>(s_A, M, 2 * M, s_col);
     __syncthreads();
     // MM = a^2 P K^{-1} P, X = I
     for (uint32_t e = t; e < M * M; e += kLB) {
@@ -418,7 +422,7 @@ __global__ __launch_bounds__(kLB) void horn_solve(MatArgs a) {
             }
             wave_fence();
         }
-        herm_eig_wave(s_X, s_V, s_ev, s_raw, M);
+        herm_eig_wave<1>(s_X, s_V, s_ev, s_raw, M);
         // z = K^{-1} P u
         const uint32_t im = imax_of(s_ev, M);
         if (r < (int)M) {
