@@ -23,8 +23,10 @@
 //      three products hi hi^H + hi lo^H + lo hi^H (MIXED_SPLIT);
 //   B  runs the iterations with R held in registers, one workgroup per
 //      instance, reductions by fixed-order lane shuffles;
-//   C  regenerates G in FP32 for q = B p; in MIXED_CLOSING it first redoes the
-//      last gamma = R p exactly as G (B p) in FP32 (the closing FP32 iteration).
+//   C  computes q = B p in FP32 from the G tiles and column sums d_j that A
+//      stored (rows < M only; reading them back costs less than generating G
+//      again); in MIXED_CLOSING it first redoes the last gamma = R p exactly as
+//      G (B p) in FP32 (the closing FP32 iteration).
 // Every reduction is per instance and in a fixed order, so an instance's result
 // does not depend on the other instances of the batch.
 #include <math.h>
@@ -59,7 +61,8 @@ struct HoloArgs {
     const DirCell* cells;
     float2* R;   // bucket: MP * MP per instance (row-major)
     float2* p;   // bucket: MP per instance
-    float2* G;   // GS: M x n_packed per instance (row-major), bucket-relative
+    float2* G;   // GS, GS-PAT: MP x n_packed per instance (row-major), bucket-relative
+    float* d;    // GS-PAT: d_j = sum_i |G_ij|, n_packed per instance, bucket-relative
     float2* q;   // GS: n_packed per instance
     float* out_amp;    // batch x n
     float* out_phase;  // batch x n
@@ -69,6 +72,17 @@ struct HoloArgs {
 __device__ __forceinline__ SrcF load_staged(const float4* staged, uint32_t j) {
     const float4 q0 = staged[3 * (uint64_t)j], q1 = staged[3 * (uint64_t)j + 1], q2 = staged[3 * (uint64_t)j + 2];
     return {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
+}
+
+// Workgroup barrier that orders LDS accesses only. __syncthreads() also waits
+// for every outstanding global store (its release fence covers global memory),
+// which stalls kernel A on its G tile stores once per chunk; the barriers of
+// the GS-PAT kernels only guard LDS data (results in global memory are read by
+// later kernels).
+__device__ __forceinline__ void lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
 // Stages every packed source once per solve: the optimizers zero every phase
@@ -123,15 +137,14 @@ __device__ __forceinline__ void load_instance(const HoloArgs& a, uint32_t b, Foc
     if (SRC == kT4010A1) stage_cells_lds(a.cells, cells);
 }
 
-// G for column j0 + (t & 31) at rows (t >> 5) + 8 r, r < MP / 8 (zero for
-// padding rows and padding columns), two rows at a time through the staged
-// term.
+// G for the thread's column (source s, staged) at rows (t >> 5) + 8 r,
+// r < MP / 8 (zero for padding rows and padding columns), two rows at a time
+// through the staged term.
 template <int SRC, int MP>
-__device__ __forceinline__ void g_tile(const HoloArgs& a, uint32_t j0, const Foci<MP>& f, const CellsLds* cells,
+__device__ __forceinline__ void g_tile(const SrcF& s, const Foci<MP>& f, const CellsLds* cells,
                                        float2 (&g)[MP / 8]) {
     const int t = threadIdx.x;
     const int grp = t >> 5;
-    const SrcF s = load_staged(a.staged, j0 + (t & 31));
 #pragma unroll
     for (int r = 0; r < MP / 8; r += 2) {
         const int i0 = grp + 8 * r, i1 = i0 + 8;
@@ -144,25 +157,117 @@ __device__ __forceinline__ void g_tile(const HoloArgs& a, uint32_t j0, const Foc
     }
 }
 
+// Kernel A's staged sources, kSB chunks at a time in LDS (SoA: v[q][j] is
+// float4 q of source j of the block), loaded one block ahead. vmcnt counts
+// loads and stores together and the compiler waits for zero, so a load issued
+// between G tile stores would make every chunk wait for the previous chunk's
+// stores; with the block loads issued kSB chunks before their data is written
+// to LDS, that wait falls once per block.
+constexpr int kSB = 8;
+static_assert(kSB * kKC * 3 == 3 * kHB, "one block is three float4 per thread");
+
+struct SrcBlock {
+    float4 v[3][kSB * kKC];
+};
+
+__device__ __forceinline__ void src_block_fetch(const HoloArgs& a, uint32_t blk, float4 (&pre)[3]) {
+    const uint64_t base = (uint64_t)blk * kSB * kKC * 3, lim = (uint64_t)a.n_packed * 3;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const uint64_t e = base + threadIdx.x + q * kHB;
+        pre[q] = e < lim ? a.staged[e] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+}
+
+__device__ __forceinline__ void src_block_put(SrcBlock& sb, const float4 (&pre)[3]) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const int e = threadIdx.x + q * kHB;
+        sb.v[e % 3][e / 3] = pre[q];
+    }
+}
+
+__device__ __forceinline__ SrcF src_from_block(const SrcBlock& sb, int cb) {
+    const int j = cb * kKC + (threadIdx.x & 31);
+    const float4 q0 = sb.v[0][j], q1 = sb.v[1][j], q2 = sb.v[2][j];
+    return {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
+}
+
 // 1 / d_j (or 1 / d_j^2) of the chunk's columns into s_inv[0..31]; d_j =
 // sum_i |G_ij| summed in a fixed order. Columns beyond n get 0.
 template <int MP>
 __device__ __forceinline__ void column_scale(const float2 (&g)[MP / 8], uint32_t j0, uint32_t n, bool squared,
-                                             float (*s_part)[kKC], float* s_inv) {
+                                             float (*s_part)[kKC], float* s_inv, float* d_out = nullptr) {
     const int t = threadIdx.x;
     float part = 0.f;
 #pragma unroll
     for (int r = 0; r < MP / 8; ++r) part += sqrtf(g[r].x * g[r].x + g[r].y * g[r].y);
     s_part[t >> 5][t & 31] = part;
-    __syncthreads();
+    lds_sync();
     if (t < kKC) {
         float d = 0.f;
 #pragma unroll
         for (int q = 0; q < 8; ++q) d += s_part[q][t];
         // a real column with d = 0 gives NaN, as conj(G) / 0 does in the reference
         s_inv[t] = (j0 + t < n) ? (squared ? 1.0f / (d * d) : 1.0f / d) : 0.f;
+        if (d_out) d_out[j0 + t] = d;
     }
-    __syncthreads();
+    lds_sync();
+}
+
+// Kernel A's copy of the G tile for kernel C: rows < M only (rows >= M are
+// zero and never read back), coalesced over the 32 columns of the chunk.
+template <int MP>
+__device__ __forceinline__ void store_g_tile(const HoloArgs& a, uint32_t k, uint32_t j0, uint32_t M,
+                                             const float2 (&g)[MP / 8]) {
+    const int t = threadIdx.x;
+    float2* G = a.G + (uint64_t)k * MP * a.n_packed + j0 + (t & 31);
+#pragma unroll
+    for (int r = 0; r < MP / 8; ++r) {
+        const int i = (t >> 5) + 8 * r;
+        if (i < (int)M) {
+            // streamed: read once, by kernel C
+            __builtin_nontemporal_store(g[r].x, &G[(uint64_t)i * a.n_packed].x);
+            __builtin_nontemporal_store(g[r].y, &G[(uint64_t)i * a.n_packed].y);
+        }
+    }
+}
+
+template <int MP>
+__device__ __forceinline__ void load_g_tile(const HoloArgs& a, uint32_t k, uint32_t j0, uint32_t M,
+                                            float2 (&g)[MP / 8]) {
+    const int t = threadIdx.x;
+    const float2* G = a.G + (uint64_t)k * MP * a.n_packed + j0 + (t & 31);
+#pragma unroll
+    for (int r = 0; r < MP / 8; ++r) {
+        const int i = (t >> 5) + 8 * r;
+        g[r] = i < (int)M ? make_float2(__builtin_nontemporal_load(&G[(uint64_t)i * a.n_packed].x),
+                                        __builtin_nontemporal_load(&G[(uint64_t)i * a.n_packed].y))
+                          : make_float2(0.f, 0.f);
+    }
+}
+
+// 1 / d_j^2 of the chunk's columns from kernel A's d_j (the same expression
+// as column_scale's); d is column j0 + t's value for threads t < kKC.
+__device__ __forceinline__ void column_scale_stored(float d, uint32_t j0, uint32_t n, float* s_inv) {
+    const int t = threadIdx.x;
+    if (t < kKC) s_inv[t] = (j0 + t < n) ? 1.0f / (d * d) : 0.f;
+    lds_sync();
+}
+
+// Kernel C's stored inputs of one chunk: the G tile and, for threads t < kKC,
+// d of column j0 + t. Loaded one chunk ahead of their use.
+template <int MP>
+struct StoredChunk {
+    float2 g[MP / 8];
+    float d;
+};
+
+template <int MP>
+__device__ __forceinline__ void load_chunk(const HoloArgs& a, uint32_t k, uint32_t j0, uint32_t M,
+                                           StoredChunk<MP>& c) {
+    load_g_tile<MP>(a, k, j0, M, c.g);
+    c.d = threadIdx.x < kKC ? a.d[(uint64_t)k * a.n_packed + j0 + threadIdx.x] : 0.f;
 }
 
 // ---------------------------------------------------------------- kernel A (FP32)
@@ -174,6 +279,7 @@ __global__ __launch_bounds__(kHB) void gspat_r_fp32(HoloArgs a) {
     __shared__ float s_part[8][kKC];
     __shared__ float s_inv[kKC];
     __shared__ float2 s_h[MP][kKC + 1];
+    __shared__ SrcBlock s_src;
     const int t = threadIdx.x;
     const uint32_t k = blockIdx.x, b = a.inst[k];
     uint32_t M, base;
@@ -187,14 +293,27 @@ __global__ __launch_bounds__(kHB) void gspat_r_fp32(HoloArgs a) {
 #pragma unroll
         for (int y = 0; y < BS; ++y) acc[x][y] = make_float2(0.f, 0.f);
 
+    float* d_out = a.d + (uint64_t)k * a.n_packed;
+    float4 pre[3];
+    src_block_fetch(a, 0, pre);
     for (uint32_t j0 = 0; j0 < a.n_packed; j0 += kKC) {
+        const uint32_t chunk = j0 / kKC, in_blk = chunk % kSB;
+        if (in_blk == 0) {
+            // every wave is past its last read of the previous block (the
+            // barrier that ends each chunk)
+            src_block_put(s_src, pre);
+            lds_sync();
+            if (j0 + kSB * kKC < a.n_packed) src_block_fetch(a, chunk / kSB + 1, pre);
+        }
+        const SrcF sc = src_from_block(s_src, in_blk);
         float2 g[MP / 8];
-        g_tile<SRC, MP>(a, j0, f, s_cells, g);
-        column_scale<MP>(g, j0, a.n, false, s_part, s_inv);
+        g_tile<SRC, MP>(sc, f, s_cells, g);
+        store_g_tile<MP>(a, k, j0, M, g);
+        column_scale<MP>(g, j0, a.n, false, s_part, s_inv, d_out);
         const float inv = s_inv[t & 31];
 #pragma unroll
         for (int r = 0; r < MP / 8; ++r) s_h[(t >> 5) + 8 * r][t & 31] = make_float2(g[r].x * inv, g[r].y * inv);
-        __syncthreads();
+        lds_sync();
         for (int jj = 0; jj < kKC; ++jj) {
             float2 hr[BS], hc[BS];
 #pragma unroll
@@ -210,7 +329,7 @@ __global__ __launch_bounds__(kHB) void gspat_r_fp32(HoloArgs a) {
                     acc[x][y].y = fmaf(hr[x].y, hc[y].x, fmaf(-hr[x].x, hc[y].y, acc[x][y].y));
                 }
         }
-        __syncthreads();
+        lds_sync();
     }
     float2* R = a.R + (uint64_t)k * MP * MP;
 #pragma unroll
@@ -241,6 +360,7 @@ __global__ __launch_bounds__(kHB) void gspat_r_mfma(HoloArgs a) {
     __shared__ float s_inv[kKC];
     __shared__ __attribute__((aligned(16))) _Float16 s_hr[NH][MP * HS];
     __shared__ __attribute__((aligned(16))) _Float16 s_hi[NH][MP * HS];
+    __shared__ SrcBlock s_src;
     const int t = threadIdx.x;
     const int w = t >> 6, l = t & 63;
     const uint32_t k = blockIdx.x, b = a.inst[k];
@@ -256,10 +376,23 @@ __global__ __launch_bounds__(kHB) void gspat_r_mfma(HoloArgs a) {
         accIR[s] = accRe[s];
         accRI[s] = accRe[s];
     }
+    float* d_out = a.d + (uint64_t)k * a.n_packed;
+    float4 pre[3];
+    src_block_fetch(a, 0, pre);
     for (uint32_t j0 = 0; j0 < a.n_packed; j0 += kKC) {
+        const uint32_t chunk = j0 / kKC, in_blk = chunk % kSB;
+        if (in_blk == 0) {
+            // every wave is past its last read of the previous block (the
+            // barrier that ends each chunk)
+            src_block_put(s_src, pre);
+            lds_sync();
+            if (j0 + kSB * kKC < a.n_packed) src_block_fetch(a, chunk / kSB + 1, pre);
+        }
+        const SrcF sc = src_from_block(s_src, in_blk);
         float2 g[MP / 8];
-        g_tile<SRC, MP>(a, j0, f, s_cells, g);
-        column_scale<MP>(g, j0, a.n, false, s_part, s_inv);
+        g_tile<SRC, MP>(sc, f, s_cells, g);
+        store_g_tile<MP>(a, k, j0, M, g);
+        column_scale<MP>(g, j0, a.n, false, s_part, s_inv, d_out);
         const float inv = s_inv[t & 31] * scale;
 #pragma unroll
         for (int r = 0; r < MP / 8; ++r) {
@@ -273,7 +406,7 @@ __global__ __launch_bounds__(kHB) void gspat_r_mfma(HoloArgs a) {
                 s_hi[NH - 1][i * HS + (t & 31)] = (_Float16)(him - (float)ih);
             }
         }
-        __syncthreads();
+        lds_sync();
         const int kb = (l >> 4) * 8;
 #pragma unroll
         for (int s = 0; s < TPW; ++s) {
@@ -304,7 +437,7 @@ __global__ __launch_bounds__(kHB) void gspat_r_mfma(HoloArgs a) {
             accIR[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ai, br, accIR[s], 0, 0, 0);
             accRI[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ar, bi, accRI[s], 0, 0, 0);
         }
-        __syncthreads();
+        lds_sync();
     }
     const float unscale = 1.0f / (scale * scale);  // exact: scale is a power of two
     float2* R = a.R + (uint64_t)k * MP * MP;
@@ -380,23 +513,22 @@ __global__ __launch_bounds__(kHB) void gspat_iter(HoloArgs a, uint32_t iters, in
 }
 
 // ---------------------------------------------------------------- kernel C
-template <int SRC, int MP>
+template <int MP>
 __global__ __launch_bounds__(kHB) void gspat_q(HoloArgs a, int closing) {
-    __shared__ std::conditional_t<SRC == kT4010A1, CellsLds, float> s_cells_mem;
-    CellsLds* s_cells = reinterpret_cast<CellsLds*>(&s_cells_mem);
-    __shared__ Foci<MP> f;
-    __shared__ float s_part[8][kKC];
     __shared__ float s_inv[kKC];
     __shared__ float2 s_qp[8][kKC];
     __shared__ float2 s_q[kKC];
     __shared__ float2 s_p[MP];
+    __shared__ float s_a[MP];
     const int t = threadIdx.x;
     const int grp = t >> 5, jj = t & 31;
     const uint32_t k = blockIdx.x, b = a.inst[k];
-    uint32_t M, base;
-    load_instance<SRC, MP>(a, b, f, s_cells, M, base);
-    for (int i = t; i < MP; i += kHB) s_p[i] = a.p[(uint64_t)k * MP + i];
-    __syncthreads();
+    const uint32_t base = a.offsets[b], M = a.offsets[b + 1] - base;
+    for (int i = t; i < MP; i += kHB) {
+        s_p[i] = a.p[(uint64_t)k * MP + i];
+        s_a[i] = i < (int)M ? a.amps[base + i] : 0.f;
+    }
+    lds_sync();
 
     // q_j partials of the chunk: w_j sum_i conj(G_ij) p_i, reduced over the 8
     // row groups in a fixed order into s_q
@@ -409,7 +541,7 @@ __global__ __launch_bounds__(kHB) void gspat_q(HoloArgs a, int closing) {
             part.y = fmaf(g[r].x, pv.y, fmaf(-g[r].y, pv.x, part.y));
         }
         s_qp[grp][jj] = part;
-        __syncthreads();
+        lds_sync();
         if (t < kKC) {
             float2 q = make_float2(0.f, 0.f);
 #pragma unroll
@@ -419,17 +551,20 @@ __global__ __launch_bounds__(kHB) void gspat_q(HoloArgs a, int closing) {
             }
             s_q[t] = make_float2(q.x * s_inv[t], q.y * s_inv[t]);
         }
-        __syncthreads();
+        lds_sync();
     };
 
     if (closing) {
         float2 gam[MP / 8];
 #pragma unroll
         for (int r = 0; r < MP / 8; ++r) gam[r] = make_float2(0.f, 0.f);
+        StoredChunk<MP> cn;
+        load_chunk<MP>(a, k, 0, M, cn);
         for (uint32_t j0 = 0; j0 < a.n_packed; j0 += kKC) {
-            float2 g[MP / 8];
-            g_tile<SRC, MP>(a, j0, f, s_cells, g);
-            column_scale<MP>(g, j0, a.n, true, s_part, s_inv);
+            const StoredChunk<MP> c = cn;
+            if (j0 + kKC < a.n_packed) load_chunk<MP>(a, k, j0 + kKC, M, cn);
+            const float2(&g)[MP / 8] = c.g;
+            column_scale_stored(c.d, j0, a.n, s_inv);
             chunk_q(g);
             const float2 qj = s_q[jj];
 #pragma unroll
@@ -437,7 +572,7 @@ __global__ __launch_bounds__(kHB) void gspat_q(HoloArgs a, int closing) {
                 gam[r].x = fmaf(g[r].x, qj.x, fmaf(-g[r].y, qj.y, gam[r].x));
                 gam[r].y = fmaf(g[r].x, qj.y, fmaf(g[r].y, qj.x, gam[r].y));
             }
-            __syncthreads();
+            lds_sync();
         }
         // reduce over the 32 columns held by the half-wave of this row group
 #pragma unroll
@@ -452,24 +587,26 @@ __global__ __launch_bounds__(kHB) void gspat_q(HoloArgs a, int closing) {
 #pragma unroll
             for (int r = 0; r < MP / 8; ++r) {
                 const int i = grp + 8 * r;
-                const float p0 = f.a[i];
+                const float p0 = s_a[i];
                 const float inv = p0 * p0 / (gam[r].x * gam[r].x + gam[r].y * gam[r].y);
                 s_p[i] = i < (int)M ? make_float2(gam[r].x * inv, gam[r].y * inv) : make_float2(0.f, 0.f);
             }
         }
-        __syncthreads();
+        lds_sync();
     }
+    StoredChunk<MP> cn;
+    load_chunk<MP>(a, k, 0, M, cn);
     for (uint32_t j0 = 0; j0 < a.n_packed; j0 += kKC) {
-        float2 g[MP / 8];
-        g_tile<SRC, MP>(a, j0, f, s_cells, g);
-        column_scale<MP>(g, j0, a.n, true, s_part, s_inv);
-        chunk_q(g);
+        const StoredChunk<MP> c = cn;
+        if (j0 + kKC < a.n_packed) load_chunk<MP>(a, k, j0 + kKC, M, cn);
+        column_scale_stored(c.d, j0, a.n, s_inv);
+        chunk_q(c.g);
         if (t < kKC && j0 + t < a.n) {
             const float2 q = s_q[t];
             a.out_amp[(uint64_t)b * a.n + j0 + t] = hypotf(q.x, q.y);
             a.out_phase[(uint64_t)b * a.n + j0 + t] = atan2f(q.y, q.x);
         }
-        __syncthreads();
+        lds_sync();
     }
 }
 
@@ -709,10 +846,7 @@ static hipError_t launch_gspat_bucket(HoloArgs a, int src, int precision, uint32
     }
     const int closing = precision == AFH_HOLO_MIXED_CLOSING ? 1 : 0;
     hipLaunchKernelGGL((gspat_iter<MP>), grid, blk, 0, st, a, iters, closing);
-    if (src == kT4010A1)
-        hipLaunchKernelGGL((gspat_q<kT4010A1, MP>), grid, blk, 0, st, a, closing);
-    else
-        hipLaunchKernelGGL((gspat_q<kSphere, MP>), grid, blk, 0, st, a, closing);
+    hipLaunchKernelGGL((gspat_q<MP>), grid, blk, 0, st, a, closing);
     return hipGetLastError();
 }
 
@@ -759,15 +893,17 @@ int32_t holo_solve(Context* ctx, const HoloBatch& hb, hipStream_t st) {
             if (kinds[q] == mp) lists[q].push_back(b);
     }
     const uint32_t np = hb.n_packed;
-    // workspace layout: offsets | instance lists | staged sources | R | p | G | q
+    // workspace layout: offsets | instance lists | staged sources | R | p | G | q | d
     size_t off_bytes = (B + 1) * sizeof(uint32_t);
     size_t list_bytes = B * sizeof(uint32_t);
-    size_t r_elems = 0, p_elems = 0, g_elems = 0, q_elems = 0;
+    size_t r_elems = 0, p_elems = 0, g_elems = 0, q_elems = 0, d_elems = 0;
     for (int q = 0; q < 4; ++q) {
         const size_t nb = lists[q].size(), mp = kinds[q];
         if (hb.algo == kAlgoGSPAT) {
             r_elems += nb * mp * mp;
             p_elems += nb * mp;
+            g_elems += nb * mp * np;
+            d_elems += nb * np;
         } else if (hb.algo == kAlgoGS) {
             g_elems += nb * mp * np;
             q_elems += nb * np;
@@ -777,7 +913,8 @@ int32_t holo_solve(Context* ctx, const HoloBatch& hb, hipStream_t st) {
     const size_t o_off = 0, o_list = align(off_bytes), o_stg = o_list + align(list_bytes);
     const size_t o_r = o_stg + align((size_t)np * AFH_SOURCE_STRIDE * sizeof(float));
     const size_t o_p = o_r + align(r_elems * sizeof(float2)), o_g = o_p + align(p_elems * sizeof(float2));
-    const size_t o_q = o_g + align(g_elems * sizeof(float2)), total = o_q + align(q_elems * sizeof(float2));
+    const size_t o_q = o_g + align(g_elems * sizeof(float2)), o_d = o_q + align(q_elems * sizeof(float2));
+    const size_t total = o_d + align(d_elems * sizeof(float));
     // Greedy: the candidate table (div float2) and their arguments (div floats)
     const uint32_t div = hb.algo == kAlgoGreedy ? hb.phase_div : 0;
     if (hb.algo == kAlgoGreedy && (div == 0 || div > kGreedyMaxDiv)) {
@@ -839,7 +976,7 @@ int32_t holo_solve(Context* ctx, const HoloBatch& hb, hipStream_t st) {
     hipLaunchKernelGGL(stage_holo_sources, dim3((np + kHB - 1) / kHB), dim3(kHB), 0, st, hb.sources, np,
                        reinterpret_cast<float4*>(wbase + o_stg));
     if (!hip_ok(hipGetLastError(), "holo staging launch")) return AFH_ERR_HIP;
-    size_t list_pos = 0, r_pos = 0, p_pos = 0, g_pos = 0, q_pos = 0;
+    size_t list_pos = 0, r_pos = 0, p_pos = 0, g_pos = 0, q_pos = 0, d_pos = 0;
     for (int q = 0; q < 4; ++q) {
         const uint32_t nb = (uint32_t)lists[q].size();
         const int mp = kinds[q];
@@ -849,6 +986,7 @@ int32_t holo_solve(Context* ctx, const HoloBatch& hb, hipStream_t st) {
         a.p = reinterpret_cast<float2*>(wbase + o_p) + p_pos;
         a.G = reinterpret_cast<float2*>(wbase + o_g) + g_pos;
         a.q = reinterpret_cast<float2*>(wbase + o_q) + q_pos;
+        a.d = reinterpret_cast<float*>(wbase + o_d) + d_pos;
         hipError_t e = hipSuccess;
         if (hb.algo == kAlgoGSPAT) {
             switch (mp) {
@@ -859,6 +997,8 @@ int32_t holo_solve(Context* ctx, const HoloBatch& hb, hipStream_t st) {
             }
             r_pos += (size_t)nb * mp * mp;
             p_pos += (size_t)nb * mp;
+            g_pos += (size_t)nb * mp * np;
+            d_pos += (size_t)nb * np;
         } else if (hb.algo == kAlgoGS) {
             const uint32_t gstride = (uint32_t)(mp * np);
             switch (mp) {
