@@ -25,8 +25,10 @@
 //      instance, reductions by fixed-order lane shuffles;
 //   C  computes q = B p in FP32 from the G tiles and column sums d_j that A
 //      stored (rows < M only; reading them back costs less than generating G
-//      again); in MIXED_CLOSING it first redoes the last gamma = R p exactly as
-//      G (B p) in FP32 (the closing FP32 iteration).
+//      again); in MIXED_CLOSING a first pass redoes the last gamma = R p
+//      exactly as G (B p) in FP32 (the closing FP32 iteration).
+// A and C run a.split workgroups per instance over ranges of source chunks;
+// A's partial R and C's partial gamma are summed in a fixed order.
 // Every reduction is per instance and in a fixed order, so an instance's result
 // does not depend on the other instances of the batch.
 #include <math.h>
@@ -59,11 +61,13 @@ struct HoloArgs {
     const uint32_t* inst;     // instance ids of this bucket
     const float4* staged;     // SrcF of every packed source (3 float4 each), phases zeroed
     const DirCell* cells;
-    float2* R;   // bucket: MP * MP per instance (row-major)
+    float2* R;   // bucket: split partial MP * MP matrices per instance (row-major)
+    uint32_t split;  // GS-PAT kernel A workgroups per instance (source chunk ranges)
     float2* p;   // bucket: MP per instance
     float2* G;   // GS, GS-PAT: MP x n_packed per instance (row-major), bucket-relative
     float* d;    // GS-PAT: d_j = sum_i |G_ij|, n_packed per instance, bucket-relative
     float2* q;   // GS: n_packed per instance
+    float2* gam; // GS-PAT MIXED_CLOSING: partial gamma (MP) per kernel C workgroup
     float* out_amp;    // batch x n
     float* out_phase;  // batch x n
 };
@@ -139,15 +143,22 @@ __device__ __forceinline__ void load_instance(const HoloArgs& a, uint32_t b, Foc
 
 // G for the thread's column (source s, staged) at rows (t >> 5) + 8 r,
 // r < MP / 8 (zero for padding rows and padding columns), two rows at a time
-// through the staged term.
+// through the staged term; pairs of rows that are both padding (>= M) are set
+// to zero without evaluating the terms.
 template <int SRC, int MP>
-__device__ __forceinline__ void g_tile(const SrcF& s, const Foci<MP>& f, const CellsLds* cells,
+__device__ __forceinline__ void g_tile(const SrcF& s, const Foci<MP>& f, const CellsLds* cells, uint32_t M,
                                        float2 (&g)[MP / 8]) {
     const int t = threadIdx.x;
     const int grp = t >> 5;
 #pragma unroll
     for (int r = 0; r < MP / 8; r += 2) {
         const int i0 = grp + 8 * r, i1 = i0 + 8;
+        if (i0 >= (int)M) {
+            // both rows are padding (i1 > i0): zero without computing the terms
+            g[r] = make_float2(0.f, 0.f);
+            g[r + 1] = make_float2(0.f, 0.f);
+            continue;
+        }
         const float px[2] = {f.x[i0], f.x[i1]}, py[2] = {f.y[i0], f.y[i1]}, pz[2] = {f.z[i0], f.z[i1]};
         const float w[2] = {f.v[i0], f.v[i1]};
         float re[2] = {0.f, 0.f}, im[2] = {0.f, 0.f};
@@ -170,8 +181,9 @@ struct SrcBlock {
     float4 v[3][kSB * kKC];
 };
 
-__device__ __forceinline__ void src_block_fetch(const HoloArgs& a, uint32_t blk, float4 (&pre)[3]) {
-    const uint64_t base = (uint64_t)blk * kSB * kKC * 3, lim = (uint64_t)a.n_packed * 3;
+// The block of kSB chunks starting at chunk c (staged sources 32 c ...).
+__device__ __forceinline__ void src_block_fetch(const HoloArgs& a, uint32_t c, float4 (&pre)[3]) {
+    const uint64_t base = (uint64_t)c * kKC * 3, lim = (uint64_t)a.n_packed * 3;
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
         const uint64_t e = base + threadIdx.x + q * kHB;
@@ -281,7 +293,9 @@ __global__ __launch_bounds__(kHB) void gspat_r_fp32(HoloArgs a) {
     __shared__ float2 s_h[MP][kKC + 1];
     __shared__ SrcBlock s_src;
     const int t = threadIdx.x;
-    const uint32_t k = blockIdx.x, b = a.inst[k];
+    const uint32_t k = blockIdx.x / a.split, part = blockIdx.x % a.split, b = a.inst[k];
+    const uint32_t n_chunks = a.n_packed / kKC;
+    const uint32_t c_begin = part * n_chunks / a.split, c_end = (part + 1) * n_chunks / a.split;
     uint32_t M, base;
     load_instance<SRC, MP>(a, b, f, s_cells, M, base);
     __syncthreads();
@@ -295,19 +309,19 @@ __global__ __launch_bounds__(kHB) void gspat_r_fp32(HoloArgs a) {
 
     float* d_out = a.d + (uint64_t)k * a.n_packed;
     float4 pre[3];
-    src_block_fetch(a, 0, pre);
-    for (uint32_t j0 = 0; j0 < a.n_packed; j0 += kKC) {
-        const uint32_t chunk = j0 / kKC, in_blk = chunk % kSB;
+    src_block_fetch(a, c_begin, pre);
+    for (uint32_t chunk = c_begin; chunk < c_end; ++chunk) {
+        const uint32_t j0 = chunk * kKC, in_blk = (chunk - c_begin) % kSB;
         if (in_blk == 0) {
             // every wave is past its last read of the previous block (the
             // barrier that ends each chunk)
             src_block_put(s_src, pre);
             lds_sync();
-            if (j0 + kSB * kKC < a.n_packed) src_block_fetch(a, chunk / kSB + 1, pre);
+            if (chunk + kSB < c_end) src_block_fetch(a, chunk + kSB, pre);
         }
         const SrcF sc = src_from_block(s_src, in_blk);
         float2 g[MP / 8];
-        g_tile<SRC, MP>(sc, f, s_cells, g);
+        g_tile<SRC, MP>(sc, f, s_cells, M, g);
         store_g_tile<MP>(a, k, j0, M, g);
         column_scale<MP>(g, j0, a.n, false, s_part, s_inv, d_out);
         const float inv = s_inv[t & 31];
@@ -331,7 +345,7 @@ __global__ __launch_bounds__(kHB) void gspat_r_fp32(HoloArgs a) {
         }
         lds_sync();
     }
-    float2* R = a.R + (uint64_t)k * MP * MP;
+    float2* R = a.R + (uint64_t)blockIdx.x * MP * MP;  // partial `part` of instance k
 #pragma unroll
     for (int x = 0; x < BS; ++x)
 #pragma unroll
@@ -363,7 +377,9 @@ __global__ __launch_bounds__(kHB) void gspat_r_mfma(HoloArgs a) {
     __shared__ SrcBlock s_src;
     const int t = threadIdx.x;
     const int w = t >> 6, l = t & 63;
-    const uint32_t k = blockIdx.x, b = a.inst[k];
+    const uint32_t k = blockIdx.x / a.split, part = blockIdx.x % a.split, b = a.inst[k];
+    const uint32_t n_chunks = a.n_packed / kKC;
+    const uint32_t c_begin = part * n_chunks / a.split, c_end = (part + 1) * n_chunks / a.split;
     uint32_t M, base;
     load_instance<SRC, MP>(a, b, f, s_cells, M, base);
     __syncthreads();
@@ -378,19 +394,19 @@ __global__ __launch_bounds__(kHB) void gspat_r_mfma(HoloArgs a) {
     }
     float* d_out = a.d + (uint64_t)k * a.n_packed;
     float4 pre[3];
-    src_block_fetch(a, 0, pre);
-    for (uint32_t j0 = 0; j0 < a.n_packed; j0 += kKC) {
-        const uint32_t chunk = j0 / kKC, in_blk = chunk % kSB;
+    src_block_fetch(a, c_begin, pre);
+    for (uint32_t chunk = c_begin; chunk < c_end; ++chunk) {
+        const uint32_t j0 = chunk * kKC, in_blk = (chunk - c_begin) % kSB;
         if (in_blk == 0) {
             // every wave is past its last read of the previous block (the
             // barrier that ends each chunk)
             src_block_put(s_src, pre);
             lds_sync();
-            if (j0 + kSB * kKC < a.n_packed) src_block_fetch(a, chunk / kSB + 1, pre);
+            if (chunk + kSB < c_end) src_block_fetch(a, chunk + kSB, pre);
         }
         const SrcF sc = src_from_block(s_src, in_blk);
         float2 g[MP / 8];
-        g_tile<SRC, MP>(sc, f, s_cells, g);
+        g_tile<SRC, MP>(sc, f, s_cells, M, g);
         store_g_tile<MP>(a, k, j0, M, g);
         column_scale<MP>(g, j0, a.n, false, s_part, s_inv, d_out);
         const float inv = s_inv[t & 31] * scale;
@@ -440,7 +456,7 @@ __global__ __launch_bounds__(kHB) void gspat_r_mfma(HoloArgs a) {
         lds_sync();
     }
     const float unscale = 1.0f / (scale * scale);  // exact: scale is a power of two
-    float2* R = a.R + (uint64_t)k * MP * MP;
+    float2* R = a.R + (uint64_t)blockIdx.x * MP * MP;  // partial `part` of instance k
 #pragma unroll
     for (int s = 0; s < TPW; ++s) {
         const int tau = w * TPW + s;
@@ -466,10 +482,18 @@ __global__ __launch_bounds__(kHB) void gspat_iter(HoloArgs a, uint32_t iters, in
     const int row = t / TPR, h = t % TPR;
     const uint32_t k = blockIdx.x, b = a.inst[k];
     const uint32_t base = a.offsets[b], M = a.offsets[b + 1] - base;
-    const float2* R = a.R + (uint64_t)k * MP * MP + (uint64_t)row * MP + h * CPT;
+    // R = sum of kernel A's partials over source chunk ranges, in order
+    const float2* R = a.R + (uint64_t)k * a.split * MP * MP + (uint64_t)row * MP + h * CPT;
     float2 r[CPT];
 #pragma unroll
     for (int c = 0; c < CPT; ++c) r[c] = R[c];
+    for (uint32_t sp = 1; sp < a.split; ++sp)
+#pragma unroll
+        for (int c = 0; c < CPT; ++c) {
+            const float2 v = R[(uint64_t)sp * MP * MP + c];
+            r[c].x += v.x;
+            r[c].y += v.y;
+        }
     const bool valid = row < (int)M;
     const float p0 = valid ? a.amps[base + row] : 0.f;
     if (h == 0) s_p[0][row] = make_float2(p0, 0.f);
@@ -513,34 +537,54 @@ __global__ __launch_bounds__(kHB) void gspat_iter(HoloArgs a, uint32_t iters, in
 }
 
 // ---------------------------------------------------------------- kernel C
+// q = B p over a range of source chunks: a.split workgroups per instance,
+// each with the chunk range kernel A's workgroup of the same index had.
+// pass kQGamma (MIXED_CLOSING only): q from the iterated p, then the range's
+//   partial gamma_i = sum_j G_ij q_j into a.gam;
+// pass kQFinal: p = a.p, or in MIXED_CLOSING p = gamma / |gamma|^2 p0^2 with
+//   gamma the sum of the partials in order; q_j -> amplitude and phase.
+enum { kQGamma = 0, kQFinal = 1 };
+
 template <int MP>
-__global__ __launch_bounds__(kHB) void gspat_q(HoloArgs a, int closing) {
+__global__ __launch_bounds__(kHB) void gspat_q(HoloArgs a, int closing, int pass) {
     __shared__ float s_inv[kKC];
     __shared__ float2 s_qp[8][kKC];
     __shared__ float2 s_q[kKC];
     __shared__ float2 s_p[MP];
-    __shared__ float s_a[MP];
     const int t = threadIdx.x;
     const int grp = t >> 5, jj = t & 31;
-    const uint32_t k = blockIdx.x, b = a.inst[k];
+    const uint32_t k = blockIdx.x / a.split, part = blockIdx.x % a.split, b = a.inst[k];
+    const uint32_t n_chunks = a.n_packed / kKC;
+    const uint32_t c_begin = part * n_chunks / a.split, c_end = (part + 1) * n_chunks / a.split;
     const uint32_t base = a.offsets[b], M = a.offsets[b + 1] - base;
     for (int i = t; i < MP; i += kHB) {
-        s_p[i] = a.p[(uint64_t)k * MP + i];
-        s_a[i] = i < (int)M ? a.amps[base + i] : 0.f;
+        if (pass == kQFinal && closing) {
+            float2 gam = make_float2(0.f, 0.f);
+            for (uint32_t sp = 0; sp < a.split; ++sp) {
+                const float2 v = a.gam[((uint64_t)k * a.split + sp) * MP + i];
+                gam.x += v.x;
+                gam.y += v.y;
+            }
+            const float p0 = i < (int)M ? a.amps[base + i] : 0.f;
+            const float inv = p0 * p0 / (gam.x * gam.x + gam.y * gam.y);
+            s_p[i] = i < (int)M ? make_float2(gam.x * inv, gam.y * inv) : make_float2(0.f, 0.f);
+        } else {
+            s_p[i] = a.p[(uint64_t)k * MP + i];
+        }
     }
     lds_sync();
 
     // q_j partials of the chunk: w_j sum_i conj(G_ij) p_i, reduced over the 8
     // row groups in a fixed order into s_q
     auto chunk_q = [&](const float2 (&g)[MP / 8]) {
-        float2 part = make_float2(0.f, 0.f);
+        float2 pq = make_float2(0.f, 0.f);
 #pragma unroll
         for (int r = 0; r < MP / 8; ++r) {
             const float2 pv = s_p[grp + 8 * r];
-            part.x = fmaf(g[r].x, pv.x, fmaf(g[r].y, pv.y, part.x));
-            part.y = fmaf(g[r].x, pv.y, fmaf(-g[r].y, pv.x, part.y));
+            pq.x = fmaf(g[r].x, pv.x, fmaf(g[r].y, pv.y, pq.x));
+            pq.y = fmaf(g[r].x, pv.y, fmaf(-g[r].y, pv.x, pq.y));
         }
-        s_qp[grp][jj] = part;
+        s_qp[grp][jj] = pq;
         lds_sync();
         if (t < kKC) {
             float2 q = make_float2(0.f, 0.f);
@@ -554,26 +598,32 @@ __global__ __launch_bounds__(kHB) void gspat_q(HoloArgs a, int closing) {
         lds_sync();
     };
 
-    if (closing) {
-        float2 gam[MP / 8];
+    float2 gam[MP / 8];
 #pragma unroll
-        for (int r = 0; r < MP / 8; ++r) gam[r] = make_float2(0.f, 0.f);
-        StoredChunk<MP> cn;
-        load_chunk<MP>(a, k, 0, M, cn);
-        for (uint32_t j0 = 0; j0 < a.n_packed; j0 += kKC) {
-            const StoredChunk<MP> c = cn;
-            if (j0 + kKC < a.n_packed) load_chunk<MP>(a, k, j0 + kKC, M, cn);
-            const float2(&g)[MP / 8] = c.g;
-            column_scale_stored(c.d, j0, a.n, s_inv);
-            chunk_q(g);
+    for (int r = 0; r < MP / 8; ++r) gam[r] = make_float2(0.f, 0.f);
+    StoredChunk<MP> cn;
+    load_chunk<MP>(a, k, c_begin * kKC, M, cn);
+    for (uint32_t chunk = c_begin; chunk < c_end; ++chunk) {
+        const uint32_t j0 = chunk * kKC;
+        const StoredChunk<MP> c = cn;
+        if (chunk + 1 < c_end) load_chunk<MP>(a, k, j0 + kKC, M, cn);
+        column_scale_stored(c.d, j0, a.n, s_inv);
+        chunk_q(c.g);
+        if (pass == kQGamma) {
             const float2 qj = s_q[jj];
 #pragma unroll
             for (int r = 0; r < MP / 8; ++r) {
-                gam[r].x = fmaf(g[r].x, qj.x, fmaf(-g[r].y, qj.y, gam[r].x));
-                gam[r].y = fmaf(g[r].x, qj.y, fmaf(g[r].y, qj.x, gam[r].y));
+                gam[r].x = fmaf(c.g[r].x, qj.x, fmaf(-c.g[r].y, qj.y, gam[r].x));
+                gam[r].y = fmaf(c.g[r].x, qj.y, fmaf(c.g[r].y, qj.x, gam[r].y));
             }
-            lds_sync();
+        } else if (t < kKC && j0 + t < a.n) {
+            const float2 q = s_q[t];
+            a.out_amp[(uint64_t)b * a.n + j0 + t] = hypotf(q.x, q.y);
+            a.out_phase[(uint64_t)b * a.n + j0 + t] = atan2f(q.y, q.x);
         }
+        lds_sync();
+    }
+    if (pass == kQGamma) {
         // reduce over the 32 columns held by the half-wave of this row group
 #pragma unroll
         for (int r = 0; r < MP / 8; ++r) {
@@ -583,30 +633,9 @@ __global__ __launch_bounds__(kHB) void gspat_q(HoloArgs a, int closing) {
                 gam[r].y += __shfl_xor(gam[r].y, off);
             }
         }
-        if (jj == 0) {
+        if (jj == 0)
 #pragma unroll
-            for (int r = 0; r < MP / 8; ++r) {
-                const int i = grp + 8 * r;
-                const float p0 = s_a[i];
-                const float inv = p0 * p0 / (gam[r].x * gam[r].x + gam[r].y * gam[r].y);
-                s_p[i] = i < (int)M ? make_float2(gam[r].x * inv, gam[r].y * inv) : make_float2(0.f, 0.f);
-            }
-        }
-        lds_sync();
-    }
-    StoredChunk<MP> cn;
-    load_chunk<MP>(a, k, 0, M, cn);
-    for (uint32_t j0 = 0; j0 < a.n_packed; j0 += kKC) {
-        const StoredChunk<MP> c = cn;
-        if (j0 + kKC < a.n_packed) load_chunk<MP>(a, k, j0 + kKC, M, cn);
-        column_scale_stored(c.d, j0, a.n, s_inv);
-        chunk_q(c.g);
-        if (t < kKC && j0 + t < a.n) {
-            const float2 q = s_q[t];
-            a.out_amp[(uint64_t)b * a.n + j0 + t] = hypotf(q.x, q.y);
-            a.out_phase[(uint64_t)b * a.n + j0 + t] = atan2f(q.y, q.x);
-        }
-        lds_sync();
+            for (int r = 0; r < MP / 8; ++r) a.gam[(uint64_t)blockIdx.x * MP + grp + 8 * r] = gam[r];
     }
 }
 
@@ -809,6 +838,19 @@ __global__ __launch_bounds__(kHB) void greedy_kernel(HoloArgs a, const float2* _
 }
 
 // ---------------------------------------------------------------- host side
+// Kernel A workgroups per GS-PAT instance: each takes a contiguous range of
+// source chunks and writes a partial R that kernel B sums. A function of the
+// array size only (the largest power of two <= chunks / 16, at most 8), so an
+// instance is computed the same way in any batch. One workgroup per instance
+// leaves most of the chip idle or waiting on a tail round (R's chunk loop is
+// serial, with four barriers per chunk).
+static uint32_t gspat_split(uint32_t n_packed) {
+    const uint32_t chunks = n_packed / kKC;
+    uint32_t s = 1;
+    while (s < 8 && 2 * s * 16 <= chunks) s *= 2;
+    return s;
+}
+
 static int padded_m(uint32_t m, bool mfma) {
     if (m <= 16) return mfma ? 32 : 16;
     if (m <= 32) return 32;
@@ -820,25 +862,25 @@ static int padded_m(uint32_t m, bool mfma) {
 template <int MP>
 static hipError_t launch_gspat_bucket(HoloArgs a, int src, int precision, uint32_t nb, uint32_t iters,
                                       hipStream_t st) {
-    const dim3 grid(nb), blk(kHB);
+    const dim3 grid(nb), grid_a(nb * a.split), blk(kHB);
     if (precision == AFH_HOLO_FP32) {
         if (src == kT4010A1)
-            hipLaunchKernelGGL((gspat_r_fp32<kT4010A1, MP>), grid, blk, 0, st, a);
+            hipLaunchKernelGGL((gspat_r_fp32<kT4010A1, MP>), grid_a, blk, 0, st, a);
         else
-            hipLaunchKernelGGL((gspat_r_fp32<kSphere, MP>), grid, blk, 0, st, a);
+            hipLaunchKernelGGL((gspat_r_fp32<kSphere, MP>), grid_a, blk, 0, st, a);
     } else {
         if constexpr (MP >= 32) {
             const bool split = precision == AFH_HOLO_MIXED_SPLIT;
             if (src == kT4010A1) {
                 if (split)
-                    hipLaunchKernelGGL((gspat_r_mfma<kT4010A1, MP, true>), grid, blk, 0, st, a);
+                    hipLaunchKernelGGL((gspat_r_mfma<kT4010A1, MP, true>), grid_a, blk, 0, st, a);
                 else
-                    hipLaunchKernelGGL((gspat_r_mfma<kT4010A1, MP, false>), grid, blk, 0, st, a);
+                    hipLaunchKernelGGL((gspat_r_mfma<kT4010A1, MP, false>), grid_a, blk, 0, st, a);
             } else {
                 if (split)
-                    hipLaunchKernelGGL((gspat_r_mfma<kSphere, MP, true>), grid, blk, 0, st, a);
+                    hipLaunchKernelGGL((gspat_r_mfma<kSphere, MP, true>), grid_a, blk, 0, st, a);
                 else
-                    hipLaunchKernelGGL((gspat_r_mfma<kSphere, MP, false>), grid, blk, 0, st, a);
+                    hipLaunchKernelGGL((gspat_r_mfma<kSphere, MP, false>), grid_a, blk, 0, st, a);
             }
         } else {
             return hipErrorInvalidValue;
@@ -846,7 +888,8 @@ static hipError_t launch_gspat_bucket(HoloArgs a, int src, int precision, uint32
     }
     const int closing = precision == AFH_HOLO_MIXED_CLOSING ? 1 : 0;
     hipLaunchKernelGGL((gspat_iter<MP>), grid, blk, 0, st, a, iters, closing);
-    hipLaunchKernelGGL((gspat_q<MP>), grid, blk, 0, st, a, closing);
+    if (closing) hipLaunchKernelGGL((gspat_q<MP>), grid_a, blk, 0, st, a, closing, (int)kQGamma);
+    hipLaunchKernelGGL((gspat_q<MP>), grid_a, blk, 0, st, a, closing, (int)kQFinal);
     return hipGetLastError();
 }
 
@@ -900,10 +943,11 @@ int32_t holo_solve(Context* ctx, const HoloBatch& hb, hipStream_t st) {
     for (int q = 0; q < 4; ++q) {
         const size_t nb = lists[q].size(), mp = kinds[q];
         if (hb.algo == kAlgoGSPAT) {
-            r_elems += nb * mp * mp;
+            r_elems += nb * mp * mp * gspat_split(np);
             p_elems += nb * mp;
             g_elems += nb * mp * np;
             d_elems += nb * np;
+            q_elems += nb * mp * gspat_split(np);  // partial gammas
         } else if (hb.algo == kAlgoGS) {
             g_elems += nb * mp * np;
             q_elems += nb * np;
@@ -969,6 +1013,7 @@ int32_t holo_solve(Context* ctx, const HoloBatch& hb, hipStream_t st) {
     a.cells = dir_cells_device();
     a.out_amp = hb.out_amp;
     a.out_phase = hb.out_phase;
+    a.split = gspat_split(np);
     if (!a.cells) {
         set_error("holo: directivity table allocation failed");
         return AFH_ERR_HIP;
@@ -987,6 +1032,7 @@ int32_t holo_solve(Context* ctx, const HoloBatch& hb, hipStream_t st) {
         a.G = reinterpret_cast<float2*>(wbase + o_g) + g_pos;
         a.q = reinterpret_cast<float2*>(wbase + o_q) + q_pos;
         a.d = reinterpret_cast<float*>(wbase + o_d) + d_pos;
+        a.gam = reinterpret_cast<float2*>(wbase + o_q) + q_pos;
         hipError_t e = hipSuccess;
         if (hb.algo == kAlgoGSPAT) {
             switch (mp) {
@@ -995,10 +1041,11 @@ int32_t holo_solve(Context* ctx, const HoloBatch& hb, hipStream_t st) {
                 case 64: e = launch_gspat_bucket<64>(a, hb.source_type, hb.precision, nb, hb.iterations, st); break;
                 default: e = launch_gspat_bucket<128>(a, hb.source_type, hb.precision, nb, hb.iterations, st); break;
             }
-            r_pos += (size_t)nb * mp * mp;
+            r_pos += (size_t)nb * mp * mp * a.split;
             p_pos += (size_t)nb * mp;
             g_pos += (size_t)nb * mp * np;
             d_pos += (size_t)nb * np;
+            q_pos += (size_t)nb * mp * a.split;
         } else if (hb.algo == kAlgoGS) {
             const uint32_t gstride = (uint32_t)(mp * np);
             switch (mp) {
