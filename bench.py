@@ -70,6 +70,20 @@ def cpu_baseline(src, k, a, order, counts, origin, res):
                       f"{t_total:.1f} s wall on {threads} threads"}
 
 
+def cpu_baseline_instances(run_one, count, what):
+    """A solver restated in C (FP32, single-threaded per call, as the
+    reference's optimizers are) over `count` instances, one instance per host
+    thread at a time."""
+    from concurrent.futures import ThreadPoolExecutor
+    threads = cpu_threads()
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        list(ex.map(run_one, range(count)))
+    dt = time.perf_counter() - t0
+    return {"value": round(count / dt, 1), "unit": "solves/s", "cores": threads, "kind": "port",
+            "sample": f"{what}: {count} instances, {dt:.2f} s wall on {threads} threads"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -196,9 +210,9 @@ def main():
                            "note": "launch + copy of the 8 B/point field into pinned host memory"},
     }
     if not args.no_gspat:
-        result["gspat"] = bench_gspat(torch, afc, D, O, src, k, a, args.gspat_batch)
+        result["gspat"] = bench_gspat(torch, afc, D, O, src, k, a, args.gspat_batch, not args.no_cpu_baseline)
     if not args.no_lm:
-        result["lm"] = bench_lm(torch, D, O)
+        result["lm"] = bench_lm(torch, D, O, not args.no_cpu_baseline)
     if not args.no_greedy:
         result["greedy"] = bench_greedy(torch, D, O)
     if not args.no_serve:
@@ -360,7 +374,7 @@ def bench_greedy(torch, D, O):
             "batch_ms": round(ms, 3), "objective_instance0": objective(ph), "objective_instance0_fp64_oracle": objective(ref)}
 
 
-def bench_lm(torch, D, O):
+def bench_lm(torch, D, O, cpu=True):
     """Batched FP64 LM (lm.rs:40-105): 64 instances on the 249- and on the
     996-transducer array, 16 foci each (as the GS-PAT workload), targets 1,
     50 iterations."""
@@ -419,6 +433,11 @@ def bench_lm(torch, D, O):
             # phases are compared after the FP32 rounding the output format applies
             entry["focal_amp_rel_err_vs_fp64_20it_max"] = max(errs)
             entry["phase_max_abs_diff_vs_fp64_20it_rad"] = max(dph)
+        if name == "n249" and cpu:
+            entry["cpu_baseline"] = cpu_baseline_instances(
+                lambda b: O.lm(1, src, k, a, foci[offs[b]:offs[b + 1]], amps[offs[b]:offs[b + 1]], "f32", k_max=50),
+                64, "the 64 instances, 50 iterations, dense f32 QR as lm.rs:72")
+            log(f"lm n249 cpu baseline: {entry['cpu_baseline']['value']} solves/s")
         if name == "n249":
             # the other two solvers of nls/ on the same machinery and workload
             for algo, key, kmax in ((D.NLS_GAUSS_NEWTON, "gauss_newton_n249", 200),
@@ -480,7 +499,7 @@ def bench_lm(torch, D, O):
     return out
 
 
-def bench_gspat(torch, afc, D, O, src, k, a, batch):
+def bench_gspat(torch, afc, D, O, src, k, a, batch, cpu=True):
     """Batched GS-PAT (gs_pat.rs:38-81) on device-resident inputs."""
     src = src.copy()
     src[:, 7] = 0.0  # GS-PAT zeroes the phases (gs_pat.rs:39-41)
@@ -555,6 +574,12 @@ def bench_gspat(torch, afc, D, O, src, k, a, batch):
     out["speedup_mixed_split_vs_plain"] = round(out["mixed_split"]["solves_per_s"] /
                                                 out["plain_fp32_unbatched"]["solves_per_s"], 1)
     log(f"gspat plain FP32 unbatched: {reps / sum(lat):.0f} solves/s")
+    if cpu:
+        count = min(256, batch)
+        out["cpu_baseline"] = cpu_baseline_instances(
+            lambda b: O.gspat(1, src, k, a, foci[offs[b]:offs[b + 1]], amps[offs[b]:offs[b + 1]], "f32"), count,
+            f"the first {count} instances of the batch")
+        log(f"gspat cpu baseline: {out['cpu_baseline']['value']} solves/s")
     return out
 
 
