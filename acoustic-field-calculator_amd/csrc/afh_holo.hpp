@@ -77,6 +77,9 @@ struct Context {
 };
 
 int32_t holo_solve(Context* ctx, const HoloBatch& hb, hipStream_t st);
+// Workspace bytes a GS-PAT holo_solve of `batch` instances on an array of
+// n_packed sources needs at most (every instance with 65..128 foci).
+size_t gspat_workspace_bound(uint32_t n_packed, uint32_t batch);
 Context& default_context();
 
 // Solves one instance on the device and returns per-source amplitude and

@@ -851,6 +851,15 @@ static uint32_t gspat_split(uint32_t n_packed) {
     return s;
 }
 
+size_t gspat_workspace_bound(uint32_t n_packed, uint32_t batch) {
+    // holo_solve's layout with every instance in the MP = 128 bucket
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t B = batch, np = n_packed, mp = 128, sp = gspat_split(n_packed);
+    return al((B + 1) * sizeof(uint32_t)) + al(B * sizeof(uint32_t)) + al(np * AFH_SOURCE_STRIDE * sizeof(float)) +
+           al(B * mp * mp * sp * sizeof(float2)) + al(B * mp * sizeof(float2)) + al(B * mp * np * sizeof(float2)) +
+           al(B * mp * sp * sizeof(float2)) + al(B * np * sizeof(float));
+}
+
 static int padded_m(uint32_t m, bool mfma) {
     if (m <= 16) return mfma ? 32 : 16;
     if (m <= 32) return 32;

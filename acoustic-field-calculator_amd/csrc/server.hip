@@ -17,6 +17,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -112,6 +113,19 @@ struct Request {
     int32_t status = AFH_OK;
 };
 
+// A completed launch's pinned staging, released once the results of all its
+// requests have been copied out.
+struct PinHold {
+    size_t off = 0, bytes = 0;
+    std::atomic<int> left{0};
+};
+
+// Results of one request to copy from pinned staging to the caller's memory.
+struct CopyJob {
+    Request* r = nullptr;
+    std::shared_ptr<PinHold> hold;
+};
+
 // One launch in flight: a field request or a GS-PAT batch.
 struct Flight {
     std::vector<Request*> reqs;
@@ -119,6 +133,7 @@ struct Flight {
     size_t dev_off = 0, dev_bytes = 0, pin_off = 0, pin_bytes = 0;
     int stream = 0;
     double work = 0;  // source-point terms, for the stream choice
+    bool gspat = false;
 };
 
 struct Server {
@@ -135,12 +150,22 @@ struct Server {
     std::deque<Request*> pending;
     std::unordered_map<uint64_t, Request*> by_ticket;
     std::deque<Flight> flights;
+    // The copies out of pinned staging run on a few threads of their own: one
+    // thread copying every result (several GB/s under load) would throttle
+    // launching and, by holding staging space, admission.
+    static constexpr int kCopiers = 4;
+    std::mutex copy_mu, pin_mu;  // pin_mu guards the pinned arena
+    std::condition_variable cv_copy;
+    std::deque<CopyJob> copies;
+    std::vector<std::thread> copiers;
+    bool copy_stop = false;
     std::vector<double> latencies[2];  // by request kind
     uint64_t next_ticket = 1, completed = 0;
     bool stop = false;
     std::thread worker;
     int next_stream = 0;
     int device = 0;
+    int gspat_inflight = 0;  // GS-PAT batches on the GPU
 
     ~Server() {
         {
@@ -149,6 +174,13 @@ struct Server {
         }
         cv_submit.notify_all();
         if (worker.joinable()) worker.join();
+        {
+            std::lock_guard<std::mutex> g(copy_mu);
+            copy_stop = true;
+        }
+        cv_copy.notify_all();
+        for (auto& t : copiers)
+            if (t.joinable()) t.join();
         for (auto s : streams) (void)hipStreamSynchronize(s);
         ctxs.clear();
         for (auto s : streams) (void)hipStreamDestroy(s);
@@ -180,12 +212,16 @@ struct Server {
     // Launches what fits, in arrival order: field requests one launch each (a
     // field request that does not fit holds back the later ones, so large
     // requests are not starved); GS-PAT requests grouped per array into batched
-    // launches of at most max_gspat_batch instances.
+    // launches of at most max_gspat_batch instances. New GS-PAT batches start
+    // only when the previous ones have completed: requests that arrive in the
+    // meantime wait and form the next, larger batch (a batch costs a few
+    // latency-bound launches whatever its size, so under load this raises the
+    // solve rate; at low load a request still starts at once).
     void dispatch(const std::vector<Request*>& todo, std::vector<Request*>& admitted) {
         std::map<uint32_t, std::vector<Request*>> groups;
         for (Request* r : todo) {
             if (r->kind == kReqGSPAT) {
-                groups[r->array_id].push_back(r);
+                if (gspat_inflight == 0) groups[r->array_id].push_back(r);
                 continue;
             }
             if (!launch_field(r)) break;
@@ -214,7 +250,7 @@ struct Server {
         f.dev_bytes = Arena::round(out_b) + Arena::round(pts_b);  // out | points
         f.pin_bytes = out_b + pts_b;
         if (!dev.alloc(f.dev_bytes, &f.dev_off)) return false;
-        if (!pin.alloc(f.pin_bytes, &f.pin_off)) {
+        if (!pin_alloc(f.pin_bytes, &f.pin_off)) {
             dev.release(f.dev_off, f.dev_bytes);
             return false;
         }
@@ -262,7 +298,7 @@ struct Server {
         f.dev_bytes = o_op + out_b;
         f.pin_bytes = f.dev_bytes;
         if (!dev.alloc(f.dev_bytes, &f.dev_off)) return false;
-        if (!pin.alloc(f.pin_bytes, &f.pin_off)) {
+        if (!pin_alloc(f.pin_bytes, &f.pin_off)) {
             dev.release(f.dev_off, f.dev_bytes);
             return false;
         }
@@ -312,37 +348,83 @@ struct Server {
             f.reqs.push_back(r);
         }
         if (e != hipSuccess || rc != AFH_OK) (void)hipStreamSynchronize(st);
+        f.gspat = true;
+        ++gspat_inflight;
         flights.push_back(f);
         return true;
     }
 
-    // Completes finished flights (caller holds no lock); returns how many.
-    int reap(std::vector<Request*>& done_out) {
+    bool pin_alloc(size_t bytes, size_t* off) {
+        std::lock_guard<std::mutex> g(pin_mu);
+        return pin.alloc(bytes, off);
+    }
+
+    // Hands finished flights to the copy threads (caller holds no lock);
+    // returns how many flights finished.
+    int reap() {
         int n = 0;
+        std::vector<CopyJob> jobs;
         for (auto it = flights.begin(); it != flights.end();) {
             const hipError_t q = it->ev ? hipEventQuery(it->ev) : hipSuccess;
             if (q == hipErrorNotReady) {
                 ++it;
                 continue;
             }
-            const double t = now_ms();
+            auto hold = std::make_shared<PinHold>();
+            hold->off = it->pin_off;
+            hold->bytes = it->pin_bytes;
+            hold->left = (int)it->reqs.size();
             for (Request* r : it->reqs) {
                 if (q != hipSuccess && r->status == AFH_OK) r->status = AFH_ERR_HIP;
-                if (r->status == AFH_OK) {
-                    memcpy(r->out, pin_base + r->pin_off, r->out_bytes);
-                    if (r->kind == kReqGSPAT) memcpy(r->out2, pin_base + r->pin_off2, r->out_bytes);
-                }
-                r->t_done = t;
-                done_out.push_back(r);
-                ++n;
+                jobs.push_back({r, hold});
             }
             if (it->ev) (void)hipEventDestroy(it->ev);
+            if (it->gspat) --gspat_inflight;
             outstanding[it->stream] -= it->work;
             dev.release(it->dev_off, it->dev_bytes);
-            pin.release(it->pin_off, it->pin_bytes);
             it = flights.erase(it);
+            ++n;
+        }
+        if (!jobs.empty()) {
+            {
+                std::lock_guard<std::mutex> g(copy_mu);
+                copies.insert(copies.end(), jobs.begin(), jobs.end());
+            }
+            cv_copy.notify_all();
         }
         return n;
+    }
+
+    // Copy thread: results to the caller's buffers, then the request is done.
+    void copy_loop() {
+        for (;;) {
+            CopyJob j;
+            {
+                std::unique_lock<std::mutex> g(copy_mu);
+                cv_copy.wait(g, [&] { return copy_stop || !copies.empty(); });
+                if (copies.empty()) return;
+                j = copies.front();
+                copies.pop_front();
+            }
+            Request* r = j.r;
+            if (r->status == AFH_OK) {
+                memcpy(r->out, pin_base + r->pin_off, r->out_bytes);
+                if (r->kind == kReqGSPAT) memcpy(r->out2, pin_base + r->pin_off2, r->out_bytes);
+            }
+            const double t = now_ms();
+            if (--j.hold->left == 0) {
+                std::lock_guard<std::mutex> g(pin_mu);
+                pin.release(j.hold->off, j.hold->bytes);
+            }
+            {
+                std::lock_guard<std::mutex> g(mu);
+                r->t_done = t;
+                r->state = kDone;
+                latencies[r->kind].push_back(t - r->t_submit);
+                ++completed;
+            }
+            cv_done.notify_all();
+        }
     }
 
     void run() {
@@ -367,19 +449,7 @@ struct Server {
                     if (r->state == kPending) keep.push_back(r);
                 pending.swap(keep);
             }
-            std::vector<Request*> done;
-            reap(done);
-            if (!done.empty()) {
-                std::lock_guard<std::mutex> g(mu);
-                for (Request* r : done) {
-                    r->state = kDone;
-                    latencies[r->kind].push_back(r->t_done - r->t_submit);
-                    ++completed;
-                }
-                cv_done.notify_all();
-            } else if (admitted.empty()) {
-                std::this_thread::sleep_for(std::chrono::microseconds(20));
-            }
+            if (reap() == 0 && admitted.empty()) std::this_thread::sleep_for(std::chrono::microseconds(20));
         }
     }
 };
@@ -433,20 +503,17 @@ int32_t AFS_Create(const AFS_Config* cfg, void** out) {
         s->outstanding.push_back(0.0);
         s->ctxs.emplace_back(new Context());
     }
-    // the GS-PAT workspace (R and p for a full batch of 128-focus instances)
-    // is reserved now rather than grown while serving
-    {
-        const size_t mb = cfg->max_gspat_batch;
-        if (!s->ctxs[0]->work.reserve(mb * (128 * 128 + 128) * sizeof(float2) + (2 * mb + 2) * 4 + 4096) ||
-            !s->ctxs[0]->staging.reserve((2 * mb + 2) * 4))
-            return AFH_ERR_HIP;
-    }
+    // the GS-PAT staging for a full batch is reserved now rather than grown
+    // while serving; the workspace, which depends on the array size, when an
+    // array is added
+    if (!s->ctxs[0]->staging.reserve((2 * (size_t)cfg->max_gspat_batch + 2) * 4)) return AFH_ERR_HIP;
     if (!dir_cells_device()) {
         set_error("AFS_Create: directivity table");
         return AFH_ERR_HIP;
     }
     Server* raw = s.get();
     s->worker = std::thread([raw] { raw->run(); });
+    for (int i = 0; i < Server::kCopiers; ++i) s->copiers.emplace_back([raw] { raw->copy_loop(); });
     *out = s.release();
     return AFH_OK;
 }
@@ -472,6 +539,20 @@ int32_t AFS_AddArray(void* srv, const void* sources, const float* wavenums, cons
     if (!hip_ok(hipMemcpy(A.packed, host.data(), host.size() * sizeof(float), hipMemcpyHostToDevice),
                 "AFS_AddArray: upload"))
         return AFH_ERR_HIP;
+    {
+        // the GS-PAT workspace for a full batch of 128-focus instances on this
+        // array, reserved now rather than grown while serving (growing frees the
+        // old block, so work queued on the GS-PAT stream must finish first)
+        Context* ctx = s->ctxs[0].get();
+        std::lock_guard<std::mutex> g(ctx->mu);
+        const size_t need = gspat_workspace_bound(A.np, s->cfg.max_gspat_batch);
+        if (need > ctx->work.cap &&
+            (!hip_ok(hipStreamSynchronize(s->streams[0]), "AFS_AddArray: hipStreamSynchronize") ||
+             !ctx->work.reserve(need))) {
+            (void)hipFree(A.packed);
+            return AFH_ERR_HIP;
+        }
+    }
     std::lock_guard<std::mutex> g(s->mu);
     *array_id = (uint32_t)s->arrays.size();
     s->arrays.push_back(A);
