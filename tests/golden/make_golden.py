@@ -2,10 +2,12 @@
 
 The reference ships no fixtures or golden vectors for the field or optimizer
 path (its only tests are in fmath), so these vectors are self-derived: FP64
-restatements of system/uniform_medium.rs:111-128, gs_pat.rs:38-81 and
-nls/lm.rs:40-105 on small seeded inputs. Each fixture is also checked by a
-closed form in tests/test_golden.py (focus identity, GS-PAT single focus,
-LM single focus), so a wrong fixture cannot pass silently.
+restatements of system/uniform_medium.rs:111-128, gs_pat.rs:38-81,
+nls/{lm,gauss_newton,gradient_descent,apo}.rs, combination/greedy.rs and
+matrix/{long,horn}.rs on small seeded inputs. Each fixture is also checked by a
+property in tests/test_golden.py that does not reuse the oracle's formulas
+(focus identity, single-focus closed forms, candidate phases, objective
+decrease, camax normalisation), so a wrong fixture cannot pass silently.
 
 Run from the repository root:  python tests/golden/make_golden.py
 Output: tests/golden/*.npz (numpy savez, no pickled objects).
@@ -74,6 +76,40 @@ def lm(m, seed, k_max):
                 iterations=np.int32(iters))
 
 
+def greedy(m, seed, div):
+    src, k, a, foci, amps = optim_problem(m, seed)
+    phase = O.greedy(1, src, k, a, foci, amps, div, "f64")
+    return dict(src=src, k=k, a=a, foci=foci, amps=amps, phase_div=np.int32(div), out_phase=phase)
+
+
+def nls(fn, m, seed, k_max):
+    src, k, a, foci, amps = optim_problem(m, seed)
+    phase, x, iters = fn(1, src, k, a, foci, amps, "f64", k_max=k_max)
+    return dict(src=src, k=k, a=a, foci=foci, amps=amps, k_max=np.int32(k_max), out_phase=phase, out_x=x,
+                iterations=np.int32(iters))
+
+
+def apo(m, seed, lam):
+    src, k, a, foci, amps = optim_problem(m, seed)
+    amps = (amps * 0.2).astype(np.float32)
+    lam = np.float32(lam)  # the C ABI takes lambda as f32
+    amp, phase, iters = O.apo(1, src, k, a, foci, amps, float(lam), "f64")
+    return dict(src=src, k=k, a=a, foci=foci, amps=amps, lam=lam, out_amp=amp, out_phase=phase,
+                iterations=np.int32(iters))
+
+
+def long_(m, seed):
+    src, k, a, foci, amps = optim_problem(m, seed)
+    amp, phase = O.long_(1, src, k, a, foci, amps)
+    return dict(src=src, k=k, a=a, foci=foci, amps=amps, out_amp=amp, out_phase=phase)
+
+
+def horn(m, seed, rseed):
+    src, k, a, foci, amps = optim_problem(m, seed)
+    amp, phase = O.horn(1, src, k, a, foci, amps, seed=rseed, lam=np.float32(0.8), alpha=np.float32(1e-5))
+    return dict(src=src, k=k, a=a, foci=foci, amps=amps, seed=np.uint64(rseed), out_amp=amp, out_phase=phase)
+
+
 def main():
     fixtures = {
         "field_t4010a1.npz": field_t4010a1(),
@@ -82,6 +118,12 @@ def main():
         "gspat_m5.npz": gspat(5, 22),
         "lm_m1.npz": lm(1, 23, 200),
         "lm_m4_20it.npz": lm(4, 24, 20),
+        "greedy_m3_div16.npz": greedy(3, 25, 16),
+        "gauss_newton_m3.npz": nls(O.gauss_newton, 3, 26, 200),
+        "gradient_descent_m2.npz": nls(O.gradient_descent, 2, 27, 10000),
+        "apo_m3.npz": apo(3, 28, 0.05),
+        "long_m4.npz": long_(4, 29),
+        "horn_m4.npz": horn(4, 30, 3),
     }
     for name, arrays in fixtures.items():
         np.savez(os.path.join(HERE, name), **arrays)

@@ -10,7 +10,13 @@ satisfies a closed form that does not use the oracle's formulas:
   * gspat_m5: focal amplitudes within 15 % of the targets (GS-PAT's closing
     normalisation gamma / |gamma|^2 p0^2);
   * lm_m1: a single focus reaches the target amplitude after 200 iterations;
-  * lm_m4_20it: the 20-iteration iterate lowers F(x) = |B e^{ix}|^2.
+  * lm_m4_20it: the 20-iteration iterate lowers F(x) = |B e^{ix}|^2;
+  * greedy_m3_div16: every phase is a candidate, the objective is below that
+    of zero phases;
+  * gauss_newton_m3, gradient_descent_m2: the iterate lowers F(x) below a
+    fraction of F(0);
+  * apo_m3, long_m4, horn_m4: outputs camax-normalised (max_j |Re q_j| +
+    |Im q_j| = 1); Long and Horn reproduce the target ratios.
 """
 import math
 import os
@@ -90,3 +96,62 @@ def test_lm_fixtures(oracle):
     f0, _ = oracle.lm_eval(1, g["src"], g["k"], g["a"], g["foci"], g["amps"], np.zeros_like(g["out_x"]))
     f1, _ = oracle.lm_eval(1, g["src"], g["k"], g["a"], g["foci"], g["amps"], g["out_x"])
     assert f1 < 0.5 * f0
+
+
+def camax_norm(amp, phase):
+    """max_j amp_j (|cos phi_j| + |sin phi_j|): 1 for outputs scaled by
+    nalgebra's camax (max_j |Re q_j| + |Im q_j|) with unit source amplitudes."""
+    return float(np.max(amp * (np.abs(np.cos(phase)) + np.abs(np.sin(phase)))))
+
+
+def test_greedy_fixture(oracle):
+    """Every phase is one of the phase_div candidates 2 pi i / div, and the
+    objective sum_i |a_i - |p_i|| is below that of all phases zero."""
+    g = load("greedy_m3_div16.npz")
+    div = int(g["phase_div"])
+    ph = oracle.greedy(1, g["src"], g["k"], g["a"], g["foci"], g["amps"], div, "f64")
+    assert np.allclose(np.angle(np.exp(1j * (ph - g["out_phase"]))), 0, atol=1e-12)
+    idx = np.mod(g["out_phase"], 2 * np.pi) / (2 * np.pi / div)
+    assert np.max(np.abs(idx - np.round(idx))) <= 1e-9
+
+    def objective(phase):
+        return float(np.sum(np.abs(g["amps"] - np.abs(focal_f64(oracle, g["src"], g["k"], g["a"], g["foci"],
+                                                                phase=phase)))))
+
+    assert objective(g["out_phase"]) < 0.5 * objective(np.zeros_like(g["out_phase"]))
+
+
+@pytest.mark.parametrize("name,fn,bound", [("gauss_newton_m3.npz", "gauss_newton", 0.01),
+                                           ("gradient_descent_m2.npz", "gradient_descent", 0.05)])
+def test_nls_fixtures(oracle, name, fn, bound):
+    """The fixture's iterate lowers F(x) = |B e^{ix}|^2 below `bound` F(0)."""
+    g = load(name)
+    phase, x, it = getattr(oracle, fn)(1, g["src"], g["k"], g["a"], g["foci"], g["amps"], "f64",
+                                       k_max=int(g["k_max"]))
+    assert it == int(g["iterations"])
+    assert np.allclose(np.angle(np.exp(1j * (phase - g["out_phase"]))), 0, atol=1e-9)
+    f0, _ = oracle.lm_eval(1, g["src"], g["k"], g["a"], g["foci"], g["amps"], np.zeros_like(g["out_x"]))
+    f1, _ = oracle.lm_eval(1, g["src"], g["k"], g["a"], g["foci"], g["amps"], g["out_x"])
+    assert f1 < bound * f0
+
+
+@pytest.mark.parametrize("name,spread", [("apo_m3.npz", None), ("long_m4.npz", 0.15), ("horn_m4.npz", 0.05)])
+def test_amp_phase_fixtures(oracle, name, spread):
+    """The oracle reproduces the fixture; the outputs are camax-normalised;
+    Long and Horn reproduce the target amplitude ratios within `spread`."""
+    g = load(name)
+    if name.startswith("apo"):
+        amp, ph, it = oracle.apo(1, g["src"], g["k"], g["a"], g["foci"], g["amps"], float(g["lam"]), "f64")
+        assert it == int(g["iterations"])
+    elif name.startswith("long"):
+        amp, ph = oracle.long_(1, g["src"], g["k"], g["a"], g["foci"], g["amps"])
+    else:
+        amp, ph = oracle.horn(1, g["src"], g["k"], g["a"], g["foci"], g["amps"], seed=int(g["seed"]),
+                              lam=np.float32(0.8), alpha=np.float32(1e-5))
+    assert np.allclose(amp, g["out_amp"], rtol=1e-10, atol=1e-14)
+    assert np.allclose(np.angle(np.exp(1j * (ph - g["out_phase"]))), 0, atol=1e-9)
+    assert camax_norm(g["out_amp"], g["out_phase"]) == pytest.approx(1.0, abs=1e-12)
+    if spread is not None:
+        p = np.abs(focal_f64(oracle, g["src"], g["k"], g["a"], g["foci"], g["out_amp"], g["out_phase"]))
+        r = p / g["amps"]
+        assert r.min() >= (1 - spread) * r.max()

@@ -4,7 +4,10 @@ vectors), through the device C ABI with the fixtures' own k and alpha.
 Tolerances: field within 1e-5 relative L2 of the FP32 CPU path and 5e-6 of
 FP64; GS-PAT focal amplitudes within 1e-3 relative of the fixture's
 (FP32 and MIXED_SPLIT); LM: single focus at its target within 1e-5 after 200
-iterations, and the 20-iteration phases within 1e-5 rad of the fixture.
+iterations, and the 20-iteration phases within 1e-5 rad of the fixture;
+Greedy: candidate phases, objective within 2 %; Gauss-Newton and gradient
+descent: phases within 1e-5 rad; APO, Long, Horn: amplitudes and phases within
+1e-5.
 """
 import os
 
@@ -115,3 +118,96 @@ def test_lm_golden(afc, oracle, torch_cuda):
             log(f"  {name}: iterations {int(iters.cpu()[0])} vs {int(g['iterations'])}, max phase diff {d.max():.2e}")
             assert int(iters.cpu()[0]) == int(g["iterations"])
             assert d.max() <= 1e-5
+
+
+class _Dev:
+    """A fixture's inputs on the device (T4010A1 sources)."""
+
+    def __init__(self, g, amps=None):
+        import torch
+        from afc import device as D
+        self.D, self.torch = D, torch
+        dev = torch.device("cuda:0")
+        self.n = g["src"].shape[0]
+        packed = D.pack_sources(1, g["src"], g["k"], g["a"])
+        self.np_ = packed.shape[0]
+        self.src = torch.from_numpy(packed).to(dev)
+        self.foci = torch.from_numpy(np.ascontiguousarray(g["foci"], np.float32)).to(dev)
+        self.amps = torch.from_numpy(np.ascontiguousarray(g["amps"] if amps is None else amps, np.float32)).to(dev)
+        self.offs = np.array([0, g["foci"].shape[0]], np.uint32)
+        self.dev = dev
+        self.ctx = D.Context()
+
+    def args(self):
+        return (self.ctx, self.src.data_ptr(), self.np_, self.n, 1, self.foci.data_ptr(), self.amps.data_ptr(),
+                self.offs)
+
+    def out(self, dtype=None):
+        return self.torch.empty((1, self.n), dtype=dtype or self.torch.float32, device=self.dev)
+
+    def done(self, *ts):
+        self.torch.cuda.synchronize()
+        self.ctx.close()
+        return [t.cpu().numpy()[0] for t in ts]
+
+
+def wrapdiff(x, y):
+    return np.abs(np.angle(np.exp(1j * (np.asarray(x, np.float64) - np.asarray(y, np.float64)))))
+
+
+def test_greedy_golden(afc, oracle, torch_cuda):
+    """Phases are candidates; the objective sum_i |a_i - |p_i|| is within 2 %
+    of the fixture's (a near-tie may resolve differently in f32)."""
+    log("case: golden greedy_m3_div16")
+    g = load("greedy_m3_div16.npz")
+    d = _Dev(g)
+    op = d.out()
+    d.D.greedy_solve(*d.args(), int(g["phase_div"]), op.data_ptr())
+    (ph,) = d.done(op)
+    div = int(g["phase_div"])
+    idx = np.mod(ph.astype(np.float64), 2 * np.pi) / (2 * np.pi / div)
+    assert np.max(np.abs(idx - np.round(idx))) <= 1e-5
+
+    def objective(phase):
+        return float(np.sum(np.abs(g["amps"] - focal(oracle, g, g["src"][:, 6], phase))))
+
+    o, o_ref = objective(ph), objective(g["out_phase"])
+    log(f"  objective {o:.6g} vs fixture {o_ref:.6g}")
+    assert abs(o - o_ref) <= 0.02 * o_ref
+
+
+@pytest.mark.parametrize("name,algo,it_tol", [("gauss_newton_m3.npz", 1, 0), ("gradient_descent_m2.npz", 2, 2)])
+def test_nls_golden(afc, oracle, torch_cuda, name, algo, it_tol):
+    log(f"case: golden {name}")
+    g = load(name)
+    d = _Dev(g)
+    op = d.out()
+    it = d.torch.zeros(1, dtype=d.torch.int32, device=d.dev)
+    d.D.nl_solve(*d.args(), algo, int(g["k_max"]), op.data_ptr(), it.data_ptr())
+    ph, its = d.done(op, it.view(1, 1))
+    diff = float(wrapdiff(ph, g["out_phase"]).max())
+    log(f"  iterations {int(its[0])} vs {int(g['iterations'])}, max phase diff {diff:.2e}")
+    assert abs(int(its[0]) - int(g["iterations"])) <= it_tol
+    assert diff <= 1e-5
+
+
+@pytest.mark.parametrize("name", ["apo_m3.npz", "long_m4.npz", "horn_m4.npz"])
+def test_amp_phase_golden(afc, oracle, torch_cuda, name):
+    """Amplitudes within 1e-5 of the largest, phases within 1e-5 rad where the
+    amplitude is above 1e-3 of the largest."""
+    log(f"case: golden {name}")
+    g = load(name)
+    d = _Dev(g)
+    oa, op = d.out(), d.out()
+    if name.startswith("apo"):
+        d.D.apo_solve(*d.args(), float(g["lam"]), 200, oa.data_ptr(), op.data_ptr())
+    elif name.startswith("long"):
+        d.D.long_solve(*d.args(), oa.data_ptr(), op.data_ptr())
+    else:
+        d.D.horn_solve(*d.args(), oa.data_ptr(), op.data_ptr(), seed=int(g["seed"]))
+    amp, ph = d.done(oa, op)
+    big = g["out_amp"] > 1e-3 * g["out_amp"].max()
+    ea = float(np.max(np.abs(amp - g["out_amp"]) / g["out_amp"].max()))
+    ep = float(wrapdiff(ph[big], g["out_phase"][big]).max())
+    log(f"  amp err {ea:.2e}, phase err {ep:.2e}")
+    assert ea <= 1e-5 and ep <= 1e-5
