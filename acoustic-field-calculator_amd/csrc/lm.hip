@@ -23,7 +23,8 @@
 //   Gauss-Newton (gauss_newton.rs:35-89): h = -pinv(A, 1e-3) g; with A = W^T W
 //     the pseudo-inverse keeps the eigenvalues of K = W W^T above 1e-3 (A and K
 //     share their nonzero eigenvalues), so h = -W^T U diag(1/lambda) U^T v over
-//     those, U and lambda from a Jacobi eigen-decomposition of K;
+//     those, U and lambda from a parallel (round-robin) Jacobi
+//     eigen-decomposition of K;
 //   gradient descent (gradient_descent.rs:34-61): x -= 0.1 g until max g <= 1e-6,
 //     g_l = sum_i Im(conj(C_il) s_i) without forming W.
 //
@@ -319,17 +320,23 @@ __device__ __attribute__((noinline)) bool chol_solve_wave_lds(double* sK, const 
 }
 
 // y = pinv(K, eps) v for the Gram matrix K (lower triangle of sK with row
-// stride kLmMaxD, diagonal in sgd), D <= 64, by one wave: cyclic Jacobi
-// rotations on a copy of K (A, row stride kLmMaxD + 1) with the eigenvectors
-// accumulated in V, sweeps until the off-diagonal mass is below 1e-30 of the
-// diagonal's; eigenvalues |lambda| <= eps are dropped (as the reference's
-// SVD pseudo-inverse drops singular values <= eps). Lane k owns row / column k.
+// stride kLmMaxD, diagonal in sgd), D <= 64 and even, by one wave: parallel
+// cyclic Jacobi on a copy of K (A, row stride kLmMaxD + 1) with the
+// eigenvectors accumulated in V. A sweep is D - 1 rounds of the round-robin
+// (circle) schedule: in round t index D - 1 meets t and (t + i) mod (D - 1)
+// meets (t - i) mod (D - 1), i = 1 .. D/2 - 1, so each round rotates D/2
+// disjoint pairs at once -- the rotations from the round's A, then all column
+// updates (A J), then all row updates (J^T A J) and V J. Sweeps run until the
+// off-diagonal mass is below 1e-30 of the diagonal's; eigenvalues |lambda| <=
+// eps are dropped (as the reference's SVD pseudo-inverse drops singular
+// values <= eps). cs holds the D/2 rotations of a round, pq their indices.
 __device__ __attribute__((noinline)) void jacobi_pinv_wave(const double* sK, const double* sgd, const double* sv,
                                                            double* sy, double* A, double* V, double* col,
-                                                           uint32_t D, double eps) {
+                                                           double2* cs, int2* pq, uint32_t D, double eps) {
     constexpr int WS = kLmMaxD + 1;
     const int k = threadIdx.x;  // < 64
     const bool live = k < (int)D;
+    const int half = (int)D / 2;
     if (live)
         for (uint32_t j = 0; j < D; ++j) {
             A[k * WS + j] = (j == (uint32_t)k) ? sgd[k] : ((j < (uint32_t)k) ? sK[k * kLmMaxD + j] : sK[j * kLmMaxD + k]);
@@ -352,30 +359,53 @@ __device__ __attribute__((noinline)) void jacobi_pinv_wave(const double* sK, con
             dg += __shfl_xor(dg, o);
         }
         if (off <= dg * 1e-30 || off == 0.0) break;
-        for (uint32_t p = 0; p + 1 < D; ++p)
-            for (uint32_t q = p + 1; q < D; ++q) {
+        const int nm = (int)D - 1;
+        for (int t = 0; t < nm; ++t) {
+            if (k < half) {
+                int p, q;
+                if (k == 0) {
+                    p = nm;
+                    q = t;
+                } else {
+                    p = (t + k) % nm;
+                    q = (t - k + nm) % nm;
+                }
                 const double apq = A[p * WS + q];
-                if (apq == 0.0) continue;
-                const double theta = (A[q * WS + q] - A[p * WS + p]) / (2.0 * apq);
-                const double tt = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-                const double c = 1.0 / sqrt(tt * tt + 1.0), sn = tt * c;
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                if (live) {
-                    const double akp = A[k * WS + p], akq = A[k * WS + q];
-                    A[k * WS + p] = c * akp - sn * akq;
-                    A[k * WS + q] = sn * akp + c * akq;
+                double c = 1.0, sn = 0.0;
+                if (apq != 0.0) {
+                    const double theta = (A[q * WS + q] - A[p * WS + p]) / (2.0 * apq);
+                    const double tt = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                    c = 1.0 / sqrt(tt * tt + 1.0);
+                    sn = tt * c;
                 }
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                if (live) {
-                    const double apk = A[p * WS + k], aqk = A[q * WS + k];
-                    A[p * WS + k] = c * apk - sn * aqk;
-                    A[q * WS + k] = sn * apk + c * aqk;
-                    const double vkp = V[k * WS + p], vkq = V[k * WS + q];
-                    V[k * WS + p] = c * vkp - sn * vkq;
-                    V[k * WS + q] = sn * vkp + c * vkq;
-                }
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                cs[k] = make_double2(c, sn);
+                pq[k] = make_int2(p, q);
             }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            // A J: rows x pairs
+            for (int i = k; i < (int)D * half; i += 64) {
+                const int row = i % (int)D, r = i / (int)D;
+                const int2 e = pq[r];
+                const double2 rot = cs[r];
+                const double akp = A[row * WS + e.x], akq = A[row * WS + e.y];
+                A[row * WS + e.x] = rot.x * akp - rot.y * akq;
+                A[row * WS + e.y] = rot.y * akp + rot.x * akq;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            // J^T (A J) and V J
+            for (int i = k; i < (int)D * half; i += 64) {
+                const int cl = i % (int)D, r = i / (int)D;
+                const int2 e = pq[r];
+                const double2 rot = cs[r];
+                const double apk = A[e.x * WS + cl], aqk = A[e.y * WS + cl];
+                A[e.x * WS + cl] = rot.x * apk - rot.y * aqk;
+                A[e.y * WS + cl] = rot.y * apk + rot.x * aqk;
+                const double vkp = V[cl * WS + e.x], vkq = V[cl * WS + e.y];
+                V[cl * WS + e.x] = rot.x * vkp - rot.y * vkq;
+                V[cl * WS + e.y] = rot.y * vkp + rot.x * vkq;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        }
     }
     // w_k = (V_:k . v) / lambda_k over the kept eigenvalues, y = V w
     if (live) {
@@ -403,6 +433,8 @@ __global__ __launch_bounds__(kLB) void lm_solve(LmArgs a) {
     __shared__ double s_col[kLmMaxD];
     __shared__ double s_work[kLmMaxD * (kLmMaxD + 1)];
     __shared__ double s_vec[kLmMaxD * (kLmMaxD + 1)];
+    __shared__ double2 s_rot[kLmMaxD / 2];
+    __shared__ int2 s_pair[kLmMaxD / 2];
     __shared__ int s_ok;
     const int t = threadIdx.x;
     const uint32_t b = blockIdx.x;
@@ -512,7 +544,7 @@ __global__ __launch_bounds__(kLB) void lm_solve(LmArgs a) {
         uint32_t it = 0;
         for (; it < a.k_max; ++it) {
             if (found) break;
-            if (t < 64) jacobi_pinv_wave(s_K, s_gd, s_v, s_y, s_work, s_vec, s_col, D, 1e-3);
+            if (t < 64) jacobi_pinv_wave(s_K, s_gd, s_v, s_y, s_work, s_vec, s_col, s_rot, s_pair, D, 1e-3);
             __syncthreads();
             double hh = 0.0, xx = 0.0;
             for (uint32_t l = t; l < n; l += kLB) {
