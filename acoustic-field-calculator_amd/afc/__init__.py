@@ -383,6 +383,14 @@ class Optimizer:
         _check("AFO_FocalPoint")
 
     @staticmethod
+    def bessel_beam(system: UniformSystem, point, direction, theta_z: float):
+        """Bessel beam phases (acoustic-field-optimizer/src/bessel.rs:31-55)."""
+        p = point if isinstance(point, Vector3) else Vector3(point)
+        d = direction if isinstance(direction, Vector3) else Vector3(direction)
+        dll().AFO_BesselBeam(system.handle, p, d, c_float(theta_z), system._source_type)
+        _check("AFO_BesselBeam")
+
+    @staticmethod
     def _run(name, system, foci, amps):
         size, f, a = _pack_foci(foci, amps)
         getattr(dll(), name)(system.handle, f.ctypes.data_as(c_void_p), a.ctypes.data_as(POINTER(c_float)),

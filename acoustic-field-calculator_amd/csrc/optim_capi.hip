@@ -1,9 +1,10 @@
 // Optimizer exports of the C ABI (capi/src/optimizer.rs).
 //
 // GS, GS-PAT, Naive, Greedy, LM, Gauss-Newton, gradient descent, APO, Long
-// and Horn run on the GPU (holo.hip / lm.hip / matrix.hip). BesselBeam and
-// IFFT are outside this backend: they record "unsupported" and leave the
-// system untouched.
+// and Horn run on the GPU (holo.hip / lm.hip / matrix.hip); FocalPoint and
+// BesselBeam set phases on the host, as the reference does. IFFT (an image
+// file read and an FFT) is outside this backend: it records "unsupported" and
+// leaves the system untouched.
 #include <math.h>
 #include <string.h>
 
@@ -138,7 +139,33 @@ void AFO_LM(void* handle, const void* foci, const float* amps, uint64_t size, in
     run_single(handle, foci, amps, size, source_type, kAlgoLM, "AFO_LM");
 }
 
-void AFO_BesselBeam(void*, AFC_Vector3, AFC_Vector3, float, int32_t) { unsupported("AFO_BesselBeam"); }
+// acoustic-field-optimizer/src/bessel.rs:31-55 (capi/src/optimizer.rs:47-56),
+// host-side like FocalPoint and in the reference's f32 operation order:
+// v = (dir_y, -dir_x, 0) (used unnormalised), theta_w = asin |v|, r = s - point
+// turned by r cos theta_w + (r x v) sin theta_w + v (v.r (1 - cos theta_w));
+// phi_j = -2 PI ((dist mod lambda) / lambda), dist = sin theta |r_xy| - cos theta r_z
+void AFO_BesselBeam(void* handle, AFC_Vector3 point, AFC_Vector3 dir, float theta, int32_t source_type) {
+    clear_error();
+    System* s = as_system(handle, source_type, "AFO_BesselBeam");
+    if (!s) return;
+    const float vx = dir.y, vy = -dir.x, vz = 0.0f;
+    const float tw = asinf(sqrtf((vx * vx + vy * vy) + vz * vz));
+    const float ct = cosf(tw), stw = sinf(tw);
+    const float sth = sinf(theta), cth = cosf(theta);
+    const size_t st = s->stride();
+    for (uint64_t j = 0; j < s->count(); ++j) {
+        float* src = s->sources.data() + st * j;
+        const float rx = src[0] - point.x, ry = src[1] - point.y, rz = src[2] - point.z;
+        const float xx = ry * vz - rz * vy, xy = rz * vx - rx * vz, xz = rx * vy - ry * vx;
+        const float vr = ((vx * rx + vy * ry) + vz * rz) * (1.0f - ct);
+        const float qx = (rx * ct + xx * stw) + vx * vr;
+        const float qy = (ry * ct + xy * stw) + vy * vr;
+        const float qz = (rz * ct + xz * stw) + vz * vr;
+        const float dist = sth * sqrtf(qx * qx + qy * qy) - cth * qz;
+        const float wl = s->sound_speed / src[st - 1];
+        src[st - 2] = -2.0f * kPiF * (fmodf(dist, wl) / wl);
+    }
+}
 void AFO_IFFT(void*, const char*, AFC_Vector3, AFC_Vector3, AFC_Vector3, float, float, int32_t) {
     unsupported("AFO_IFFT");
 }

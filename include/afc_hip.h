@@ -151,9 +151,11 @@ void AFO_Horn(void* handle, const void* foci, const float* amps, uint64_t size, 
  * reference reads is taken with ascending eigenvalues and each vector's
  * largest-modulus entry real positive (nalgebra fixes neither). */
 void AFO_Long(void* handle, const void* foci, const float* amps, uint64_t size, int32_t source_type);
-/* Exported for binding compatibility only; each records "unsupported".
- * capi/src/optimizer.rs:49, :61 */
+/* capi/src/optimizer.rs:47-56, acoustic-field-optimizer/src/bessel.rs:31-55:
+ * sets the phases on the host, in the reference's f32 operation order */
 void AFO_BesselBeam(void* handle, AFC_Vector3 point, AFC_Vector3 dir, float theta, int32_t source_type);
+/* Exported for binding compatibility only; records "unsupported".
+ * capi/src/optimizer.rs:59-77 */
 void AFO_IFFT(void* handle, const char* path, AFC_Vector3 bottom_left, AFC_Vector3 top_left,
               AFC_Vector3 bottom_right, float spacing, float z, int32_t source_type);
 

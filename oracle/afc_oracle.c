@@ -28,6 +28,7 @@
  *   orc_long                 multiple_foci/matrix/long.rs:46-106
  *   orc_horn                 multiple_foci/matrix/horn.rs:58-129 (seeded row choice)
  *   orc_herm_eig             the Hermitian eigen-decomposition both use (Jacobi)
+ *   orc_bessel_phases        acoustic-field-optimizer/src/bessel.rs:31-55
  *   orc_focal_point_phases   acoustic-field-optimizer/src/focal_point.rs:28-39
  *   orc_grid_points          observe_area/grid/builder.rs:144-209, grid_area.rs:268-438
  */
@@ -83,6 +84,7 @@ static inline size_t orc_stride(int source_type) { return source_type == 1 ? 9 :
 #define R float
 #define SFX(name) name##_f32
 #define RACOS acosf
+#define RASIN asinf
 #define RCOS cosf
 #define RSIN sinf
 #define REXP expf
@@ -98,6 +100,7 @@ static inline size_t orc_stride(int source_type) { return source_type == 1 ? 9 :
 #undef R
 #undef SFX
 #undef RACOS
+#undef RASIN
 #undef RCOS
 #undef RSIN
 #undef REXP
@@ -114,6 +117,7 @@ static inline size_t orc_stride(int source_type) { return source_type == 1 ? 9 :
 #define R double
 #define SFX(name) name##_f64
 #define RACOS acos
+#define RASIN asin
 #define RCOS cos
 #define RSIN sin
 #define REXP exp

@@ -1285,6 +1285,34 @@ R SFX(orc_lm_eval)(int source_type, const float* sources, const float* wavenums,
 }
 
 /* FocalPoint phases: acoustic-field-optimizer/src/focal_point.rs:28-39 */
+/* BesselBeam phases: acoustic-field-optimizer/src/bessel.rs:31-55, in its
+ * operation order. v = (dir_y, -dir_x, 0), theta_w = asin |v|; r = s - point;
+ * r' = r cos theta_w + (r x v) sin theta_w + v (v.r (1 - cos theta_w)) (the
+ * axis v is used unnormalised, as written); dist = sin theta |r'_xy| -
+ * cos theta r'_z; phase = -2 pi ((dist mod lambda) / lambda). */
+void SFX(orc_bessel_phases)(int source_type, const float* sources, uint64_t n, R sound_speed, R px, R py, R pz,
+                            R dx, R dy, R dz, R theta, R* out_phase) {
+    const size_t st = orc_stride(source_type);
+    const R pi = SFX(ORC_PI);
+    const R vx = dy, vy = -dx, vz = 0;
+    const R tw = RASIN(RSQRT_F((vx * vx + vy * vy) + vz * vz));
+    const R ct = RCOS(tw), stw = RSIN(tw);
+    uint64_t j;
+    (void)dz;
+    for (j = 0; j < n; ++j) {
+        const float* s = sources + st * j;
+        const R rx = (R)s[0] - px, ry = (R)s[1] - py, rz = (R)s[2] - pz;
+        const R xx = ry * vz - rz * vy, xy = rz * vx - rx * vz, xz = rx * vy - ry * vx;
+        const R vr = ((vx * rx + vy * ry) + vz * rz) * ((R)1 - ct);
+        const R qx = (rx * ct + xx * stw) + vx * vr;
+        const R qy = (ry * ct + xy * stw) + vy * vr;
+        const R qz = (rz * ct + xz * stw) + vz * vr;
+        const R dist = RSIN(theta) * RSQRT_F(qx * qx + qy * qy) - RCOS(theta) * qz;
+        const R wl = sound_speed / (R)s[st - 1];
+        out_phase[j] = (R)-2.0 * pi * (RFMOD(dist, wl) / wl);
+    }
+}
+
 void SFX(orc_focal_point_phases)(int source_type, const float* sources, uint64_t n, R sound_speed, R fx, R fy, R fz,
                                  R* out_phase) {
     const size_t st = orc_stride(source_type);

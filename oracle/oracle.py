@@ -91,6 +91,9 @@ def lib():
             f = getattr(L, f"orc_focal_point_phases_{sfx}")
             f.restype = None
             f.argtypes = [c_int, fp, c_uint64, R, R, R, R, fp]
+            f = getattr(L, f"orc_bessel_phases_{sfx}")
+            f.restype = None
+            f.argtypes = [c_int, fp, c_uint64, R, R, R, R, R, R, R, R, fp]
         L.orc_grid_points.restype = c_uint64
         L.orc_grid_points.argtypes = [c_int, fp, fp, c_float, fp, fp]
         L.orc_reduce_f32.restype = None
@@ -285,6 +288,17 @@ def lm_eval(source_type, sources, wavenums, attens, foci, amps, x):
     g = np.empty(n + m, np.float64)
     F = lib().orc_lm_eval_f64(source_type, _p(src), _p(k), _p(a), n, _p(f), _p(am), m, _p(xx), _p(g))
     return F, g
+
+
+def bessel_phases(source_type, sources, sound_speed_mm_s, point, direction, theta, prec="f32") -> np.ndarray:
+    """BesselBeam phases (bessel.rs:31-55) per source."""
+    dt, sfx = _dt(prec)
+    src = _f32(sources)
+    n = src.shape[0]
+    out = np.empty(n, dt)
+    getattr(lib(), f"orc_bessel_phases_{sfx}")(source_type, _p(src), n, sound_speed_mm_s, *[float(v) for v in point],
+                                               *[float(v) for v in direction], float(theta), _p(out))
+    return out
 
 
 def focal_point_phases(source_type, sources, sound_speed_mm_s, focus, prec="f32") -> np.ndarray:

@@ -121,7 +121,8 @@ def test_cpu_calculator_rejected(afc):
 
 def test_unsupported_optimizer_reports(afc):
     s = _system(afc, 300.0, [(0, 0, 0)])
-    afc.dll().AFO_BesselBeam(s.handle, afc.Vector3((0, 0, 100)), afc.Vector3((0, 0, 1)), 0.1, 1)
+    v = afc.Vector3((0, 0, 0))
+    afc.dll().AFO_IFFT(s.handle, b"image.png", v, v, v, ctypes.c_float(1.0), ctypes.c_float(100.0), 1)
     assert b"not provided" in afc.dll().AFC_GetLastError()
 
 
@@ -140,4 +141,17 @@ def test_focal_point_matches_oracle(afc, oracle):
     afc.Optimizer.focus(s, (90.0, 70.0, 150.0))
     got = s.source_array()[:, 7]
     want = oracle.focal_point_phases(1, s.source_array().copy(), s.sound_speed(), (90.0, 70.0, 150.0))
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("direction,theta", [((0.0, 0.0, 1.0), 0.3), ((0.2, -0.1, 0.97), 0.25), ((0.6, 0.0, 0.8), 0.5)])
+def test_bessel_beam_matches_oracle(afc, oracle, direction, theta):
+    """AFO_BesselBeam (bessel.rs:31-55) sets the same f32 phases as the oracle's
+    restatement, bit for bit (both in the reference's operation order)."""
+    pos = oracle.array_249()
+    s = _system(afc, oracle.TEMP_340, [tuple(p) for p in pos])
+    point = (90.0, 70.0, 0.0)
+    afc.Optimizer.bessel_beam(s, point, direction, theta)
+    got = s.source_array()[:, 7]
+    want = oracle.bessel_phases(1, s.source_array().copy(), s.sound_speed(), point, direction, theta)
     assert np.array_equal(got, want)

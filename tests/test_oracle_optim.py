@@ -233,3 +233,22 @@ def test_apo_single_focus_stays_matched(oracle):
     assert np.allclose(np.angle(np.exp(1j * (ph + np.angle(G)))), 0, atol=1e-6)
     r = amp / np.abs(G)
     assert np.max(r) / np.min(r) - 1 < 1e-6
+
+
+def test_bessel_phases_axial_closed_form(oracle):
+    """dir = +z: the axis v = (0, 0, 0) leaves r unturned, so dist = sin(theta)
+    rho - cos(theta) (z_j - z_p) with rho the distance from the beam axis, and
+    phi_j = -2 pi ((dist mod lambda) / lambda) (C fmod: the sign of dist)."""
+    pos = oracle.array_249()
+    src = oracle.t4010a1_sources(pos)
+    c = 340e3
+    point, theta = np.array([90.0, 70.0, -20.0]), 0.3
+    ph = oracle.bessel_phases(1, src, c, point, (0.0, 0.0, 1.0), theta, "f64")
+    d = src[:, 0:3].astype(np.float64) - point
+    dist = np.sin(theta) * np.hypot(d[:, 0], d[:, 1]) - np.cos(theta) * d[:, 2]
+    lam = c / 40e3
+    want = -2 * np.pi * (np.fmod(dist, lam) / lam)
+    assert np.allclose(ph, want, rtol=0, atol=1e-12)
+    # the f32 restatement agrees to f32 rounding
+    ph32 = oracle.bessel_phases(1, src, c, point, (0.0, 0.0, 1.0), theta, "f32")
+    assert np.max(np.abs(ph32 - want)) < 2e-5
