@@ -211,6 +211,7 @@ def main():
     }
     if not args.no_gspat:
         result["gspat"] = bench_gspat(torch, afc, D, O, src, k, a, args.gspat_batch, not args.no_cpu_baseline)
+    result["capi_config1"] = bench_capi_config1(afc, O, not args.no_cpu_baseline)
     if not args.no_lm:
         result["lm"] = bench_lm(torch, D, O, not args.no_cpu_baseline)
     if not args.no_greedy:
@@ -496,6 +497,48 @@ def bench_lm(torch, D, O, cpu=True):
         ctx.close()
         out[name] = entry
         log(f"lm {name}: {ms:.3f} ms per batch of 64, {64 / (ms * 1e-3):.0f} solves/s")
+    return out
+
+
+def bench_capi_config1(afc, O, cpu=True):
+    """Config 1 through the reference's own entry point, AFC_Calculate (one
+    synchronous call: the sources packed and uploaded, the field computed on the
+    GPU, |p| copied back), beside the CPU restatement of the reference's
+    calculator on the same volume."""
+    src, k, a, area = O.config1()
+    system = afc.UniformSystem(O.TEMP_340)
+    for row in src:
+        system.add_wave_source(afc.T4010A1(row[0:3], row[3:6], row[6], row[7], row[8]))
+    grid = afc.GridAreaBuilder().x_range((40, 139)).y_range((20, 119)).z_at(150).resolution(1.0).generate()
+    field = afc.PressureFieldBuffer()
+    calc = afc.GpuCalculator()
+    for _ in range(5):
+        got = calc.calculate(system, grid, field)
+    lat = []
+    for _ in range(100):
+        t0 = time.perf_counter()
+        got = calc.calculate(system, grid, field)
+        lat.append(time.perf_counter() - t0)
+    pts, _ = O.grid_points(area["order"], area["ranges"], area["res"])
+    terms = pts.shape[0] * src.shape[0]
+    ref = np.abs(O.field(1, src, k, a, pts, "f64"))
+    p50 = float(np.percentile(lat, 50))
+    out = {"workload": "config 1: 249 T4010A1 x 100x100 points at z = 150 mm, |p| (PressureField), AFC_Calculate",
+           "p50_ms": round(p50 * 1e3, 4), "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 4),
+           "gsrc_pt_per_s": round(terms / p50 / 1e9, 3),
+           "rel_l2_vs_fp64": float(np.linalg.norm(got - ref) / np.linalg.norm(ref))}
+    if cpu:
+        threads = cpu_threads()
+        runs = []
+        for th in (threads, 1):
+            t0 = time.perf_counter()
+            O.field(1, src, k, a, pts, "f32", threads=th)
+            runs.append(time.perf_counter() - t0)
+        out["cpu_baseline"] = {"ms": round(runs[0] * 1e3, 3), "ms_single_thread": round(runs[1] * 1e3, 3),
+                               "cores": threads, "kind": "port",
+                               "sample": "the whole volume (2.49e6 terms), FP32 restatement of the reference's "
+                                         "CpuCalculator"}
+    log(f"AFC_Calculate config 1: p50 {out['p50_ms']:.3f} ms; cpu {out.get('cpu_baseline', {}).get('ms')} ms")
     return out
 
 
