@@ -385,12 +385,13 @@ __global__ __launch_bounds__(kHB) void gspat_r_mfma(HoloArgs a) {
     __syncthreads();
     // s = 2^floor(log2(M)) / 2 (>= 1) for SPLIT, 1 otherwise
     const float scale = SPLIT ? fmaxf(1.0f, exp2f(floorf(log2f((float)M)) - 1.0f)) : 1.0f;
-    floatx4 accRe[TPW], accIR[TPW], accRI[TPW];
+    // R = H H^H: Re R += Hr Hr^T + Hi Hi^T, Im R += Hi Hr^T + (-Hr) Hi^T, the
+    // sign flip applied to the FP16 A operand (exact)
+    floatx4 accRe[TPW], accIm[TPW];
 #pragma unroll
     for (int s = 0; s < TPW; ++s) {
         accRe[s] = floatx4{0.f, 0.f, 0.f, 0.f};
-        accIR[s] = accRe[s];
-        accRI[s] = accRe[s];
+        accIm[s] = accRe[s];
     }
     float* d_out = a.d + (uint64_t)k * a.n_packed;
     float4 pre[3];
@@ -433,25 +434,27 @@ __global__ __launch_bounds__(kHB) void gspat_r_mfma(HoloArgs a) {
             const half8 ai = *reinterpret_cast<const half8*>(&s_hi[0][arow * HS + kb]);
             const half8 br = *reinterpret_cast<const half8*>(&s_hr[0][brow * HS + kb]);
             const half8 bi = *reinterpret_cast<const half8*>(&s_hi[0][brow * HS + kb]);
+            const half8 nar = -ar;
             if (SPLIT) {
                 const half8 arl = *reinterpret_cast<const half8*>(&s_hr[NH - 1][arow * HS + kb]);
                 const half8 ail = *reinterpret_cast<const half8*>(&s_hi[NH - 1][arow * HS + kb]);
                 const half8 brl = *reinterpret_cast<const half8*>(&s_hr[NH - 1][brow * HS + kb]);
                 const half8 bil = *reinterpret_cast<const half8*>(&s_hi[NH - 1][brow * HS + kb]);
+                const half8 narl = -arl;
                 // cross terms first: they are small, the hi products dominate the sum
                 accRe[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ar, brl, accRe[s], 0, 0, 0);
                 accRe[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(arl, br, accRe[s], 0, 0, 0);
                 accRe[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ai, bil, accRe[s], 0, 0, 0);
                 accRe[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ail, bi, accRe[s], 0, 0, 0);
-                accIR[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ai, brl, accIR[s], 0, 0, 0);
-                accIR[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ail, br, accIR[s], 0, 0, 0);
-                accRI[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ar, bil, accRI[s], 0, 0, 0);
-                accRI[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(arl, bi, accRI[s], 0, 0, 0);
+                accIm[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ai, brl, accIm[s], 0, 0, 0);
+                accIm[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ail, br, accIm[s], 0, 0, 0);
+                accIm[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(nar, bil, accIm[s], 0, 0, 0);
+                accIm[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(narl, bi, accIm[s], 0, 0, 0);
             }
             accRe[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ar, br, accRe[s], 0, 0, 0);
             accRe[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ai, bi, accRe[s], 0, 0, 0);
-            accIR[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ai, br, accIR[s], 0, 0, 0);
-            accRI[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ar, bi, accRI[s], 0, 0, 0);
+            accIm[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ai, br, accIm[s], 0, 0, 0);
+            accIm[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(nar, bi, accIm[s], 0, 0, 0);
         }
         lds_sync();
     }
@@ -464,7 +467,7 @@ __global__ __launch_bounds__(kHB) void gspat_r_mfma(HoloArgs a) {
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             const int row = (tau / T) * 16 + 4 * (l >> 4) + e;
-            R[row * MP + col] = make_float2(accRe[s][e] * unscale, (accIR[s][e] - accRI[s][e]) * unscale);
+            R[row * MP + col] = make_float2(accRe[s][e] * unscale, accIm[s][e] * unscale);
         }
     }
 }
