@@ -93,6 +93,7 @@ struct Request {
     uint64_t ticket = 0;
     int kind = 0;
     uint32_t array_id = 0;
+    const SrvArray* arr = nullptr;  // taken under the server lock at submission
     // field
     GridArea grid;
     bool scatter = false;
@@ -144,7 +145,9 @@ struct Server {
     std::vector<hipStream_t> streams;
     std::vector<double> outstanding;  // queued terms per stream
     std::vector<std::unique_ptr<Context>> ctxs;
-    std::vector<SrvArray> arrays;
+    // arrays are added while serving: the worker reads them through the
+    // pointers requests took at submission, which a push_back does not move
+    std::vector<std::unique_ptr<SrvArray>> arrays;
     std::mutex mu;
     std::condition_variable cv_submit, cv_done;
     std::deque<Request*> pending;
@@ -185,7 +188,7 @@ struct Server {
         ctxs.clear();
         for (auto s : streams) (void)hipStreamDestroy(s);
         for (auto& a : arrays)
-            if (a.packed) (void)hipFree(a.packed);
+            if (a->packed) (void)hipFree(a->packed);
         if (dev_base) (void)hipFree(dev_base);
         if (pin_base) (void)hipHostFree(pin_base);
         for (auto& kv : by_ticket) delete kv.second;  // the worker drained `pending` first
@@ -242,7 +245,7 @@ struct Server {
     }
 
     bool launch_field(Request* r) {
-        const SrvArray& A = arrays[r->array_id];
+        const SrvArray& A = *r->arr;
         const size_t per = r->field_type == AFC_FIELD_COMPLEX ? 2 : 1;
         const size_t out_b = r->n_points * per * sizeof(float);
         const size_t pts_b = r->scatter ? r->n_points * 3 * sizeof(float) : 0;
@@ -287,7 +290,7 @@ struct Server {
     }
 
     bool launch_gspat(uint32_t array_id, Request** rs, size_t cnt) {
-        const SrvArray& A = arrays[array_id];
+        const SrvArray& A = *rs[0]->arr;  // every request of the group is on array_id
         size_t sum_m = 0;
         for (size_t i = 0; i < cnt; ++i) sum_m += rs[i]->m;
         const size_t foci_b = sum_m * 3 * sizeof(float), amps_b = sum_m * sizeof(float);
@@ -537,8 +540,10 @@ int32_t AFS_AddArray(void* srv, const void* sources, const float* wavenums, cons
     pack_sources(static_cast<const float*>(sources), wavenums, attens, n, source_type, host.data());
     if (!hip_ok(hipMalloc(&A.packed, host.size() * sizeof(float)), "AFS_AddArray: hipMalloc")) return AFH_ERR_HIP;
     if (!hip_ok(hipMemcpy(A.packed, host.data(), host.size() * sizeof(float), hipMemcpyHostToDevice),
-                "AFS_AddArray: upload"))
+                "AFS_AddArray: upload")) {
+        (void)hipFree(A.packed);
         return AFH_ERR_HIP;
+    }
     {
         // the GS-PAT workspace for a full batch of 128-focus instances on this
         // array, reserved now rather than grown while serving (growing frees the
@@ -555,7 +560,7 @@ int32_t AFS_AddArray(void* srv, const void* sources, const float* wavenums, cons
     }
     std::lock_guard<std::mutex> g(s->mu);
     *array_id = (uint32_t)s->arrays.size();
-    s->arrays.push_back(A);
+    s->arrays.emplace_back(new SrvArray(A));
     return AFH_OK;
 }
 
@@ -587,6 +592,7 @@ int32_t AFS_SubmitField(void* srv, uint32_t array_id, const AFH_PointSet* pts, i
             set_error("AFS_SubmitField: unknown array");
             return AFH_ERR_ARG;
         }
+        r->arr = s->arrays[array_id].get();
     }
     r->kind = kReqField;
     r->array_id = array_id;
@@ -643,7 +649,8 @@ int32_t AFS_SubmitGSPAT(void* srv, uint32_t array_id, const float* foci, const f
             set_error("AFS_SubmitGSPAT: unknown array");
             return AFH_ERR_ARG;
         }
-        n = s->arrays[array_id].n;
+        r->arr = s->arrays[array_id].get();
+        n = r->arr->n;
     }
     const uint64_t need = (uint64_t)m * 16 + 2ull * n * 4 + 1024;
     if (need > s->cfg.hbm_budget_bytes || need > s->cfg.pinned_budget_bytes) {
