@@ -123,7 +123,7 @@ def main():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O  # inputs are built with the shared workload helpers; accuracy check only
 
-    src, k, a, area = O.config2(seed=2)
+    src, k, a, area = O.volume_3984(seed=2)
     n = src.shape[0]
     ps, cnt = D.grid_from_area(area["order"], area["ranges"], area["res"])
     n_pts = cnt[0] * cnt[1] * cnt[2]
@@ -211,7 +211,7 @@ def main():
     }
     if not args.no_gspat:
         result["gspat"] = bench_gspat(torch, afc, D, O, src, k, a, args.gspat_batch, not args.no_cpu_baseline)
-    result["capi_config1"] = bench_capi_config1(afc, O, not args.no_cpu_baseline)
+    result["capi_focus_plane_249"] = bench_capi_focus_plane_249(afc, O, not args.no_cpu_baseline)
     if not args.no_lm:
         result["lm"] = bench_lm(torch, D, O, not args.no_cpu_baseline)
     if not args.no_greedy:
@@ -500,12 +500,12 @@ def bench_lm(torch, D, O, cpu=True):
     return out
 
 
-def bench_capi_config1(afc, O, cpu=True):
-    """Config 1 through the reference's own entry point, AFC_Calculate (one
+def bench_capi_focus_plane_249(afc, O, cpu=True):
+    """The 249-source focus plane (100x100 points at z = 150 mm) through the reference's own entry point, AFC_Calculate (one
     synchronous call: the sources packed and uploaded, the field computed on the
     GPU, |p| copied back), beside the CPU restatement of the reference's
     calculator on the same volume."""
-    src, k, a, area = O.config1()
+    src, k, a, area = O.focus_plane_249()
     system = afc.UniformSystem(O.TEMP_340)
     for row in src:
         system.add_wave_source(afc.T4010A1(row[0:3], row[3:6], row[6], row[7], row[8]))
@@ -523,7 +523,7 @@ def bench_capi_config1(afc, O, cpu=True):
     terms = pts.shape[0] * src.shape[0]
     ref = np.abs(O.field(1, src, k, a, pts, "f64"))
     p50 = float(np.percentile(lat, 50))
-    out = {"workload": "config 1: 249 T4010A1 x 100x100 points at z = 150 mm, |p| (PressureField), AFC_Calculate",
+    out = {"workload": "249 T4010A1 x 100x100 points at z = 150 mm, |p| (PressureField), AFC_Calculate",
            "p50_ms": round(p50 * 1e3, 4), "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 4),
            "gsrc_pt_per_s": round(terms / p50 / 1e9, 3),
            "rel_l2_vs_fp64": float(np.linalg.norm(got - ref) / np.linalg.norm(ref))}
@@ -538,7 +538,7 @@ def bench_capi_config1(afc, O, cpu=True):
                                "cores": threads, "kind": "port",
                                "sample": "the whole volume (2.49e6 terms), FP32 restatement of the reference's "
                                          "CpuCalculator"}
-    log(f"AFC_Calculate config 1: p50 {out['p50_ms']:.3f} ms; cpu {out.get('cpu_baseline', {}).get('ms')} ms")
+    log(f"AFC_Calculate, 249 sources x 100x100 points: p50 {out['p50_ms']:.3f} ms; cpu {out.get('cpu_baseline', {}).get('ms')} ms")
     return out
 
 

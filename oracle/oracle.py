@@ -413,7 +413,7 @@ TEMP_340 = float(np.float32(273.15 * (340.0 / 331.3) ** 2))
 WAVENUM_40K = float(np.float32(2.0 * np.pi * 40e3 / 340e3))
 
 
-def config1():
+def focus_plane_249():
     """249-transducer array, focus at (90,70,150) mm, 100x100 XY plane at z=150."""
     pos = array_249()
     alpha = np.float32(attenuation_coef(40e3, 30.0, 1.0, 1.0, TEMP_340))
@@ -427,7 +427,7 @@ def config1():
     return src, k, a, area
 
 
-def config2(seed: int = 2):
+def volume_3984(seed: int = 2):
     """16 tiled arrays (3984 transducers), random phases; 256x256x128 volume at 1 mm."""
     rng = np.random.Generator(np.random.PCG64(seed))
     pos = tiled_array(4, 4)

@@ -25,7 +25,7 @@ import oracle as O  # noqa: E402
 def field_t4010a1():
     """Config-1 array (249 T4010A1, 40 kHz, c = 340 m/s) with FocalPoint phases
     for (90, 70, 150) mm; 64 seeded scatter points plus the focus (last)."""
-    src, k, a, _ = O.config1()
+    src, k, a, _ = O.focus_plane_249()
     rng = np.random.default_rng(11)
     pts = np.stack([rng.uniform(0, 180, 64), rng.uniform(0, 140, 64), rng.uniform(20, 300, 64)], 1)
     pts = np.vstack([pts, [[90.0, 70.0, 150.0]]]).astype(np.float32)

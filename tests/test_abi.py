@@ -96,7 +96,7 @@ def test_grid_2d_with_at_axis(afc):
 
 
 def test_packing_layout(afc, oracle):
-    src, k, a, _ = oracle.config1()
+    src, k, a, _ = oracle.focus_plane_249()
     from afc import device
     p = device.pack_sources(1, src, k, a)
     n = src.shape[0]

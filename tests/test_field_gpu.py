@@ -46,8 +46,8 @@ def consts(afc, s, n):
 
 
 @pytest.fixture(scope="module")
-def cfg1(afc, oracle, torch_cuda):
-    src, k, a, area = oracle.config1()
+def plane249(afc, oracle, torch_cuda):
+    src, k, a, area = oracle.focus_plane_249()
     s = build_system(afc, src, oracle.TEMP_340)
     kk, aa = consts(afc, s, src.shape[0])
     pts, bounds = oracle.grid_points(**area)
@@ -56,39 +56,39 @@ def cfg1(afc, oracle, torch_cuda):
     return dict(src=src, sys=s, k=kk, a=aa, pts=pts, bounds=bounds, ref32=ref32, ref64=ref64)
 
 
-def config1_area(afc):
+def plane249_area(afc):
     return afc.GridAreaBuilder().x_range((40, 139)).y_range((20, 119)).z_at(150).resolution(1.0).generate()
 
 
-def test_config1_pressure_capi(afc, oracle, cfg1):
-    log("case: config1 PressureField via AFC_Calculate (GpuCalculator)")
+def test_plane249_pressure_capi(afc, oracle, plane249):
+    log("case: focus_plane_249 PressureField via AFC_Calculate (GpuCalculator)")
     calc = afc.GpuCalculator()
-    area = config1_area(afc)
-    assert area.get_size() == cfg1["bounds"]
-    got = calc.calculate(cfg1["sys"], area, afc.PressureFieldBuffer())
+    area = plane249_area(afc)
+    assert area.get_size() == plane249["bounds"]
+    got = calc.calculate(plane249["sys"], area, afc.PressureFieldBuffer())
     assert got.shape == (10000,)
-    want32 = oracle.reduce_f32(0, cfg1["ref32"])
+    want32 = oracle.reduce_f32(0, plane249["ref32"])
     e32 = float(np.linalg.norm(got - want32) / np.linalg.norm(want32))
-    e64 = float(np.linalg.norm(got - np.abs(cfg1["ref64"])) / np.linalg.norm(np.abs(cfg1["ref64"])))
+    e64 = float(np.linalg.norm(got - np.abs(plane249["ref64"])) / np.linalg.norm(np.abs(plane249["ref64"])))
     log(f"  |p| rel-L2 vs FP32 CPU path {e32:.3e}, vs FP64 {e64:.3e}")
     assert e32 <= 1e-5
     assert e64 <= 5e-6
     # the focus is where FocalPoint put it
-    assert np.argmax(got) == np.where((cfg1["pts"][:, 0] == 90) & (cfg1["pts"][:, 1] == 70))[0][0]
+    assert np.argmax(got) == np.where((plane249["pts"][:, 0] == 90) & (plane249["pts"][:, 1] == 70))[0][0]
 
 
-def test_config1_complex_and_power(afc, oracle, cfg1):
-    log("case: config1 Complex/Power via AFC_Calculate")
+def test_plane249_complex_and_power(afc, oracle, plane249):
+    log("case: focus_plane_249 Complex/Power via AFC_Calculate")
     calc = afc.GpuCalculator()
-    area = config1_area(afc)
-    cp = calc.calculate(cfg1["sys"], area, afc.ComplexPressureFieldBuffer())
-    e32 = rel_l2(cp, cfg1["ref32"])
-    e64 = rel_l2(cp, cfg1["ref64"])
+    area = plane249_area(afc)
+    cp = calc.calculate(plane249["sys"], area, afc.ComplexPressureFieldBuffer())
+    e32 = rel_l2(cp, plane249["ref32"])
+    e64 = rel_l2(cp, plane249["ref64"])
     log(f"  complex rel-L2 vs FP32 CPU {e32:.3e}, vs FP64 {e64:.3e}; CPU FP32 vs FP64 "
-        f"{rel_l2(cfg1['ref32'], cfg1['ref64']):.3e}")
+        f"{rel_l2(plane249['ref32'], plane249['ref64']):.3e}")
     assert e32 <= 1e-5 and e64 <= 5e-6
-    pw = calc.calculate(cfg1["sys"], area, afc.PowerFieldBuffer())
-    pr = calc.calculate(cfg1["sys"], area, afc.PressureFieldBuffer())
+    pw = calc.calculate(plane249["sys"], area, afc.PowerFieldBuffer())
+    pr = calc.calculate(plane249["sys"], area, afc.PressureFieldBuffer())
     # the reductions are applied to the same complex sum: |p|^2 and sqrt(|p|^2)
     c32 = cp.astype(np.complex64)
     ns = (c32.real * c32.real + c32.imag * c32.imag).astype(np.float32)
@@ -96,39 +96,39 @@ def test_config1_complex_and_power(afc, oracle, cfg1):
     assert np.array_equal(pr, np.sqrt(ns))
 
 
-def test_config1_accurate_mode(afc, oracle, cfg1):
-    log("case: config1 AccurateCalculator (polynomial sincos)")
+def test_plane249_accurate_mode(afc, oracle, plane249):
+    log("case: focus_plane_249 AccurateCalculator (polynomial sincos)")
     calc = afc.AccurateCalculator()
-    cp = calc.calculate(cfg1["sys"], config1_area(afc), afc.ComplexPressureFieldBuffer())
-    e64 = rel_l2(cp, cfg1["ref64"])
+    cp = calc.calculate(plane249["sys"], plane249_area(afc), afc.ComplexPressureFieldBuffer())
+    e64 = rel_l2(cp, plane249["ref64"])
     log(f"  accurate rel-L2 vs FP64 {e64:.3e}")
     assert e64 <= 2e-6
-    assert rel_l2(cp, cfg1["ref32"]) <= 1e-5
+    assert rel_l2(cp, plane249["ref32"]) <= 1e-5
 
 
-def test_hip_tag_equals_gpu_tag(afc, cfg1):
-    a = afc.GpuCalculator().calculate(cfg1["sys"], config1_area(afc), afc.ComplexPressureFieldBuffer())
-    b = afc.HipCalculator().calculate(cfg1["sys"], config1_area(afc), afc.ComplexPressureFieldBuffer())
+def test_hip_tag_equals_gpu_tag(afc, plane249):
+    a = afc.GpuCalculator().calculate(plane249["sys"], plane249_area(afc), afc.ComplexPressureFieldBuffer())
+    b = afc.HipCalculator().calculate(plane249["sys"], plane249_area(afc), afc.ComplexPressureFieldBuffer())
     assert np.array_equal(a, b)
 
 
 @pytest.mark.parametrize("axes", [(1, 2, 3), (3, 1, 2), (2, 3, 1), (1, 3, 2), (2, 1, 3), (3, 2, 1)])
-def test_grid3d_orders(afc, oracle, cfg1, axes):
+def test_grid3d_orders(afc, oracle, plane249, axes):
     log(f"case: grid3d axes {axes}")
     rng = {1: (60.0, 75.0), 2: (55.0, 66.0), 3: (140.0, 149.0)}
     b = afc.GridAreaBuilder()
     for ax in axes:
         getattr(b, {1: "x_range", 2: "y_range", 3: "z_range"}[ax])(rng[ax])
     area = b.resolution(1.5).generate()
-    got = afc.GpuCalculator().calculate(cfg1["sys"], area, afc.ComplexPressureFieldBuffer())
+    got = afc.GpuCalculator().calculate(plane249["sys"], area, afc.ComplexPressureFieldBuffer())
     pts, _ = oracle.grid_points([a - 1 for a in axes], (rng[1], rng[2], rng[3]), 1.5)
-    want = oracle.field(1, cfg1["src"], cfg1["k"], cfg1["a"], pts, "f32")
+    want = oracle.field(1, plane249["src"], plane249["k"], plane249["a"], pts, "f32")
     assert got.shape == want.shape
     assert rel_l2(got, want) <= 1e-5
 
 
 @pytest.mark.parametrize("builder", ["x", "y", "z", "xz", "zy", "yx"])
-def test_grid_1d_2d(afc, oracle, cfg1, builder):
+def test_grid_1d_2d(afc, oracle, plane249, builder):
     log(f"case: grid {builder}")
     rng = {"x": (60.0, 120.0), "y": (40.0, 100.0), "z": (100.0, 180.0)}
     at = {"x": 90.0, "y": 70.0, "z": 150.0}
@@ -140,23 +140,23 @@ def test_grid_1d_2d(afc, oracle, cfg1, builder):
         if c not in builder:
             getattr(b, f"{c}_at")(at[c])
     area = b.resolution(0.75).generate()
-    got = afc.GpuCalculator().calculate(cfg1["sys"], area, afc.ComplexPressureFieldBuffer())
+    got = afc.GpuCalculator().calculate(plane249["sys"], area, afc.ComplexPressureFieldBuffer())
     ranges = [rng[c] if c in builder else (at[c], at[c]) for c in "xyz"]
     pts, _ = oracle.grid_points(order, ranges, 0.75)
-    want = oracle.field(1, cfg1["src"], cfg1["k"], cfg1["a"], pts, "f32")
+    want = oracle.field(1, plane249["src"], plane249["k"], plane249["a"], pts, "f32")
     assert rel_l2(got, want) <= 1e-5
 
 
-def test_scatter_equals_grid_bitwise(afc, oracle, cfg1):
+def test_scatter_equals_grid_bitwise(afc, oracle, plane249):
     """Grid points are generated on the device with the reference's rounding, so
     the same points given as a scatter area produce identical sums."""
     log("case: scatter vs grid")
-    area = config1_area(afc)
-    grid = afc.GpuCalculator().calculate(cfg1["sys"], area, afc.ComplexPressureFieldBuffer())
+    area = plane249_area(afc)
+    grid = afc.GpuCalculator().calculate(plane249["sys"], area, afc.ComplexPressureFieldBuffer())
     sc = afc.ScatterArea()
-    for p in cfg1["pts"]:
+    for p in plane249["pts"]:
         sc.add_observe_point(tuple(p))
-    scat = afc.GpuCalculator().calculate(cfg1["sys"], sc, afc.ComplexPressureFieldBuffer())
+    scat = afc.GpuCalculator().calculate(plane249["sys"], sc, afc.ComplexPressureFieldBuffer())
     assert np.array_equal(grid, scat)
 
 
@@ -209,43 +209,43 @@ def test_t4010a1_random_directions_all_angles(afc, oracle, torch_cuda):
     assert np.max(np.abs(got - w64)) <= 1e-5 * scale
 
 
-def test_in_place_source_edit_is_seen(afc, oracle, cfg1):
+def test_in_place_source_edit_is_seen(afc, oracle, plane249):
     """AFC_GetWaveSources hands out the system's own array (python/main.py:58-63)."""
     log("case: in-place phase edit")
-    s = build_system(afc, cfg1["src"], oracle.TEMP_340)
+    s = build_system(afc, plane249["src"], oracle.TEMP_340)
     calc = afc.GpuCalculator()
-    area = config1_area(afc)
+    area = plane249_area(afc)
     before = calc.calculate(s, area, afc.ComplexPressureFieldBuffer())
     srcs = s.get_wave_sources()
     for i in range(len(srcs)):
         srcs[i].phase = 0.0
     after = calc.calculate(s, area, afc.ComplexPressureFieldBuffer())
-    src0 = cfg1["src"].copy()
+    src0 = plane249["src"].copy()
     src0[:, 7] = 0
-    want = oracle.field(1, src0, cfg1["k"], cfg1["a"], cfg1["pts"], "f32")
+    want = oracle.field(1, src0, plane249["k"], plane249["a"], plane249["pts"], "f32")
     assert rel_l2(after, want) <= 1e-5 and rel_l2(before, after) > 0.1
 
 
-def test_empty_inputs(afc, oracle, cfg1):
+def test_empty_inputs(afc, oracle, plane249):
     log("case: empty source list / empty area")
     calc = afc.GpuCalculator()
     empty = afc.UniformSystem(300.0)
     empty.add_wave_source(afc.T4010A1((0, 0, 0), (0, 0, 1), 1.0, 0.0, 40e3))
     # no points
-    assert calc.calculate(cfg1["sys"], afc.ScatterArea(), afc.PressureFieldBuffer()).shape == (0,)
+    assert calc.calculate(plane249["sys"], afc.ScatterArea(), afc.PressureFieldBuffer()).shape == (0,)
     # amplitude zero everywhere -> exactly zero field
     srcs = empty.get_wave_sources()
     srcs[0].amp = 0.0
-    out = calc.calculate(empty, config1_area(afc), afc.PressureFieldBuffer())
+    out = calc.calculate(empty, plane249_area(afc), afc.PressureFieldBuffer())
     assert np.all(out == 0)
 
 
-def test_point_on_a_source_is_not_finite(afc, oracle, cfg1):
+def test_point_on_a_source_is_not_finite(afc, oracle, plane249):
     """At r = 0 the reference divides by zero (uniform_medium.rs:117): non-finite."""
     sc = afc.ScatterArea()
-    sc.add_observe_point(tuple(cfg1["src"][5, 0:3]))
+    sc.add_observe_point(tuple(plane249["src"][5, 0:3]))
     sc.add_observe_point((90.0, 70.0, 150.0))
-    out = afc.GpuCalculator().calculate(cfg1["sys"], sc, afc.PressureFieldBuffer())
+    out = afc.GpuCalculator().calculate(plane249["sys"], sc, afc.PressureFieldBuffer())
     assert not np.isfinite(out[0]) and np.isfinite(out[1])
 
 
@@ -258,7 +258,7 @@ def test_split_and_point_parallel_kernels_agree(afc, oracle, torch_cuda):
     log("case: 640x640 grid (point-parallel) vs scatter subsets (source-split)")
     import torch
     from afc import device as D
-    src, k, a, _ = oracle.config2(seed=7)
+    src, k, a, _ = oracle.volume_3984(seed=7)
     src = src[:996]
     k, a = k[:996], a[:996]
     packed = D.pack_sources(1, src, k, a)
@@ -294,7 +294,7 @@ def test_ragged_point_counts(afc, oracle, torch_cuda, n):
     log(f"case: ragged scatter set of {n} points")
     import torch
     from afc import device as D
-    src, k, a, _ = oracle.config1()
+    src, k, a, _ = oracle.focus_plane_249()
     packed = D.pack_sources(1, src, k, a)
     dev = torch.device("cuda:0")
     src_d = torch.from_numpy(packed).to(dev)

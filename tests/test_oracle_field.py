@@ -132,7 +132,7 @@ def test_t4010a1_on_axis_equals_sphere(oracle):
 
 # ---------------------------------------------------------------- many sources
 def test_linearity_in_sources(oracle):
-    src, k, a, area = oracle.config1()
+    src, k, a, area = oracle.focus_plane_249()
     pts, _ = oracle.grid_points(**area)
     pts = pts[::37]
     full = oracle.field(1, src, k, a, pts, "f64")
@@ -142,7 +142,7 @@ def test_linearity_in_sources(oracle):
 
 
 def test_translation_invariance(oracle):
-    src, k, a, area = oracle.config1()
+    src, k, a, area = oracle.focus_plane_249()
     pts, _ = oracle.grid_points(**area)
     pts = pts[::53].astype(np.float64)
     # positions on a 1/64 mm lattice so that the shifted inputs are exact in f32
@@ -159,7 +159,7 @@ def test_translation_invariance(oracle):
 def test_focus_identity(oracle):
     """With FocalPoint phases every term arrives in phase at the focus, so |p(f)|
     equals the sum of the term amplitudes."""
-    src, k, a, _ = oracle.config1()
+    src, k, a, _ = oracle.focus_plane_249()
     f = np.array([[90.0, 70.0, 150.0]], np.float32)
     p = oracle.field(1, src, k, a, f, "f64")[0]
     pos = src[:, :3].astype(np.float64)
@@ -190,8 +190,8 @@ def test_c_f64_matches_numpy_twin(oracle):
         assert rel_l2(c, n) < 1e-13
 
 
-def test_fp32_path_error_vs_fp64_config1(oracle):
-    src, k, a, area = oracle.config1()
+def test_fp32_path_error_vs_fp64_focus_plane(oracle):
+    src, k, a, area = oracle.focus_plane_249()
     pts, _ = oracle.grid_points(**area)
     e = rel_l2(oracle.field(1, src, k, a, pts, "f32"), oracle.field(1, src, k, a, pts, "f64"))
     assert 1e-7 < e < 1e-5  # FP32 rounding is visible but small
@@ -200,7 +200,7 @@ def test_fp32_path_error_vs_fp64_config1(oracle):
 def test_accurate_sum_is_order_independent(oracle):
     """The ascending-order sum (system_acc.rs:46-51) does not depend on the order
     in which sources were added; the naive sum does."""
-    src, k, a, area = oracle.config1()
+    src, k, a, area = oracle.focus_plane_249()
     pts, _ = oracle.grid_points(**area)
     pts = pts[::7]
     perm = np.random.default_rng(1).permutation(src.shape[0])
