@@ -738,6 +738,7 @@ __global__ __launch_bounds__(kLB) void apo_solve(LmArgs a) {
         const uint32_t i = e / M, k = e % M;
         if (k > i) continue;
         double re = 0.0, im = 0.0;
+#pragma unroll 8
         for (uint32_t l = 0; l < N; ++l) {
             const double2 gi = s.G[(uint64_t)i * N + l], gk = s.G[(uint64_t)k * N + l];
             re += gi.x * gk.x + gi.y * gk.y;  // G_il conj(G_kl)
@@ -770,6 +771,7 @@ __global__ __launch_bounds__(kLB) void apo_solve(LmArgs a) {
         }
         s.q[j] = make_double2(re, im);
     }
+#pragma unroll 4
     for (uint64_t e = t; e < (uint64_t)N * N; e += kLB)
         s.H[e] = make_double2(e % ((uint64_t)N + 1) == 0 ? 1.0 : 0.0, 0.0);
     __syncthreads();
@@ -780,6 +782,7 @@ __global__ __launch_bounds__(kLB) void apo_solve(LmArgs a) {
         __syncthreads();
         for (uint32_t j = t; j < N; j += kLB) {
             double re = 0.0, im = 0.0;
+#pragma unroll 4
             for (uint32_t i = 0; i < M; ++i) {
                 const double2 g = s.G[(uint64_t)i * N + j], gq = s_gq[i];
                 const double w = s_p2[i] - (gq.x * gq.x + gq.y * gq.y);
@@ -796,6 +799,7 @@ __global__ __launch_bounds__(kLB) void apo_solve(LmArgs a) {
         // d = -H grad
         for (uint32_t j = t; j < N; j += kLB) {
             double re = 0.0, im = 0.0;
+#pragma unroll 8
             for (uint32_t k = 0; k < N; ++k) {
                 const double2 h = s.H[(uint64_t)k * N + j], g = s.gr[k];
                 re += h.x * g.x - h.y * g.y;
@@ -824,6 +828,7 @@ __global__ __launch_bounds__(kLB) void apo_solve(LmArgs a) {
         for (uint32_t l = t; l < kApoLineSearch; l += kLB) {
             const double al = (double)l / (double)kApoLineSearch;
             double v = 0.0;
+#pragma unroll 4
             for (uint32_t i = 0; i < M; ++i) {
                 const double re = s_gq[i].x + al * s_gd[i].x, im = s_gq[i].y + al * s_gd[i].y;
                 const double sdiff = re * re + im * im - s_p2[i];
@@ -871,6 +876,7 @@ __global__ __launch_bounds__(kLB) void apo_solve(LmArgs a) {
         double2 ys = make_double2(0.0, 0.0), su = ys;
         for (uint32_t j = t; j < N; j += kLB) {
             double re = 0.0, im = 0.0;
+#pragma unroll 8
             for (uint32_t k = 0; k < N; ++k) {
                 const double2 h = s.H[(uint64_t)k * N + j], v = s.sv[k];
                 re += h.x * v.x - h.y * v.y;
@@ -889,6 +895,7 @@ __global__ __launch_bounds__(kLB) void apo_solve(LmArgs a) {
         const double ny = ys.x * ys.x + ys.y * ys.y, nu = su.x * su.x + su.y * su.y;
         const double2 iy = make_double2(ys.x / ny, -ys.y / ny), iu = make_double2(su.x / nu, -su.y / nu);
         // H += y y^T / (y . s) - u u^T / (s . u)
+#pragma unroll 4
         for (uint64_t e = t; e < (uint64_t)N * N; e += kLB) {
             const uint32_t j = (uint32_t)(e % N), k = (uint32_t)(e / N);
             const double2 a1 = cmul(cmul(s.d[j], s.d[k]), iy), a2 = cmul(cmul(s.u[j], s.u[k]), iu);

@@ -222,6 +222,7 @@ __device__ inline void gram_herm(const double2* G, const double* w, uint32_t N, 
         const uint32_t i = e / M, k = e % M;
         if (k > i) continue;
         double re = 0.0, im = 0.0;
+#pragma unroll 8
         for (uint32_t l = 0; l < N; ++l) {
             const double2 gi = G[(uint64_t)i * N + l], gk = G[(uint64_t)k * N + l];
             double pr = gi.x * gk.x + gi.y * gk.y, pi = gi.y * gk.x - gi.x * gk.y;
@@ -276,6 +277,7 @@ __global__ __launch_bounds__(kLB) void long_solve(MatArgs a) {
     // sigma_j^2 = sum_i |G_ij| amps_i / M
     for (uint32_t j = t; j < N; j += kLB) {
         double acc = 0.0;
+#pragma unroll 4
         for (uint32_t i = 0; i < M; ++i) {
             const double2 g = s.G[(uint64_t)i * N + j];
             acc += sqrt(g.x * g.x + g.y * g.y) * (double)amps[i];
@@ -337,6 +339,7 @@ __global__ __launch_bounds__(kLB) void long_solve(MatArgs a) {
     // q = D^{-1} G^H y
     for (uint32_t j = t; j < N; j += kLB) {
         double re = 0.0, im = 0.0;
+#pragma unroll 4
         for (uint32_t i = 0; i < M; ++i) {
             const double2 g = s.G[(uint64_t)i * N + j], y = s_A[i * kAS + M];
             re += g.x * y.x + g.y * y.y;
@@ -441,6 +444,7 @@ __global__ __launch_bounds__(kLB) void horn_solve(MatArgs a) {
     // q = G^H z
     for (uint32_t j = t; j < N; j += kLB) {
         double re = 0.0, im = 0.0;
+#pragma unroll 4
         for (uint32_t i = 0; i < M; ++i) {
             const double2 g = s.G[(uint64_t)i * N + j], z = s_z[i];
             re += g.x * z.x + g.y * z.y;
