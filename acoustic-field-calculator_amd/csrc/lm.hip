@@ -448,19 +448,24 @@ __global__ __launch_bounds__(kLB) void lm_solve(LmArgs a) {
     // the Gram matrix W W^T into s_K (lower triangle and diagonal), F = |s|^2.
     // Returns (F, max_l g_l, max_l diag(A)_l).
     auto linearize = [&](double& F, double& gmax, double& amax) {
+        // G and W are separate regions of the scratch: restrict lets the next
+        // G loads issue ahead of this row's W stores
+        const double2* __restrict__ Gp = s.G;
+        double* __restrict__ Wp = s.W;
         for (uint32_t l = t; l < n; l += kLB) {
             const double2 e = s.E[l];
             double colsq = 0.0;
+#pragma unroll 4
             for (uint32_t i = 0; i < M; ++i) {
                 double2 bv;
                 if (l < N) {
-                    bv = s.G[(uint64_t)i * N + l];
+                    bv = Gp[(uint64_t)i * N + l];
                 } else {
                     bv = make_double2(l - N == i ? -(double)amps[i] : 0.0, 0.0);
                 }
                 const double cr = bv.x * e.x - bv.y * e.y, ci = bv.x * e.y + bv.y * e.x;
-                s.W[(uint64_t)i * n + l] = cr;
-                s.W[(uint64_t)(M + i) * n + l] = ci;
+                Wp[(uint64_t)i * n + l] = cr;
+                Wp[(uint64_t)(M + i) * n + l] = ci;
                 colsq += cr * cr + ci * ci;
             }
             s.h[l] = colsq;  // diag(A)_l, reused as scratch here
@@ -479,6 +484,7 @@ __global__ __launch_bounds__(kLB) void lm_solve(LmArgs a) {
         double gm = -INFINITY, am = -INFINITY;
         for (uint32_t l = t; l < n; l += kLB) {
             double gl = 0.0;
+#pragma unroll 8
             for (uint32_t q = 0; q < D; ++q) gl += s.W[(uint64_t)q * n + l] * s_v[q];
             s.g[l] = gl;
             gm = fmax(gm, gl);
@@ -509,6 +515,7 @@ __global__ __launch_bounds__(kLB) void lm_solve(LmArgs a) {
             for (uint32_t l = t; l < n; l += kLB) {
                 const double2 e = s.E[l];
                 double gl = 0.0;
+#pragma unroll 4
                 for (uint32_t i = 0; i < M; ++i) {
                     double2 bv;
                     if (l < N)
@@ -549,6 +556,7 @@ __global__ __launch_bounds__(kLB) void lm_solve(LmArgs a) {
             double hh = 0.0, xx = 0.0;
             for (uint32_t l = t; l < n; l += kLB) {
                 double acc = 0.0;
+#pragma unroll 8
                 for (uint32_t q = 0; q < D; ++q) acc += s.W[(uint64_t)q * n + l] * s_y[q];
                 const double hl = -acc;
                 const double xl = s.x[l];
@@ -613,6 +621,7 @@ __global__ __launch_bounds__(kLB) void lm_solve(LmArgs a) {
         double hh = 0.0, xx = 0.0, lp = 0.0;
         for (uint32_t l = t; l < n; l += kLB) {
             double acc = 0.0;
+#pragma unroll 8
             for (uint32_t q = 0; q < D; ++q) acc += s.W[(uint64_t)q * n + l] * s_y[q];
             const double hl = -acc;
             s.h[l] = hl;
