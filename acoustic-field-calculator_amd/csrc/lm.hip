@@ -238,12 +238,19 @@ __device__ __attribute__((noinline)) bool chol_solve_wave(double* sK, const doub
 #pragma unroll
         for (int j = 0; j < DP; ++j)
             if (j <= i) sK[i * kLmMaxD + j] = r[j];
+    // 1 / L_ii, once per lane, so that no division sits on the substitutions'
+    // lane-to-lane critical path
+    double diag = 1.0;
+#pragma unroll
+    for (int j = 0; j < DP; ++j)
+        if (i == j) diag = r[j];
+    const double dinv = 1.0 / diag;
     // L z = v
     double acc = real ? sv[i] : 0.0, z = 0.0;
 #pragma unroll
     for (int j = 0; j < DP; ++j) {
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        if (i == j) col[j] = acc / r[j];
+        if (i == j) col[j] = acc * dinv;
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         const double zj = col[j];
         if (i == j) z = zj;
@@ -255,7 +262,7 @@ __device__ __attribute__((noinline)) bool chol_solve_wave(double* sK, const doub
 #pragma unroll
     for (int j = DP - 1; j >= 0; --j) {
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        if (i == j) col[j] = acc / r[j];
+        if (i == j) col[j] = acc * dinv;
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         const double yj = col[j];
         if (i == j) y = yj;
@@ -296,10 +303,11 @@ __device__ __attribute__((noinline)) bool chol_solve_wave_lds(double* sK, const 
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     if (real)
         for (int j = 0; j <= i; ++j) sK[i * kLmMaxD + j] = work[i * WS + j];
+    const double dinv = 1.0 / work[i * WS + i];  // 1 / L_ii (1 on the identity rows)
     double acc = real ? sv[i] : 0.0, z = 0.0;
     for (int j = 0; j < DP; ++j) {
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        if (i == j) col[j] = acc / work[i * WS + j];
+        if (i == j) col[j] = acc * dinv;
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         const double zj = col[j];
         if (i == j) z = zj;
@@ -309,7 +317,7 @@ __device__ __attribute__((noinline)) bool chol_solve_wave_lds(double* sK, const 
     double y = 0.0;
     for (int j = DP - 1; j >= 0; --j) {
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        if (i == j) col[j] = acc / work[i * WS + j];
+        if (i == j) col[j] = acc * dinv;
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         const double yj = col[j];
         if (i == j) y = yj;
