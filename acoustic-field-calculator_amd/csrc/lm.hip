@@ -193,22 +193,33 @@ __device__ void gram(const double* __restrict__ W, uint32_t n, uint32_t D, doubl
         }
 }
 
+// Lane `lane`'s value of v (a wave-uniform result, read with v_readlane).
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, lane);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), lane);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
 // One wave (lanes 0..63) factors K = L L^T (D x D, lower triangle of sK with
 // row stride kLmMaxD) and solves L z = v, L^T y = z. Lane i keeps row i of
 // the matrix in registers (DP >= D entries, rows i >= D are identity rows, so
-// the padded system is block-diagonal); a column is passed to all lanes
-// through the LDS array col (a wave's LDS operations complete in order).
-// Right-looking elimination, forward substitution in increasing j, backward
-// in decreasing i, every update an fma. Writes L back to sK (the lower
-// triangle only: the upper triangle holds the Gram copy) and y to sy[0..D).
-// Returns false when a pivot is not positive (or NaN). Site gives each calling
-// kernel its own instance, so that the LDS arrays it passes stay compile-time
-// addresses inside the function (with two callers they become runtime pointers
-// and every unrolled access pays for address arithmetic).
+// the padded system is block-diagonal); the loops are unrolled, so the values
+// one lane needs from another (a pivot, a column entry, a solved unknown) are
+// read with v_readlane, without LDS round trips or fences. Right-looking
+// elimination, forward substitution in increasing j, backward in decreasing
+// j (the backward step reads row j of L, written back to sK), every update an
+// fma, each division by L_ii a multiplication by its reciprocal. Writes L back
+// to sK (the lower triangle only: the upper triangle holds the Gram copy) and
+// y to sy[0..D). Returns false when a pivot is not positive (or NaN). Site
+// gives each calling kernel its own instance, so that the LDS arrays it passes
+// stay compile-time addresses inside the function (with two callers they
+// become runtime pointers and every unrolled access pays for address
+// arithmetic).
 template <int DP, int Site>
 __device__ __attribute__((noinline)) bool chol_solve_wave(double* sK, const double* sv, double* sy, double* col, uint32_t D) {
-    const int i = threadIdx.x;  // < 64
-    const bool live = i < DP;
+    (void)col;
+    const int i = threadIdx.x;  // < 64, the whole wave active
     const bool real = i < (int)D;
     double r[DP];
 #pragma unroll
@@ -216,30 +227,24 @@ __device__ __attribute__((noinline)) bool chol_solve_wave(double* sK, const doub
     bool ok = true;
 #pragma unroll
     for (int k = 0; k < DP; ++k) {
-        if (live) col[i] = r[k];
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        const double dkk = col[k];
+        const double dkk = readlane_f64(r[k], k);
         ok = ok && dkk > 0.0;
         const double lkk = sqrt(dkk);
         if (i > k) r[k] = r[k] / lkk;
         if (i == k) r[k] = lkk;
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        if (live) col[i] = r[k];
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 #pragma unroll
         for (int j = k + 1; j < DP; ++j) {
-            const double ljk = col[j];
+            const double ljk = readlane_f64(r[k], j);
             if (i >= j) r[j] = fma(-r[k], ljk, r[j]);
         }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     }
-    // L back to LDS for the backward substitution (row i' read across lanes)
+    // L back to LDS for the backward substitution (row j read across lanes)
     if (real)
 #pragma unroll
         for (int j = 0; j < DP; ++j)
             if (j <= i) sK[i * kLmMaxD + j] = r[j];
-    // 1 / L_ii, once per lane, so that no division sits on the substitutions'
-    // lane-to-lane critical path
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    // 1 / L_ii, once per lane
     double diag = 1.0;
 #pragma unroll
     for (int j = 0; j < DP; ++j)
@@ -249,10 +254,7 @@ __device__ __attribute__((noinline)) bool chol_solve_wave(double* sK, const doub
     double acc = real ? sv[i] : 0.0, z = 0.0;
 #pragma unroll
     for (int j = 0; j < DP; ++j) {
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        if (i == j) col[j] = acc * dinv;
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        const double zj = col[j];
+        const double zj = readlane_f64(acc * dinv, j);
         if (i == j) z = zj;
         if (i > j) acc = fma(-r[j], zj, acc);
     }
@@ -261,10 +263,7 @@ __device__ __attribute__((noinline)) bool chol_solve_wave(double* sK, const doub
     double y = 0.0;
 #pragma unroll
     for (int j = DP - 1; j >= 0; --j) {
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        if (i == j) col[j] = acc * dinv;
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        const double yj = col[j];
+        const double yj = readlane_f64(acc * dinv, j);
         if (i == j) y = yj;
         // L[j][i] for i < j: row j of L, read by lane i
         if (i < j && j < (int)D) acc = fma(-sK[j * kLmMaxD + i], yj, acc);
