@@ -327,35 +327,48 @@ __device__ __attribute__((noinline)) bool chol_solve_wave_lds(double* sK, const 
 }
 
 // y = pinv(K, eps) v for the Gram matrix K (lower triangle of sK with row
-// stride kLmMaxD, diagonal in sgd), D <= 64 and even, by one wave: parallel
-// cyclic Jacobi on a copy of K (A, row stride kLmMaxD + 1) with the
-// eigenvectors accumulated in V. A sweep is D - 1 rounds of the round-robin
-// (circle) schedule: in round t index D - 1 meets t and (t + i) mod (D - 1)
-// meets (t - i) mod (D - 1), i = 1 .. D/2 - 1, so each round rotates D/2
-// disjoint pairs at once -- the rotations from the round's A, then all column
-// updates (A J), then all row updates (J^T A J) and V J. Sweeps run until the
-// off-diagonal mass is below 1e-30 of the diagonal's; eigenvalues |lambda| <=
-// eps are dropped (as the reference's SVD pseudo-inverse drops singular
-// values <= eps). cs holds the D/2 rotations of a round, pq their indices.
+// stride kLmMaxD, diagonal in sgd), D <= DP, by one wave: parallel cyclic
+// Jacobi on a copy of K padded to DP x DP with zeros (A, row stride
+// kLmMaxD + 1), the eigenvectors accumulated in V. The padding rows and
+// columns stay exactly zero (every rotation that pairs a padding index has
+// a_pq = 0 and is the identity), so they add eigenvalues 0, which the
+// threshold drops. A sweep is DP - 1 rounds of the round-robin (circle)
+// schedule: in round t index DP - 1 meets t and (t + r) mod (DP - 1) meets
+// (t - r) mod (DP - 1), r = 1 .. DP/2 - 1, so each round rotates DP/2
+// disjoint pairs at once. The rotations are computed from the round's A
+// (lanes r < DP/2, into cs and pq); then each 2 x 2 block of A at pairs
+// (r, s) becomes R_r^T A_rs R_s (the column rotation of pair s first, then
+// the row rotation of pair r), independently of every other block, and V's
+// rows take the column rotations. Lane k always holds pair s = k mod DP/2,
+// so it reads one column rotation per round. Sweeps run until the
+// off-diagonal mass is below 1e-30 of the diagonal's; eigenvalues |lambda|
+// <= eps are dropped (as the reference's SVD pseudo-inverse drops singular
+// values <= eps).
+template <int DP>
 __device__ __attribute__((noinline)) void jacobi_pinv_wave(const double* sK, const double* sgd, const double* sv,
                                                            double* sy, double* A, double* V, double* col,
                                                            double2* cs, int2* pq, uint32_t D, double eps) {
-    constexpr int WS = kLmMaxD + 1;
-    const int k = threadIdx.x;  // < 64
-    const bool live = k < (int)D;
-    const int half = (int)D / 2;
-    if (live)
-        for (uint32_t j = 0; j < D; ++j) {
-            A[k * WS + j] = (j == (uint32_t)k) ? sgd[k] : ((j < (uint32_t)k) ? sK[k * kLmMaxD + j] : sK[j * kLmMaxD + k]);
-            V[k * WS + j] = (j == (uint32_t)k) ? 1.0 : 0.0;
+    constexpr int WS = kLmMaxD + 1, H = DP / 2, NM = DP - 1;
+    constexpr int RSTEP = 64 / H;               // block rows / V rows one lane steps by
+    constexpr int NB = H * H / 64, NV = DP * H / 64;
+    static_assert(DP == 32 || DP == 64, "one wave, pairs a multiple of 64 / (DP / 2)");
+    const int k = threadIdx.x;  // < 64, the whole wave active
+    const int s = k % H, r0 = k / H;
+    if (k < DP)
+#pragma unroll 4
+        for (int j = 0; j < DP; ++j) {
+            const bool in = k < (int)D && j < (int)D;
+            A[k * WS + j] = !in ? 0.0 : (j == k ? sgd[k] : (j < k ? sK[k * kLmMaxD + j] : sK[j * kLmMaxD + k]));
+            V[k * WS + j] = (j == k) ? 1.0 : 0.0;
         }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     for (int sweep = 0; sweep < 100; ++sweep) {
         double off = 0.0, dg = 0.0;
-        if (live)
-            for (uint32_t j = 0; j < D; ++j) {
+        if (k < DP)
+#pragma unroll 8
+            for (int j = 0; j < DP; ++j) {
                 const double v = A[k * WS + j];
-                if (j == (uint32_t)k)
+                if (j == k)
                     dg += v * v;
                 else
                     off += v * v;
@@ -366,17 +379,10 @@ __device__ __attribute__((noinline)) void jacobi_pinv_wave(const double* sK, con
             dg += __shfl_xor(dg, o);
         }
         if (off <= dg * 1e-30 || off == 0.0) break;
-        const int nm = (int)D - 1;
-        for (int t = 0; t < nm; ++t) {
-            if (k < half) {
-                int p, q;
-                if (k == 0) {
-                    p = nm;
-                    q = t;
-                } else {
-                    p = (t + k) % nm;
-                    q = (t - k + nm) % nm;
-                }
+        for (int t = 0; t < NM; ++t) {
+            if (k < H) {
+                const int p = k == 0 ? NM : (t + k) % NM;
+                const int q = k == 0 ? t : (t - k + NM) % NM;
                 const double apq = A[p * WS + q];
                 double c = 1.0, sn = 0.0;
                 if (apq != 0.0) {
@@ -389,42 +395,45 @@ __device__ __attribute__((noinline)) void jacobi_pinv_wave(const double* sK, con
                 pq[k] = make_int2(p, q);
             }
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            // A J: rows x pairs
-            for (int i = k; i < (int)D * half; i += 64) {
-                const int row = i % (int)D, r = i / (int)D;
-                const int2 e = pq[r];
-                const double2 rot = cs[r];
-                const double akp = A[row * WS + e.x], akq = A[row * WS + e.y];
-                A[row * WS + e.x] = rot.x * akp - rot.y * akq;
-                A[row * WS + e.y] = rot.y * akp + rot.x * akq;
+            const int2 es = pq[s];
+            const double2 rs = cs[s];
+#pragma unroll
+            for (int u = 0; u < NB; ++u) {
+                const int r = r0 + RSTEP * u;
+                const int2 er = pq[r];
+                const double2 rr = cs[r];
+                const double a = A[er.x * WS + es.x], b = A[er.x * WS + es.y];
+                const double c = A[er.y * WS + es.x], d = A[er.y * WS + es.y];
+                const double a1 = rs.x * a - rs.y * b, b1 = rs.y * a + rs.x * b;
+                const double c1 = rs.x * c - rs.y * d, d1 = rs.y * c + rs.x * d;
+                A[er.x * WS + es.x] = rr.x * a1 - rr.y * c1;
+                A[er.y * WS + es.x] = rr.y * a1 + rr.x * c1;
+                A[er.x * WS + es.y] = rr.x * b1 - rr.y * d1;
+                A[er.y * WS + es.y] = rr.y * b1 + rr.x * d1;
             }
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            // J^T (A J) and V J
-            for (int i = k; i < (int)D * half; i += 64) {
-                const int cl = i % (int)D, r = i / (int)D;
-                const int2 e = pq[r];
-                const double2 rot = cs[r];
-                const double apk = A[e.x * WS + cl], aqk = A[e.y * WS + cl];
-                A[e.x * WS + cl] = rot.x * apk - rot.y * aqk;
-                A[e.y * WS + cl] = rot.y * apk + rot.x * aqk;
-                const double vkp = V[cl * WS + e.x], vkq = V[cl * WS + e.y];
-                V[cl * WS + e.x] = rot.x * vkp - rot.y * vkq;
-                V[cl * WS + e.y] = rot.y * vkp + rot.x * vkq;
+#pragma unroll
+            for (int u = 0; u < NV; ++u) {
+                const int row = r0 + RSTEP * u;
+                const double vp = V[row * WS + es.x], vq = V[row * WS + es.y];
+                V[row * WS + es.x] = rs.x * vp - rs.y * vq;
+                V[row * WS + es.y] = rs.y * vp + rs.x * vq;
             }
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         }
     }
     // w_k = (V_:k . v) / lambda_k over the kept eigenvalues, y = V w
-    if (live) {
+    if (k < DP) {
         double d = 0.0;
-        for (uint32_t j = 0; j < D; ++j) d += V[j * WS + k] * sv[j];
+#pragma unroll 8
+        for (int j = 0; j < DP; ++j) d += V[j * WS + k] * (j < (int)D ? sv[j] : 0.0);
         const double lam = A[k * WS + k];
         col[k] = fabs(lam) > eps ? d / lam : 0.0;
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    if (live) {
+    if (k < (int)D) {
         double y = 0.0;
-        for (uint32_t j = 0; j < D; ++j) y += V[k * WS + j] * col[j];
+#pragma unroll 8
+        for (int j = 0; j < DP; ++j) y += V[k * WS + j] * col[j];
         sy[k] = y;
     }
 }
@@ -558,7 +567,12 @@ __global__ __launch_bounds__(kLB) void lm_solve(LmArgs a) {
         uint32_t it = 0;
         for (; it < a.k_max; ++it) {
             if (found) break;
-            if (t < 64) jacobi_pinv_wave(s_K, s_gd, s_v, s_y, s_work, s_vec, s_col, s_rot, s_pair, D, 1e-3);
+            if (t < 64) {
+                if (D <= 32)
+                    jacobi_pinv_wave<32>(s_K, s_gd, s_v, s_y, s_work, s_vec, s_col, s_rot, s_pair, D, 1e-3);
+                else
+                    jacobi_pinv_wave<64>(s_K, s_gd, s_v, s_y, s_work, s_vec, s_col, s_rot, s_pair, D, 1e-3);
+            }
             __syncthreads();
             double hh = 0.0, xx = 0.0;
             for (uint32_t l = t; l < n; l += kLB) {
