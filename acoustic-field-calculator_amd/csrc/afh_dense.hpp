@@ -48,6 +48,14 @@ __device__ inline double2 term_f64(const float* src, int source_type, double tx,
     return make_double2(r * cs, r * sn);
 }
 
+// Workgroup barrier that orders LDS accesses only (__syncthreads() also
+// waits for outstanding global stores).
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Block reductions in a fixed order: lanes by xor shuffles, then the 4 waves.
 __device__ inline double block_sum(double v, double* red) {
 #pragma unroll

@@ -327,114 +327,132 @@ __device__ __attribute__((noinline)) bool chol_solve_wave_lds(double* sK, const 
 }
 
 // y = pinv(K, eps) v for the Gram matrix K (lower triangle of sK with row
-// stride kLmMaxD, diagonal in sgd), D <= DP, by one wave: parallel cyclic
-// Jacobi on a copy of K padded to DP x DP with zeros (A, row stride
-// kLmMaxD + 1), the eigenvectors accumulated in V. The padding rows and
-// columns stay exactly zero (every rotation that pairs a padding index has
-// a_pq = 0 and is the identity), so they add eigenvalues 0, which the
-// threshold drops. A sweep is DP - 1 rounds of the round-robin (circle)
-// schedule: in round t index DP - 1 meets t and (t + r) mod (DP - 1) meets
-// (t - r) mod (DP - 1), r = 1 .. DP/2 - 1, so each round rotates DP/2
-// disjoint pairs at once. The rotations are computed from the round's A
-// (lanes r < DP/2, into cs and pq); then each 2 x 2 block of A at pairs
-// (r, s) becomes R_r^T A_rs R_s (the column rotation of pair s first, then
-// the row rotation of pair r), independently of every other block, and V's
-// rows take the column rotations. Lane k always holds pair s = k mod DP/2,
-// so it reads one column rotation per round. Sweeps run until the
-// off-diagonal mass is below 1e-30 of the diagonal's; eigenvalues |lambda|
-// <= eps are dropped (as the reference's SVD pseudo-inverse drops singular
-// values <= eps).
+// stride kLmMaxD, diagonal in sgd), D <= DP, by the whole workgroup (kLB
+// threads): parallel cyclic Jacobi on a copy of K padded to DP x DP with
+// zeros (A, row stride kLmMaxD + 1), the eigenvectors accumulated in V. The
+// padding rows and columns stay exactly zero (every rotation that pairs a
+// padding index has a_pq = 0 and is the identity), so they add eigenvalues 0,
+// which the threshold drops. A sweep is DP - 1 rounds of the round-robin
+// (circle) schedule: in round t index DP - 1 meets t and (t + r) mod (DP - 1)
+// meets (t - r) mod (DP - 1), r = 1 .. DP/2 - 1, so each round rotates DP/2
+// disjoint pairs at once. Threads r < DP/2 compute the round's rotations from
+// its A (into cs and pq); then each 2 x 2 block of A at pairs (r, s) becomes
+// R_r^T A_rs R_s (the column rotation of pair s first, then the row rotation
+// of pair r), independently of every other block, and V's rows take the
+// column rotations: thread T holds pair s = T mod DP/2 throughout, the blocks
+// and V rows of its group. Sweeps run until the off-diagonal mass is below
+// 1e-30 of the diagonal's; eigenvalues |lambda| <= eps are dropped (as the
+// reference's SVD pseudo-inverse drops singular values <= eps).
 template <int DP>
-__device__ __attribute__((noinline)) void jacobi_pinv_wave(const double* sK, const double* sgd, const double* sv,
-                                                           double* sy, double* A, double* V, double* col,
-                                                           double2* cs, int2* pq, uint32_t D, double eps) {
+__device__ __attribute__((noinline)) void jacobi_pinv_block(const double* sK, const double* sgd, const double* sv,
+                                                            double* sy, double* A, double* V, double* col,
+                                                            double2* cs, int2* pq, double* red, uint32_t D,
+                                                            double eps) {
     constexpr int WS = kLmMaxD + 1, H = DP / 2, NM = DP - 1;
-    constexpr int RSTEP = 64 / H;               // block rows / V rows one lane steps by
-    constexpr int NB = H * H / 64, NV = DP * H / 64;
-    static_assert(DP == 32 || DP == 64, "one wave, pairs a multiple of 64 / (DP / 2)");
-    const int k = threadIdx.x;  // < 64, the whole wave active
-    const int s = k % H, r0 = k / H;
-    if (k < DP)
-#pragma unroll 4
-        for (int j = 0; j < DP; ++j) {
-            const bool in = k < (int)D && j < (int)D;
-            A[k * WS + j] = !in ? 0.0 : (j == k ? sgd[k] : (j < k ? sK[k * kLmMaxD + j] : sK[j * kLmMaxD + k]));
-            V[k * WS + j] = (j == k) ? 1.0 : 0.0;
-        }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    constexpr int RSTEP = kLB / H;              // block rows / V rows a thread steps by
+    constexpr int NB = H * H / kLB;             // blocks per thread; 2 NB V row pairs
+    static_assert(kLB == 256 && (DP == 32 || DP == 64), "4 waves; 1 or 4 blocks per thread");
+    const int T = threadIdx.x;
+    const int s = T % H, r0 = T / H;
+    for (int e = T; e < DP * DP; e += kLB) {
+        const int k = e / DP, j = e % DP;
+        const bool in = k < (int)D && j < (int)D;
+        A[k * WS + j] = !in ? 0.0 : (j == k ? sgd[k] : (j < k ? sK[k * kLmMaxD + j] : sK[j * kLmMaxD + k]));
+        V[k * WS + j] = (j == k) ? 1.0 : 0.0;
+    }
+    lds_barrier();
     for (int sweep = 0; sweep < 100; ++sweep) {
         double off = 0.0, dg = 0.0;
-        if (k < DP)
-#pragma unroll 8
-            for (int j = 0; j < DP; ++j) {
-                const double v = A[k * WS + j];
-                if (j == k)
-                    dg += v * v;
-                else
-                    off += v * v;
-            }
-#pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) {
-            off += __shfl_xor(off, o);
-            dg += __shfl_xor(dg, o);
+        for (int e = T; e < DP * DP; e += kLB) {
+            const int k = e / DP, j = e % DP;
+            const double v = A[k * WS + j];
+            if (j == k)
+                dg += v * v;
+            else
+                off += v * v;
         }
+        off = block_sum(off, red);
+        dg = block_sum(dg, red);
         if (off <= dg * 1e-30 || off == 0.0) break;
         for (int t = 0; t < NM; ++t) {
-            if (k < H) {
-                const int p = k == 0 ? NM : (t + k) % NM;
-                const int q = k == 0 ? t : (t - k + NM) % NM;
+            if (T < H) {
+                const int p = T == 0 ? NM : (t + T) % NM;
+                const int q = T == 0 ? t : (t - T + NM) % NM;
                 const double apq = A[p * WS + q];
                 double c = 1.0, sn = 0.0;
                 if (apq != 0.0) {
-                    const double theta = (A[q * WS + q] - A[p * WS + p]) / (2.0 * apq);
-                    const double tt = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-                    c = 1.0 / sqrt(tt * tt + 1.0);
+                    // t = sgn(theta) / (|theta| + sqrt(theta^2 + 1)) with theta = delta / tau,
+                    // delta = a_qq - a_pp, tau = 2 a_pq, as sgn(theta) |tau| / (|delta| +
+                    // hypot(delta, tau)); sgn(+-0) = +1
+                    const double delta = A[q * WS + q] - A[p * WS + p], tau = 2.0 * apq;
+                    const double sg = (delta == 0.0 || (delta > 0.0) == (tau > 0.0)) ? 1.0 : -1.0;
+                    const double tt = sg * fabs(tau) / (fabs(delta) + sqrt(fma(delta, delta, tau * tau)));
+                    c = rsqrt(fma(tt, tt, 1.0));
                     sn = tt * c;
                 }
-                cs[k] = make_double2(c, sn);
-                pq[k] = make_int2(p, q);
+                cs[T] = make_double2(c, sn);
+                pq[T] = make_int2(p, q);
             }
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            lds_barrier();
+            // every load of the thread's group is issued before its first
+            // store (the elements are disjoint, which the compiler cannot
+            // see), so the group pays one LDS round trip
             const int2 es = pq[s];
             const double2 rs = cs[s];
+            int2 er[NB];
+            double2 rr[NB];
+            double va[NB], vb[NB], vc[NB], vd[NB], vp[2 * NB], vq[2 * NB];
 #pragma unroll
             for (int u = 0; u < NB; ++u) {
                 const int r = r0 + RSTEP * u;
-                const int2 er = pq[r];
-                const double2 rr = cs[r];
-                const double a = A[er.x * WS + es.x], b = A[er.x * WS + es.y];
-                const double c = A[er.y * WS + es.x], d = A[er.y * WS + es.y];
-                const double a1 = rs.x * a - rs.y * b, b1 = rs.y * a + rs.x * b;
-                const double c1 = rs.x * c - rs.y * d, d1 = rs.y * c + rs.x * d;
-                A[er.x * WS + es.x] = rr.x * a1 - rr.y * c1;
-                A[er.y * WS + es.x] = rr.y * a1 + rr.x * c1;
-                A[er.x * WS + es.y] = rr.x * b1 - rr.y * d1;
-                A[er.y * WS + es.y] = rr.y * b1 + rr.x * d1;
+                er[u] = pq[r];
+                rr[u] = cs[r];
             }
 #pragma unroll
-            for (int u = 0; u < NV; ++u) {
-                const int row = r0 + RSTEP * u;
-                const double vp = V[row * WS + es.x], vq = V[row * WS + es.y];
-                V[row * WS + es.x] = rs.x * vp - rs.y * vq;
-                V[row * WS + es.y] = rs.y * vp + rs.x * vq;
+            for (int u = 0; u < NB; ++u) {
+                va[u] = A[er[u].x * WS + es.x];
+                vb[u] = A[er[u].x * WS + es.y];
+                vc[u] = A[er[u].y * WS + es.x];
+                vd[u] = A[er[u].y * WS + es.y];
             }
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+#pragma unroll
+            for (int u = 0; u < 2 * NB; ++u) {
+                const int row = r0 + RSTEP * u;
+                vp[u] = V[row * WS + es.x];
+                vq[u] = V[row * WS + es.y];
+            }
+#pragma unroll
+            for (int u = 0; u < NB; ++u) {
+                const double a1 = rs.x * va[u] - rs.y * vb[u], b1 = rs.y * va[u] + rs.x * vb[u];
+                const double c1 = rs.x * vc[u] - rs.y * vd[u], d1 = rs.y * vc[u] + rs.x * vd[u];
+                A[er[u].x * WS + es.x] = rr[u].x * a1 - rr[u].y * c1;
+                A[er[u].y * WS + es.x] = rr[u].y * a1 + rr[u].x * c1;
+                A[er[u].x * WS + es.y] = rr[u].x * b1 - rr[u].y * d1;
+                A[er[u].y * WS + es.y] = rr[u].y * b1 + rr[u].x * d1;
+            }
+#pragma unroll
+            for (int u = 0; u < 2 * NB; ++u) {
+                const int row = r0 + RSTEP * u;
+                V[row * WS + es.x] = rs.x * vp[u] - rs.y * vq[u];
+                V[row * WS + es.y] = rs.y * vp[u] + rs.x * vq[u];
+            }
+            lds_barrier();
         }
     }
     // w_k = (V_:k . v) / lambda_k over the kept eigenvalues, y = V w
-    if (k < DP) {
+    if (T < DP) {
         double d = 0.0;
 #pragma unroll 8
-        for (int j = 0; j < DP; ++j) d += V[j * WS + k] * (j < (int)D ? sv[j] : 0.0);
-        const double lam = A[k * WS + k];
-        col[k] = fabs(lam) > eps ? d / lam : 0.0;
+        for (int j = 0; j < DP; ++j) d += V[j * WS + T] * (j < (int)D ? sv[j] : 0.0);
+        const double lam = A[T * WS + T];
+        col[T] = fabs(lam) > eps ? d / lam : 0.0;
     }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    if (k < (int)D) {
+    lds_barrier();
+    if (T < (int)D) {
         double y = 0.0;
 #pragma unroll 8
-        for (int j = 0; j < DP; ++j) y += V[k * WS + j] * col[j];
-        sy[k] = y;
+        for (int j = 0; j < DP; ++j) y += V[T * WS + j] * col[j];
+        sy[T] = y;
     }
 }
 
@@ -567,12 +585,10 @@ __global__ __launch_bounds__(kLB) void lm_solve(LmArgs a) {
         uint32_t it = 0;
         for (; it < a.k_max; ++it) {
             if (found) break;
-            if (t < 64) {
-                if (D <= 32)
-                    jacobi_pinv_wave<32>(s_K, s_gd, s_v, s_y, s_work, s_vec, s_col, s_rot, s_pair, D, 1e-3);
-                else
-                    jacobi_pinv_wave<64>(s_K, s_gd, s_v, s_y, s_work, s_vec, s_col, s_rot, s_pair, D, 1e-3);
-            }
+            if (D <= 32)
+                jacobi_pinv_block<32>(s_K, s_gd, s_v, s_y, s_work, s_vec, s_col, s_rot, s_pair, s_red, D, 1e-3);
+            else
+                jacobi_pinv_block<64>(s_K, s_gd, s_v, s_y, s_work, s_vec, s_col, s_rot, s_pair, s_red, D, 1e-3);
             __syncthreads();
             double hh = 0.0, xx = 0.0;
             for (uint32_t l = t; l < n; l += kLB) {
