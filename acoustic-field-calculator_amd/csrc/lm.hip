@@ -759,6 +759,138 @@ __device__ inline void rows_times(const double2* G, const double2* q, uint32_t N
     }
 }
 
+// Gradient descent (gradient_descent.rs:49-56) with G held in registers:
+// thread T owns the columns l = T + kLB c (c < CPT, l < N) of G for MP >= M
+// foci (zero rows beyond M), its phases x_l and E_l = e^{i x_l}; thread i < M
+// also owns the amplitude parameter x_{N+i} (column N + i of B = [G |
+// -diag(amps)]). A step: the thread's partial field sums over its columns, 2MP
+// values, reduced over the wave by a transposed butterfly (at offset o = 32,
+// 16, .. the lane with bit o set keeps the upper half of its list and sends
+// the lower half, so 2MP values cost 2MP - 1 shuffles and lane L ends with
+// value L (MP = 32) or L >> 1 (MP = 16)), then over the 4 waves in a fixed
+// order; g_l = Im(conj(E_l) sum_i conj(G_il) s_i); the block maximum of g
+// (the reference's stop test); x -= 0.1 g. No global memory is touched inside
+// the loop.
+template <int MP, int CPT>
+__global__ __launch_bounds__(kLB) void gd_solve(LmArgs a) {
+    constexpr int K = 2 * MP;  // re s_i at i, im s_i at MP + i
+    static_assert(K == 32 || K == 64, "16 or 32 foci");
+    __shared__ double s_part[4][64];
+    __shared__ double s_f[K];
+    __shared__ double2 s_ea[MP];
+    __shared__ double s_red[4];
+    const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+    const uint32_t b = blockIdx.x;
+    const uint32_t base = a.offsets[b], M = a.offsets[b + 1] - base;
+    const uint32_t N = a.n;
+    LmScratch sc = lm_scratch(a.scratch + b * a.scratch_stride, N, M);
+    double gr[CPT][MP], gi[CPT][MP], x[CPT];
+    double2 e[CPT];
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) {
+        const uint32_t l = t + kLB * c;
+#pragma unroll
+        for (int i = 0; i < MP; ++i) {
+            const double2 v = (l < N && i < (int)M) ? sc.G[(uint64_t)i * N + l] : make_double2(0.0, 0.0);
+            gr[c][i] = v.x;
+            gi[c][i] = v.y;
+        }
+        x[c] = 0.0;
+        e[c] = make_double2(1.0, 0.0);
+    }
+    const bool own_amp = t < (int)M;
+    const double amp = own_amp ? (double)a.amps[base + t] : 0.0;
+    const bool amp_row = t < K && t % MP < (int)M;  // thread t < K sums value t
+    const double amp_f = amp_row ? (double)a.amps[base + t % MP] : 0.0;
+    double xa = 0.0;
+    if (t < MP) s_ea[t] = make_double2(1.0, 0.0);
+    uint32_t it = 0;
+    for (; it < a.k_max; ++it) {
+        // partial sums over this thread's columns: s_i = sum_l G_il E_l
+        double v[K];
+#pragma unroll
+        for (int i = 0; i < MP; ++i) {
+            double re = 0.0, im = 0.0;
+#pragma unroll
+            for (int c = 0; c < CPT; ++c) {
+                re += gr[c][i] * e[c].x - gi[c][i] * e[c].y;
+                im += gr[c][i] * e[c].y + gi[c][i] * e[c].x;
+            }
+            v[i] = re;
+            v[MP + i] = im;
+        }
+        // transposed butterfly over the wave
+#pragma unroll
+        for (int o = 32, n = K; n > 1; o >>= 1, n >>= 1) {
+            const bool up = (lane & o) != 0;
+#pragma unroll
+            for (int j = 0; j < n / 2; ++j) {
+                const double send = up ? v[j] : v[j + n / 2];
+                const double keep = up ? v[j + n / 2] : v[j];
+                v[j] = keep + __shfl_xor(send, o);
+            }
+        }
+        double f = v[0];
+        if (K == 32) f += __shfl_xor(f, 1);
+        const int idx = K == 64 ? lane : lane >> 1;
+        if (K == 64 || (lane & 1) == 0) s_part[w][idx] = f;
+        lds_barrier();
+        if (t < K) {
+            double sum = ((s_part[0][t] + s_part[1][t]) + s_part[2][t]) + s_part[3][t];
+            if (amp_row) {
+                // the -diag(amps) block of B: column N + t mod MP
+                const double2 ea = s_ea[t % MP];
+                sum -= amp_f * (t < MP ? ea.x : ea.y);
+            }
+            s_f[t] = sum;
+        }
+        lds_barrier();
+        // g_l = Im(conj(C_il) s_i) summed over i, C_il = G_il E_l
+        double gm = -INFINITY, g[CPT];
+#pragma unroll
+        for (int c = 0; c < CPT; ++c) {
+            double tr = 0.0, ti = 0.0;
+#pragma unroll
+            for (int i = 0; i < MP; ++i) {
+                const double fr = s_f[i], fi = s_f[MP + i];
+                tr += gr[c][i] * fr + gi[c][i] * fi;
+                ti += gr[c][i] * fi - gi[c][i] * fr;
+            }
+            g[c] = e[c].x * ti - e[c].y * tr;
+            if (t + kLB * c < N) gm = fmax(gm, g[c]);
+        }
+        double ga = 0.0;
+        if (own_amp) {
+            const double2 ea = s_ea[t];
+            ga = -amp * (ea.x * s_f[MP + t] - ea.y * s_f[t]);
+            gm = fmax(gm, ga);
+        }
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) gm = fmax(gm, __shfl_xor(gm, o));
+        if (lane == 0) s_red[w] = gm;
+        lds_barrier();
+        const double gmax = fmax(fmax(s_red[0], s_red[1]), fmax(s_red[2], s_red[3]));
+        if (gmax <= 1e-6) break;
+#pragma unroll
+        for (int c = 0; c < CPT; ++c) {
+            x[c] -= 0.1 * g[c];
+            double sn, cs;
+            sincos(x[c], &sn, &cs);
+            e[c] = make_double2(cs, sn);
+        }
+        if (own_amp) {
+            xa -= 0.1 * ga;
+            double sn, cs;
+            sincos(xa, &sn, &cs);
+            s_ea[t] = make_double2(cs, sn);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < CPT; ++c)
+        if (t + kLB * c < N) a.out_phase[(uint64_t)b * N + t + kLB * c] = (float)x[c];
+    if (t == 0 && a.out_iters) a.out_iters[b] = it;
+}
+
 __global__ __launch_bounds__(kLB) void apo_solve(LmArgs a) {
     __shared__ double s_K[kLmMaxD * kLmMaxD];
     __shared__ double s_v[kLmMaxD], s_y[kLmMaxD], s_col[kLmMaxD];
@@ -1066,7 +1198,14 @@ int32_t lm_solve_batch(Context* ctx, const float* sources, uint32_t n, uint32_t 
     a.out_phase = out_phase;
     a.out_iters = out_iters;
     hipLaunchKernelGGL(lm_gen_g, dim3((n + kLB - 1) / kLB, batch), dim3(kLB), 0, st, a);
-    hipLaunchKernelGGL(lm_solve, dim3(batch), dim3(kLB), 0, st, a);
+    if (algo == kNlsGD && maxm <= 16 && n <= kLB)
+        hipLaunchKernelGGL((gd_solve<16, 1>), dim3(batch), dim3(kLB), 0, st, a);
+    else if (algo == kNlsGD && maxm <= 16 && n <= 2 * kLB)
+        hipLaunchKernelGGL((gd_solve<16, 2>), dim3(batch), dim3(kLB), 0, st, a);
+    else if (algo == kNlsGD && n <= kLB)
+        hipLaunchKernelGGL((gd_solve<32, 1>), dim3(batch), dim3(kLB), 0, st, a);
+    else
+        hipLaunchKernelGGL(lm_solve, dim3(batch), dim3(kLB), 0, st, a);
     if (!hip_ok(hipGetLastError(), "LM kernel launch")) return AFH_ERR_HIP;
     return AFH_OK;
 }

@@ -92,6 +92,28 @@ def test_gradient_descent_vs_oracle(afc, oracle, torch_cuda, m, seed):
     assert d <= 1e-6
 
 
+@pytest.mark.parametrize("tiles,n_src,m,seed", [((2, 1), 300, 3, 11), ((1, 1), 200, 20, 12), ((2, 2), 600, 3, 13)])
+def test_gradient_descent_sizes_vs_oracle(afc, oracle, torch_cuda, tiles, n_src, m, seed):
+    """Gradient descent at the sizes that select each kernel: G in registers
+    for M <= 16 with 256 < N <= 512 (two columns per thread) and for
+    16 < M <= 32 with N <= 256, G in global memory for N > 512. 2000 steps."""
+    log(f"case: gradient descent tiles={tiles} N={n_src} M={m} seed={seed}")
+    from afc import device as D
+    import torch
+    rng = np.random.default_rng(seed)
+    pos = oracle.tiled_array(*tiles)[:n_src]
+    src = oracle.t4010a1_sources(pos)
+    k = np.full(n_src, oracle.WAVENUM_40K, np.float32)
+    a = np.full(n_src, np.float32(oracle.attenuation_coef(40e3, 30, 1, 1, oracle.TEMP_340)), np.float32)
+    foci, amps = problem(rng, m)
+    ph, it = run_nls(torch, src, k, a, foci, amps, np.array([0, m], np.uint32), D.NLS_GRADIENT_DESCENT, 2000)
+    ref, x_ref, it_ref = oracle.gradient_descent(1, src, k, a, foci, amps, "f64", k_max=2000)
+    d = wrapdiff(ph[0], ref).max()
+    log(f"  steps gpu {it[0]} oracle {it_ref}; max phase diff {d:.2e}")
+    assert abs(int(it[0]) - it_ref) <= 2
+    assert d <= 1e-6
+
+
 def test_nls_capi_sets_phases_only(afc, oracle, torch_cuda):
     log("case: AFO_GaussNewton / AFO_GradientDescent through the C ABI")
     rng = np.random.default_rng(6)
