@@ -205,26 +205,27 @@ __device__ __forceinline__ SrcF src_from_block(const SrcBlock& sb, int cb) {
     return {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
 }
 
-// 1 / d_j (or 1 / d_j^2) of the chunk's columns into s_inv[0..31]; d_j =
-// sum_i |G_ij| summed in a fixed order. Columns beyond n get 0.
+// 1 / d of the thread's column j0 + (t & 31), d = sum_i |G_ij| summed in a
+// fixed order (each row group's partial, then the 8 groups in order); 0 for
+// columns beyond n. Threads t < kKC write d to d_out[j0 + t]. Every thread sums
+// the 8 partials of its own column, so one barrier suffices; s_part is written
+// again only in the next chunk, after the chunk's tile barrier, which every
+// thread reaches after its reads here.
 template <int MP>
-__device__ __forceinline__ void column_scale(const float2 (&g)[MP / 8], uint32_t j0, uint32_t n, bool squared,
-                                             float (*s_part)[kKC], float* s_inv, float* d_out = nullptr) {
+__device__ __forceinline__ float column_inv(const float2 (&g)[MP / 8], uint32_t j0, uint32_t n,
+                                            float (*s_part)[kKC], float* d_out) {
     const int t = threadIdx.x;
     float part = 0.f;
 #pragma unroll
     for (int r = 0; r < MP / 8; ++r) part += sqrtf(g[r].x * g[r].x + g[r].y * g[r].y);
     s_part[t >> 5][t & 31] = part;
     lds_sync();
-    if (t < kKC) {
-        float d = 0.f;
+    float d = 0.f;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) d += s_part[q][t];
-        // a real column with d = 0 gives NaN, as conj(G) / 0 does in the reference
-        s_inv[t] = (j0 + t < n) ? (squared ? 1.0f / (d * d) : 1.0f / d) : 0.f;
-        if (d_out) d_out[j0 + t] = d;
-    }
-    lds_sync();
+    for (int q = 0; q < 8; ++q) d += s_part[q][t & 31];
+    if (t < kKC) d_out[j0 + t] = d;
+    // a real column with d = 0 gives NaN, as conj(G) / 0 does in the reference
+    return (j0 + (t & 31) < n) ? 1.0f / d : 0.f;
 }
 
 // Kernel A's copy of the G tile for kernel C: rows < M only (rows >= M are
@@ -245,25 +246,39 @@ __device__ __forceinline__ void store_g_tile(const HoloArgs& a, uint32_t k, uint
     }
 }
 
-template <int MP>
+// Kernel C's read of the G tile: FP32 G, or (H16) the FP16 H = G / d_j that
+// kernel A fed its MFMA in MIXED mode, widened to FP32.
+template <int MP, bool H16>
 __device__ __forceinline__ void load_g_tile(const HoloArgs& a, uint32_t k, uint32_t j0, uint32_t M,
                                             float2 (&g)[MP / 8]) {
     const int t = threadIdx.x;
-    const float2* G = a.G + (uint64_t)k * MP * a.n_packed + j0 + (t & 31);
+    const uint64_t e0 = (uint64_t)k * MP * a.n_packed + j0 + (t & 31);
 #pragma unroll
     for (int r = 0; r < MP / 8; ++r) {
+        // the selects keep the loads of a chunk in flight together (a branch
+        // around each load made the compiler wait for it inside the branch)
         const int i = (t >> 5) + 8 * r;
-        g[r] = i < (int)M ? make_float2(__builtin_nontemporal_load(&G[(uint64_t)i * a.n_packed].x),
-                                        __builtin_nontemporal_load(&G[(uint64_t)i * a.n_packed].y))
-                          : make_float2(0.f, 0.f);
+        if constexpr (H16) {
+            const uint32_t h = i < (int)M ? __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(a.G) + e0 +
+                                                                       (uint64_t)i * a.n_packed)
+                                          : 0u;
+            g[r] = make_float2((float)__builtin_bit_cast(_Float16, (uint16_t)(h & 0xffffu)),
+                               (float)__builtin_bit_cast(_Float16, (uint16_t)(h >> 16)));
+        } else {
+            const float2* G = a.G + e0 + (uint64_t)i * a.n_packed;
+            g[r] = i < (int)M ? make_float2(__builtin_nontemporal_load(&G->x), __builtin_nontemporal_load(&G->y))
+                              : make_float2(0.f, 0.f);
+        }
     }
 }
 
-// 1 / d_j^2 of the chunk's columns from kernel A's d_j (the same expression
-// as column_scale's); d is column j0 + t's value for threads t < kKC.
+// The column factor of kernel C's chunk from kernel A's d_j: 1 / d_j^2 for
+// FP32 G (conj(G_ij) / d_j^2, the same d as column_inv's), 1 / d_j for H16
+// (conj(H_ij) / d_j); d is column j0 + t's value for threads t < kKC.
+template <bool H16>
 __device__ __forceinline__ void column_scale_stored(float d, uint32_t j0, uint32_t n, float* s_inv) {
     const int t = threadIdx.x;
-    if (t < kKC) s_inv[t] = (j0 + t < n) ? 1.0f / (d * d) : 0.f;
+    if (t < kKC) s_inv[t] = (j0 + t < n) ? (H16 ? 1.0f / d : 1.0f / (d * d)) : 0.f;
     lds_sync();
 }
 
@@ -275,10 +290,10 @@ struct StoredChunk {
     float d;
 };
 
-template <int MP>
+template <int MP, bool H16>
 __device__ __forceinline__ void load_chunk(const HoloArgs& a, uint32_t k, uint32_t j0, uint32_t M,
                                            StoredChunk<MP>& c) {
-    load_g_tile<MP>(a, k, j0, M, c.g);
+    load_g_tile<MP, H16>(a, k, j0, M, c.g);
     c.d = threadIdx.x < kKC ? a.d[(uint64_t)k * a.n_packed + j0 + threadIdx.x] : 0.f;
 }
 
@@ -289,7 +304,6 @@ __global__ __launch_bounds__(kHB) void gspat_r_fp32(HoloArgs a) {
     CellsLds* s_cells = reinterpret_cast<CellsLds*>(&s_cells_mem);
     __shared__ Foci<MP> f;
     __shared__ float s_part[8][kKC];
-    __shared__ float s_inv[kKC];
     __shared__ float2 s_h[MP][kKC + 1];
     __shared__ SrcBlock s_src;
     const int t = threadIdx.x;
@@ -314,7 +328,7 @@ __global__ __launch_bounds__(kHB) void gspat_r_fp32(HoloArgs a) {
         const uint32_t j0 = chunk * kKC, in_blk = (chunk - c_begin) % kSB;
         if (in_blk == 0) {
             // every wave is past its last read of the previous block (the
-            // barrier that ends each chunk)
+            // previous chunk's tile barrier)
             src_block_put(s_src, pre);
             lds_sync();
             if (chunk + kSB < c_end) src_block_fetch(a, chunk + kSB, pre);
@@ -323,10 +337,11 @@ __global__ __launch_bounds__(kHB) void gspat_r_fp32(HoloArgs a) {
         float2 g[MP / 8];
         g_tile<SRC, MP>(sc, f, s_cells, M, g);
         store_g_tile<MP>(a, k, j0, M, g);
-        column_scale<MP>(g, j0, a.n, false, s_part, s_inv, d_out);
-        const float inv = s_inv[t & 31];
+        const float inv = column_inv<MP>(g, j0, a.n, s_part, d_out);
 #pragma unroll
         for (int r = 0; r < MP / 8; ++r) s_h[(t >> 5) + 8 * r][t & 31] = make_float2(g[r].x * inv, g[r].y * inv);
+        // the tile barrier: s_h is written again only after the next chunk's
+        // column barrier, which every thread reaches after the products below
         lds_sync();
         for (int jj = 0; jj < kKC; ++jj) {
             float2 hr[BS], hc[BS];
@@ -343,7 +358,6 @@ __global__ __launch_bounds__(kHB) void gspat_r_fp32(HoloArgs a) {
                     acc[x][y].y = fmaf(hr[x].y, hc[y].x, fmaf(-hr[x].x, hc[y].y, acc[x][y].y));
                 }
         }
-        lds_sync();
     }
     float2* R = a.R + (uint64_t)blockIdx.x * MP * MP;  // partial `part` of instance k
 #pragma unroll
@@ -360,9 +374,17 @@ __global__ __launch_bounds__(kHB) void gspat_r_fp32(HoloArgs a) {
 // R = hi hi^H + hi lo^H + lo hi^H, FP32 accumulation; H is first scaled by a
 // per-instance power of two s ~ M / 2 (undone exactly on output) so that lo
 // stays in the normal FP16 range.
-template <int SRC, int MP, bool SPLIT>
-__global__ __launch_bounds__(kHB) void gspat_r_mfma(HoloArgs a) {
+// MP = 64: at most 168 registers (accumulators included), so that 3 waves
+// share a SIMD and 3 workgroups a CU (LDS allows 3); at the 196 the compiler
+// otherwise takes, 2 workgroups per CU ran the 2052 workgroups of a
+// 513-instance bucket in 5 rounds, the last with 4 of them.
+// H16 (MIXED): the FP16 H = G / d_j of the MFMA operand is what kernel A
+// stores for kernel C, 4 bytes per element instead of FP32 G's 8.
+template <int SRC, int MP, bool SPLIT, bool H16>
+__global__ __launch_bounds__(kHB) __attribute__((amdgpu_waves_per_eu(MP == 64 ? 3 : 1))) void gspat_r_mfma(
+        HoloArgs a) {
     static_assert(MP >= 32, "one 16x16 tile per wave at least");
+    static_assert(!(SPLIT && H16), "SPLIT keeps FP32 G");
     constexpr int T = MP / 16;
     constexpr int TPW = T * T / 4;  // tiles per wave
     constexpr int HS = kKC + 8;     // row stride in halves (80 bytes)
@@ -371,7 +393,6 @@ __global__ __launch_bounds__(kHB) void gspat_r_mfma(HoloArgs a) {
     CellsLds* s_cells = reinterpret_cast<CellsLds*>(&s_cells_mem);
     __shared__ Foci<MP> f;
     __shared__ float s_part[8][kKC];
-    __shared__ float s_inv[kKC];
     __shared__ __attribute__((aligned(16))) _Float16 s_hr[NH][MP * HS];
     __shared__ __attribute__((aligned(16))) _Float16 s_hi[NH][MP * HS];
     __shared__ SrcBlock s_src;
@@ -400,7 +421,7 @@ __global__ __launch_bounds__(kHB) void gspat_r_mfma(HoloArgs a) {
         const uint32_t j0 = chunk * kKC, in_blk = (chunk - c_begin) % kSB;
         if (in_blk == 0) {
             // every wave is past its last read of the previous block (the
-            // barrier that ends each chunk)
+            // previous chunk's tile barrier)
             src_block_put(s_src, pre);
             lds_sync();
             if (chunk + kSB < c_end) src_block_fetch(a, chunk + kSB, pre);
@@ -408,9 +429,8 @@ __global__ __launch_bounds__(kHB) void gspat_r_mfma(HoloArgs a) {
         const SrcF sc = src_from_block(s_src, in_blk);
         float2 g[MP / 8];
         g_tile<SRC, MP>(sc, f, s_cells, M, g);
-        store_g_tile<MP>(a, k, j0, M, g);
-        column_scale<MP>(g, j0, a.n, false, s_part, s_inv, d_out);
-        const float inv = s_inv[t & 31] * scale;
+        if constexpr (!H16) store_g_tile<MP>(a, k, j0, M, g);
+        const float inv = column_inv<MP>(g, j0, a.n, s_part, d_out) * scale;
 #pragma unroll
         for (int r = 0; r < MP / 8; ++r) {
             const int i = (t >> 5) + 8 * r;
@@ -418,11 +438,20 @@ __global__ __launch_bounds__(kHB) void gspat_r_mfma(HoloArgs a) {
             const _Float16 rh = (_Float16)hre, ih = (_Float16)him;
             s_hr[0][i * HS + (t & 31)] = rh;
             s_hi[0][i * HS + (t & 31)] = ih;
+            if (H16 && i < (int)M) {
+                // streamed: read once, by kernel C
+                const uint32_t h = (uint32_t)__builtin_bit_cast(uint16_t, rh) |
+                                   ((uint32_t)__builtin_bit_cast(uint16_t, ih) << 16);
+                __builtin_nontemporal_store(h, reinterpret_cast<uint32_t*>(a.G) +
+                                                   ((uint64_t)k * MP + i) * a.n_packed + j0 + (t & 31));
+            }
             if (SPLIT) {
                 s_hr[NH - 1][i * HS + (t & 31)] = (_Float16)(hre - (float)rh);
                 s_hi[NH - 1][i * HS + (t & 31)] = (_Float16)(him - (float)ih);
             }
         }
+        // the tile barrier: s_hr / s_hi are written again only after the next
+        // chunk's column barrier, which every wave reaches after its MFMAs
         lds_sync();
         const int kb = (l >> 4) * 8;
 #pragma unroll
@@ -456,7 +485,6 @@ __global__ __launch_bounds__(kHB) void gspat_r_mfma(HoloArgs a) {
             accIm[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ai, br, accIm[s], 0, 0, 0);
             accIm[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(nar, bi, accIm[s], 0, 0, 0);
         }
-        lds_sync();
     }
     const float unscale = 1.0f / (scale * scale);  // exact: scale is a power of two
     float2* R = a.R + (uint64_t)blockIdx.x * MP * MP;  // partial `part` of instance k
@@ -548,7 +576,7 @@ __global__ __launch_bounds__(kHB) void gspat_iter(HoloArgs a, uint32_t iters, in
 //   gamma the sum of the partials in order; q_j -> amplitude and phase.
 enum { kQGamma = 0, kQFinal = 1 };
 
-template <int MP>
+template <int MP, bool H16>
 __global__ __launch_bounds__(kHB) void gspat_q(HoloArgs a, int closing, int pass) {
     __shared__ float s_inv[kKC];
     __shared__ float2 s_qp[8][kKC];
@@ -605,12 +633,12 @@ __global__ __launch_bounds__(kHB) void gspat_q(HoloArgs a, int closing, int pass
 #pragma unroll
     for (int r = 0; r < MP / 8; ++r) gam[r] = make_float2(0.f, 0.f);
     StoredChunk<MP> cn;
-    load_chunk<MP>(a, k, c_begin * kKC, M, cn);
+    load_chunk<MP, H16>(a, k, c_begin * kKC, M, cn);
     for (uint32_t chunk = c_begin; chunk < c_end; ++chunk) {
         const uint32_t j0 = chunk * kKC;
         const StoredChunk<MP> c = cn;
-        if (chunk + 1 < c_end) load_chunk<MP>(a, k, j0 + kKC, M, cn);
-        column_scale_stored(c.d, j0, a.n, s_inv);
+        if (chunk + 1 < c_end) load_chunk<MP, H16>(a, k, j0 + kKC, M, cn);
+        column_scale_stored<H16>(c.d, j0, a.n, s_inv);
         chunk_q(c.g);
         if (pass == kQGamma) {
             const float2 qj = s_q[jj];
@@ -882,17 +910,20 @@ static hipError_t launch_gspat_bucket(HoloArgs a, int src, int precision, uint32
             hipLaunchKernelGGL((gspat_r_fp32<kSphere, MP>), grid_a, blk, 0, st, a);
     } else {
         if constexpr (MP >= 32) {
-            const bool split = precision == AFH_HOLO_MIXED_SPLIT;
             if (src == kT4010A1) {
-                if (split)
-                    hipLaunchKernelGGL((gspat_r_mfma<kT4010A1, MP, true>), grid_a, blk, 0, st, a);
+                if (precision == AFH_HOLO_MIXED_SPLIT)
+                    hipLaunchKernelGGL((gspat_r_mfma<kT4010A1, MP, true, false>), grid_a, blk, 0, st, a);
+                else if (precision == AFH_HOLO_MIXED)
+                    hipLaunchKernelGGL((gspat_r_mfma<kT4010A1, MP, false, true>), grid_a, blk, 0, st, a);
                 else
-                    hipLaunchKernelGGL((gspat_r_mfma<kT4010A1, MP, false>), grid_a, blk, 0, st, a);
+                    hipLaunchKernelGGL((gspat_r_mfma<kT4010A1, MP, false, false>), grid_a, blk, 0, st, a);
             } else {
-                if (split)
-                    hipLaunchKernelGGL((gspat_r_mfma<kSphere, MP, true>), grid_a, blk, 0, st, a);
+                if (precision == AFH_HOLO_MIXED_SPLIT)
+                    hipLaunchKernelGGL((gspat_r_mfma<kSphere, MP, true, false>), grid_a, blk, 0, st, a);
+                else if (precision == AFH_HOLO_MIXED)
+                    hipLaunchKernelGGL((gspat_r_mfma<kSphere, MP, false, true>), grid_a, blk, 0, st, a);
                 else
-                    hipLaunchKernelGGL((gspat_r_mfma<kSphere, MP, false>), grid_a, blk, 0, st, a);
+                    hipLaunchKernelGGL((gspat_r_mfma<kSphere, MP, false, false>), grid_a, blk, 0, st, a);
             }
         } else {
             return hipErrorInvalidValue;
@@ -900,8 +931,11 @@ static hipError_t launch_gspat_bucket(HoloArgs a, int src, int precision, uint32
     }
     const int closing = precision == AFH_HOLO_MIXED_CLOSING ? 1 : 0;
     hipLaunchKernelGGL((gspat_iter<MP>), grid, blk, 0, st, a, iters, closing);
-    if (closing) hipLaunchKernelGGL((gspat_q<MP>), grid_a, blk, 0, st, a, closing, (int)kQGamma);
-    hipLaunchKernelGGL((gspat_q<MP>), grid_a, blk, 0, st, a, closing, (int)kQFinal);
+    if (closing) hipLaunchKernelGGL((gspat_q<MP, false>), grid_a, blk, 0, st, a, closing, (int)kQGamma);
+    if (precision == AFH_HOLO_MIXED)
+        hipLaunchKernelGGL((gspat_q<MP, true>), grid_a, blk, 0, st, a, closing, (int)kQFinal);
+    else
+        hipLaunchKernelGGL((gspat_q<MP, false>), grid_a, blk, 0, st, a, closing, (int)kQFinal);
     return hipGetLastError();
 }
 
