@@ -246,55 +246,49 @@ __device__ __forceinline__ void store_g_tile(const HoloArgs& a, uint32_t k, uint
     }
 }
 
-// Kernel C's read of the G tile: FP32 G, or (H16) the FP16 H = G / d_j that
-// kernel A fed its MFMA in MIXED mode, widened to FP32.
+// Kernel C's stored inputs of one chunk as loaded: the G tile (FP32 G, or
+// H16 words: the FP16 H = G / d_j that kernel A fed its MFMA in MIXED mode)
+// and, for threads t < kKC, d of column j0 + t.
 template <int MP, bool H16>
-__device__ __forceinline__ void load_g_tile(const HoloArgs& a, uint32_t k, uint32_t j0, uint32_t M,
-                                            float2 (&g)[MP / 8]) {
-    const int t = threadIdx.x;
-    const uint64_t e0 = (uint64_t)k * MP * a.n_packed + j0 + (t & 31);
-#pragma unroll
-    for (int r = 0; r < MP / 8; ++r) {
-        // the selects keep the loads of a chunk in flight together (a branch
-        // around each load made the compiler wait for it inside the branch)
-        const int i = (t >> 5) + 8 * r;
-        if constexpr (H16) {
-            const uint32_t h = i < (int)M ? __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(a.G) + e0 +
-                                                                       (uint64_t)i * a.n_packed)
-                                          : 0u;
-            g[r] = make_float2((float)__builtin_bit_cast(_Float16, (uint16_t)(h & 0xffffu)),
-                               (float)__builtin_bit_cast(_Float16, (uint16_t)(h >> 16)));
-        } else {
-            const float2* G = a.G + e0 + (uint64_t)i * a.n_packed;
-            g[r] = i < (int)M ? make_float2(__builtin_nontemporal_load(&G->x), __builtin_nontemporal_load(&G->y))
-                              : make_float2(0.f, 0.f);
-        }
-    }
-}
-
-// The column factor of kernel C's chunk from kernel A's d_j: 1 / d_j^2 for
-// FP32 G (conj(G_ij) / d_j^2, the same d as column_inv's), 1 / d_j for H16
-// (conj(H_ij) / d_j); d is column j0 + t's value for threads t < kKC.
-template <bool H16>
-__device__ __forceinline__ void column_scale_stored(float d, uint32_t j0, uint32_t n, float* s_inv) {
-    const int t = threadIdx.x;
-    if (t < kKC) s_inv[t] = (j0 + t < n) ? (H16 ? 1.0f / d : 1.0f / (d * d)) : 0.f;
-    lds_sync();
-}
-
-// Kernel C's stored inputs of one chunk: the G tile and, for threads t < kKC,
-// d of column j0 + t. Loaded one chunk ahead of their use.
-template <int MP>
 struct StoredChunk {
-    float2 g[MP / 8];
+    std::conditional_t<H16, uint32_t, float2> g[MP / 8];
     float d;
 };
 
 template <int MP, bool H16>
 __device__ __forceinline__ void load_chunk(const HoloArgs& a, uint32_t k, uint32_t j0, uint32_t M,
-                                           StoredChunk<MP>& c) {
-    load_g_tile<MP, H16>(a, k, j0, M, c.g);
-    c.d = threadIdx.x < kKC ? a.d[(uint64_t)k * a.n_packed + j0 + threadIdx.x] : 0.f;
+                                           StoredChunk<MP, H16>& c) {
+    const int t = threadIdx.x;
+    const uint64_t e0 = (uint64_t)k * MP * a.n_packed + j0 + (t & 31);
+#pragma unroll
+    for (int r = 0; r < MP / 8; ++r) {
+        // selects, not branches: a branch around each load made the compiler
+        // wait for it inside the branch
+        const int i = (t >> 5) + 8 * r;
+        if constexpr (H16) {
+            c.g[r] = i < (int)M ? __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(a.G) + e0 +
+                                                             (uint64_t)i * a.n_packed)
+                                : 0u;
+        } else {
+            const float2* G = a.G + e0 + (uint64_t)i * a.n_packed;
+            c.g[r] = i < (int)M ? make_float2(__builtin_nontemporal_load(&G->x), __builtin_nontemporal_load(&G->y))
+                                : make_float2(0.f, 0.f);
+        }
+    }
+    c.d = t < kKC ? a.d[(uint64_t)k * a.n_packed + j0 + t] : 0.f;
+}
+
+// The chunk's G tile in FP32 (H16 words widened).
+template <int MP, bool H16>
+__device__ __forceinline__ void chunk_g(const StoredChunk<MP, H16>& c, float2 (&g)[MP / 8]) {
+#pragma unroll
+    for (int r = 0; r < MP / 8; ++r) {
+        if constexpr (H16)
+            g[r] = make_float2((float)__builtin_bit_cast(_Float16, (uint16_t)(c.g[r] & 0xffffu)),
+                               (float)__builtin_bit_cast(_Float16, (uint16_t)(c.g[r] >> 16)));
+        else
+            g[r] = c.g[r];
+    }
 }
 
 // ---------------------------------------------------------------- kernel A (FP32)
@@ -578,9 +572,12 @@ enum { kQGamma = 0, kQFinal = 1 };
 
 template <int MP, bool H16>
 __global__ __launch_bounds__(kHB) void gspat_q(HoloArgs a, int closing, int pass) {
-    __shared__ float s_inv[kKC];
-    __shared__ float2 s_qp[8][kKC];
-    __shared__ float2 s_q[kKC];
+    // double-buffered by chunk parity: a buffer is written again two chunks
+    // later, after a barrier that every reader of its previous contents has
+    // reached
+    __shared__ float2 s_qp[2][8][kKC];
+    __shared__ float2 s_q[2][kKC];
+    __shared__ float2 s_qall[8][kKC];  // kQFinal: q of up to 8 chunks, written out together
     __shared__ float2 s_p[MP];
     const int t = threadIdx.x;
     const int grp = t >> 5, jj = t & 31;
@@ -605,9 +602,28 @@ __global__ __launch_bounds__(kHB) void gspat_q(HoloArgs a, int closing, int pass
     }
     lds_sync();
 
-    // q_j partials of the chunk: w_j sum_i conj(G_ij) p_i, reduced over the 8
-    // row groups in a fixed order into s_q
-    auto chunk_q = [&](const float2 (&g)[MP / 8]) {
+    float2 gam[MP / 8];
+#pragma unroll
+    for (int r = 0; r < MP / 8; ++r) gam[r] = make_float2(0.f, 0.f);
+    // chunks loaded ahead of their use: H16 holds 9 words a chunk and reads
+    // 4 bytes per element, so it keeps three chunks in flight to cover the
+    // memory latency; FP32 G (17 words, 8 bytes) keeps one
+    constexpr int PD = H16 ? 3 : 1;
+    StoredChunk<MP, H16> cq[PD];
+#pragma unroll
+    for (int u = 0; u < PD; ++u)
+        if (c_begin + u < c_end) load_chunk<MP, H16>(a, k, (c_begin + u) * kKC, M, cq[u]);
+    for (uint32_t chunk = c_begin; chunk < c_end; ++chunk) {
+        const uint32_t j0 = chunk * kKC;
+        const int pb = (chunk - c_begin) & 1, slot = (chunk - c_begin) % 8;
+        float2 g[MP / 8];
+        chunk_g<MP, H16>(cq[0], g);
+        const float dj = cq[0].d;
+#pragma unroll
+        for (int u = 0; u + 1 < PD; ++u) cq[u] = cq[u + 1];
+        if (chunk + PD < c_end) load_chunk<MP, H16>(a, k, (chunk + PD) * kKC, M, cq[PD - 1]);
+        // q_j = w_j sum_i conj(G_ij) p_i: this row group's partial, then the 8
+        // groups in a fixed order; w_j = 1 / d_j^2 (FP32 G) or 1 / d_j (H16)
         float2 pq = make_float2(0.f, 0.f);
 #pragma unroll
         for (int r = 0; r < MP / 8; ++r) {
@@ -615,44 +631,45 @@ __global__ __launch_bounds__(kHB) void gspat_q(HoloArgs a, int closing, int pass
             pq.x = fmaf(g[r].x, pv.x, fmaf(g[r].y, pv.y, pq.x));
             pq.y = fmaf(g[r].x, pv.y, fmaf(-g[r].y, pv.x, pq.y));
         }
-        s_qp[grp][jj] = pq;
+        s_qp[pb][grp][jj] = pq;
         lds_sync();
         if (t < kKC) {
             float2 q = make_float2(0.f, 0.f);
 #pragma unroll
             for (int q8 = 0; q8 < 8; ++q8) {
-                q.x += s_qp[q8][t].x;
-                q.y += s_qp[q8][t].y;
+                q.x += s_qp[pb][q8][t].x;
+                q.y += s_qp[pb][q8][t].y;
             }
-            s_q[t] = make_float2(q.x * s_inv[t], q.y * s_inv[t]);
+            const float w = (j0 + t < a.n) ? (H16 ? 1.0f / dj : 1.0f / (dj * dj)) : 0.f;
+            q = make_float2(q.x * w, q.y * w);
+            if (pass == kQGamma)
+                s_q[pb][t] = q;
+            else
+                s_qall[slot][t] = q;
         }
-        lds_sync();
-    };
-
-    float2 gam[MP / 8];
-#pragma unroll
-    for (int r = 0; r < MP / 8; ++r) gam[r] = make_float2(0.f, 0.f);
-    StoredChunk<MP> cn;
-    load_chunk<MP, H16>(a, k, c_begin * kKC, M, cn);
-    for (uint32_t chunk = c_begin; chunk < c_end; ++chunk) {
-        const uint32_t j0 = chunk * kKC;
-        const StoredChunk<MP> c = cn;
-        if (chunk + 1 < c_end) load_chunk<MP, H16>(a, k, j0 + kKC, M, cn);
-        column_scale_stored<H16>(c.d, j0, a.n, s_inv);
-        chunk_q(c.g);
+        if (pass == kQFinal && (slot == 7 || chunk + 1 == c_end)) {
+            // amplitude and phase of the group's columns, one per thread (slot
+            // sl holds chunk - (slot - sl)); a slot is written again in the
+            // next group, after the next chunk's barrier, which every thread
+            // reaches after its read here
+            lds_sync();
+            const int sl = t >> 5;
+            const uint32_t j = j0 - (uint32_t)(slot - sl) * kKC + jj;
+            if (sl <= slot && j < a.n) {
+                const float2 q = s_qall[sl][jj];
+                a.out_amp[(uint64_t)b * a.n + j] = hypotf(q.x, q.y);
+                a.out_phase[(uint64_t)b * a.n + j] = atan2f(q.y, q.x);
+            }
+        }
         if (pass == kQGamma) {
-            const float2 qj = s_q[jj];
+            lds_sync();
+            const float2 qj = s_q[pb][jj];
 #pragma unroll
             for (int r = 0; r < MP / 8; ++r) {
-                gam[r].x = fmaf(c.g[r].x, qj.x, fmaf(-c.g[r].y, qj.y, gam[r].x));
-                gam[r].y = fmaf(c.g[r].x, qj.y, fmaf(c.g[r].y, qj.x, gam[r].y));
+                gam[r].x = fmaf(g[r].x, qj.x, fmaf(-g[r].y, qj.y, gam[r].x));
+                gam[r].y = fmaf(g[r].x, qj.y, fmaf(g[r].y, qj.x, gam[r].y));
             }
-        } else if (t < kKC && j0 + t < a.n) {
-            const float2 q = s_q[t];
-            a.out_amp[(uint64_t)b * a.n + j0 + t] = hypotf(q.x, q.y);
-            a.out_phase[(uint64_t)b * a.n + j0 + t] = atan2f(q.y, q.x);
         }
-        lds_sync();
     }
     if (pass == kQGamma) {
         // reduce over the 32 columns held by the half-wave of this row group
