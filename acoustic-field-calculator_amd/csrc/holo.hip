@@ -142,29 +142,40 @@ __device__ __forceinline__ void load_instance(const HoloArgs& a, uint32_t b, Foc
 }
 
 // G for the thread's column (source s, staged) at rows (t >> 5) + 8 r,
-// r < MP / 8 (zero for padding rows and padding columns), two rows at a time
-// through the staged term; pairs of rows that are both padding (>= M) are set
-// to zero without evaluating the terms.
-template <int SRC, int MP>
+// r < MP / 8 (zero for padding rows and padding columns), RT rows at a time
+// through the staged term; steps whose rows are all padding (>= M) are set to
+// zero without evaluating the terms. Kernel A takes RT = 2: with 4 the MP = 64
+// kernels reach their 168-register bound and ran slower.
+template <int SRC, int MP, int RT>
 __device__ __forceinline__ void g_tile(const SrcF& s, const Foci<MP>& f, const CellsLds* cells, uint32_t M,
                                        float2 (&g)[MP / 8]) {
+    static_assert(RT <= MP / 8 && (MP / 8) % RT == 0, "rows per step");
     const int t = threadIdx.x;
     const int grp = t >> 5;
 #pragma unroll
-    for (int r = 0; r < MP / 8; r += 2) {
-        const int i0 = grp + 8 * r, i1 = i0 + 8;
+    for (int r = 0; r < MP / 8; r += RT) {
+        const int i0 = grp + 8 * r;
         if (i0 >= (int)M) {
-            // both rows are padding (i1 > i0): zero without computing the terms
-            g[r] = make_float2(0.f, 0.f);
-            g[r + 1] = make_float2(0.f, 0.f);
+            // every row of the step is padding (the rest are above i0): zero
+            // without computing the terms
+#pragma unroll
+            for (int u = 0; u < RT; ++u) g[r + u] = make_float2(0.f, 0.f);
             continue;
         }
-        const float px[2] = {f.x[i0], f.x[i1]}, py[2] = {f.y[i0], f.y[i1]}, pz[2] = {f.z[i0], f.z[i1]};
-        const float w[2] = {f.v[i0], f.v[i1]};
-        float re[2] = {0.f, 0.f}, im[2] = {0.f, 0.f};
-        add_terms_staged<SRC, 2>(s, px, py, pz, cells->p, cells->q, re, im, w);
-        g[r] = make_float2(re[0], im[0]);
-        g[r + 1] = make_float2(re[1], im[1]);
+        float px[RT], py[RT], pz[RT], w[RT], re[RT], im[RT];
+#pragma unroll
+        for (int u = 0; u < RT; ++u) {
+            const int i = i0 + 8 * u;
+            px[u] = f.x[i];
+            py[u] = f.y[i];
+            pz[u] = f.z[i];
+            w[u] = f.v[i];
+            re[u] = 0.f;
+            im[u] = 0.f;
+        }
+        add_terms_staged<SRC, RT>(s, px, py, pz, cells->p, cells->q, re, im, w);
+#pragma unroll
+        for (int u = 0; u < RT; ++u) g[r + u] = make_float2(re[u], im[u]);
     }
 }
 
@@ -329,7 +340,7 @@ __global__ __launch_bounds__(kHB) void gspat_r_fp32(HoloArgs a) {
         }
         const SrcF sc = src_from_block(s_src, in_blk);
         float2 g[MP / 8];
-        g_tile<SRC, MP>(sc, f, s_cells, M, g);
+        g_tile<SRC, MP, 2>(sc, f, s_cells, M, g);
         store_g_tile<MP>(a, k, j0, M, g);
         const float inv = column_inv<MP>(g, j0, a.n, s_part, d_out);
 #pragma unroll
@@ -422,7 +433,7 @@ __global__ __launch_bounds__(kHB) __attribute__((amdgpu_waves_per_eu(MP == 64 ? 
         }
         const SrcF sc = src_from_block(s_src, in_blk);
         float2 g[MP / 8];
-        g_tile<SRC, MP>(sc, f, s_cells, M, g);
+        g_tile<SRC, MP, 2>(sc, f, s_cells, M, g);
         if constexpr (!H16) store_g_tile<MP>(a, k, j0, M, g);
         const float inv = column_inv<MP>(g, j0, a.n, s_part, d_out) * scale;
 #pragma unroll
