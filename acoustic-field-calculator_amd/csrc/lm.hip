@@ -383,12 +383,25 @@ __device__ __attribute__((noinline)) void jacobi_pinv_block(const double* sK, co
                 if (apq != 0.0) {
                     // t = sgn(theta) / (|theta| + sqrt(theta^2 + 1)) with theta = delta / tau,
                     // delta = a_qq - a_pp, tau = 2 a_pq, as sgn(theta) |tau| / (|delta| +
-                    // hypot(delta, tau)); sgn(+-0) = +1
+                    // hypot(delta, tau)); sgn(+-0) = +1. The angle is taken in FP32 from
+                    // delta and tau scaled by a power of two (exact), which leaves a_pq at
+                    // ~1e-7 of its value instead of 0 and does not change the sweep count;
+                    // c and s are then renormalised in FP64 (c^2 + s^2 - 1 = e, times
+                    // 1 - e/2 + 3e^2/8), so every rotation, and V, stays orthogonal to FP64
+                    // rounding
                     const double delta = A[q * WS + q] - A[p * WS + p], tau = 2.0 * apq;
-                    const double sg = (delta == 0.0 || (delta > 0.0) == (tau > 0.0)) ? 1.0 : -1.0;
-                    const double tt = sg * fabs(tau) / (fabs(delta) + sqrt(fma(delta, delta, tau * tau)));
-                    c = rsqrt(fma(tt, tt, 1.0));
-                    sn = tt * c;
+                    const int ex = __builtin_amdgcn_frexp_exp(fmax(fabs(delta), fabs(tau)));
+                    const float d32 = (float)__builtin_amdgcn_ldexp(delta, -ex);
+                    const float t32 = (float)__builtin_amdgcn_ldexp(tau, -ex);
+                    const float sg = (d32 == 0.f || (d32 > 0.f) == (t32 > 0.f)) ? 1.f : -1.f;
+                    const float tt = sg * fabsf(t32) *
+                                     __builtin_amdgcn_rcpf(fabsf(d32) + __builtin_amdgcn_sqrtf(fmaf(d32, d32, t32 * t32)));
+                    const float c32 = __builtin_amdgcn_rsqf(fmaf(tt, tt, 1.f));
+                    const double c0 = c32, s0 = (double)(tt * c32);
+                    const double e2 = fma(c0, c0, fma(s0, s0, -1.0));
+                    const double f = fma(e2, fma(0.375, e2, -0.5), 1.0);
+                    c = c0 * f;
+                    sn = s0 * f;
                 }
                 cs[T] = make_double2(c, sn);
                 pq[T] = make_int2(p, q);
