@@ -506,6 +506,24 @@ __global__ __launch_bounds__(kHB) __attribute__((amdgpu_waves_per_eu(MP == 64 ? 
 }
 
 // ---------------------------------------------------------------- kernel B
+// v summed over the TPR consecutive lanes of a row group (TPR = 1, 2, 4, 8 or
+// 16) with DPP lane moves instead of LDS-crossbar shuffles: lane i ^ 8 (row
+// rotate by 8), 7 - i within 8 lanes (row half mirror), i ^ 2 and i ^ 1 (quad
+// permutes). Each step pairs lanes symmetrically, so every lane of the group
+// ends with the same bits.
+#define AFH_DPP_ADD(v, ctrl) \
+    ((v) + __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, (v)), (ctrl), 0xf, 0xf, false)))
+template <int TPR>
+__device__ __forceinline__ float row_group_sum(float v) {
+    static_assert(TPR == 1 || TPR == 2 || TPR == 4 || TPR == 8 || TPR == 16, "row group within a DPP row");
+    if constexpr (TPR >= 16) v = AFH_DPP_ADD(v, 0x128);  // row_ror:8
+    if constexpr (TPR >= 8) v = AFH_DPP_ADD(v, 0x141);   // row_half_mirror
+    if constexpr (TPR >= 4) v = AFH_DPP_ADD(v, 0x4e);    // quad_perm [2, 3, 0, 1]
+    if constexpr (TPR >= 2) v = AFH_DPP_ADD(v, 0xb1);    // quad_perm [1, 0, 3, 2]
+    return v;
+}
+#undef AFH_DPP_ADD
+
 // gamma = R p0; iters x {p = gamma / |gamma| p0; gamma = R p}. Writes the
 // last p (closing) or p = gamma / |gamma|^2 p0^2 to a.p.
 template <int MP>
@@ -536,27 +554,23 @@ __global__ __launch_bounds__(kHB) void gspat_iter(HoloArgs a, uint32_t iters, in
     __syncthreads();
     int cur = 0;
     auto matvec = [&](int buf) {
-        float2 acc = make_float2(0.f, 0.f);
+        // two accumulators (even and odd columns) halve the FMA chain
+        float2 a0 = make_float2(0.f, 0.f), a1 = make_float2(0.f, 0.f);
 #pragma unroll
         for (int c = 0; c < CPT; ++c) {
             const float2 pv = s_p[buf][h * CPT + c];
+            float2& acc = (c & 1) ? a1 : a0;
             acc.x = fmaf(r[c].x, pv.x, fmaf(-r[c].y, pv.y, acc.x));
             acc.y = fmaf(r[c].x, pv.y, fmaf(r[c].y, pv.x, acc.y));
         }
-#pragma unroll
-        for (int off = TPR / 2; off >= 1; off >>= 1) {
-            acc.x += __shfl_xor(acc.x, off);
-            acc.y += __shfl_xor(acc.y, off);
-        }
-        return acc;
+        return make_float2(row_group_sum<TPR>(a0.x + a1.x), row_group_sum<TPR>(a0.y + a1.y));
     };
     float2 gam = matvec(cur);
     for (uint32_t it = 0; it < iters; ++it) {
-        if (h == 0) {
-            const float inv = p0 / sqrtf(gam.x * gam.x + gam.y * gam.y);
-            s_p[cur ^ 1][row] = valid ? make_float2(gam.x * inv, gam.y * inv) : make_float2(0.f, 0.f);
-        }
-        __syncthreads();
+        // every lane of the row group holds the same gamma; one writes p
+        const float inv = p0 * __builtin_amdgcn_rsqf(fmaf(gam.x, gam.x, gam.y * gam.y));
+        if (h == 0) s_p[cur ^ 1][row] = valid ? make_float2(gam.x * inv, gam.y * inv) : make_float2(0.f, 0.f);
+        lds_sync();
         cur ^= 1;
         gam = matvec(cur);
     }
