@@ -56,6 +56,58 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// v of lane (lane ^ O). O = 1, 2 and 8 are single DPP moves within a 16-lane
+// row (quad permutes, row rotate by 8); O = 4 takes lane + 4 (row shift left)
+// or lane - 4 (row shift right) by the lane's bit 2; larger O use the LDS
+// crossbar (__shfl_xor). Every lane of the wave must be active.
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, 0xf, 0xf, false);
+}
+
+template <int O>
+__device__ __forceinline__ double xor_lane_f64(double v) {
+    if constexpr (O >= 16) {
+        return __shfl_xor(v, O);
+    } else {
+        const uint64_t u = __builtin_bit_cast(uint64_t, v);
+        uint32_t lo = (uint32_t)u, hi = (uint32_t)(u >> 32);
+        if constexpr (O == 1) {
+            lo = dpp_u32<0xb1>(lo);  // quad_perm [1, 0, 3, 2]
+            hi = dpp_u32<0xb1>(hi);
+        } else if constexpr (O == 2) {
+            lo = dpp_u32<0x4e>(lo);  // quad_perm [2, 3, 0, 1]
+            hi = dpp_u32<0x4e>(hi);
+        } else if constexpr (O == 8) {
+            lo = dpp_u32<0x128>(lo);  // row_ror:8
+            hi = dpp_u32<0x128>(hi);
+        } else {
+            static_assert(O == 4, "xor offset");
+            const bool up = (threadIdx.x & 4) != 0;
+            const uint32_t l_dn = dpp_u32<0x104>(lo), h_dn = dpp_u32<0x104>(hi);  // row_shl:4, lane + 4
+            const uint32_t l_up = dpp_u32<0x114>(lo), h_up = dpp_u32<0x114>(hi);  // row_shr:4, lane - 4
+            lo = up ? l_up : l_dn;
+            hi = up ? h_up : h_dn;
+        }
+        return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+    }
+}
+
+// One step of a transposed butterfly over N values per lane: the lane with
+// bit O set keeps the upper half of its list and sends the lower half to
+// lane ^ O, which keeps the lower half; each lane adds what it receives to
+// what it keeps (N / 2 values remain).
+template <int O, int N>
+__device__ __forceinline__ void butterfly_step(double* v, int lane) {
+    const bool up = (lane & O) != 0;
+#pragma unroll
+    for (int j = 0; j < N / 2; ++j) {
+        const double send = up ? v[j] : v[j + N / 2];
+        const double keep = up ? v[j + N / 2] : v[j];
+        v[j] = keep + xor_lane_f64<O>(send);
+    }
+}
+
 // Block reductions in a fixed order: lanes by xor shuffles, then the 4 waves.
 __device__ inline double block_sum(double v, double* red) {
 #pragma unroll

@@ -777,10 +777,11 @@ __device__ inline void rows_times(const double2* G, const double2* q, uint32_t N
 // foci (zero rows beyond M), its phases x_l and E_l = e^{i x_l}; thread i < M
 // also owns the amplitude parameter x_{N+i} (column N + i of B = [G |
 // -diag(amps)]). A step: the thread's partial field sums over its columns, 2MP
-// values, reduced over the wave by a transposed butterfly (at offset o = 32,
-// 16, .. the lane with bit o set keeps the upper half of its list and sends
-// the lower half, so 2MP values cost 2MP - 1 shuffles and lane L ends with
-// value L (MP = 32) or L >> 1 (MP = 16)), then over the 4 waves in a fixed
+// values, reduced over the wave by a transposed butterfly (at offset o = 1, 2,
+// .. 32 the lane with bit o set keeps the upper half of its list and sends the
+// lower half, so 2MP values cost 2MP - 1 lane moves, most of them DPP, and
+// lane L ends with the value whose index is L's bits reversed), then over the
+// 4 waves in a fixed
 // order; g_l = Im(conj(E_l) sum_i conj(G_il) s_i); the block maximum of g
 // (the reference's stop test); x -= 0.1 g. No global memory is touched inside
 // the loop.
@@ -832,21 +833,24 @@ __global__ __launch_bounds__(kLB) void gd_solve(LmArgs a) {
             v[i] = re;
             v[MP + i] = im;
         }
-        // transposed butterfly over the wave
-#pragma unroll
-        for (int o = 32, n = K; n > 1; o >>= 1, n >>= 1) {
-            const bool up = (lane & o) != 0;
-#pragma unroll
-            for (int j = 0; j < n / 2; ++j) {
-                const double send = up ? v[j] : v[j + n / 2];
-                const double keep = up ? v[j + n / 2] : v[j];
-                v[j] = keep + __shfl_xor(send, o);
-            }
-        }
+        // transposed butterfly over the wave, low lane bits first: the steps
+        // with many values move them by DPP, only the last one or two values
+        // cross 16-lane rows through the LDS crossbar
+        butterfly_step<1, K>(v, lane);
+        butterfly_step<2, K / 2>(v, lane);
+        butterfly_step<4, K / 4>(v, lane);
+        butterfly_step<8, K / 8>(v, lane);
+        butterfly_step<16, K / 16>(v, lane);
+        if constexpr (K == 64) butterfly_step<32, 2>(v, lane);
         double f = v[0];
-        if (K == 32) f += __shfl_xor(f, 1);
-        const int idx = K == 64 ? lane : lane >> 1;
-        if (K == 64 || (lane & 1) == 0) s_part[w][idx] = f;
+        if constexpr (K == 32) f += xor_lane_f64<32>(f);
+        // lane bit b chose the half of weight K / 2^(b + 1): the value index is
+        // the lane's bits reversed (6 bits for K = 64, bits 0..4 for K = 32)
+        const int idx = K == 64 ? (((lane & 1) << 5) | ((lane & 2) << 3) | ((lane & 4) << 1) | ((lane & 8) >> 1) |
+                                   ((lane & 16) >> 3) | ((lane & 32) >> 5))
+                                : (((lane & 1) << 4) | ((lane & 2) << 2) | (lane & 4) | ((lane & 8) >> 2) |
+                                   ((lane & 16) >> 4));
+        if (K == 64 || (lane & 32) == 0) s_part[w][idx] = f;
         lds_barrier();
         if (t < K) {
             double sum = ((s_part[0][t] + s_part[1][t]) + s_part[2][t]) + s_part[3][t];
@@ -878,8 +882,12 @@ __global__ __launch_bounds__(kLB) void gd_solve(LmArgs a) {
             ga = -amp * (ea.x * s_f[MP + t] - ea.y * s_f[t]);
             gm = fmax(gm, ga);
         }
-#pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) gm = fmax(gm, __shfl_xor(gm, o));
+        gm = fmax(gm, xor_lane_f64<1>(gm));
+        gm = fmax(gm, xor_lane_f64<2>(gm));
+        gm = fmax(gm, xor_lane_f64<4>(gm));
+        gm = fmax(gm, xor_lane_f64<8>(gm));
+        gm = fmax(gm, xor_lane_f64<16>(gm));
+        gm = fmax(gm, xor_lane_f64<32>(gm));
         if (lane == 0) s_red[w] = gm;
         lds_barrier();
         const double gmax = fmax(fmax(s_red[0], s_red[1]), fmax(s_red[2], s_red[3]));
