@@ -108,10 +108,31 @@ __device__ __forceinline__ void butterfly_step(double* v, int lane) {
     }
 }
 
-// Block reductions in a fixed order: lanes by xor shuffles, then the 4 waves.
+// Sum (maximum) over the 64 lanes by xor steps at offsets 32, 16, .. 1 (the
+// last four by DPP); every lane gets the same bits. All lanes must be active.
+__device__ __forceinline__ double wave_sum_f64(double v) {
+    v += xor_lane_f64<32>(v);
+    v += xor_lane_f64<16>(v);
+    v += xor_lane_f64<8>(v);
+    v += xor_lane_f64<4>(v);
+    v += xor_lane_f64<2>(v);
+    v += xor_lane_f64<1>(v);
+    return v;
+}
+
+__device__ __forceinline__ double wave_max_f64(double v) {
+    v = fmax(v, xor_lane_f64<32>(v));
+    v = fmax(v, xor_lane_f64<16>(v));
+    v = fmax(v, xor_lane_f64<8>(v));
+    v = fmax(v, xor_lane_f64<4>(v));
+    v = fmax(v, xor_lane_f64<2>(v));
+    v = fmax(v, xor_lane_f64<1>(v));
+    return v;
+}
+
+// Block reductions in a fixed order: lanes by xor steps, then the 4 waves.
 __device__ inline double block_sum(double v, double* red) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+    v = wave_sum_f64(v);
     const int w = threadIdx.x >> 6;
     __syncthreads();
     if ((threadIdx.x & 63) == 0) red[w] = v;
@@ -120,8 +141,7 @@ __device__ inline double block_sum(double v, double* red) {
 }
 
 __device__ inline double block_max(double v, double* red) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+    v = wave_max_f64(v);
     const int w = threadIdx.x >> 6;
     __syncthreads();
     if ((threadIdx.x & 63) == 0) red[w] = v;

@@ -129,11 +129,8 @@ __device__ inline void field_at_foci(const LmScratch& s, const double2* E, uint3
             re += gv.x * e.x - gv.y * e.y;
             im += gv.x * e.y + gv.y * e.x;
         }
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) {
-            re += __shfl_xor(re, off);
-            im += __shfl_xor(im, off);
-        }
+        re = wave_sum_f64(re);
+        im = wave_sum_f64(im);
         if (l == 0) {
             // the -diag(amps) block of B: column N + i
             const double2 e = E[N + i];
@@ -763,11 +760,8 @@ __device__ inline void rows_times(const double2* G, const double2* q, uint32_t N
             re += g.x * v.x - g.y * v.y;
             im += g.x * v.y + g.y * v.x;
         }
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) {
-            re += __shfl_xor(re, off);
-            im += __shfl_xor(im, off);
-        }
+        re = wave_sum_f64(re);
+        im = wave_sum_f64(im);
         if (l == 0) out[i] = make_double2(re, im);
     }
 }

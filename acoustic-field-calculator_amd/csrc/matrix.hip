@@ -119,11 +119,8 @@ __device__ __attribute__((noinline)) void herm_eig_wave(double2* A, double2* V, 
                 else
                     off += m2;
             }
-#pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) {
-            off += __shfl_xor(off, o);
-            dg += __shfl_xor(dg, o);
-        }
+        off = wave_sum_f64(off);
+        dg = wave_sum_f64(dg);
         if (off <= dg * 1e-30 || off == 0.0) break;
         for (uint32_t p = 0; p + 1 < M; ++p)
             for (uint32_t q = p + 1; q < M; ++q) {
@@ -294,11 +291,8 @@ __global__ __launch_bounds__(kLB) void long_solve(MatArgs a) {
                 re += g.x;
                 im += g.y;
             }
-#pragma unroll
-            for (int o = 32; o >= 1; o >>= 1) {
-                re += __shfl_xor(re, o);
-                im += __shfl_xor(im, o);
-            }
+            re = wave_sum_f64(re);
+            im = wave_sum_f64(im);
             if (l == 0) s_den[i] = make_double2(re, im);
         }
     }
@@ -412,8 +406,7 @@ __global__ __launch_bounds__(kLB) void horn_solve(MatArgs a) {
                 const double2 mr = s_MM[r * kMS + ii];
                 gre = xb.x * mr.x + xb.y * mr.y;  // Re(conj(xb_r) MM[r][ii])
             }
-#pragma unroll
-            for (int o = 32; o >= 1; o >>= 1) gre += __shfl_xor(gre, o);
+            gre = wave_sum_f64(gre);
             if (act) {
                 double2 v = make_double2(0.0, 0.0);
                 if (gre > 0.0) {
