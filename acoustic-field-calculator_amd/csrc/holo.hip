@@ -743,16 +743,21 @@ __device__ __forceinline__ float2 wave_sum(float2 v) {
     return v;
 }
 
-template <int MP>
+// QLDS: q (n_packed <= kGsLdsQ) lives in LDS, so that the G q pass reads only G
+// from memory; otherwise in the workspace.
+constexpr uint32_t kGsLdsQ = 4096;
+
+template <int MP, bool QLDS>
 __global__ __launch_bounds__(kHB) void gs_iter(HoloArgs a, uint32_t iters, uint32_t g_stride_inst) {
     __shared__ float2 s_gam[MP];
     __shared__ float2 s_p[MP];
+    __shared__ float2 s_qbuf[QLDS ? kGsLdsQ : 1];
     const int t = threadIdx.x, w = t >> 6, l = t & 63;
     const uint32_t k = blockIdx.x, b = a.inst[k];
     const uint32_t base = a.offsets[b], M = a.offsets[b + 1] - base;
     const uint32_t n = a.n, np = a.n_packed;
     const float2* G = a.G + (uint64_t)k * g_stride_inst;
-    float2* q = a.q + (uint64_t)k * np;
+    float2* q = QLDS ? s_qbuf : a.q + (uint64_t)k * np;
     // q0 = current source amplitudes (gs.rs:48-54)
     for (uint32_t j = t; j < np; j += kHB) q[j] = make_float2(j < n ? a.sources[AFH_SOURCE_STRIDE * (uint64_t)j + 6] : 0.f, 0.f);
     __syncthreads();
@@ -760,6 +765,8 @@ __global__ __launch_bounds__(kHB) void gs_iter(HoloArgs a, uint32_t iters, uint3
         // gamma = G q, a wave per row, lanes over columns
         for (uint32_t i = w; i < M; i += 4) {
             float2 acc = make_float2(0.f, 0.f);
+            // unrolled so that several loads of G and q are in flight per lane
+#pragma unroll 8
             for (uint32_t j = l; j < np; j += 64) {
                 const float2 gv = G[(uint64_t)i * np + j], qv = q[j];
                 acc.x = fmaf(gv.x, qv.x, fmaf(-gv.y, qv.y, acc.x));
@@ -778,6 +785,7 @@ __global__ __launch_bounds__(kHB) void gs_iter(HoloArgs a, uint32_t iters, uint3
         // xi = G^H p, q = xi / |xi| q0
         for (uint32_t j = t; j < n; j += kHB) {
             float2 xi = make_float2(0.f, 0.f);
+#pragma unroll 8
             for (uint32_t i = 0; i < M; ++i) {
                 const float2 gv = G[(uint64_t)i * np + j], pv = s_p[i];
                 xi.x = fmaf(gv.x, pv.x, fmaf(gv.y, pv.y, xi.x));
@@ -989,7 +997,10 @@ static hipError_t launch_gs_bucket(HoloArgs a, int src, uint32_t nb, uint32_t it
         hipLaunchKernelGGL((gen_g<kT4010A1>), ggrid, dim3(kHB), 0, st, a, g_stride);
     else
         hipLaunchKernelGGL((gen_g<kSphere>), ggrid, dim3(kHB), 0, st, a, g_stride);
-    hipLaunchKernelGGL((gs_iter<MP>), dim3(nb), dim3(kHB), 0, st, a, iters, g_stride);
+    if (a.n_packed <= kGsLdsQ)
+        hipLaunchKernelGGL((gs_iter<MP, true>), dim3(nb), dim3(kHB), 0, st, a, iters, g_stride);
+    else
+        hipLaunchKernelGGL((gs_iter<MP, false>), dim3(nb), dim3(kHB), 0, st, a, iters, g_stride);
     return hipGetLastError();
 }
 

@@ -596,6 +596,37 @@ def bench_gspat(torch, afc, D, O, src, k, a, batch, cpu=True):
                      "focal_amp_rel_err_max": max(errs), "sample_instances": int(len(sample))}
         log(f"gspat {name}: {ms:.3f} ms per batch of {batch}, {batch / (ms * 1e-3):.0f} solves/s, "
             f"max focal-amplitude rel err vs FP64 {max(errs):.2e}")
+    # GS (gs.rs:36-77, 100 iterations) and Naive (naive.rs:32-51) on the same batch, FP32, against the
+    # oracle's FP64 runs of the sampled instances
+    for name, algo, ref_fn in (("gs", D.ALGO_GS, lambda f, am: O.gs(1, src, k, a, f, am, "f64")),
+                               ("naive", D.ALGO_NAIVE, lambda f, am: O.naive(1, src, k, a, f, am, "f64"))):
+        def run():
+            D.holo_solve(ctx, src_d.data_ptr(), packed.shape[0], n, 1, foci_d.data_ptr(), amps_d.data_ptr(), offs,
+                         algo, D.HOLO_FP32, 100, out_a.data_ptr(), out_p.data_ptr(), stream.cuda_stream)
+        run()
+        torch.cuda.synchronize()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(3)]
+        for e0, e1 in ev:
+            e0.record(stream)
+            run()
+            e1.record(stream)
+        torch.cuda.synchronize()
+        ms = float(np.median([e0.elapsed_time(e1) for e0, e1 in ev]))
+        am, ph = out_a.cpu().numpy(), out_p.cpu().numpy()
+        errs = []
+        for b in sample:
+            f, amp_t = foci[offs[b]:offs[b + 1]], amps[offs[b]:offs[b + 1]]
+            a64, p64 = ref_fn(f, amp_t)
+            s_ref, s_got = src.copy(), src.copy()
+            s_ref[:, 6], s_ref[:, 7] = a64, p64
+            s_got[:, 6], s_got[:, 7] = am[b], ph[b]
+            want = np.abs(O.field(1, s_ref, k, a, f, "f64"))
+            got = np.abs(O.field(1, s_got, k, a, f, "f64"))
+            errs.append(float(np.max(np.abs(got - want) / want)))
+        out[name] = {"solves_per_s": round(batch / (ms * 1e-3), 1), "batch_ms": round(ms, 4),
+                     "focal_amp_rel_err_max_vs_fp64_run": max(errs), "sample_instances": int(len(sample))}
+        log(f"{name}: {ms:.3f} ms per batch of {batch}, {batch / (ms * 1e-3):.0f} solves/s, "
+            f"max focal-amplitude rel err vs the FP64 run {max(errs):.2e}")
     ctx.close()
     # the plain FP32 path: AFO_GSPAT, one instance per call through the C ABI
     system = afc.UniformSystem(O.TEMP_340)

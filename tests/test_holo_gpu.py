@@ -164,6 +164,28 @@ def test_gs_and_naive_batched(afc, oracle, torch_cuda):
             assert np.max(np.abs(got - want) / want) <= 1e-3, (algo, b)
 
 
+def test_gs_large_array(afc, oracle, torch_cuda):
+    """GS on 4980 sources (5x4 tiles): more than 4096 padded sources, so the
+    iterate q stays in the workspace instead of LDS."""
+    log("case: batched GS on the 4980-source array")
+    src, k, a = setup_array(oracle, (5, 4))
+    rng = np.random.default_rng(5)
+    ms = np.array([3, 12], np.int64)
+    offs = np.zeros(3, np.uint32)
+    offs[1:] = np.cumsum(ms)
+    foci = random_foci(rng, int(offs[-1]))
+    amps = rng.uniform(0.5, 1.0, int(offs[-1])).astype(np.float32)
+    amp, ph = run_batch(afc, torch_cuda, oracle, src, k, a, foci, amps, offs, 1, 0)
+    for b in range(2):
+        f, am = foci[offs[b]:offs[b + 1]], amps[offs[b]:offs[b + 1]]
+        a64, p64 = oracle.gs(1, src, k, a, f, am, "f64")
+        got = focal_amps(oracle, src, k, a, f, amp[b], ph[b])
+        want = focal_amps(oracle, src, k, a, f, a64, p64)
+        err = float(np.max(np.abs(got - want) / want))
+        log(f"  M={ms[b]}: max rel focal-amplitude error {err:.2e}")
+        assert err <= 1e-3
+
+
 def greedy_objective(oracle, src, k, a, foci, amps, phase):
     p = focal_amps(oracle, src, k, a, foci, src[:, 6], phase)
     return float(np.sum(np.abs(amps - p))), p
