@@ -286,6 +286,40 @@ def test_split_and_point_parallel_kernels_agree(afc, oracle, torch_cuda):
     assert rel_l2(small, want) <= 5e-5 and rel_l2(big, want) <= 5e-5
 
 
+@pytest.mark.parametrize("order,ranges,res", [
+    ((0, 1, 2), ((40.0, 189.0), (20.0, 520.5), (150.0, 150.5)), 0.5),      # second axis y, 1002 points
+    ((1, 2, 0), ((60.0, 61.0), (30.0, 159.0), (100.0, 358.0)), 0.5),      # second axis z, 517
+    ((2, 0, 1), ((0.0, 375.0), (70.0, 70.5), (120.0, 189.0)), 0.25),      # second axis x, 1501
+    ((0, 1), ((10.0, 329.5), (0.0, 250.5), (130.0, 130.0)), 0.5),         # 2-D, 502
+])
+def test_large_grid_runs_vs_oracle(afc, oracle, torch_cuda, order, ranges, res):
+    """Grids of 2e5+ points (the point-parallel kernel) in every axis order and
+    with counts that are not multiples of 4 or 256: every point is written and
+    sampled points match the FP32 CPU path to 1e-5."""
+    import torch
+    from afc import device as D
+    src, k, a, _ = oracle.volume_3984(seed=11)
+    src, k, a = src[:249], k[:249], a[:249]
+    packed = D.pack_sources(1, src, k, a)
+    dev = torch.device("cuda:0")
+    src_d = torch.from_numpy(packed).to(dev)
+    ps, cnt = D.grid_from_area(order, ranges, res)
+    n = cnt[0] * cnt[1] * cnt[2]
+    log(f"case: grid order {order} counts {cnt} ({n} points)")
+    assert n >= 200000
+    out = torch.full((n, 2), float("nan"), dtype=torch.float32, device=dev)
+    D.field_eval(src_d.data_ptr(), packed.shape[0], 1, ps, 2, 0, out.data_ptr())
+    torch.cuda.synchronize()
+    got_all = out.cpu().numpy().view(np.complex64).reshape(-1)
+    assert np.all(np.isfinite(got_all))
+    pts_all, _ = oracle.grid_points(order, ranges, res)
+    sel = np.random.default_rng(1).choice(n, 4000, replace=False)
+    want = oracle.field(1, src, k, a, pts_all[sel], "f32")
+    err = rel_l2(got_all[sel], want)
+    log(f"  rel-L2 vs FP32 CPU path at 4000 sampled points {err:.2e}")
+    assert err <= 1e-5
+
+
 @pytest.mark.parametrize("n", [1, 63, 255, 256, 257, 199999, 200000, 200001])
 def test_ragged_point_counts(afc, oracle, torch_cuda, n):
     """Point counts around the workgroup size (256 points) and the source-split
