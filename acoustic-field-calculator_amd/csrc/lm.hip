@@ -140,54 +140,58 @@ __device__ inline void field_at_foci(const LmScratch& s, const double2* E, uint3
     }
 }
 
-// Gram matrix W W^T of the D x n matrix W (row-major, D <= 16 BS): the columns
-// are staged through LDS kGramChunk at a time and thread t accumulates the
-// BS x BS block at rows (t / 16) BS.., columns (t % 16) BS.., each entry summed
-// over l in increasing order. Writes the lower triangle (p >= q) into sK (row
-// stride kLmMaxD) and the diagonal into sgd.
-constexpr int kGramChunk = 32;
+typedef double doublex4 __attribute__((ext_vector_type(4)));
 
-template <int BS>
-__device__ void gram(const double* __restrict__ W, uint32_t n, uint32_t D, double* tile, double* sK, double* sgd) {
-    constexpr int R = 16 * BS;  // rows held in the tile
-    constexpr int TS = kGramChunk + 1;
-    const int t = threadIdx.x;
-    const int p0 = (t >> 4) * BS, q0 = (t & 15) * BS;
-    double acc[BS][BS];
-#pragma unroll
-    for (int x = 0; x < BS; ++x)
-#pragma unroll
-        for (int y = 0; y < BS; ++y) acc[x][y] = 0.0;
-    for (uint32_t l0 = 0; l0 < n; l0 += kGramChunk) {
-        for (int e = t; e < R * kGramChunk; e += kLB) {
-            const int r = e / kGramChunk, c = e % kGramChunk;
-            tile[r * TS + c] = ((uint32_t)r < D && l0 + c < n) ? W[(uint64_t)r * n + l0 + c] : 0.0;
+// Gram matrix W W^T of the D x n matrix W (row-major, D <= 64), lower
+// triangle (p >= q) into sK (row stride kLmMaxD) and the diagonal into sgd,
+// with FP64 MFMA (v_mfma_f64_16x16x4f64, A 16 x 4, B 4 x 16): the
+// lower-triangle 16 x 16 tiles (I, J), I >= J, of the D x D
+// result are dealt to the 4 waves in turn; a tile sums W_I W_J^T over the
+// n / 4 column blocks, lane l supplying W[16 I + l % 16][4 kb + l / 16] as A
+// and W[16 J + l % 16][4 kb + l / 16] as B and receiving the tile's rows
+// l / 16 + 4 e, e = 0..3, column l % 16 (the FP64 MFMA interleaves its
+// output rows, unlike the FP32 16 x 16 MFMAs' 4 (l / 16) + e). Even and odd column blocks go to two
+// accumulators (summed at the end), so consecutive MFMAs do not wait on each
+// other. Rows >= D and columns >= n enter as zeros.
+__device__ void gram_mfma(const double* __restrict__ W, uint32_t n, uint32_t D, double* sK, double* sgd) {
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int T = (int)(D + 15) / 16, ntiles = T * (T + 1) / 2;
+    const uint32_t kq = l >> 4;
+    for (int tile = w; tile < ntiles; tile += kLB / 64) {
+        int I = 0, J = tile;
+        while (J > I) {
+            J -= I + 1;
+            ++I;
         }
-        __syncthreads();
-#pragma unroll 4
-        for (int c = 0; c < kGramChunk; ++c) {
-            double u[BS], v[BS];
-#pragma unroll
-            for (int x = 0; x < BS; ++x) u[x] = tile[(p0 + x) * TS + c];
-#pragma unroll
-            for (int y = 0; y < BS; ++y) v[y] = tile[(q0 + y) * TS + c];
-#pragma unroll
-            for (int x = 0; x < BS; ++x)
-#pragma unroll
-                for (int y = 0; y < BS; ++y) acc[x][y] = fma(u[x], v[y], acc[x][y]);
+        const uint32_t ra = 16 * I + (l & 15), rb = 16 * J + (l & 15);
+        const bool va = ra < D, vb = rb < D;
+        const double* __restrict__ wa = W + (uint64_t)(va ? ra : 0) * n;
+        const double* __restrict__ wb = W + (uint64_t)(vb ? rb : 0) * n;
+        doublex4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = acc0;
+        uint32_t k0 = 0;
+#pragma unroll 2
+        for (; k0 + 8 <= n; k0 += 8) {
+            const uint32_t k = k0 + kq;
+            const double a0 = va ? wa[k] : 0.0, b0 = vb ? wb[k] : 0.0;
+            const double a1 = va ? wa[k + 4] : 0.0, b1 = vb ? wb[k + 4] : 0.0;
+            acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc1, 0, 0, 0);
         }
-        __syncthreads();
-    }
+        for (; k0 < n; k0 += 4) {
+            const uint32_t k = k0 + kq;
+            const double a0 = (va && k < n) ? wa[k] : 0.0, b0 = (vb && k < n) ? wb[k] : 0.0;
+            acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc0, 0, 0, 0);
+        }
+        const doublex4 acc = acc0 + acc1;
 #pragma unroll
-    for (int x = 0; x < BS; ++x)
-#pragma unroll
-        for (int y = 0; y < BS; ++y) {
-            const uint32_t p = p0 + x, q = q0 + y;
+        for (int e = 0; e < 4; ++e) {
+            const uint32_t p = 16 * I + kq + 4 * e, q = 16 * J + (l & 15);
             if (p < D && q <= p) {
-                sK[p * kLmMaxD + q] = acc[x][y];
-                if (p == q) sgd[p] = acc[x][y];
+                sK[p * kLmMaxD + q] = acc[e];
+                if (p == q) sgd[p] = acc[e];
             }
         }
+    }
 }
 
 // Lane `lane`'s value of v (a wave-uniform result, read with v_readlane).
@@ -473,7 +477,6 @@ __global__ __launch_bounds__(kLB) void lm_solve(LmArgs a) {
     __shared__ double s_y[kLmMaxD];
     __shared__ double2 s_s[kLmMaxM];
     __shared__ double s_red[4];
-    __shared__ double s_tile[kLmMaxD * (kGramChunk + 1)];
     __shared__ double s_col[kLmMaxD];
     __shared__ double s_work[kLmMaxD * (kLmMaxD + 1)];
     __shared__ double s_vec[kLmMaxD * (kLmMaxD + 1)];
@@ -537,10 +540,7 @@ __global__ __launch_bounds__(kLB) void lm_solve(LmArgs a) {
         gmax = block_max(gm, s_red);
         amax = block_max(am, s_red);
         // Gram matrix W W^T (p >= q) into s_K and its diagonal into s_gd
-        if (D <= 32)
-            gram<2>(s.W, n, D, s_tile, s_K, s_gd);
-        else
-            gram<4>(s.W, n, D, s_tile, s_K, s_gd);
+        gram_mfma(s.W, n, D, s_K, s_gd);
         __syncthreads();
     };
 
