@@ -274,11 +274,12 @@ __device__ __forceinline__ int floor_to_int(float u) {
 // cos are computed. The table is read as two float4 arrays (cp = p0 p1 p2 k,
 // cq = q0 q1 q2 s of each DirCell), 16 bytes per read.
 // With a non-null weight, the amplitude of point k is also multiplied by
-// weight[k] (0 or 1 masks padding rows without a select).
+// weight[k] (0 or 1 masks padding rows without a select); with a non-null
+// amp_out, amp_out[k] receives that amplitude (the term's modulus, >= 0).
 template <int SRC, int NP>
 __device__ __forceinline__ void add_terms_staged(const SrcF& s, const float* px, const float* py, const float* pz,
                                                  const float4* cp, const float4* cq, float* re, float* im,
-                                                 const float* weight = nullptr) {
+                                                 const float* weight = nullptr, float* amp_out = nullptr) {
     float d2[NP], rinv[NP], t[NP];
     float4 P[NP], Q[NP];
 #pragma unroll
@@ -321,6 +322,7 @@ __device__ __forceinline__ void add_terms_staged(const SrcF& s, const float* px,
             a *= fmaf(step, q, p);
         }
         if (weight) a *= weight[k];
+        if (amp_out) amp_out[k] = a;
         re[k] = fmaf(a, cs[k], re[k]);
         im[k] = fmaf(a, sn[k], im[k]);
     }

@@ -263,7 +263,7 @@ __device__ __forceinline__ void store_g_tile(const HoloArgs& a, uint32_t k, uint
 template <int MP, bool H16>
 struct StoredChunk {
     std::conditional_t<H16, uint32_t, float2> g[MP / 8];
-    float d;
+    float d, la;  // la (H16): log2 |amp| of the column's source, which sets its FP16 scale
 };
 
 template <int MP, bool H16>
@@ -287,6 +287,7 @@ __device__ __forceinline__ void load_chunk(const HoloArgs& a, uint32_t k, uint32
         }
     }
     c.d = t < kKC ? a.d[(uint64_t)k * a.n_packed + j0 + t] : 0.f;
+    c.la = (H16 && t < kKC) ? a.staged[3 * (uint64_t)(j0 + t)].w : 0.f;
 }
 
 // The chunk's G tile in FP32 (H16 words widened).
@@ -505,6 +506,149 @@ __global__ __launch_bounds__(kHB) __attribute__((amdgpu_waves_per_eu(MP == 64 ? 
     }
 }
 
+// ---------------------------------------------------------------- MIXED: kernels A1, A2
+// Exponent s_j of column j's FP16 copy G_ij 2^s_j: s_j = 8 - floor(log2 |amp_j|),
+// so that |G_ij 2^s_j| ~ 256 D / r (r in mm) sits in the FP16 normal range;
+// 0 for a zero amplitude (the column is 0).
+__device__ __forceinline__ int gen16_scale(float la) {
+    return (la > -64.f && la < 64.f) ? 8 - (int)floorf(la) : 0;
+}
+
+__device__ __forceinline__ uint32_t pack_h2(float re, float im) {
+    return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)re) |
+           ((uint32_t)__builtin_bit_cast(uint16_t, (_Float16)im) << 16);
+}
+
+// A1: G of every instance of the bucket, one thread per source column j and
+// the instance's foci in turn (two at a time through the staged term, as the
+// field kernel evaluates them), stored for rows < M as FP16 G_ij 2^s_j (4
+// bytes, non-temporal), and d_j = sum_i |G_ij| over i in order (FP32, from
+// each term's modulus).
+template <int SRC, int MP>
+__global__ __launch_bounds__(kHB) void gspat_gen16(HoloArgs a) {
+    __shared__ std::conditional_t<SRC == kT4010A1, CellsLds, float> s_cells_mem;
+    CellsLds* s_cells = reinterpret_cast<CellsLds*>(&s_cells_mem);
+    __shared__ float4 s_f[MP];
+    const uint32_t k = blockIdx.y, b = a.inst[k];
+    const uint32_t base = a.offsets[b], M = a.offsets[b + 1] - base;
+    for (int i = threadIdx.x; i < MP; i += kHB) {
+        const uint64_t q = 3 * (uint64_t)(base + i);
+        s_f[i] = i < (int)M ? make_float4(a.foci[q], a.foci[q + 1], a.foci[q + 2], 0.f) : make_float4(0.f, 0.f, 1.f, 0.f);
+    }
+    if (SRC == kT4010A1) stage_cells_lds(a.cells, s_cells);
+    __syncthreads();
+    const uint32_t j = blockIdx.x * kHB + threadIdx.x;
+    if (j >= a.n_packed) return;
+    const SrcF src = load_staged(a.staged, j);
+    const int sc = gen16_scale(src.la);
+    uint32_t* G16 = reinterpret_cast<uint32_t*>(a.G) + (uint64_t)k * MP * a.n_packed + j;
+    float d = 0.f;
+    for (uint32_t i = 0; i < M; i += 2) {
+        // i + 1 < MP: M <= MP and both even steps; row M (M odd) is padding, not stored
+        const float4 f0 = s_f[i], f1 = s_f[i + 1];
+        const float px[2] = {f0.x, f1.x}, py[2] = {f0.y, f1.y}, pz[2] = {f0.z, f1.z};
+        float re[2] = {0.f, 0.f}, im[2] = {0.f, 0.f}, am[2];
+        add_terms_staged<SRC, 2>(src, px, py, pz, s_cells->p, s_cells->q, re, im, nullptr, am);
+        d += fabsf(am[0]);
+        __builtin_nontemporal_store(pack_h2(ldexpf(re[0], sc), ldexpf(im[0], sc)), G16 + (uint64_t)i * a.n_packed);
+        if (i + 1 < M) {
+            d += fabsf(am[1]);
+            __builtin_nontemporal_store(pack_h2(ldexpf(re[1], sc), ldexpf(im[1], sc)),
+                                        G16 + (uint64_t)(i + 1) * a.n_packed);
+        }
+    }
+    a.d[(uint64_t)k * a.n_packed + j] = d;
+}
+
+// A2: R = H H^H over the split's source chunk range from A1's copy, H_ij =
+// G_ij / d_j = FP16(G_ij 2^s_j) 2^-s_j / d_j rounded to FP16 in LDS, then the
+// same MFMA tiles as kernel A (Re R += Hr Hr^T + Hi Hi^T, Im R += Hi Hr^T +
+// (-Hr) Hi^T). The tile is double-buffered by chunk parity: a buffer is written
+// again two chunks later, after a barrier every wave reaches after its MFMAs
+// on it. Each thread's words of the next chunk are loaded during this one.
+template <int MP>
+__global__ __launch_bounds__(kHB) __attribute__((amdgpu_waves_per_eu(MP == 64 ? 3 : 1))) void gspat_r16(HoloArgs a) {
+    static_assert(MP >= 32, "one 16x16 tile per wave at least");
+    constexpr int T = MP / 16;
+    constexpr int TPW = T * T / 4;
+    constexpr int HS = kKC + 8;
+    __shared__ __attribute__((aligned(16))) _Float16 s_hr[2][MP * HS];
+    __shared__ __attribute__((aligned(16))) _Float16 s_hi[2][MP * HS];
+    const int t = threadIdx.x;
+    const int w = t >> 6, l = t & 63;
+    const uint32_t k = blockIdx.x / a.split, part = blockIdx.x % a.split, b = a.inst[k];
+    const uint32_t n_chunks = a.n_packed / kKC;
+    const uint32_t c_begin = part * n_chunks / a.split, c_end = (part + 1) * n_chunks / a.split;
+    const uint32_t M = a.offsets[b + 1] - a.offsets[b];
+    const uint32_t* G16 = reinterpret_cast<const uint32_t*>(a.G) + (uint64_t)k * MP * a.n_packed + (t & 31);
+    const float* dk = a.d + (uint64_t)k * a.n_packed + (t & 31);
+    floatx4 accRe[TPW], accIm[TPW];
+#pragma unroll
+    for (int s = 0; s < TPW; ++s) {
+        accRe[s] = floatx4{0.f, 0.f, 0.f, 0.f};
+        accIm[s] = accRe[s];
+    }
+    // the thread's words (rows (t >> 5) + 8 r, column j0 + t % 32), d_j and la_j
+    uint32_t nx[MP / 8];
+    float nd = 0.f, nla = 0.f;
+    auto fetch = [&](uint32_t j0) {
+#pragma unroll
+        for (int r = 0; r < MP / 8; ++r) {
+            const int i = (t >> 5) + 8 * r;
+            nx[r] = i < (int)M ? __builtin_nontemporal_load(G16 + (uint64_t)i * a.n_packed + j0) : 0u;
+        }
+        nd = dk[j0];
+        nla = a.staged[3 * (uint64_t)(j0 + (t & 31))].w;
+    };
+    if (c_begin < c_end) fetch(c_begin * kKC);
+    for (uint32_t chunk = c_begin; chunk < c_end; ++chunk) {
+        const uint32_t j0 = chunk * kKC;
+        const int pb = (chunk - c_begin) & 1;
+        uint32_t x[MP / 8];
+#pragma unroll
+        for (int r = 0; r < MP / 8; ++r) x[r] = nx[r];
+        const float dj = nd, laj = nla;
+        if (chunk + 1 < c_end) fetch(j0 + kKC);
+        const float f = (j0 + (t & 31) < a.n) ? ldexpf(1.0f / dj, -gen16_scale(laj)) : 0.f;
+#pragma unroll
+        for (int r = 0; r < MP / 8; ++r) {
+            const int i = (t >> 5) + 8 * r;
+            const float hre = (float)__builtin_bit_cast(_Float16, (uint16_t)(x[r] & 0xffffu)) * f;
+            const float him = (float)__builtin_bit_cast(_Float16, (uint16_t)(x[r] >> 16)) * f;
+            s_hr[pb][i * HS + (t & 31)] = (_Float16)hre;
+            s_hi[pb][i * HS + (t & 31)] = (_Float16)him;
+        }
+        lds_sync();
+        const int kb = (l >> 4) * 8;
+#pragma unroll
+        for (int s = 0; s < TPW; ++s) {
+            const int tau = w * TPW + s;
+            const int arow = (tau / T) * 16 + (l & 15);
+            const int brow = (tau % T) * 16 + (l & 15);
+            const half8 ar = *reinterpret_cast<const half8*>(&s_hr[pb][arow * HS + kb]);
+            const half8 ai = *reinterpret_cast<const half8*>(&s_hi[pb][arow * HS + kb]);
+            const half8 br = *reinterpret_cast<const half8*>(&s_hr[pb][brow * HS + kb]);
+            const half8 bi = *reinterpret_cast<const half8*>(&s_hi[pb][brow * HS + kb]);
+            const half8 nar = -ar;
+            accRe[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ar, br, accRe[s], 0, 0, 0);
+            accRe[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ai, bi, accRe[s], 0, 0, 0);
+            accIm[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ai, br, accIm[s], 0, 0, 0);
+            accIm[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(nar, bi, accIm[s], 0, 0, 0);
+        }
+    }
+    float2* R = a.R + (uint64_t)blockIdx.x * MP * MP;  // partial `part` of instance k
+#pragma unroll
+    for (int s = 0; s < TPW; ++s) {
+        const int tau = w * TPW + s;
+        const int col = (tau % T) * 16 + (l & 15);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int row = (tau / T) * 16 + 4 * (l >> 4) + e;
+            R[row * MP + col] = make_float2(accRe[s][e], accIm[s][e]);
+        }
+    }
+}
+
 // ---------------------------------------------------------------- kernel B
 // v summed over the TPR consecutive lanes of a row group (TPR = 1, 2, 4, 8 or
 // 16) with DPP lane moves instead of LDS-crossbar shuffles: lane i ^ 8 (row
@@ -643,7 +787,7 @@ __global__ __launch_bounds__(kHB) void gspat_q(HoloArgs a, int closing, int pass
         const int pb = (chunk - c_begin) & 1, slot = (chunk - c_begin) % 8;
         float2 g[MP / 8];
         chunk_g<MP, H16>(cq[0], g);
-        const float dj = cq[0].d;
+        const float dj = cq[0].d, laj = cq[0].la;
 #pragma unroll
         for (int u = 0; u + 1 < PD; ++u) cq[u] = cq[u + 1];
         if (chunk + PD < c_end) load_chunk<MP, H16>(a, k, (chunk + PD) * kKC, M, cq[PD - 1]);
@@ -665,7 +809,8 @@ __global__ __launch_bounds__(kHB) void gspat_q(HoloArgs a, int closing, int pass
                 q.x += s_qp[pb][q8][t].x;
                 q.y += s_qp[pb][q8][t].y;
             }
-            const float w = (j0 + t < a.n) ? (H16 ? 1.0f / dj : 1.0f / (dj * dj)) : 0.f;
+            const float w = (j0 + t < a.n) ? (H16 ? ldexpf(1.0f / (dj * dj), -gen16_scale(laj)) : 1.0f / (dj * dj))
+                                           : 0.f;
             q = make_float2(q.x * w, q.y * w);
             if (pass == kQGamma)
                 s_q[pb][t] = q;
@@ -960,18 +1105,22 @@ static hipError_t launch_gspat_bucket(HoloArgs a, int src, int precision, uint32
             hipLaunchKernelGGL((gspat_r_fp32<kSphere, MP>), grid_a, blk, 0, st, a);
     } else {
         if constexpr (MP >= 32) {
-            if (src == kT4010A1) {
+            if (precision == AFH_HOLO_MIXED) {
+                // A1 (G, one thread per source column) then A2 (R from A1's FP16 copy)
+                const dim3 grid_g(a.n_packed / kHB + (a.n_packed % kHB ? 1 : 0), nb);
+                if (src == kT4010A1)
+                    hipLaunchKernelGGL((gspat_gen16<kT4010A1, MP>), grid_g, blk, 0, st, a);
+                else
+                    hipLaunchKernelGGL((gspat_gen16<kSphere, MP>), grid_g, blk, 0, st, a);
+                hipLaunchKernelGGL((gspat_r16<MP>), grid_a, blk, 0, st, a);
+            } else if (src == kT4010A1) {
                 if (precision == AFH_HOLO_MIXED_SPLIT)
                     hipLaunchKernelGGL((gspat_r_mfma<kT4010A1, MP, true, false>), grid_a, blk, 0, st, a);
-                else if (precision == AFH_HOLO_MIXED)
-                    hipLaunchKernelGGL((gspat_r_mfma<kT4010A1, MP, false, true>), grid_a, blk, 0, st, a);
                 else
                     hipLaunchKernelGGL((gspat_r_mfma<kT4010A1, MP, false, false>), grid_a, blk, 0, st, a);
             } else {
                 if (precision == AFH_HOLO_MIXED_SPLIT)
                     hipLaunchKernelGGL((gspat_r_mfma<kSphere, MP, true, false>), grid_a, blk, 0, st, a);
-                else if (precision == AFH_HOLO_MIXED)
-                    hipLaunchKernelGGL((gspat_r_mfma<kSphere, MP, false, true>), grid_a, blk, 0, st, a);
                 else
                     hipLaunchKernelGGL((gspat_r_mfma<kSphere, MP, false, false>), grid_a, blk, 0, st, a);
             }

@@ -134,7 +134,7 @@ def test_gspat_batched_vs_fp64(afc, oracle, torch_cuda, batch16, precision, tol)
 def test_gspat_instance_isolation(afc, oracle, torch_cuda, batch16):
     """An instance's result does not depend on its batch-mates."""
     d = batch16
-    for precision in (0, 3):
+    for precision in (0, 1, 3):
         full_a, full_p = run_batch(afc, torch_cuda, oracle, d["src"], d["k"], d["a"], d["foci"], d["amps"], d["offs"],
                                    0, precision)
         for b in (0, 8, 10):
