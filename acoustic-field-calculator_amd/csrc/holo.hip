@@ -520,10 +520,9 @@ __device__ __forceinline__ uint32_t pack_h2(float re, float im) {
 }
 
 // A1: G of every instance of the bucket, one thread per source column j and
-// the instance's foci in turn (two at a time through the staged term, as the
-// field kernel evaluates them), stored for rows < M as FP16 G_ij 2^s_j (4
-// bytes, non-temporal), and d_j = sum_i |G_ij| over i in order (FP32, from
-// each term's modulus).
+// the instance's foci in turn (four at a time through the staged term),
+// stored for rows < M as FP16 G_ij 2^s_j (4 bytes, non-temporal), and d_j =
+// sum_i |G_ij| over i in order (FP32, from each term's modulus).
 template <int SRC, int MP>
 __global__ __launch_bounds__(kHB) void gspat_gen16(HoloArgs a) {
     __shared__ std::conditional_t<SRC == kT4010A1, CellsLds, float> s_cells_mem;
@@ -539,25 +538,34 @@ __global__ __launch_bounds__(kHB) void gspat_gen16(HoloArgs a) {
     __syncthreads();
     const uint32_t j = blockIdx.x * kHB + threadIdx.x;
     if (j >= a.n_packed) return;
-    const SrcF src = load_staged(a.staged, j);
+    SrcF src = load_staged(a.staged, j);
     const int sc = gen16_scale(src.la);
+    // the scale rides on the amplitude's exponent: every term comes out as
+    // G_ij 2^s_j, and d_j is scaled back exactly at the end
+    src.la += (float)sc;
     uint32_t* G16 = reinterpret_cast<uint32_t*>(a.G) + (uint64_t)k * MP * a.n_packed + j;
     float d = 0.f;
-    for (uint32_t i = 0; i < M; i += 2) {
-        // i + 1 < MP: M <= MP and both even steps; row M (M odd) is padding, not stored
-        const float4 f0 = s_f[i], f1 = s_f[i + 1];
-        const float px[2] = {f0.x, f1.x}, py[2] = {f0.y, f1.y}, pz[2] = {f0.z, f1.z};
-        float re[2] = {0.f, 0.f}, im[2] = {0.f, 0.f}, am[2];
-        add_terms_staged<SRC, 2>(src, px, py, pz, s_cells->p, s_cells->q, re, im, nullptr, am);
-        d += fabsf(am[0]);
-        __builtin_nontemporal_store(pack_h2(ldexpf(re[0], sc), ldexpf(im[0], sc)), G16 + (uint64_t)i * a.n_packed);
-        if (i + 1 < M) {
-            d += fabsf(am[1]);
-            __builtin_nontemporal_store(pack_h2(ldexpf(re[1], sc), ldexpf(im[1], sc)),
-                                        G16 + (uint64_t)(i + 1) * a.n_packed);
+    for (uint32_t i = 0; i < M; i += 4) {
+        // i + 3 < MP (i is a multiple of 4 below M <= MP, MP a multiple of
+        // 16); rows from M on are padding foci, evaluated and not stored
+        float px[4], py[4], pz[4], re[4], im[4], am[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const float4 f = s_f[i + u];
+            px[u] = f.x;
+            py[u] = f.y;
+            pz[u] = f.z;
+            re[u] = im[u] = 0.f;
         }
+        add_terms_staged<SRC, 4>(src, px, py, pz, s_cells->p, s_cells->q, re, im, nullptr, am);
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (i + u < M) {
+                d += fabsf(am[u]);
+                __builtin_nontemporal_store(pack_h2(re[u], im[u]), G16 + (uint64_t)(i + u) * a.n_packed);
+            }
     }
-    a.d[(uint64_t)k * a.n_packed + j] = d;
+    a.d[(uint64_t)k * a.n_packed + j] = ldexpf(d, -sc);
 }
 
 // A2: R = H H^H over the split's source chunk range from A1's copy, H_ij =
