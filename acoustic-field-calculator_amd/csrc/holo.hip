@@ -258,7 +258,7 @@ __device__ __forceinline__ void store_g_tile(const HoloArgs& a, uint32_t k, uint
 }
 
 // Kernel C's stored inputs of one chunk as loaded: the G tile (FP32 G, or
-// H16 words: the FP16 H = G / d_j that kernel A fed its MFMA in MIXED mode)
+// H16 words: the FP16 copy G_ij 2^s_j that kernel A1 stores in MIXED mode)
 // and, for threads t < kKC, d of column j0 + t.
 template <int MP, bool H16>
 struct StoredChunk {
@@ -384,13 +384,10 @@ __global__ __launch_bounds__(kHB) void gspat_r_fp32(HoloArgs a) {
 // share a SIMD and 3 workgroups a CU (LDS allows 3); at the 196 the compiler
 // otherwise takes, 2 workgroups per CU ran the 2052 workgroups of a
 // 513-instance bucket in 5 rounds, the last with 4 of them.
-// H16 (MIXED): the FP16 H = G / d_j of the MFMA operand is what kernel A
-// stores for kernel C, 4 bytes per element instead of FP32 G's 8.
-template <int SRC, int MP, bool SPLIT, bool H16>
+template <int SRC, int MP, bool SPLIT>
 __global__ __launch_bounds__(kHB) __attribute__((amdgpu_waves_per_eu(MP == 64 ? 3 : 1))) void gspat_r_mfma(
         HoloArgs a) {
     static_assert(MP >= 32, "one 16x16 tile per wave at least");
-    static_assert(!(SPLIT && H16), "SPLIT keeps FP32 G");
     constexpr int T = MP / 16;
     constexpr int TPW = T * T / 4;  // tiles per wave
     constexpr int HS = kKC + 8;     // row stride in halves (80 bytes)
@@ -435,7 +432,7 @@ __global__ __launch_bounds__(kHB) __attribute__((amdgpu_waves_per_eu(MP == 64 ? 
         const SrcF sc = src_from_block(s_src, in_blk);
         float2 g[MP / 8];
         g_tile<SRC, MP, 2>(sc, f, s_cells, M, g);
-        if constexpr (!H16) store_g_tile<MP>(a, k, j0, M, g);
+        store_g_tile<MP>(a, k, j0, M, g);
         const float inv = column_inv<MP>(g, j0, a.n, s_part, d_out) * scale;
 #pragma unroll
         for (int r = 0; r < MP / 8; ++r) {
@@ -444,13 +441,6 @@ __global__ __launch_bounds__(kHB) __attribute__((amdgpu_waves_per_eu(MP == 64 ? 
             const _Float16 rh = (_Float16)hre, ih = (_Float16)him;
             s_hr[0][i * HS + (t & 31)] = rh;
             s_hi[0][i * HS + (t & 31)] = ih;
-            if (H16 && i < (int)M) {
-                // streamed: read once, by kernel C
-                const uint32_t h = (uint32_t)__builtin_bit_cast(uint16_t, rh) |
-                                   ((uint32_t)__builtin_bit_cast(uint16_t, ih) << 16);
-                __builtin_nontemporal_store(h, reinterpret_cast<uint32_t*>(a.G) +
-                                                   ((uint64_t)k * MP + i) * a.n_packed + j0 + (t & 31));
-            }
             if (SPLIT) {
                 s_hr[NH - 1][i * HS + (t & 31)] = (_Float16)(hre - (float)rh);
                 s_hi[NH - 1][i * HS + (t & 31)] = (_Float16)(him - (float)ih);
@@ -800,7 +790,7 @@ __global__ __launch_bounds__(kHB) void gspat_q(HoloArgs a, int closing, int pass
         for (int u = 0; u + 1 < PD; ++u) cq[u] = cq[u + 1];
         if (chunk + PD < c_end) load_chunk<MP, H16>(a, k, (chunk + PD) * kKC, M, cq[PD - 1]);
         // q_j = w_j sum_i conj(G_ij) p_i: this row group's partial, then the 8
-        // groups in a fixed order; w_j = 1 / d_j^2 (FP32 G) or 1 / d_j (H16)
+        // groups in a fixed order; w_j = 1 / d_j^2 (FP32 G) or 2^-s_j / d_j^2 (H16)
         float2 pq = make_float2(0.f, 0.f);
 #pragma unroll
         for (int r = 0; r < MP / 8; ++r) {
@@ -1123,14 +1113,14 @@ static hipError_t launch_gspat_bucket(HoloArgs a, int src, int precision, uint32
                 hipLaunchKernelGGL((gspat_r16<MP>), grid_a, blk, 0, st, a);
             } else if (src == kT4010A1) {
                 if (precision == AFH_HOLO_MIXED_SPLIT)
-                    hipLaunchKernelGGL((gspat_r_mfma<kT4010A1, MP, true, false>), grid_a, blk, 0, st, a);
+                    hipLaunchKernelGGL((gspat_r_mfma<kT4010A1, MP, true>), grid_a, blk, 0, st, a);
                 else
-                    hipLaunchKernelGGL((gspat_r_mfma<kT4010A1, MP, false, false>), grid_a, blk, 0, st, a);
+                    hipLaunchKernelGGL((gspat_r_mfma<kT4010A1, MP, false>), grid_a, blk, 0, st, a);
             } else {
                 if (precision == AFH_HOLO_MIXED_SPLIT)
-                    hipLaunchKernelGGL((gspat_r_mfma<kSphere, MP, true, false>), grid_a, blk, 0, st, a);
+                    hipLaunchKernelGGL((gspat_r_mfma<kSphere, MP, true>), grid_a, blk, 0, st, a);
                 else
-                    hipLaunchKernelGGL((gspat_r_mfma<kSphere, MP, false, false>), grid_a, blk, 0, st, a);
+                    hipLaunchKernelGGL((gspat_r_mfma<kSphere, MP, false>), grid_a, blk, 0, st, a);
             }
         } else {
             return hipErrorInvalidValue;
