@@ -511,10 +511,11 @@ __device__ __forceinline__ uint32_t pack_h2(float re, float im) {
 
 // A1: G of every instance of the bucket, one thread per source column j and
 // the instance's foci in turn (four at a time through the staged term),
-// stored for rows < M as FP16 G_ij 2^s_j (4 bytes, non-temporal), and d_j =
+// stored for rows < M as FP16 G_ij 2^s_j (MIXED, 4 bytes) or FP32 G_ij (F32:
+// FP32 mode, 8 bytes, the layout kernel C reads), non-temporal, and d_j =
 // sum_i |G_ij| over i in order (FP32, from each term's modulus).
-template <int SRC, int MP>
-__global__ __launch_bounds__(kHB) void gspat_gen16(HoloArgs a) {
+template <int SRC, int MP, bool F32>
+__global__ __launch_bounds__(kHB) void gspat_gen(HoloArgs a) {
     __shared__ std::conditional_t<SRC == kT4010A1, CellsLds, float> s_cells_mem;
     CellsLds* s_cells = reinterpret_cast<CellsLds*>(&s_cells_mem);
     __shared__ float4 s_f[MP];
@@ -529,11 +530,12 @@ __global__ __launch_bounds__(kHB) void gspat_gen16(HoloArgs a) {
     const uint32_t j = blockIdx.x * kHB + threadIdx.x;
     if (j >= a.n_packed) return;
     SrcF src = load_staged(a.staged, j);
-    const int sc = gen16_scale(src.la);
+    const int sc = F32 ? 0 : gen16_scale(src.la);
     // the scale rides on the amplitude's exponent: every term comes out as
     // G_ij 2^s_j, and d_j is scaled back exactly at the end
     src.la += (float)sc;
     uint32_t* G16 = reinterpret_cast<uint32_t*>(a.G) + (uint64_t)k * MP * a.n_packed + j;
+    float2* G32 = a.G + (uint64_t)k * MP * a.n_packed + j;
     float d = 0.f;
     for (uint32_t i = 0; i < M; i += 4) {
         // i + 3 < MP (i is a multiple of 4 below M <= MP, MP a multiple of
@@ -552,7 +554,13 @@ __global__ __launch_bounds__(kHB) void gspat_gen16(HoloArgs a) {
         for (int u = 0; u < 4; ++u)
             if (i + u < M) {
                 d += fabsf(am[u]);
-                __builtin_nontemporal_store(pack_h2(re[u], im[u]), G16 + (uint64_t)(i + u) * a.n_packed);
+                if constexpr (F32) {
+                    float2* g = G32 + (uint64_t)(i + u) * a.n_packed;
+                    __builtin_nontemporal_store(re[u], &g->x);
+                    __builtin_nontemporal_store(im[u], &g->y);
+                } else {
+                    __builtin_nontemporal_store(pack_h2(re[u], im[u]), G16 + (uint64_t)(i + u) * a.n_packed);
+                }
             }
     }
     a.d[(uint64_t)k * a.n_packed + j] = ldexpf(d, -sc);
@@ -643,6 +651,124 @@ __global__ __launch_bounds__(kHB) __attribute__((amdgpu_waves_per_eu(MP == 64 ? 
         for (int e = 0; e < 4; ++e) {
             const int row = (tau / T) * 16 + 4 * (l >> 4) + e;
             R[row * MP + col] = make_float2(accRe[s][e], accIm[s][e]);
+        }
+    }
+}
+
+// A2 for FP32 mode (MP >= 32): R = H H^H over the split's source chunk range
+// from A1's FP32 G, H_ij = G_ij / d_j in FP32 (an LDS tile, double-buffered by
+// chunk parity as in A2), on FP32 MFMA (v_mfma_f32_16x16x4f32: lane l supplies
+// A[l % 16][l / 16] and B[l / 16][l % 16] and receives D[4 (l / 16) + e][l %
+// 16]). R is Hermitian, so only the tiles (I, J) with I <= J are formed, dealt
+// to the 4 waves in turn, and each with three real products (P1 = Hr Hr^T, P2 =
+// Hi Hi^T, P3 = (Hr + Hi)(Hr - Hi)^T; Re R = P1 + P2, Im R = P3 - P1 + P2, the
+// sums in FP32); the tile (J, I) is written as the conjugate transpose. The sum
+// over columns is taken in a permuted order so that each lane reads 4 adjacent
+// columns at once: MFMA step (m, q) gives lane group c = l / 16 the column
+// 16 m + 4 c + q, for both operands (rows 36 floats apart: the 128-bit reads of
+// 8 or 16 lanes fall in distinct banks of a 32- or 64-bank LDS).
+template <int MP>
+__global__ __launch_bounds__(kHB) void gspat_r32(HoloArgs a) {
+    static_assert(MP >= 32, "one 16x16 tile per wave at least");
+    constexpr int T = MP / 16;
+    constexpr int NT = T * (T + 1) / 2;  // upper-triangle tiles
+    constexpr int TPW = (NT + 3) / 4;
+    constexpr int HS = kKC + 4;
+    __shared__ __attribute__((aligned(16))) float s_hr[2][MP * HS];
+    __shared__ __attribute__((aligned(16))) float s_hi[2][MP * HS];
+    const int t = threadIdx.x;
+    const int w = t >> 6, l = t & 63;
+    const uint32_t k = blockIdx.x / a.split, part = blockIdx.x % a.split, b = a.inst[k];
+    const uint32_t n_chunks = a.n_packed / kKC;
+    const uint32_t c_begin = part * n_chunks / a.split, c_end = (part + 1) * n_chunks / a.split;
+    const uint32_t M = a.offsets[b + 1] - a.offsets[b];
+    const float2* G = a.G + (uint64_t)k * MP * a.n_packed + (t & 31);
+    const float* dk = a.d + (uint64_t)k * a.n_packed + (t & 31);
+    // the wave's tiles: tau = w + 4 s in row-major order of the upper triangle;
+    // a wave with fewer than TPW tiles forms tile 0 again in the last slot, so
+    // every wave runs the same straight-line MFMA sequence, and does not store it
+    int tI[TPW], tJ[TPW];
+#pragma unroll
+    for (int s = 0; s < TPW; ++s) {
+        int r = w + 4 * s < NT ? w + 4 * s : 0, I = 0;
+        while (r >= T - I) {
+            r -= T - I;
+            ++I;
+        }
+        tI[s] = I;
+        tJ[s] = I + r;
+    }
+    floatx4 p1[TPW], p2[TPW], p3[TPW];
+#pragma unroll
+    for (int s = 0; s < TPW; ++s) {
+        p1[s] = floatx4{0.f, 0.f, 0.f, 0.f};
+        p2[s] = p1[s];
+        p3[s] = p1[s];
+    }
+    // the thread's G (rows (t >> 5) + 8 r, column j0 + t % 32) and d_j, one chunk ahead
+    float2 ng[MP / 8];
+    float nd = 0.f;
+    auto fetch = [&](uint32_t j0) {
+#pragma unroll
+        for (int r = 0; r < MP / 8; ++r) {
+            const int i = (t >> 5) + 8 * r;
+            const float2* g = G + (uint64_t)i * a.n_packed + j0;
+            ng[r] = i < (int)M ? make_float2(__builtin_nontemporal_load(&g->x), __builtin_nontemporal_load(&g->y))
+                               : make_float2(0.f, 0.f);
+        }
+        nd = dk[j0];
+    };
+    if (c_begin < c_end) fetch(c_begin * kKC);
+    for (uint32_t chunk = c_begin; chunk < c_end; ++chunk) {
+        const uint32_t j0 = chunk * kKC;
+        const int pb = (chunk - c_begin) & 1;
+        float2 g[MP / 8];
+#pragma unroll
+        for (int r = 0; r < MP / 8; ++r) g[r] = ng[r];
+        const float dj = nd;
+        if (chunk + 1 < c_end) fetch(j0 + kKC);
+        // a real column with d = 0 gives NaN, as conj(G) / 0 does in the reference
+        const float inv = (j0 + (t & 31) < a.n) ? 1.0f / dj : 0.f;
+#pragma unroll
+        for (int r = 0; r < MP / 8; ++r) {
+            const int i = (t >> 5) + 8 * r;
+            s_hr[pb][i * HS + (t & 31)] = g[r].x * inv;
+            s_hi[pb][i * HS + (t & 31)] = g[r].y * inv;
+        }
+        lds_sync();
+#pragma unroll
+        for (int m = 0; m < kKC / 16; ++m) {
+            const int cb = 16 * m + 4 * (l >> 4);
+#pragma unroll
+            for (int s = 0; s < TPW; ++s) {
+                const int arow = tI[s] * 16 + (l & 15);
+                const int brow = tJ[s] * 16 + (l & 15);
+                const float4 ar = *reinterpret_cast<const float4*>(&s_hr[pb][arow * HS + cb]);
+                const float4 ai = *reinterpret_cast<const float4*>(&s_hi[pb][arow * HS + cb]);
+                const float4 br = *reinterpret_cast<const float4*>(&s_hr[pb][brow * HS + cb]);
+                const float4 bi = *reinterpret_cast<const float4*>(&s_hi[pb][brow * HS + cb]);
+                const float xr[4] = {ar.x, ar.y, ar.z, ar.w}, xi[4] = {ai.x, ai.y, ai.z, ai.w};
+                const float yr[4] = {br.x, br.y, br.z, br.w}, yi[4] = {bi.x, bi.y, bi.z, bi.w};
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    p1[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(xr[q], yr[q], p1[s], 0, 0, 0);
+                    p2[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(xi[q], yi[q], p2[s], 0, 0, 0);
+                    p3[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(xr[q] + xi[q], yr[q] - yi[q], p3[s], 0, 0, 0);
+                }
+            }
+        }
+    }
+    float2* R = a.R + (uint64_t)blockIdx.x * MP * MP;  // partial `part` of instance k
+#pragma unroll
+    for (int s = 0; s < TPW; ++s) {
+        if (w + 4 * s >= NT) continue;  // wave-uniform
+        const int col = tJ[s] * 16 + (l & 15);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int row = tI[s] * 16 + 4 * (l >> 4) + e;
+            const float re = p1[s][e] + p2[s][e], im = p3[s][e] - p1[s][e] + p2[s][e];
+            R[row * MP + col] = make_float2(re, im);
+            if (tI[s] != tJ[s]) R[col * MP + row] = make_float2(re, -im);
         }
     }
 }
@@ -1096,20 +1222,28 @@ template <int MP>
 static hipError_t launch_gspat_bucket(HoloArgs a, int src, int precision, uint32_t nb, uint32_t iters,
                                       hipStream_t st) {
     const dim3 grid(nb), grid_a(nb * a.split), blk(kHB);
+    const dim3 grid_g(a.n_packed / kHB + (a.n_packed % kHB ? 1 : 0), nb);
     if (precision == AFH_HOLO_FP32) {
-        if (src == kT4010A1)
+        if constexpr (MP >= 32) {
+            // A1 (FP32 G, one thread per source column) then A2 on FP32 MFMA
+            if (src == kT4010A1)
+                hipLaunchKernelGGL((gspat_gen<kT4010A1, MP, true>), grid_g, blk, 0, st, a);
+            else
+                hipLaunchKernelGGL((gspat_gen<kSphere, MP, true>), grid_g, blk, 0, st, a);
+            hipLaunchKernelGGL((gspat_r32<MP>), grid_a, blk, 0, st, a);
+        } else if (src == kT4010A1) {
             hipLaunchKernelGGL((gspat_r_fp32<kT4010A1, MP>), grid_a, blk, 0, st, a);
-        else
+        } else {
             hipLaunchKernelGGL((gspat_r_fp32<kSphere, MP>), grid_a, blk, 0, st, a);
+        }
     } else {
         if constexpr (MP >= 32) {
             if (precision == AFH_HOLO_MIXED) {
                 // A1 (G, one thread per source column) then A2 (R from A1's FP16 copy)
-                const dim3 grid_g(a.n_packed / kHB + (a.n_packed % kHB ? 1 : 0), nb);
                 if (src == kT4010A1)
-                    hipLaunchKernelGGL((gspat_gen16<kT4010A1, MP>), grid_g, blk, 0, st, a);
+                    hipLaunchKernelGGL((gspat_gen<kT4010A1, MP, false>), grid_g, blk, 0, st, a);
                 else
-                    hipLaunchKernelGGL((gspat_gen16<kSphere, MP>), grid_g, blk, 0, st, a);
+                    hipLaunchKernelGGL((gspat_gen<kSphere, MP, false>), grid_g, blk, 0, st, a);
                 hipLaunchKernelGGL((gspat_r16<MP>), grid_a, blk, 0, st, a);
             } else if (src == kT4010A1) {
                 if (precision == AFH_HOLO_MIXED_SPLIT)
