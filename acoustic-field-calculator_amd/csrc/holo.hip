@@ -1074,6 +1074,115 @@ __global__ __launch_bounds__(kHB) void gs_iter(HoloArgs a, uint32_t iters, uint3
     }
 }
 
+// GS with one pass over G per iteration (gs.rs:56-72, the same steps in the
+// same order): q_{t+1,j} = xi_{t,j} / |xi_{t,j}| q0_j depends on column j
+// alone, so the pass that forms xi_t = G^H p_t column by column also forms
+// gamma_{t+1} = G q_{t+1} from the same loaded values of G. Pass 0 forms
+// gamma_0 = G q0; pass t + 1 forms xi_t, q_{t+1} and, before the last, gamma_{t+1};
+// the last writes |q| and arg q. Wave w takes the 16-column groups w, w + 4,
+// ...; lane (rg, c) = (l / 16, l % 16) holds rows rg + 4 r of column 16 g + c,
+// the next group's loaded during this one. Every load is issued unconditionally
+// and its value kept as loaded (rows >= M read row M - 1, columns >= n source
+// n - 1, past the last group the current one is read again); the 0 / 1 masks
+// multiply the values where they are used. A select right after the load let
+// the compiler move the load under a branch and wait for it there. xi_j sums the 4 row
+// groups by lane exchange; gamma's partials are summed over a wave's 16
+// columns by lane exchange and over the 4 waves in LDS, in fixed order.
+template <int MP>
+__global__ __launch_bounds__(kHB) void gs_fused(HoloArgs a, uint32_t iters, uint32_t g_stride_inst) {
+    constexpr int RPT = MP / 4;
+    __shared__ float2 s_gam[4][MP];
+    __shared__ float2 s_p[MP];
+    const int t = threadIdx.x, w = t >> 6, l = t & 63, rg = l >> 4, c = l & 15;
+    const uint32_t k = blockIdx.x, b = a.inst[k];
+    const uint32_t base = a.offsets[b], M = a.offsets[b + 1] - base;  // M >= 1
+    const uint32_t n = a.n, np = a.n_packed;                          // np: a multiple of kKC
+    const float2* G = a.G + (uint64_t)k * g_stride_inst;
+    const float* src_amp = a.sources + 6;
+    float2 p[RPT], g[RPT], ng[RPT];
+    float nq0 = 0.f, rmask[RPT];
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        p[r] = make_float2(0.f, 0.f);
+        rmask[r] = rg + 4 * r < (int)M ? 1.f : 0.f;
+    }
+    auto fetch = [&](uint32_t j) {
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+            const uint32_t i = rg + 4 * r;
+            ng[r] = G[(uint64_t)(i < M ? i : M - 1) * np + j];
+        }
+        nq0 = src_amp[AFH_SOURCE_STRIDE * (uint64_t)(j < n ? j : n - 1)];
+    };
+    for (uint32_t pass = 0; pass <= iters; ++pass) {
+        const bool first = pass == 0, last = pass == iters;
+        float2 gam[RPT];
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) gam[r] = make_float2(0.f, 0.f);
+        fetch((16 * w < np ? 16 * w : 0) + c);  // np may be 32: waves 2 and 3 then have no group
+        for (uint32_t g0 = 16 * w; g0 < np; g0 += 64) {
+            const uint32_t j = g0 + c;
+#pragma unroll
+            for (int r = 0; r < RPT; ++r) g[r] = make_float2(ng[r].x * rmask[r], ng[r].y * rmask[r]);
+            const float q0 = nq0 * (j < n ? 1.f : 0.f);
+            fetch((g0 + 64 < np ? g0 + 64 : g0) + c);
+            float2 q = make_float2(q0, 0.f);
+            if (!first) {
+                float2 xi = make_float2(0.f, 0.f);
+#pragma unroll
+                for (int r = 0; r < RPT; ++r) {
+                    xi.x = fmaf(g[r].x, p[r].x, fmaf(g[r].y, p[r].y, xi.x));
+                    xi.y = fmaf(g[r].x, p[r].y, fmaf(-g[r].y, p[r].x, xi.y));
+                }
+                xi.x += __shfl_xor(xi.x, 16);
+                xi.y += __shfl_xor(xi.y, 16);
+                xi.x += __shfl_xor(xi.x, 32);
+                xi.y += __shfl_xor(xi.y, 32);
+                const float inv = q0 / sqrtf(xi.x * xi.x + xi.y * xi.y);
+                q = j < n ? make_float2(xi.x * inv, xi.y * inv) : make_float2(0.f, 0.f);
+                if (last) {
+                    if (rg == 0 && j < n) {
+                        a.out_amp[(uint64_t)b * n + j] = hypotf(q.x, q.y);
+                        a.out_phase[(uint64_t)b * n + j] = atan2f(q.y, q.x);
+                    }
+                    continue;
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < RPT; ++r) {
+                gam[r].x = fmaf(g[r].x, q.x, fmaf(-g[r].y, q.y, gam[r].x));
+                gam[r].y = fmaf(g[r].x, q.y, fmaf(g[r].y, q.x, gam[r].y));
+            }
+        }
+        if (last) break;
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+#pragma unroll
+            for (int off = 1; off < 16; off <<= 1) {
+                gam[r].x += __shfl_xor(gam[r].x, off);
+                gam[r].y += __shfl_xor(gam[r].y, off);
+            }
+        }
+        if (c == 0) {
+#pragma unroll
+            for (int r = 0; r < RPT; ++r) s_gam[w][rg + 4 * r] = gam[r];
+        }
+        __syncthreads();
+        if (t < (int)M) {
+            const float2 g0v = s_gam[0][t], g1v = s_gam[1][t], g2v = s_gam[2][t], g3v = s_gam[3][t];
+            const float2 gm = make_float2(((g0v.x + g1v.x) + g2v.x) + g3v.x, ((g0v.y + g1v.y) + g2v.y) + g3v.y);
+            const float inv = a.amps[base + t] / sqrtf(gm.x * gm.x + gm.y * gm.y);
+            s_p[t] = make_float2(gm.x * inv, gm.y * inv);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+            const uint32_t i = rg + 4 * r;
+            p[r] = i < M ? s_p[i] : make_float2(0.f, 0.f);
+        }
+    }
+}
+
 // ---------------------------------------------------------------- naive
 template <int SRC>
 __global__ __launch_bounds__(kHB) void naive_q(HoloArgs a) {
@@ -1278,7 +1387,9 @@ static hipError_t launch_gs_bucket(HoloArgs a, int src, uint32_t nb, uint32_t it
         hipLaunchKernelGGL((gen_g<kT4010A1>), ggrid, dim3(kHB), 0, st, a, g_stride);
     else
         hipLaunchKernelGGL((gen_g<kSphere>), ggrid, dim3(kHB), 0, st, a, g_stride);
-    if (a.n_packed <= kGsLdsQ)
+    if constexpr (MP <= 64)
+        hipLaunchKernelGGL((gs_fused<MP>), dim3(nb), dim3(kHB), 0, st, a, iters, g_stride);
+    else if (a.n_packed <= kGsLdsQ)
         hipLaunchKernelGGL((gs_iter<MP, true>), dim3(nb), dim3(kHB), 0, st, a, iters, g_stride);
     else
         hipLaunchKernelGGL((gs_iter<MP, false>), dim3(nb), dim3(kHB), 0, st, a, iters, g_stride);
