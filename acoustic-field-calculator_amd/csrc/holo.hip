@@ -1079,39 +1079,40 @@ __global__ __launch_bounds__(kHB) void gs_iter(HoloArgs a, uint32_t iters, uint3
 // alone, so the pass that forms xi_t = G^H p_t column by column also forms
 // gamma_{t+1} = G q_{t+1} from the same loaded values of G. Pass 0 forms
 // gamma_0 = G q0; pass t + 1 forms xi_t, q_{t+1} and, before the last, gamma_{t+1};
-// the last writes |q| and arg q. Wave w takes the 16-column groups w, w + 4,
-// ...; lane (rg, c) = (l / 16, l % 16) holds rows rg + 4 r of column 16 g + c,
-// the next group's loaded during this one. Every load is issued unconditionally
-// and its value kept as loaded (rows >= M read row M - 1, columns >= n source
-// n - 1, past the last group the current one is read again); the 0 / 1 masks
-// multiply the values where they are used. A select right after the load let
-// the compiler move the load under a branch and wait for it there. xi_j sums the 4 row
-// groups by lane exchange; gamma's partials are summed over a wave's 16
-// columns by lane exchange and over the 4 waves in LDS, in fixed order.
-template <int MP>
-__global__ __launch_bounds__(kHB) void gs_fused(HoloArgs a, uint32_t iters, uint32_t g_stride_inst) {
+// the last writes |q| and arg q. W waves per instance (more for small MP, so
+// that a bucket of a few hundred instances still fills the CUs); wave w takes
+// the 16-column groups w, w + W, ...; lane (rg, c) = (l / 16, l % 16) holds
+// rows rg + 4 r of column 16 g + c, the next group's loaded during this one.
+// G is read through a buffer descriptor of M rows, so rows >= M read as 0
+// without a memory access. Every load is issued unconditionally and its value
+// kept as loaded (columns >= n read source n - 1, past the last group the
+// current one is read again; the 0 / 1 mask multiplies q0 where it is used):
+// a select right after a load let the compiler move the load under a branch
+// and wait for it there. xi_j sums the 4 row groups by lane exchange; gamma's
+// partials are summed over a wave's 16 columns by lane exchange and over the W
+// waves in LDS, in fixed order.
+template <int MP, int W>
+__global__ __launch_bounds__(64 * W) void gs_fused(HoloArgs a, uint32_t iters, uint32_t g_stride_inst) {
     constexpr int RPT = MP / 4;
-    __shared__ float2 s_gam[4][MP];
+    __shared__ float2 s_gam[W][MP];
     __shared__ float2 s_p[MP];
     const int t = threadIdx.x, w = t >> 6, l = t & 63, rg = l >> 4, c = l & 15;
     const uint32_t k = blockIdx.x, b = a.inst[k];
-    const uint32_t base = a.offsets[b], M = a.offsets[b + 1] - base;  // M >= 1
+    const uint32_t base = a.offsets[b], M = a.offsets[b + 1] - base;  // 1 <= M <= MP
     const uint32_t n = a.n, np = a.n_packed;                          // np: a multiple of kKC
-    const float2* G = a.G + (uint64_t)k * g_stride_inst;
+    // rows < M of the instance's G: M np 8 bytes (MP np 8 < 2^32 is checked at launch)
+    const __amdgpu_buffer_rsrc_t gr = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<float2*>(a.G + (uint64_t)k * g_stride_inst), (short)0, (int)(M * np * 8), 0x00020000);
     const float* src_amp = a.sources + 6;
     float2 p[RPT], g[RPT], ng[RPT];
-    float nq0 = 0.f, rmask[RPT];
+    float nq0 = 0.f;
 #pragma unroll
-    for (int r = 0; r < RPT; ++r) {
-        p[r] = make_float2(0.f, 0.f);
-        rmask[r] = rg + 4 * r < (int)M ? 1.f : 0.f;
-    }
+    for (int r = 0; r < RPT; ++r) p[r] = make_float2(0.f, 0.f);
     auto fetch = [&](uint32_t j) {
 #pragma unroll
-        for (int r = 0; r < RPT; ++r) {
-            const uint32_t i = rg + 4 * r;
-            ng[r] = G[(uint64_t)(i < M ? i : M - 1) * np + j];
-        }
+        for (int r = 0; r < RPT; ++r)
+            ng[r] = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(
+                                                   gr, (int)(((rg + 4 * r) * np + j) * 8), 0, 0));
         nq0 = src_amp[AFH_SOURCE_STRIDE * (uint64_t)(j < n ? j : n - 1)];
     };
     for (uint32_t pass = 0; pass <= iters; ++pass) {
@@ -1119,13 +1120,13 @@ __global__ __launch_bounds__(kHB) void gs_fused(HoloArgs a, uint32_t iters, uint
         float2 gam[RPT];
 #pragma unroll
         for (int r = 0; r < RPT; ++r) gam[r] = make_float2(0.f, 0.f);
-        fetch((16 * w < np ? 16 * w : 0) + c);  // np may be 32: waves 2 and 3 then have no group
-        for (uint32_t g0 = 16 * w; g0 < np; g0 += 64) {
+        fetch((16 * w < np ? 16 * w : 0) + c);  // a wave past np has no group
+        for (uint32_t g0 = 16 * w; g0 < np; g0 += 16 * W) {
             const uint32_t j = g0 + c;
 #pragma unroll
-            for (int r = 0; r < RPT; ++r) g[r] = make_float2(ng[r].x * rmask[r], ng[r].y * rmask[r]);
+            for (int r = 0; r < RPT; ++r) g[r] = ng[r];
             const float q0 = nq0 * (j < n ? 1.f : 0.f);
-            fetch((g0 + 64 < np ? g0 + 64 : g0) + c);
+            fetch((g0 + 16 * W < np ? g0 + 16 * W : g0) + c);
             float2 q = make_float2(q0, 0.f);
             if (!first) {
                 float2 xi = make_float2(0.f, 0.f);
@@ -1169,8 +1170,12 @@ __global__ __launch_bounds__(kHB) void gs_fused(HoloArgs a, uint32_t iters, uint
         }
         __syncthreads();
         if (t < (int)M) {
-            const float2 g0v = s_gam[0][t], g1v = s_gam[1][t], g2v = s_gam[2][t], g3v = s_gam[3][t];
-            const float2 gm = make_float2(((g0v.x + g1v.x) + g2v.x) + g3v.x, ((g0v.y + g1v.y) + g2v.y) + g3v.y);
+            float2 gm = s_gam[0][t];
+#pragma unroll
+            for (int v = 1; v < W; ++v) {
+                gm.x += s_gam[v][t].x;
+                gm.y += s_gam[v][t].y;
+            }
             const float inv = a.amps[base + t] / sqrtf(gm.x * gm.x + gm.y * gm.y);
             s_p[t] = make_float2(gm.x * inv, gm.y * inv);
         }
@@ -1387,9 +1392,17 @@ static hipError_t launch_gs_bucket(HoloArgs a, int src, uint32_t nb, uint32_t it
         hipLaunchKernelGGL((gen_g<kT4010A1>), ggrid, dim3(kHB), 0, st, a, g_stride);
     else
         hipLaunchKernelGGL((gen_g<kSphere>), ggrid, dim3(kHB), 0, st, a, g_stride);
-    if constexpr (MP <= 64)
-        hipLaunchKernelGGL((gs_fused<MP>), dim3(nb), dim3(kHB), 0, st, a, iters, g_stride);
-    else if (a.n_packed <= kGsLdsQ)
+    if constexpr (MP <= 64) {
+        // gs_fused addresses an instance's G with 32-bit buffer offsets
+        if ((uint64_t)MP * a.n_packed * sizeof(float2) <= 0xFFFFFFFFull) {
+            // waves per instance: 16 at MP = 16 (1.52 against 2.56 ms with 4 for a
+            // bucket of 253 instances); 4 at MP = 32 (8 measured slower)
+            constexpr int W = MP == 16 ? 16 : 4;
+            hipLaunchKernelGGL((gs_fused<MP, W>), dim3(nb), dim3(64 * W), 0, st, a, iters, g_stride);
+            return hipGetLastError();
+        }
+    }
+    if (a.n_packed <= kGsLdsQ)
         hipLaunchKernelGGL((gs_iter<MP, true>), dim3(nb), dim3(kHB), 0, st, a, iters, g_stride);
     else
         hipLaunchKernelGGL((gs_iter<MP, false>), dim3(nb), dim3(kHB), 0, st, a, iters, g_stride);
