@@ -149,14 +149,15 @@ def test_gs_and_naive_batched(afc, oracle, torch_cuda):
     log("case: batched GS / Naive on the 249 array")
     src, k, a = setup_array(oracle)
     rng = np.random.default_rng(3)
-    ms = np.array([2, 7, 20], np.int64)
-    offs = np.zeros(4, np.uint32)
+    # M = 80 pads to 128 foci, which GS runs on its two-pass kernel
+    ms = np.array([2, 7, 20, 80], np.int64)
+    offs = np.zeros(len(ms) + 1, np.uint32)
     offs[1:] = np.cumsum(ms)
     foci = random_foci(rng, int(offs[-1]))
     amps = rng.uniform(0.5, 1.0, int(offs[-1])).astype(np.float32)
     for algo, ref in ((1, oracle.gs), (2, oracle.naive)):
         amp, ph = run_batch(afc, torch_cuda, oracle, src, k, a, foci, amps, offs, algo, 0)
-        for b in range(3):
+        for b in range(len(ms)):
             f, am = foci[offs[b]:offs[b + 1]], amps[offs[b]:offs[b + 1]]
             a64, p64 = ref(1, src, k, a, f, am, "f64")
             got = focal_amps(oracle, src, k, a, f, amp[b], ph[b])
@@ -165,8 +166,8 @@ def test_gs_and_naive_batched(afc, oracle, torch_cuda):
 
 
 def test_gs_large_array(afc, oracle, torch_cuda):
-    """GS on 4980 sources (5x4 tiles): more than 4096 padded sources, so the
-    iterate q stays in the workspace instead of LDS."""
+    """GS on 4980 sources (5x4 tiles): more than 4096 padded sources, past
+    the size up to which the two-pass kernel keeps q in LDS."""
     log("case: batched GS on the 4980-source array")
     src, k, a = setup_array(oracle, (5, 4))
     rng = np.random.default_rng(5)
