@@ -46,12 +46,22 @@ def cpu_threads():
     return max(1, n)
 
 
+def cpu_leg_oracle():
+    """The cpu_baseline leg's oracle: the CPU restatement of the reference under
+    oracle/ (test infrastructure). This file uses it for two things only, both
+    outside every timed GPU region: the CPU baselines it times, and the FP64
+    references it checks the GPU results against on samples of the same
+    workloads. Every input comes from afc.workloads."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    return oracle
+
+
 def cpu_baseline(src, k, a, order, counts, origin, res):
     """The reference CPU path (FP32, restated in C, OpenMP over points) on whole
     z-slices of the same volume, until ~10 s of wall time or 8 slices."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
-
+    O = cpu_leg_oracle()
     threads = cpu_threads()
     nx, ny = counts[0], counts[1]
     done_pts, t_total, slices = 0, 0.0, 0
@@ -120,10 +130,13 @@ def main():
     import afc
     from afc import device as D
 
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O  # inputs are built with the shared workload helpers; accuracy check only
+    from afc import workloads as W
 
-    src, k, a, area = O.volume_3984(seed=2)
+    # the oracle comes in through the cpu_baseline leg only, as the checker and
+    # the timed CPU baseline, never for inputs or in a timed GPU region
+    O = cpu_leg_oracle()
+
+    src, k, a, area = W.volume_3984(seed=2)
     n = src.shape[0]
     ps, cnt = D.grid_from_area(area["order"], area["ranges"], area["res"])
     n_pts = cnt[0] * cnt[1] * cnt[2]
@@ -217,7 +230,7 @@ def main():
     if not args.no_greedy:
         result["greedy"] = bench_greedy(torch, D, O)
     if not args.no_serve:
-        result["serve"] = bench_serve(D, O, [float(q) for q in args.serve_qps.split(",")], args.serve_seconds,
+        result["serve"] = bench_serve(D, [float(q) for q in args.serve_qps.split(",")], args.serve_seconds,
                                       args.serve_gspat_cus)
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
@@ -225,24 +238,22 @@ def main():
     print(json.dumps(result), flush=True)
 
 
-def bench_serve(D, O, qps_list, seconds, gspat_cus=32):
+def bench_serve(D, qps_list, seconds, gspat_cus=32):
     """Served mixed stream: Poisson arrivals at a fixed QPS, 70 % field
     evaluations (grids of 1e4..1e6 points, log-uniform, 1 mm, complex output)
     and 30 % GS-PAT solves (2..128 foci), each on one of the 249-, 996- or
     3984-transducer arrays chosen uniformly; random seeded phases and targets."""
+    from afc import workloads as W
+
     import queue
     import threading
     from afc.server import Server
 
     arrs = []
     for tiles in ((1, 1), (2, 2), (4, 4)):
-        pos = O.array_249() if tiles == (1, 1) else O.tiled_array(*tiles)
-        rng = np.random.default_rng(len(pos))
-        src = O.t4010a1_sources(pos, phase=rng.uniform(0, 2 * np.pi, len(pos)).astype(np.float32))
-        n = len(pos)
-        k = np.full(n, O.WAVENUM_40K, np.float32)
-        a = np.full(n, np.float32(O.attenuation_coef(40e3, 30, 1, 1, O.TEMP_340)), np.float32)
-        ext = pos.max(0)
+        n = 249 * tiles[0] * tiles[1]
+        src, k, a = W.array_rows(tiles, phase_seed=n)
+        ext = src[:, 0:3].max(0)
         arrs.append((src, k, a, ext))
     out = {"workload": "Poisson arrivals; 70% field (1e4..1e6 pts log-uniform, complex) / 30% GS-PAT (M~U[2,128], "
                        "MIXED_SPLIT); arrays 249/996/3984 uniform", "gspat_cus": gspat_cus, "runs": []}
@@ -332,13 +343,12 @@ def bench_greedy(torch, D, O):
     """Batched greedy phase search (combination/greedy.rs:23-83): 256
     instances on the 996-transducer array, M ~ U[2,16] foci drawn as in the
     GS-PAT workload, 16 phase candidates (the reference's default)."""
+    from afc import workloads as W
+
     dev = torch.device("cuda:0")
-    pos = O.tiled_array(2, 2)
-    src = O.t4010a1_sources(pos)
-    n = pos.shape[0]
-    k = np.full(n, O.WAVENUM_40K, np.float32)
-    a = np.full(n, np.float32(O.attenuation_coef(40e3, 30, 1, 1, O.TEMP_340)), np.float32)
-    foci, amps, offs = O.gspat_batch(256, m_range=(2, 16), seed=8)
+    src, k, a = W.array_rows((2, 2))
+    n = src.shape[0]
+    foci, amps, offs = W.focus_batch(256, m_range=(2, 16), seed=8)
     packed = D.pack_sources(1, src, k, a)
     src_d = torch.from_numpy(packed).to(dev)
     foci_d = torch.from_numpy(foci).to(dev)
@@ -379,15 +389,14 @@ def bench_lm(torch, D, O, cpu=True):
     """Batched FP64 LM (lm.rs:40-105): 64 instances on the 249- and on the
     996-transducer array, 16 foci each (as the GS-PAT workload), targets 1,
     50 iterations."""
+    from afc import workloads as W
+
     out = {}
     dev = torch.device("cuda:0")
     for name, tiles in (("n249", (1, 1)), ("n996", (2, 2))):
-        pos = O.array_249() if tiles == (1, 1) else O.tiled_array(*tiles)
-        src = O.t4010a1_sources(pos)
-        n = pos.shape[0]
-        k = np.full(n, O.WAVENUM_40K, np.float32)
-        a = np.full(n, np.float32(O.attenuation_coef(40e3, 30, 1, 1, O.TEMP_340)), np.float32)
-        foci, amps, offs = O.gspat_batch(64, m_range=(16, 16), seed=4)
+        src, k, a = W.array_rows(tiles)
+        n = src.shape[0]
+        foci, amps, offs = W.focus_batch(64, m_range=(16, 16), seed=4)
         amps = np.ones_like(amps)
         packed = D.pack_sources(1, src, k, a)
         src_d = torch.from_numpy(packed).to(dev)
@@ -505,8 +514,10 @@ def bench_capi_focus_plane_249(afc, O, cpu=True):
     synchronous call: the sources packed and uploaded, the field computed on the
     GPU, |p| copied back), beside the CPU restatement of the reference's
     calculator on the same volume."""
-    src, k, a, area = O.focus_plane_249()
-    system = afc.UniformSystem(O.TEMP_340)
+    from afc import workloads as W
+
+    src, k, a, area = W.focus_plane_249()
+    system = afc.UniformSystem(W.TEMP_340)
     for row in src:
         system.add_wave_source(afc.T4010A1(row[0:3], row[3:6], row[6], row[7], row[8]))
     grid = afc.GridAreaBuilder().x_range((40, 139)).y_range((20, 119)).z_at(150).resolution(1.0).generate()
@@ -544,10 +555,12 @@ def bench_capi_focus_plane_249(afc, O, cpu=True):
 
 def bench_gspat(torch, afc, D, O, src, k, a, batch, cpu=True):
     """Batched GS-PAT (gs_pat.rs:38-81) on device-resident inputs."""
+    from afc import workloads as W
+
     src = src.copy()
     src[:, 7] = 0.0  # GS-PAT zeroes the phases (gs_pat.rs:39-41)
     n = src.shape[0]
-    foci, amps, offs = O.gspat_batch(batch, m_range=(2, 64), seed=3)
+    foci, amps, offs = W.focus_batch(batch, m_range=(2, 64), seed=3)
     dev = torch.device("cuda:0")
     packed = D.pack_sources(1, src, k, a)
     src_d = torch.from_numpy(packed).to(dev)
@@ -629,7 +642,7 @@ def bench_gspat(torch, afc, D, O, src, k, a, batch, cpu=True):
             f"max focal-amplitude rel err vs the FP64 run {max(errs):.2e}")
     ctx.close()
     # the plain FP32 path: AFO_GSPAT, one instance per call through the C ABI
-    system = afc.UniformSystem(O.TEMP_340)
+    system = afc.UniformSystem(W.TEMP_340)
     for row in src:
         system.add_wave_source(afc.T4010A1(row[0:3], row[3:6], row[6], row[7], row[8]))
     import time as _t
