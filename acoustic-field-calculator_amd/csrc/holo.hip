@@ -266,28 +266,40 @@ struct StoredChunk {
     float d, la;  // la (H16): log2 |amp| of the column's source, which sets its FP16 scale
 };
 
+// cache-policy operand of a buffer load: bit 1 is the non-temporal hint (nt)
+constexpr int kNonTemporal = 2;
+
 template <int MP, bool H16>
-__device__ __forceinline__ void load_chunk(const HoloArgs& a, uint32_t k, uint32_t j0, uint32_t M,
-                                           StoredChunk<MP, H16>& c) {
+__device__ __forceinline__ void load_chunk(const HoloArgs& a, __amdgpu_buffer_rsrc_t gr, uint32_t k, uint32_t j0,
+                                           uint32_t M, StoredChunk<MP, H16>& c) {
     const int t = threadIdx.x;
-    const uint64_t e0 = (uint64_t)k * MP * a.n_packed + j0 + (t & 31);
+    const uint32_t col = j0 + (t & 31);
+    if constexpr (H16) {
+        // every load unconditional: the descriptor ends at row M (rows >= M
+        // read as 0 without a memory access), and la is read by every thread
+        // (only t < kKC use it); with a select per load and the ring rotated
+        // by copies, the compiler waited for each chunk's loads at once
 #pragma unroll
-    for (int r = 0; r < MP / 8; ++r) {
-        // selects, not branches: a branch around each load made the compiler
-        // wait for it inside the branch
-        const int i = (t >> 5) + 8 * r;
-        if constexpr (H16) {
-            c.g[r] = i < (int)M ? __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(a.G) + e0 +
-                                                             (uint64_t)i * a.n_packed)
-                                : 0u;
-        } else {
-            const float2* G = a.G + e0 + (uint64_t)i * a.n_packed;
-            c.g[r] = i < (int)M ? make_float2(__builtin_nontemporal_load(&G->x), __builtin_nontemporal_load(&G->y))
+        for (int r = 0; r < MP / 8; ++r) {
+            const uint32_t e = (uint32_t)((t >> 5) + 8 * r) * a.n_packed + col;
+            c.g[r] = __builtin_amdgcn_raw_buffer_load_b32(gr, (int)(e * 4), 0, kNonTemporal);
+        }
+        c.d = a.d[(uint64_t)k * a.n_packed + col];
+        c.la = a.staged[3 * (uint64_t)col].w;
+    } else {
+        // FP32 G, one chunk ahead: rows >= M are skipped (measured faster here
+        // than the bounded buffer loads, which still issue those rows)
+        const float2* G = a.G + (uint64_t)k * MP * a.n_packed + col;
+#pragma unroll
+        for (int r = 0; r < MP / 8; ++r) {
+            const int i = (t >> 5) + 8 * r;
+            const float2* g = G + (uint64_t)i * a.n_packed;
+            c.g[r] = i < (int)M ? make_float2(__builtin_nontemporal_load(&g->x), __builtin_nontemporal_load(&g->y))
                                 : make_float2(0.f, 0.f);
         }
+        c.d = t < kKC ? a.d[(uint64_t)k * a.n_packed + col] : 0.f;
+        c.la = 0.f;
     }
-    c.d = t < kKC ? a.d[(uint64_t)k * a.n_packed + j0 + t] : 0.f;
-    c.la = (H16 && t < kKC) ? a.staged[3 * (uint64_t)(j0 + t)].w : 0.f;
 }
 
 // The chunk's G tile in FP32 (H16 words widened).
@@ -902,19 +914,15 @@ __global__ __launch_bounds__(kHB) void gspat_q(HoloArgs a, int closing, int pass
     // 4 bytes per element, so it keeps three chunks in flight to cover the
     // memory latency; FP32 G (17 words, 8 bytes) keeps one
     constexpr int PD = H16 ? 3 : 1;
-    StoredChunk<MP, H16> cq[PD];
-#pragma unroll
-    for (int u = 0; u < PD; ++u)
-        if (c_begin + u < c_end) load_chunk<MP, H16>(a, k, (c_begin + u) * kKC, M, cq[u]);
-    for (uint32_t chunk = c_begin; chunk < c_end; ++chunk) {
+    constexpr uint32_t kElem = H16 ? 4 : 8;
+    // H16: rows < M of the instance's G words (MP n_packed kElem < 2^32 is checked on the host)
+    const __amdgpu_buffer_rsrc_t gr = __builtin_amdgcn_make_buffer_rsrc(
+        reinterpret_cast<char*>(a.G) + (uint64_t)k * MP * a.n_packed * kElem, (short)0,
+        (int)(M * a.n_packed * kElem), 0x00020000);
+    // one chunk of columns from its G tile (FP32, H16 words widened), d_j and la_j
+    auto run_chunk = [&](const float2 (&g)[MP / 8], float dj, float laj, uint32_t chunk) {
         const uint32_t j0 = chunk * kKC;
         const int pb = (chunk - c_begin) & 1, slot = (chunk - c_begin) % 8;
-        float2 g[MP / 8];
-        chunk_g<MP, H16>(cq[0], g);
-        const float dj = cq[0].d, laj = cq[0].la;
-#pragma unroll
-        for (int u = 0; u + 1 < PD; ++u) cq[u] = cq[u + 1];
-        if (chunk + PD < c_end) load_chunk<MP, H16>(a, k, (chunk + PD) * kKC, M, cq[PD - 1]);
         // q_j = w_j sum_i conj(G_ij) p_i: this row group's partial, then the 8
         // groups in a fixed order; w_j = 1 / d_j^2 (FP32 G) or 2^-s_j / d_j^2 (H16)
         float2 pq = make_float2(0.f, 0.f);
@@ -963,6 +971,38 @@ __global__ __launch_bounds__(kHB) void gspat_q(HoloArgs a, int closing, int pass
                 gam[r].x = fmaf(g[r].x, qj.x, fmaf(-g[r].y, qj.y, gam[r].x));
                 gam[r].y = fmaf(g[r].x, qj.y, fmaf(g[r].y, qj.x, gam[r].y));
             }
+        }
+    };
+    if constexpr (H16) {
+        // slot u of the ring always holds chunks c_begin + u (mod PD): the
+        // loop is unrolled over the slots, so no loaded register is copied to
+        // another before its use; past the range a slot re-reads the last chunk
+        auto clamp_chunk = [&](uint32_t c) { return c < c_end ? c : c_end - 1; };
+        StoredChunk<MP, H16> cq[PD];
+        if (c_begin < c_end) {
+#pragma unroll
+            for (int u = 0; u < PD; ++u) load_chunk<MP, H16>(a, gr, k, clamp_chunk(c_begin + u) * kKC, M, cq[u]);
+        }
+        for (uint32_t c0 = c_begin; c0 < c_end; c0 += PD) {
+#pragma unroll
+            for (int u = 0; u < PD; ++u) {
+                const uint32_t chunk = c0 + u;
+                float2 g[MP / 8];
+                chunk_g<MP, H16>(cq[u], g);
+                const float dj = cq[u].d, laj = cq[u].la;
+                load_chunk<MP, H16>(a, gr, k, clamp_chunk(chunk + PD) * kKC, M, cq[u]);
+                if (chunk < c_end) run_chunk(g, dj, laj, chunk);  // wave-uniform
+            }
+        }
+    } else {
+        StoredChunk<MP, H16> cq;
+        if (c_begin < c_end) load_chunk<MP, H16>(a, gr, k, c_begin * kKC, M, cq);
+        for (uint32_t chunk = c_begin; chunk < c_end; ++chunk) {
+            float2 g[MP / 8];
+            chunk_g<MP, H16>(cq, g);
+            const float dj = cq.d, laj = cq.la;
+            if (chunk + 1 < c_end) load_chunk<MP, H16>(a, gr, k, (chunk + 1) * kKC, M, cq);
+            run_chunk(g, dj, laj, chunk);
         }
     }
     if (pass == kQGamma) {
@@ -1539,6 +1579,11 @@ int32_t holo_solve(Context* ctx, const HoloBatch& hb, hipStream_t st) {
         a.gam = reinterpret_cast<float2*>(wbase + o_q) + q_pos;
         hipError_t e = hipSuccess;
         if (hb.algo == kAlgoGSPAT) {
+            // kernel C reads an instance's G through 32-bit buffer offsets
+            if ((uint64_t)mp * np * sizeof(float2) > 0xFFFFFFFFull) {
+                set_error("holo: GS-PAT supports at most 2^32 / (8 * 128) padded sources");
+                return AFH_ERR_UNSUPPORTED;
+            }
             switch (mp) {
                 case 16: e = launch_gspat_bucket<16>(a, hb.source_type, hb.precision, nb, hb.iterations, st); break;
                 case 32: e = launch_gspat_bucket<32>(a, hb.source_type, hb.precision, nb, hb.iterations, st); break;
