@@ -149,8 +149,9 @@ def test_gs_and_naive_batched(afc, oracle, torch_cuda):
     log("case: batched GS / Naive on the 249 array")
     src, k, a = setup_array(oracle)
     rng = np.random.default_rng(3)
-    # M = 80 pads to 128 foci, which GS runs on its two-pass kernel
-    ms = np.array([2, 7, 20, 80], np.int64)
+    # M = 40 pads to 64 foci; M = 80 pads to 128, which GS runs on its
+    # two-pass kernel
+    ms = np.array([2, 7, 20, 40, 80], np.int64)
     offs = np.zeros(len(ms) + 1, np.uint32)
     offs[1:] = np.cumsum(ms)
     foci = random_foci(rng, int(offs[-1]))
