@@ -3,9 +3,13 @@
 Mirrors python/afc/afc.py:30-567 (class names, methods, argument meaning): a
 script written for the reference's `afc` package runs against this backend by
 importing this package instead. Differences:
-  * ``CpuCalculator`` is the reference's CPU path and is not provided here;
-    constructing it raises. ``GpuCalculator`` and ``AccurateCalculator`` run
-    on the GPU, and ``HipCalculator`` is this backend's own tag (3).
+  * Every calculator runs on the GPU: ``CpuCalculator`` (tag 0),
+    ``GpuCalculator`` (2) and this backend's own ``HipCalculator`` (3) take
+    the FAST kernels, ``AccurateCalculator`` (1) the ACCURATE ones. There is
+    no CPU path.
+  * ``Optimizer.ifft`` (an image read and an FFT, outside the hot path)
+    raises ``AFCError`` ("not provided") and leaves the system as it was.
+  * ``afc.plot_helper`` (matplotlib plotting) is not part of this package.
   * Failures raise ``AFCError`` with the library's message instead of
     aborting the process.
   * There is no binary download: the library is built in-tree
@@ -151,7 +155,7 @@ class Calculator:
 
 
 class CpuCalculator(Calculator):
-    """The reference's CPU calculator (tag 0): not part of this GPU backend."""
+    """The reference's CPU calculator tag (0); computed by the GPU FAST kernels here."""
 
     @staticmethod
     def get_type():
@@ -383,12 +387,22 @@ class Optimizer:
         _check("AFO_FocalPoint")
 
     @staticmethod
-    def bessel_beam(system: UniformSystem, point, direction, theta_z: float):
-        """Bessel beam phases (acoustic-field-optimizer/src/bessel.rs:31-55)."""
-        p = point if isinstance(point, Vector3) else Vector3(point)
+    def bessel(system: UniformSystem, p, direction, theta_z: float):
+        """Bessel beam phases (python/afc/afc.py:427, acoustic-field-optimizer/src/bessel.rs:31-55)."""
+        p = p if isinstance(p, Vector3) else Vector3(p)
         d = direction if isinstance(direction, Vector3) else Vector3(direction)
         dll().AFO_BesselBeam(system.handle, p, d, c_float(theta_z), system._source_type)
         _check("AFO_BesselBeam")
+
+    bessel_beam = bessel
+
+    @staticmethod
+    def ifft(system: UniformSystem, path, bottom_left, top_left, bottom_right, spacing, z):
+        """python/afc/afc.py:435; not provided by this backend: raises AFCError."""
+        args = [v if isinstance(v, Vector3) else Vector3(v) for v in (bottom_left, top_left, bottom_right)]
+        dll().AFO_IFFT(system.handle, str(path).encode("utf-8"), *args, c_float(spacing), c_float(z),
+                       system._source_type)
+        _check("AFO_IFFT")
 
     @staticmethod
     def _run(name, system, foci, amps):
@@ -426,8 +440,11 @@ class Optimizer:
         Optimizer._run("AFO_Horn", system, foci, amps)
 
     @staticmethod
-    def long(system: UniformSystem, foci, amps):
+    def long2014(system: UniformSystem, foci, amps):
+        """python/afc/afc.py:492."""
         Optimizer._run("AFO_Long", system, foci, amps)
+
+    long = long2014
 
     @staticmethod
     def apo(system: UniformSystem, foci, amps, lambda_reg: float):

@@ -74,6 +74,9 @@ class T4010A1(Structure):
     def direction(self, d):
         self.nx, self.ny, self.nz = (float(v) for v in d)
 
+    # the reference binding's getter is named `dir` (python/afc/nativemethods.py:86-88)
+    dir = property(lambda self: (self.nx, self.ny, self.nz))
+
     @staticmethod
     def get_type():
         return 1

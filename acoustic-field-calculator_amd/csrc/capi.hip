@@ -233,9 +233,11 @@ const char* AFC_GetLastError(void) { return g_last_error.c_str(); }
 void AFC_CreateCalculator(void** out, int32_t calc_type) {
     clear_error();
     if (!out) return;
-    if (calc_type < AFC_CALC_ACCURATE || calc_type > AFC_CALC_HIP) {
-        set_error(calc_type == AFC_CALC_CPU ? "CpuCalculator is not provided by libafc_hip (GPU backend only)"
-                                            : "unknown calculator type");
+    // CpuCalculator (0) is accepted and runs the same FAST kernels as Gpu (2)
+    // and Hip (3): this library has no CPU path, and the reference's CPU and
+    // Vulkan calculators compute the same field (cpu_calculator.rs:43-65)
+    if (calc_type < AFC_CALC_CPU || calc_type > AFC_CALC_HIP) {
+        set_error("unknown calculator type");
         *out = nullptr;
         return;
     }
@@ -266,8 +268,8 @@ uint64_t AFC_Calculate(void* calc_handle, void* system_handle, void* obs_handle,
         set_error("AFC_Calculate: null handle");
         return 0;
     }
-    if (calc_type < AFC_CALC_ACCURATE || calc_type > AFC_CALC_HIP || calc_type != calc->calc_type) {
-        set_error("AFC_Calculate: calculator type not provided by this library or not matching the handle");
+    if (calc_type < AFC_CALC_CPU || calc_type > AFC_CALC_HIP || calc_type != calc->calc_type) {
+        set_error("AFC_Calculate: unknown calculator type or not matching the handle");
         return 0;
     }
     if (source_type != sys->source_type || (source_type != kSphere && source_type != kT4010A1)) {

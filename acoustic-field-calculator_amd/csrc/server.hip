@@ -110,7 +110,7 @@ struct Request {
     size_t pin_off = 0;     // this request's results in the pinned arena
     size_t pin_off2 = 0;    // GS-PAT: its phases
     double t_submit = 0, t_done = 0;
-    int state = kPending;
+    std::atomic<int> state{kPending};  // written by the dispatcher and copy threads, read by AFS_Wait
     int32_t status = AFH_OK;
 };
 

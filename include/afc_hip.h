@@ -13,15 +13,17 @@
  * plain pointers and sizes and return an int32 status (AFH_OK on success).
  *
  * Differences from the reference (intentional):
- *   - CalculatorType 0 (CpuCalculator) is not provided by this library: the
- *     CPU path stays in the reference's own library. AFC_Calculate with it
- *     returns 0 and sets the error string. Types 1 (Accurate) and 2 (Gpu) run
- *     on the GPU; type 3 (AFC_CALC_HIP) is this backend's own tag.
+ *   - Every CalculatorType runs on the GPU: 0 (Cpu), 2 (Gpu) and this
+ *     backend's own 3 (AFC_CALC_HIP) take the FAST kernels, 1 (Accurate) the
+ *     ACCURATE ones. There is no CPU path in this library.
  *   - Errors never abort the process: calls that fail record a message
  *     retrievable with AFC_GetLastError() (thread-local).
- *   - Optimizers outside the hot path (BesselBeam, IFFT, Horn, Long, APO,
- *     GaussNewton, GradientDescent, Greedy) are exported so that bindings
- *     load, but they only record "unsupported" and leave the system as is.
+ *   - AFO_IFFT (an image file read and an FFT) is exported so that bindings
+ *     load, but it only records "not provided" and leaves the system as is.
+ *     Every other AFO_* symbol is implemented.
+ *   - Size limits: LM, Gauss-Newton, gradient descent, APO, Long and Horn
+ *     take at most 32 foci; GS, GS-PAT, Naive and Greedy at most 128; one
+ *     AFC_Calculate at most 2^31 observation points.
  */
 #ifndef AFC_HIP_H
 #define AFC_HIP_H

@@ -19,7 +19,7 @@ HEADER = os.path.join(ROOT, "include", "afc_hip.h")
 def declared_functions():
     txt = open(HEADER).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    names = re.findall(r"\b((?:AFC|AFO|AFH)_\w+)\s*\(", txt)
+    names = re.findall(r"\b((?:AFC|AFO|AFH|AFS)_\w+)\s*\(", txt)
     return sorted(set(names))
 
 
@@ -114,16 +114,56 @@ def test_packing_layout(afc, oracle):
     assert np.all(p[n:, 6] == 0)  # padding has zero amplitude
 
 
-def test_cpu_calculator_rejected(afc):
-    with pytest.raises(afc.AFCError, match="CpuCalculator"):
-        afc.CpuCalculator()
+def test_every_calculator_type_constructs(afc):
+    """CalculatorType 0 (Cpu), 1 (Accurate), 2 (Gpu) of type_inference_aux.rs:55-59
+    and this backend's 3 all create a handle; an unknown tag is refused."""
+    for cls in (afc.CpuCalculator, afc.AccurateCalculator, afc.GpuCalculator, afc.HipCalculator):
+        assert cls().handle
+    h = c_void_p()
+    afc.dll().AFC_CreateCalculator(byref(h), 4)
+    assert not h and b"unknown" in afc.dll().AFC_GetLastError()
 
 
-def test_unsupported_optimizer_reports(afc):
+# The public names of the reference's binding: python/afc/__init__.py:20-40
+# (package exports, plot_helper aside) and the methods of python/afc/afc.py
+# (UniformSystem :30-85, Calculator :88-150, GridArea :233-301,
+# GridAreaBuilder :305-416, Optimizer :419-567).
+REF_EXPORTS = ["UniformSystem", "CpuCalculator", "AccurateCalculator", "GpuCalculator", "PressureFieldBuffer",
+               "PowerFieldBuffer", "ComplexPressureFieldBuffer", "ScatterArea", "GridAreaBuilder", "SphereWaveSource",
+               "T4010A1", "Vector3", "Optimizer"]
+REF_METHODS = {
+    "UniformSystem": ["set_source_type", "sound_speed", "info", "info_of_source", "add_wave_source",
+                      "get_wave_sources"],
+    "CpuCalculator": ["calculate", "get_type"],
+    "GridAreaBuilder": ["x_at", "y_at", "z_at", "x_range", "y_range", "z_range", "resolution", "generate"],
+    "ScatterArea": ["add_observe_point", "get_type"],
+    "Optimizer": ["focus", "bessel", "ifft", "gs_pat", "gs", "naive", "horn", "long2014", "apo", "lm", "gauss_newton",
+                  "gradient_descent", "greedy"],
+}
+
+
+def test_mirror_has_every_reference_name(afc):
+    for name in REF_EXPORTS:
+        assert hasattr(afc, name), name
+    for cls, methods in REF_METHODS.items():
+        for m in methods:
+            assert callable(getattr(getattr(afc, cls), m, None)), f"{cls}.{m}"
+    for cls in ("GridArea1D", "GridArea2D", "GridArea3D"):
+        assert callable(getattr(afc, cls).get_size)
+    # struct properties the reference's scripts use (python/main.py:58-63, foci.py:105)
+    t = afc.T4010A1((1, 2, 3), (0, 0, 1), 1.0, 0.5, 40e3)
+    assert tuple(t.pos) == (1, 2, 3) and tuple(t.dir) == (0, 0, 1)
+
+
+def test_ifft_reports_not_provided(afc):
     s = _system(afc, 300.0, [(0, 0, 0)])
+    before = s.source_array().copy()
     v = afc.Vector3((0, 0, 0))
     afc.dll().AFO_IFFT(s.handle, b"image.png", v, v, v, ctypes.c_float(1.0), ctypes.c_float(100.0), 1)
     assert b"not provided" in afc.dll().AFC_GetLastError()
+    with pytest.raises(afc.AFCError, match="not provided"):
+        afc.Optimizer.ifft(s, "star.bmp", v, v, v, 10.18, 150.0)
+    assert np.array_equal(s.source_array(), before)
 
 
 def test_bad_tags_do_not_abort(afc):
@@ -151,7 +191,7 @@ def test_bessel_beam_matches_oracle(afc, oracle, direction, theta):
     pos = oracle.array_249()
     s = _system(afc, oracle.TEMP_340, [tuple(p) for p in pos])
     point = (90.0, 70.0, 0.0)
-    afc.Optimizer.bessel_beam(s, point, direction, theta)
+    afc.Optimizer.bessel(s, point, direction, theta)
     got = s.source_array()[:, 7]
     want = oracle.bessel_phases(1, s.source_array().copy(), s.sound_speed(), point, direction, theta)
     assert np.array_equal(got, want)

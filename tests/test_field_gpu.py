@@ -346,3 +346,36 @@ def test_ragged_point_counts(afc, oracle, torch_cuda, n):
     want = oracle.field(1, src, k, a, pts[sel], "f32")
     assert np.all(np.isfinite(got))
     assert rel_l2(got[sel], want) <= 1e-5
+
+
+def test_volume_3984_full_size(afc, oracle, torch_cuda):
+    """The whole 3984-source x 256x256x128 volume (8 388 608 points, complex
+    output) through AFH_FieldEval: every point finite, and 4096 sampled points
+    (plus the eight corners) within 1e-5 relative L2 of the reference's FP32
+    CPU path and 5e-6 of FP64 (cpu_calculator.rs:43-65, uniform_medium.rs:122-128)."""
+    import torch
+    from afc import device as D
+    src, k, a, area = oracle.volume_3984()
+    packed = D.pack_sources(1, src, k, a)
+    dev = torch.device("cuda:0")
+    src_d = torch.from_numpy(packed).to(dev)
+    ps, cnt = D.grid_from_area(area["order"], area["ranges"], area["res"])
+    n = cnt[0] * cnt[1] * cnt[2]
+    log(f"case: volume_3984 full size, {src.shape[0]} sources x counts {cnt} ({n} points)")
+    assert n == 256 * 256 * 128
+    out = torch.full((n, 2), float("nan"), dtype=torch.float32, device=dev)
+    D.field_eval(src_d.data_ptr(), packed.shape[0], 1, ps, D.FIELD_COMPLEX, D.PREC_FAST, out.data_ptr())
+    torch.cuda.synchronize()
+    assert bool(torch.isfinite(out).all())
+    got_all = out.cpu().numpy().view(np.complex64).reshape(-1)
+    del out
+    pts_all, bounds = oracle.grid_points(area["order"], area["ranges"], area["res"])
+    assert pts_all.shape[0] == n
+    corners = [0, cnt[0] - 1, cnt[0] * (cnt[1] - 1), cnt[0] * cnt[1] - 1]
+    corners += [c + cnt[0] * cnt[1] * (cnt[2] - 1) for c in corners]
+    sel = np.unique(np.concatenate([corners, np.random.default_rng(2).choice(n, 4096, replace=False)]))
+    want32 = oracle.field(1, src, k, a, pts_all[sel], "f32")
+    want64 = oracle.field(1, src, k, a, pts_all[sel], "f64")
+    e32, e64 = rel_l2(got_all[sel], want32), rel_l2(got_all[sel], want64)
+    log(f"  {sel.size} sampled points: rel-L2 vs FP32 CPU path {e32:.2e}, vs FP64 {e64:.2e}")
+    assert e32 <= 1e-5 and e64 <= 5e-6

@@ -51,6 +51,75 @@ def test_gs_keeps_source_amplitudes(oracle):
     assert np.allclose(amp, 1.0, rtol=1e-12)
 
 
+def gs_problem(oracle, m, seed):
+    """small_problem with non-uniform source amplitudes, so that q0 (the current
+    amplitudes, gs.rs:48-54) is not the constant 1."""
+    src, k, a, foci, amps = small_problem(oracle, m=m, seed=seed)
+    src[:, 6] = np.random.default_rng(seed + 100).uniform(0.5, 1.0, src.shape[0]).astype(np.float32)
+    # G as gs.rs builds it: the current amplitudes, phases zeroed (macros.rs:17-53)
+    G = oracle.propagation_matrix(1, with_solution(src, src[:, 6], 0.0), k, a, foci, "f64")
+    return src, k, a, foci, amps, G
+
+
+def gs_single_focus_pin(amp, ph, q0, g, tol):
+    """The M = 1 closed form of gs.rs:56-72: p is a scalar, xi_j = conj(G_j) p, so
+    |q_j| = q0_j and arg q_j = -arg G_j + c for one constant c; every term then
+    arrives at the focus in phase."""
+    c = np.angle(np.exp(1j * (ph + np.angle(g))))
+    spread = np.max(np.abs(np.angle(np.exp(1j * (c - c[0])))))
+    return bool(np.allclose(amp, q0, rtol=tol) and spread <= 10 * tol)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_gs_single_focus_closed_form(oracle, prec):
+    src, k, a, foci, amps, G = gs_problem(oracle, 1, 5)
+    amp, ph = oracle.gs(1, src, k, a, foci, amps, prec)
+    q0 = src[:, 6].astype(np.float64)
+    # f32: G's phase k r (r ~ 200 mm) carries ~1e-5 rad of rounding
+    tol = 1e-5 if prec == "f32" else 1e-12
+    assert gs_single_focus_pin(amp, ph, q0, G[0], tol)
+    # |p(f)| = sum_j q0_j |G_j| with |G_j| the unit-amplitude term's modulus
+    g_unit = oracle.propagation_matrix(1, with_solution(src, 1.0, 0.0), k, a, foci, "f64")[0]
+    p = focal(oracle, with_solution(src, amp, ph), k, a, foci)[0]
+    assert abs(p) == pytest.approx(float(np.sum(q0 * np.abs(g_unit))), rel=max(tol, 1e-6))
+    # the pin rejects what two plausible mistakes would produce: xi without the
+    # conjugate (arg q_j = +arg G_j + c) and q normalised without q0 (|q_j| = 1)
+    assert not gs_single_focus_pin(amp, np.angle(G[0]) + 0.3, q0, G[0], 1e-3)
+    assert not gs_single_focus_pin(np.ones_like(amp), ph, q0, G[0], 1e-3)
+
+
+def gs_ascent_objective(G, p0, q):
+    """max over |p_i| = p0_i of Re <p, G q>, i.e. sum_i p0_i |(G q)_i|."""
+    return np.sum(p0 * np.abs(G @ q))
+
+
+@pytest.mark.parametrize("m,seed", [(3, 6), (12, 7), (30, 8)])
+def test_gs_is_block_coordinate_ascent(oracle, m, seed):
+    """GS maximises f(p, q) = Re <p, G q> over |p_i| = p0_i and |q_j| = q0_j by
+    blocks: p = gamma / |gamma| p0 is the best p for the current q, and
+    q = xi / |xi| q0 with xi = G^H p the best q for that p (gs.rs:56-69). Hence
+    max_p f(p, q_r) never decreases with the iteration count r. A conjugate
+    dropped from xi, or a transposed G, breaks the ascent."""
+    src, k, a, foci, amps, G = gs_problem(oracle, m, seed)
+    p0 = amps.astype(np.float64)
+    f = []
+    for r in range(0, 21):
+        amp, ph = oracle.gs(1, src, k, a, foci, amps, "f64", repeat=r)
+        f.append(gs_ascent_objective(G, p0, amp * np.exp(1j * ph)))
+    assert np.all(np.diff(f) >= -1e-12 * f[-1]), np.diff(f)
+    assert f[-1] > 2 * f[0]
+    # a planted mistake in a CPU restatement fails the same check
+    q0 = src[:, 6].astype(np.float64)
+    q, bad = q0.astype(np.complex128), []
+    for _ in range(20):
+        g = G @ q
+        p = g / np.abs(g) * p0
+        xi = G.T @ p  # conjugate dropped
+        q = xi / np.abs(xi) * q0
+        bad.append(gs_ascent_objective(G, p0, q))
+    assert np.min(np.diff(bad)) < 0
+
+
 def test_naive_single_focus(oracle):
     src, k, a, foci, amps = small_problem(oracle, m=1, seed=2)
     amp, ph = oracle.naive(1, src, k, a, foci, amps, "f64")

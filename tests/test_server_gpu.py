@@ -17,9 +17,9 @@ def log(*a):
     print(*a, flush=True)
 
 
-def arrays(oracle):
+def arrays(oracle, tiles_list=((1, 1), (2, 2))):
     out = []
-    for tiles in ((1, 1), (2, 2)):
+    for tiles in tiles_list:
         pos = oracle.array_249() if tiles == (1, 1) else oracle.tiled_array(*tiles)
         rng = np.random.default_rng(len(pos))
         src = oracle.t4010a1_sources(pos, phase=rng.uniform(0, 6.28, len(pos)).astype(np.float32))
@@ -129,4 +129,59 @@ def test_small_budget_queues_and_rejects(afc, oracle, torch_cuda):
     ps = D.grid_pointset((0, 1, 2), (100, 100, 100), (30.0, 20.0, 100.0), 1.0)
     with pytest.raises(AFCError, match="budget"):
         srv.submit_field(aid, ps, 2, np.empty(100 * 100 * 100 * 2, np.float32))
+    srv.close()
+
+
+def test_mix_equals_alone_3984_and_large_grids(afc, oracle, torch_cuda):
+    """The isolation check with the three arrays of the served stream (249, 996
+    and 3984 transducers) and field requests of 10^6+ points: every result of
+    a mixed, concurrently submitted stream equals the same request run alone,
+    bit for bit; one 10^6-point result on the 3984 array matches the FP64
+    oracle at sampled points."""
+    from afc.server import Server
+    arrs = arrays(oracle, ((1, 1), (2, 2), (4, 4)))
+    rng = np.random.default_rng(5)
+    reqs = make_requests(rng, len(arrs), 24)
+    big = [("field", 2, (128, 128, 64), (300.0, 220.0, 90.0), 2), ("field", 2, (256, 128, 32), (200.0, 150.0, 60.0), 0),
+           ("field", 1, (100, 100, 101), (60.0, 40.0, 80.0), 1)]
+    reqs = reqs[:8] + big[:1] + reqs[8:16] + big[1:2] + reqs[16:] + big[2:]
+    reqs += [("gspat", 2, np.stack([rng.uniform(0, 730, m), rng.uniform(0, 570, m), rng.uniform(100, 250, m)],
+                                   1).astype(np.float32), rng.uniform(0.5, 1.0, m).astype(np.float32))
+             for m in (2, 64, 128)]
+    log(f"case: server isolation, arrays 249/996/3984, {len(reqs)} requests incl. 3 grids of 10^6+ points")
+    srv = Server(hbm_budget=512 << 20, pinned_budget=256 << 20, n_streams=4, max_gspat_batch=64, gspat_cus=32)
+    ids = [srv.add_array(1, *a) for a in arrs]
+    arr_n = [a[0].shape[0] for a in arrs]
+    alone = []
+    for r in reqs:
+        t, bufs = submit(srv, ids, arr_n, r)
+        srv.wait(t)
+        alone.append(bufs)
+    mixed = [None] * len(reqs)
+    tickets = [None] * len(reqs)
+
+    def worker(idx):
+        for i in idx:
+            tickets[i], mixed[i] = submit(srv, ids, arr_n, reqs[i])
+
+    th = [threading.Thread(target=worker, args=(list(range(j, len(reqs), 3)),)) for j in range(3)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    for i in range(len(reqs)):
+        srv.wait(tickets[i])
+    for i, r in enumerate(reqs):
+        for x, y in zip(alone[i], mixed[i]):
+            assert np.array_equal(x, y), (i, r[0])
+    i = reqs.index(big[0])
+    _, aid, cnt, origin, _ = reqs[i]
+    src, kk, aa = arrs[aid]
+    pts, _ = oracle.grid_points((0, 1, 2), [(origin[a], origin[a] + cnt[a] - 1) for a in range(3)], 1.0)
+    sel = np.random.default_rng(9).choice(pts.shape[0], 2048, replace=False)
+    want = oracle.field(1, src, kk, aa, pts[sel], "f64")
+    got = alone[i][0].view(np.complex64).astype(np.complex128)[sel]
+    err = np.linalg.norm(got - want) / np.linalg.norm(want)
+    log(f"  10^6-point field on the 3984 array: rel-L2 vs FP64 at 2048 points {err:.2e}")
+    assert err <= 5e-6
     srv.close()
