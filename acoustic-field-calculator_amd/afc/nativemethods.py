@@ -164,6 +164,9 @@ def _declare(d):
     _sig(d, "AFH_HoloSolve", [c_void_p, c_void_p, c_uint64, c_uint64, ctypes.c_int32, c_void_p, c_void_p, c_void_p,
                               c_uint32, ctypes.c_int32, ctypes.c_int32, c_uint32, c_void_p, c_void_p, c_void_p],
          ctypes.c_int32)
+    _sig(d, "AFH_HoloSolveNoAlloc", [c_void_p, c_uint64, c_uint64, ctypes.c_int32, c_void_p, c_void_p, c_void_p,
+                                     c_uint32, ctypes.c_int32, ctypes.c_int32, c_uint32, c_void_p, c_void_p, c_void_p],
+         ctypes.c_int32)
     _sig(d, "AFH_LMSolve", [c_void_p, c_void_p, c_uint64, c_uint64, ctypes.c_int32, c_void_p, c_void_p, c_void_p,
                             c_uint32, c_uint32, c_void_p, c_void_p, c_void_p], ctypes.c_int32)
     _sig(d, "AFH_NLSolve", [c_void_p, c_void_p, c_uint64, c_uint64, ctypes.c_int32, c_void_p, c_void_p, c_void_p,

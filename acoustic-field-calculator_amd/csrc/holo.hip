@@ -1457,35 +1457,43 @@ bool Context::reserve_staging(size_t bytes) {
     return staging.reserve(bytes);
 }
 
-int32_t holo_solve(Context* ctx, const HoloBatch& hb, hipStream_t st) {
-    const uint32_t B = hb.batch;
-    if (B == 0) return AFH_OK;
-    const bool mfma = hb.precision != AFH_HOLO_FP32;
-    // bucket the instances by padded focus count
-    const int kinds[4] = {16, 32, 64, 128};
-    std::vector<uint32_t> lists[4];
+// Workspace plan of one holo_solve call: instances per padded-focus bucket and
+// the byte offsets of offsets | instance lists | staged sources | R | p | G |
+// q | d | Greedy candidates.
+struct HoloPlan {
+    uint32_t nb[4] = {0, 0, 0, 0};  // instances of the MP = 16, 32, 64, 128 buckets
+    uint32_t div = 0;               // Greedy phase candidates
+    size_t o_off = 0, o_list = 0, o_stg = 0, o_r = 0, o_p = 0, o_g = 0, o_q = 0, o_d = 0, o_cand = 0, total = 0;
+};
+
+constexpr int kBucketMP[4] = {16, 32, 64, 128};
+
+static int bucket_of(const HoloBatch& hb, uint32_t b) {
+    const uint32_t m = hb.offsets_host[b + 1] - hb.offsets_host[b];
+    const int mp = padded_m(m, hb.precision != AFH_HOLO_FP32 && hb.algo == kAlgoGSPAT);
+    for (int q = 0; q < 4; ++q)
+        if (kBucketMP[q] == mp) return q;
+    return -1;
+}
+
+static int32_t plan_holo(const HoloBatch& hb, HoloPlan& pl) {
+    const uint32_t B = hb.batch, np = hb.n_packed;
     for (uint32_t b = 0; b < B; ++b) {
         const uint32_t m = hb.offsets_host[b + 1] - hb.offsets_host[b];
-        if (m == 0) {
+        if (m == 0 || hb.offsets_host[b + 1] < hb.offsets_host[b]) {
             set_error("holo: an instance has no foci");
             return AFH_ERR_ARG;
         }
-        const int mp = padded_m(m, mfma && hb.algo == kAlgoGSPAT);
-        if (mp == 0) {
+        const int q = bucket_of(hb, b);
+        if (q < 0) {
             set_error("holo: more than 128 foci in one instance is not supported");
             return AFH_ERR_UNSUPPORTED;
         }
-        if (hb.algo == kAlgoGreedy) continue;  // no buckets
-        for (int q = 0; q < 4; ++q)
-            if (kinds[q] == mp) lists[q].push_back(b);
+        if (hb.algo != kAlgoGreedy) ++pl.nb[q];  // Greedy runs without buckets
     }
-    const uint32_t np = hb.n_packed;
-    // workspace layout: offsets | instance lists | staged sources | R | p | G | q | d
-    size_t off_bytes = (B + 1) * sizeof(uint32_t);
-    size_t list_bytes = B * sizeof(uint32_t);
     size_t r_elems = 0, p_elems = 0, g_elems = 0, q_elems = 0, d_elems = 0;
     for (int q = 0; q < 4; ++q) {
-        const size_t nb = lists[q].size(), mp = kinds[q];
+        const size_t nb = pl.nb[q], mp = kBucketMP[q];
         if (hb.algo == kAlgoGSPAT) {
             r_elems += nb * mp * mp * gspat_split(np);
             p_elems += nb * mp;
@@ -1497,54 +1505,45 @@ int32_t holo_solve(Context* ctx, const HoloBatch& hb, hipStream_t st) {
             q_elems += nb * np;
         }
     }
-    auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    const size_t o_off = 0, o_list = align(off_bytes), o_stg = o_list + align(list_bytes);
-    const size_t o_r = o_stg + align((size_t)np * AFH_SOURCE_STRIDE * sizeof(float));
-    const size_t o_p = o_r + align(r_elems * sizeof(float2)), o_g = o_p + align(p_elems * sizeof(float2));
-    const size_t o_q = o_g + align(g_elems * sizeof(float2)), o_d = o_q + align(q_elems * sizeof(float2));
-    const size_t total = o_d + align(d_elems * sizeof(float));
-    // Greedy: the candidate table (div float2) and their arguments (div floats)
-    const uint32_t div = hb.algo == kAlgoGreedy ? hb.phase_div : 0;
-    if (hb.algo == kAlgoGreedy && (div == 0 || div > kGreedyMaxDiv)) {
+    pl.div = hb.algo == kAlgoGreedy ? hb.phase_div : 0;
+    if (hb.algo == kAlgoGreedy && (pl.div == 0 || pl.div > kGreedyMaxDiv)) {
         set_error("greedy: phase_div must be in 1..4096");
         return AFH_ERR_ARG;
     }
-    const size_t cand_bytes = (size_t)div * (sizeof(float2) + sizeof(float));
-    const size_t o_cand = total;
-    const size_t total_all = total + (div ? ((cand_bytes + 255) & ~(size_t)255) : 0);
-    // growing the workspace frees the old block: work queued on this stream
-    // may still use it
-    if (total_all > ctx->work.cap && !hip_ok(hipStreamSynchronize(st), "hipStreamSynchronize")) return AFH_ERR_HIP;
-    if (!ctx->work.reserve(total_all)) return AFH_ERR_HIP;
-    if (!ctx->reserve_staging(off_bytes + list_bytes + cand_bytes)) return AFH_ERR_HIP;
-    uint32_t* stage = static_cast<uint32_t*>(ctx->staging.ptr);
-    memcpy(stage, hb.offsets_host, off_bytes);
-    {
-        uint32_t* lp = stage + (B + 1);
-        for (int q = 0; q < 4; ++q)
-            for (uint32_t b : lists[q]) *lp++ = b;
-    }
-    char* wbase = static_cast<char*>(ctx->work.ptr);
-    if (!hip_ok(hipMemcpyAsync(wbase + o_off, stage, off_bytes, hipMemcpyHostToDevice, st), "H2D offsets") ||
-        !hip_ok(hipMemcpyAsync(wbase + o_list, stage + (B + 1), list_bytes, hipMemcpyHostToDevice, st), "H2D lists"))
-        return AFH_ERR_HIP;
-    if (div) {
-        // greedy.rs:23-29: Complex::new(0, 2 PI i / div).exp(), in f32, and arg()
-        char* cs = reinterpret_cast<char*>(stage) + off_bytes + list_bytes;
-        float2* c2 = reinterpret_cast<float2*>(cs);
-        float* cph = reinterpret_cast<float*>(cs + (size_t)div * sizeof(float2));
-        for (uint32_t q = 0; q < div; ++q) {
-            const float th = 2.0f * 3.14159265358979323846f * (float)q / (float)div;
-            c2[q] = make_float2(expf(0.0f) * cosf(th), expf(0.0f) * sinf(th));
-            cph[q] = atan2f(c2[q].y, c2[q].x);
-        }
-        if (!hip_ok(hipMemcpyAsync(wbase + o_cand, cs, cand_bytes, hipMemcpyHostToDevice, st), "H2D candidates"))
-            return AFH_ERR_HIP;
-    }
-    if (!ctx->upload_done && !hip_ok(hipEventCreateWithFlags(&ctx->upload_done, hipEventDisableTiming), "event"))
-        return AFH_ERR_HIP;
-    if (!hip_ok(hipEventRecord(ctx->upload_done, st), "hipEventRecord")) return AFH_ERR_HIP;
+    auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    pl.o_off = 0;
+    pl.o_list = align((B + 1) * sizeof(uint32_t));
+    pl.o_stg = pl.o_list + align(B * sizeof(uint32_t));
+    pl.o_r = pl.o_stg + align((size_t)np * AFH_SOURCE_STRIDE * sizeof(float));
+    pl.o_p = pl.o_r + align(r_elems * sizeof(float2));
+    pl.o_g = pl.o_p + align(p_elems * sizeof(float2));
+    pl.o_q = pl.o_g + align(g_elems * sizeof(float2));
+    pl.o_d = pl.o_q + align(q_elems * sizeof(float2));
+    pl.o_cand = pl.o_d + align(d_elems * sizeof(float));
+    pl.total = pl.o_cand + align((size_t)pl.div * (sizeof(float2) + sizeof(float)));
+    return AFH_OK;
+}
 
+// Calls put(b) for every instance in bucket order (the instance lists).
+template <class F>
+static void for_each_listed(const HoloBatch& hb, F&& put) {
+    if (hb.algo == kAlgoGreedy) return;
+    for (int q = 0; q < 4; ++q)
+        for (uint32_t b = 0; b < hb.batch; ++b)
+            if (bucket_of(hb, b) == q) put(b);
+}
+
+// greedy.rs:23-29: Complex::new(0, 2 PI i / div).exp() in f32, and its arg().
+static void greedy_candidate(uint32_t q, uint32_t div, float2& c, float& ph) {
+    const float th = 2.0f * 3.14159265358979323846f * (float)q / (float)div;
+    c = make_float2(expf(0.0f) * cosf(th), expf(0.0f) * sinf(th));
+    ph = atan2f(c.y, c.x);
+}
+
+// Launches the solver on a workspace whose offsets, instance lists and Greedy
+// candidates are already on the device at wbase (laid out by `pl`).
+static int32_t run_holo(const HoloBatch& hb, const HoloPlan& pl, char* wbase, const DirCell* cells, hipStream_t st) {
+    const uint32_t B = hb.batch, np = hb.n_packed;
     HoloArgs a;
     memset(&a, 0, sizeof(a));
     a.sources = hb.sources;
@@ -1552,9 +1551,9 @@ int32_t holo_solve(Context* ctx, const HoloBatch& hb, hipStream_t st) {
     a.n_packed = np;
     a.foci = hb.foci;
     a.amps = hb.amps;
-    a.offsets = reinterpret_cast<const uint32_t*>(wbase + o_off);
-    a.staged = reinterpret_cast<const float4*>(wbase + o_stg);
-    a.cells = dir_cells_device();
+    a.offsets = reinterpret_cast<const uint32_t*>(wbase + pl.o_off);
+    a.staged = reinterpret_cast<const float4*>(wbase + pl.o_stg);
+    a.cells = cells;
     a.out_amp = hb.out_amp;
     a.out_phase = hb.out_phase;
     a.split = gspat_split(np);
@@ -1563,20 +1562,20 @@ int32_t holo_solve(Context* ctx, const HoloBatch& hb, hipStream_t st) {
         return AFH_ERR_HIP;
     }
     hipLaunchKernelGGL(stage_holo_sources, dim3((np + kHB - 1) / kHB), dim3(kHB), 0, st, hb.sources, np,
-                       reinterpret_cast<float4*>(wbase + o_stg));
+                       reinterpret_cast<float4*>(wbase + pl.o_stg));
     if (!hip_ok(hipGetLastError(), "holo staging launch")) return AFH_ERR_HIP;
     size_t list_pos = 0, r_pos = 0, p_pos = 0, g_pos = 0, q_pos = 0, d_pos = 0;
     for (int q = 0; q < 4; ++q) {
-        const uint32_t nb = (uint32_t)lists[q].size();
-        const int mp = kinds[q];
+        const uint32_t nb = pl.nb[q];
+        const int mp = kBucketMP[q];
         if (nb == 0) continue;
-        a.inst = reinterpret_cast<const uint32_t*>(wbase + o_list) + list_pos;
-        a.R = reinterpret_cast<float2*>(wbase + o_r) + r_pos;
-        a.p = reinterpret_cast<float2*>(wbase + o_p) + p_pos;
-        a.G = reinterpret_cast<float2*>(wbase + o_g) + g_pos;
-        a.q = reinterpret_cast<float2*>(wbase + o_q) + q_pos;
-        a.d = reinterpret_cast<float*>(wbase + o_d) + d_pos;
-        a.gam = reinterpret_cast<float2*>(wbase + o_q) + q_pos;
+        a.inst = reinterpret_cast<const uint32_t*>(wbase + pl.o_list) + list_pos;
+        a.R = reinterpret_cast<float2*>(wbase + pl.o_r) + r_pos;
+        a.p = reinterpret_cast<float2*>(wbase + pl.o_p) + p_pos;
+        a.G = reinterpret_cast<float2*>(wbase + pl.o_g) + g_pos;
+        a.q = reinterpret_cast<float2*>(wbase + pl.o_q) + q_pos;
+        a.d = reinterpret_cast<float*>(wbase + pl.o_d) + d_pos;
+        a.gam = reinterpret_cast<float2*>(wbase + pl.o_q) + q_pos;
         hipError_t e = hipSuccess;
         if (hb.algo == kAlgoGSPAT) {
             // kernel C reads an instance's G through 32-bit buffer offsets
@@ -1610,17 +1609,17 @@ int32_t holo_solve(Context* ctx, const HoloBatch& hb, hipStream_t st) {
         list_pos += nb;
     }
     if (hb.algo == kAlgoGreedy) {
-        const float2* cand = reinterpret_cast<const float2*>(wbase + o_cand);
-        const float* cph = reinterpret_cast<const float*>(wbase + o_cand + (size_t)div * sizeof(float2));
+        const float2* cand = reinterpret_cast<const float2*>(wbase + pl.o_cand);
+        const float* cph = reinterpret_cast<const float*>(wbase + pl.o_cand + (size_t)pl.div * sizeof(float2));
         for (uint32_t b = 0; b < B; ++b)
             if (hb.offsets_host[b + 1] - hb.offsets_host[b] > kGreedyMaxM) {
                 set_error("greedy: more than 128 foci in one instance is not supported");
                 return AFH_ERR_UNSUPPORTED;
             }
         if (hb.source_type == kT4010A1)
-            hipLaunchKernelGGL((greedy_kernel<kT4010A1>), dim3(B), dim3(kHB), 0, st, a, cand, cph, div);
+            hipLaunchKernelGGL((greedy_kernel<kT4010A1>), dim3(B), dim3(kHB), 0, st, a, cand, cph, pl.div);
         else
-            hipLaunchKernelGGL((greedy_kernel<kSphere>), dim3(B), dim3(kHB), 0, st, a, cand, cph, div);
+            hipLaunchKernelGGL((greedy_kernel<kSphere>), dim3(B), dim3(kHB), 0, st, a, cand, cph, pl.div);
         if (!hip_ok(hipGetLastError(), "greedy kernel launch")) return AFH_ERR_HIP;
     }
     if (hb.algo == kAlgoNaive) {
@@ -1632,6 +1631,174 @@ int32_t holo_solve(Context* ctx, const HoloBatch& hb, hipStream_t st) {
         if (!hip_ok(hipGetLastError(), "naive kernel launch")) return AFH_ERR_HIP;
     }
     return AFH_OK;
+}
+
+int32_t holo_solve(Context* ctx, const HoloBatch& hb, hipStream_t st) {
+    const uint32_t B = hb.batch;
+    if (B == 0) return AFH_OK;
+    HoloPlan pl;
+    const int32_t rc = plan_holo(hb, pl);
+    if (rc != AFH_OK) return rc;
+    const size_t off_bytes = (B + 1) * sizeof(uint32_t), list_bytes = B * sizeof(uint32_t);
+    const size_t cand_bytes = (size_t)pl.div * (sizeof(float2) + sizeof(float));
+    // growing the workspace frees the old block: work queued on this stream
+    // may still use it
+    if (pl.total > ctx->work.cap && !hip_ok(hipStreamSynchronize(st), "hipStreamSynchronize")) return AFH_ERR_HIP;
+    if (!ctx->work.reserve(pl.total)) return AFH_ERR_HIP;
+    if (!ctx->reserve_staging(off_bytes + list_bytes + cand_bytes)) return AFH_ERR_HIP;
+    uint32_t* stage = static_cast<uint32_t*>(ctx->staging.ptr);
+    memcpy(stage, hb.offsets_host, off_bytes);
+    {
+        uint32_t* lp = stage + (B + 1);
+        for_each_listed(hb, [&](uint32_t b) { *lp++ = b; });
+    }
+    char* wbase = static_cast<char*>(ctx->work.ptr);
+    if (!hip_ok(hipMemcpyAsync(wbase + pl.o_off, stage, off_bytes, hipMemcpyHostToDevice, st), "H2D offsets") ||
+        !hip_ok(hipMemcpyAsync(wbase + pl.o_list, stage + (B + 1), list_bytes, hipMemcpyHostToDevice, st),
+                "H2D lists"))
+        return AFH_ERR_HIP;
+    if (pl.div) {
+        char* cs = reinterpret_cast<char*>(stage) + off_bytes + list_bytes;
+        float2* c2 = reinterpret_cast<float2*>(cs);
+        float* cph = reinterpret_cast<float*>(cs + (size_t)pl.div * sizeof(float2));
+        for (uint32_t q = 0; q < pl.div; ++q) greedy_candidate(q, pl.div, c2[q], cph[q]);
+        if (!hip_ok(hipMemcpyAsync(wbase + pl.o_cand, cs, cand_bytes, hipMemcpyHostToDevice, st), "H2D candidates"))
+            return AFH_ERR_HIP;
+    }
+    if (!ctx->upload_done && !hip_ok(hipEventCreateWithFlags(&ctx->upload_done, hipEventDisableTiming), "event"))
+        return AFH_ERR_HIP;
+    if (!hip_ok(hipEventRecord(ctx->upload_done, st), "hipEventRecord")) return AFH_ERR_HIP;
+    return run_holo(hb, pl, wbase, dir_cells_device(), st);
+}
+
+// ---------------------------------------------------------------- allocation-free entry point
+// AFH_HoloSolveNoAlloc runs holo_solve's kernels on a file-scope device array
+// instead of a context's workspace, for callers that cannot allocate (code
+// under graph capture, serving loops that forbid hipMalloc). Each device
+// holds its own copy of `scratch` until the process ends: the first
+// kHoloScratchTable bytes keep the directivity table (written once per
+// device), the rest is the workspace of one call. Host data (offsets,
+// instance lists, Greedy candidates, the table) reaches it as kernel
+// arguments, so no staging buffer is needed either. Calls that overlap on one
+// device share the array and race.
+constexpr size_t kHoloScratchBytes = 128ull << 20;
+constexpr size_t kHoloScratchTable = 16384;
+static_assert(sizeof(DirCell) * kDirCells <= kHoloScratchTable, "directivity table fits its slot");
+
+static __device__ __align__(256) unsigned char scratch[kHoloScratchBytes];
+
+constexpr uint32_t kUploadWords = 960;  // 3.8 KB of kernel arguments per launch
+struct UploadChunk {
+    uint32_t* dst;
+    uint32_t n;
+    uint32_t w[kUploadWords];
+};
+
+static_assert(sizeof(UploadChunk) <= 4096, "kernel arguments stay within 4 KiB");
+
+__global__ __launch_bounds__(kHB) void upload_words(UploadChunk c) {
+    for (uint32_t i = threadIdx.x; i < c.n; i += kHB) c.dst[i] = c.w[i];
+}
+
+// Writes a sequence of 32-bit words to consecutive device addresses from dst
+// on, kUploadWords per launch.
+struct WordUploader {
+    hipStream_t st;
+    uint32_t* dst;
+    UploadChunk c;
+    bool ok = true;
+    WordUploader(hipStream_t s, void* d) : st(s), dst(static_cast<uint32_t*>(d)) {
+        c.dst = dst;
+        c.n = 0;
+    }
+    void flush() {
+        if (c.n == 0 || !ok) return;
+        hipLaunchKernelGGL(upload_words, dim3(1), dim3(kHB), 0, st, c);
+        ok = hip_ok(hipGetLastError(), "upload launch");
+        dst += c.n;
+        c.dst = dst;
+        c.n = 0;
+    }
+    void push(uint32_t v) {
+        c.w[c.n++] = v;
+        if (c.n == kUploadWords) flush();
+    }
+    void push(float v) { push(__builtin_bit_cast(uint32_t, v)); }
+};
+
+// The device's scratch address, from hipGetSymbolAddress once per device,
+// with the directivity table written into its first bytes.
+static char* holo_scratch(hipStream_t st) {
+    static std::mutex mu;
+    static char* ptr[64] = {};
+    int dev = 0;
+    if (!hip_ok(hipGetDevice(&dev), "hipGetDevice")) return nullptr;
+    if (dev < 0 || dev >= 64) {
+        set_error("holo: device index out of range");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> g(mu);
+    if (ptr[dev]) return ptr[dev];
+    char* p = nullptr;
+    if (!hip_ok(hipGetSymbolAddress(reinterpret_cast<void**>(&p), HIP_SYMBOL(scratch)), "hipGetSymbolAddress"))
+        return nullptr;
+    static DirCell host[kDirCells];
+    build_dir_cells(host);
+    WordUploader up(st, p);
+    const uint32_t* words = reinterpret_cast<const uint32_t*>(host);
+    for (size_t i = 0; i < sizeof(host) / sizeof(uint32_t); ++i) up.push(words[i]);
+    up.flush();
+    // the table must be in place before a call on another stream reads it
+    if (!up.ok || !hip_ok(hipStreamSynchronize(st), "hipStreamSynchronize")) return nullptr;
+    ptr[dev] = p;
+    return p;
+}
+
+int32_t holo_solve_noalloc(const HoloBatch& hb, hipStream_t st) {
+    const uint32_t B = hb.batch;
+    if (B == 0) return AFH_OK;
+    HoloPlan pl;
+    const int32_t rc = plan_holo(hb, pl);
+    if (rc != AFH_OK) return rc;
+    if (pl.total > kHoloScratchBytes - kHoloScratchTable) {
+        set_error("AFH_HoloSolveNoAlloc: the batch needs " + std::to_string(pl.total) + " bytes of workspace, more than the " +
+                  std::to_string(kHoloScratchBytes - kHoloScratchTable) + " bytes of the device scratch array");
+        return AFH_ERR_BUDGET;
+    }
+    char* base = holo_scratch(st);
+    if (!base) return AFH_ERR_HIP;
+    char* wbase = base + kHoloScratchTable;
+    {
+        WordUploader up(st, wbase + pl.o_off);
+        for (uint32_t b = 0; b <= B; ++b) up.push(hb.offsets_host[b]);
+        up.flush();
+        if (!up.ok) return AFH_ERR_HIP;
+    }
+    if (hb.algo != kAlgoGreedy) {
+        WordUploader up(st, wbase + pl.o_list);
+        for_each_listed(hb, [&](uint32_t b) { up.push(b); });
+        up.flush();
+        if (!up.ok) return AFH_ERR_HIP;
+    }
+    if (pl.div) {
+        WordUploader up(st, wbase + pl.o_cand);
+        for (uint32_t q = 0; q < pl.div; ++q) {
+            float2 c;
+            float ph;
+            greedy_candidate(q, pl.div, c, ph);
+            up.push(c.x);
+            up.push(c.y);
+        }
+        for (uint32_t q = 0; q < pl.div; ++q) {
+            float2 c;
+            float ph;
+            greedy_candidate(q, pl.div, c, ph);
+            up.push(ph);
+        }
+        up.flush();
+        if (!up.ok) return AFH_ERR_HIP;
+    }
+    return run_holo(hb, pl, wbase, reinterpret_cast<const DirCell*>(base), st);
 }
 
 // ---------------------------------------------------------------- single instance (AFO_*)
@@ -1743,6 +1910,36 @@ int32_t AFH_HoloSolve(void* ctx, const float* packed_sources_dev, uint64_t n_pac
     hb.out_amp = out_amp_dev;
     hb.out_phase = out_phase_dev;
     return holo_solve(c, hb, (hipStream_t)stream);
+}
+
+int32_t AFH_HoloSolveNoAlloc(const float* packed_sources_dev, uint64_t n_packed, uint64_t n_sources,
+                             int32_t source_type, const float* foci_dev, const float* amps_dev,
+                             const uint32_t* offsets_host, uint32_t batch, int32_t algo, int32_t precision,
+                             uint32_t iterations, float* out_amp_dev, float* out_phase_dev, void* stream) {
+    clear_error();
+    if (!packed_sources_dev || !foci_dev || !amps_dev || !offsets_host || !out_amp_dev || !out_phase_dev ||
+        n_packed % AFH_SOURCE_PAD != 0 || n_packed < n_sources || n_packed > 0xFFFFFFFFull ||
+        (source_type != kSphere && source_type != kT4010A1) || algo < kAlgoGSPAT || algo > kAlgoNaive ||
+        precision < AFH_HOLO_FP32 || precision > AFH_HOLO_MIXED_SPLIT ||
+        (algo != kAlgoGSPAT && precision != AFH_HOLO_FP32) || offsets_host[0] != 0) {
+        set_error("AFH_HoloSolveNoAlloc: bad argument");
+        return AFH_ERR_ARG;
+    }
+    HoloBatch hb;
+    hb.sources = packed_sources_dev;
+    hb.n = (uint32_t)n_sources;
+    hb.n_packed = (uint32_t)n_packed;
+    hb.source_type = source_type;
+    hb.foci = foci_dev;
+    hb.amps = amps_dev;
+    hb.offsets_host = offsets_host;
+    hb.batch = batch;
+    hb.algo = algo;
+    hb.precision = precision;
+    hb.iterations = iterations;
+    hb.out_amp = out_amp_dev;
+    hb.out_phase = out_phase_dev;
+    return holo_solve_noalloc(hb, (hipStream_t)stream);
 }
 
 int32_t AFH_GreedySolve(void* ctx_handle, const float* packed, uint64_t n_packed, uint64_t n, int32_t source_type,

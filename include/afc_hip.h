@@ -247,6 +247,21 @@ int32_t AFH_HoloSolve(void* ctx, const float* packed_sources_dev, uint64_t n_pac
                       uint32_t batch, int32_t algo, int32_t precision, uint32_t iterations, float* out_amp_dev,
                       float* out_phase_dev, void* stream);
 
+/* No counterpart in the reference. AFH_HoloSolve without a context, for
+ * callers that cannot allocate (code under HIP graph capture, serving loops
+ * that forbid hipMalloc): the same kernels and results, with the workspace in
+ * a 128 MiB device array of the library (each GPU holds its own copy until
+ * the process ends) and the host-side tables passed as kernel arguments, so
+ * the call allocates neither device nor pinned memory. A batch whose
+ * workspace exceeds the array (128 MiB less 16 KiB) returns AFH_ERR_BUDGET.
+ * Calls that overlap on one GPU share the array and race: serialize them.
+ * The first call on a device writes the directivity table into the array and
+ * synchronizes `stream` once. Arguments as AFH_HoloSolve. */
+int32_t AFH_HoloSolveNoAlloc(const float* packed_sources_dev, uint64_t n_packed, uint64_t n_sources,
+                             int32_t source_type, const float* foci_dev, const float* amps_dev,
+                             const uint32_t* offsets_host, uint32_t batch, int32_t algo, int32_t precision,
+                             uint32_t iterations, float* out_amp_dev, float* out_phase_dev, void* stream);
+
 /* Greedy phase search (combination/greedy.rs:23-83) for `batch` instances:
  * candidates e^{i 2 pi idx / phase_div}, sources in index order; writes the
  * phase of every source of instance b to out_phase_dev[b * n_sources ...].
