@@ -50,6 +50,7 @@ constexpr int kKC = 32;   // G columns per chunk
 
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef uint32_t uintx4 __attribute__((ext_vector_type(4)));
 
 struct HoloArgs {
     const float* sources;  // packed, AFH_SOURCE_STRIDE floats each
@@ -119,10 +120,11 @@ __device__ __forceinline__ void stage_cells_lds(const DirCell* __restrict__ cell
     }
 }
 
-// Per-instance foci in LDS; rows >= M are padding.
+// Per-instance foci in LDS, positions split as by split3; rows >= M are
+// padding.
 template <int MP>
 struct Foci {
-    float x[MP], y[MP], z[MP], a[MP], v[MP];  // v: 1 for a focus, 0 for padding
+    float xh[MP], yh[MP], zh[MP], xl[MP], yl[MP], zl[MP], a[MP], v[MP];  // v: 1 for a focus, 0 for padding
 };
 
 template <int SRC, int MP>
@@ -132,9 +134,15 @@ __device__ __forceinline__ void load_instance(const HoloArgs& a, uint32_t b, Foc
     M = a.offsets[b + 1] - base;
     for (int i = threadIdx.x; i < MP; i += kHB) {
         const bool v = i < (int)M;
-        f.x[i] = v ? a.foci[3 * (uint64_t)(base + i)] : 0.f;
-        f.y[i] = v ? a.foci[3 * (uint64_t)(base + i) + 1] : 0.f;
-        f.z[i] = v ? a.foci[3 * (uint64_t)(base + i) + 2] : 1.f;
+        const Split3 q = split3(v ? a.foci[3 * (uint64_t)(base + i)] : 0.f,
+                                v ? a.foci[3 * (uint64_t)(base + i) + 1] : 0.f,
+                                v ? a.foci[3 * (uint64_t)(base + i) + 2] : 1.f);
+        f.xh[i] = q.xh;
+        f.yh[i] = q.yh;
+        f.zh[i] = q.zh;
+        f.xl[i] = q.xl;
+        f.yl[i] = q.yl;
+        f.zl[i] = q.zl;
         f.a[i] = v ? a.amps[base + i] : 0.f;
         f.v[i] = v ? 1.f : 0.f;
     }
@@ -147,8 +155,8 @@ __device__ __forceinline__ void load_instance(const HoloArgs& a, uint32_t b, Foc
 // zero without evaluating the terms. Kernel A takes RT = 2: with 4 the MP = 64
 // kernels reach their 168-register bound and ran slower.
 template <int SRC, int MP, int RT>
-__device__ __forceinline__ void g_tile(const SrcF& s, const Foci<MP>& f, const CellsLds* cells, uint32_t M,
-                                       float2 (&g)[MP / 8]) {
+__device__ __forceinline__ void g_tile(const SrcF& s, const Split3& ss, const Foci<MP>& f, const CellsLds* cells,
+                                       uint32_t M, float2 (&g)[MP / 8]) {
     static_assert(RT <= MP / 8 && (MP / 8) % RT == 0, "rows per step");
     const int t = threadIdx.x;
     const int grp = t >> 5;
@@ -162,18 +170,17 @@ __device__ __forceinline__ void g_tile(const SrcF& s, const Foci<MP>& f, const C
             for (int u = 0; u < RT; ++u) g[r + u] = make_float2(0.f, 0.f);
             continue;
         }
-        float px[RT], py[RT], pz[RT], w[RT], re[RT], im[RT];
+        Split3 pt[RT];
+        float w[RT], re[RT], im[RT];
 #pragma unroll
         for (int u = 0; u < RT; ++u) {
             const int i = i0 + 8 * u;
-            px[u] = f.x[i];
-            py[u] = f.y[i];
-            pz[u] = f.z[i];
+            pt[u] = {f.xh[i], f.yh[i], f.zh[i], f.xl[i], f.yl[i], f.zl[i]};
             w[u] = f.v[i];
             re[u] = 0.f;
             im[u] = 0.f;
         }
-        add_terms_staged<SRC, RT>(s, px, py, pz, cells->p, cells->q, re, im, w);
+        add_terms_exact<SRC, RT>(s, ss, pt, cells->p, cells->q, re, im, w, nullptr);
 #pragma unroll
         for (int u = 0; u < RT; ++u) g[r + u] = make_float2(re[u], im[u]);
     }
@@ -258,12 +265,12 @@ __device__ __forceinline__ void store_g_tile(const HoloArgs& a, uint32_t k, uint
 }
 
 // Kernel C's stored inputs of one chunk as loaded: the G tile (FP32 G, or
-// H16 words: the FP16 copy G_ij 2^s_j that kernel A1 stores in MIXED mode)
-// and, for threads t < kKC, d of column j0 + t.
+// H16 words: the FP16 copy of H = s G_ij / d_j that the fused kernel A stores
+// in MIXED mode) and, for threads t < kKC, d of column j0 + t.
 template <int MP, bool H16>
 struct StoredChunk {
     std::conditional_t<H16, uint32_t, float2> g[MP / 8];
-    float d, la;  // la (H16): log2 |amp| of the column's source, which sets its FP16 scale
+    float d;
 };
 
 // cache-policy operand of a buffer load: bit 1 is the non-temporal hint (nt)
@@ -276,7 +283,7 @@ __device__ __forceinline__ void load_chunk(const HoloArgs& a, __amdgpu_buffer_rs
     const uint32_t col = j0 + (t & 31);
     if constexpr (H16) {
         // every load unconditional: the descriptor ends at row M (rows >= M
-        // read as 0 without a memory access), and la is read by every thread
+        // read as 0 without a memory access), and d is read by every thread
         // (only t < kKC use it); with a select per load and the ring rotated
         // by copies, the compiler waited for each chunk's loads at once
 #pragma unroll
@@ -285,7 +292,6 @@ __device__ __forceinline__ void load_chunk(const HoloArgs& a, __amdgpu_buffer_rs
             c.g[r] = __builtin_amdgcn_raw_buffer_load_b32(gr, (int)(e * 4), 0, kNonTemporal);
         }
         c.d = a.d[(uint64_t)k * a.n_packed + col];
-        c.la = a.staged[3 * (uint64_t)col].w;
     } else {
         // FP32 G, one chunk ahead: rows >= M are skipped (measured faster here
         // than the bounded buffer loads, which still issue those rows)
@@ -298,7 +304,6 @@ __device__ __forceinline__ void load_chunk(const HoloArgs& a, __amdgpu_buffer_rs
                                 : make_float2(0.f, 0.f);
         }
         c.d = t < kKC ? a.d[(uint64_t)k * a.n_packed + col] : 0.f;
-        c.la = 0.f;
     }
 }
 
@@ -353,7 +358,7 @@ __global__ __launch_bounds__(kHB) void gspat_r_fp32(HoloArgs a) {
         }
         const SrcF sc = src_from_block(s_src, in_blk);
         float2 g[MP / 8];
-        g_tile<SRC, MP, 2>(sc, f, s_cells, M, g);
+        g_tile<SRC, MP, 2>(sc, split3(sc.x, sc.y, sc.z), f, s_cells, M, g);
         store_g_tile<MP>(a, k, j0, M, g);
         const float inv = column_inv<MP>(g, j0, a.n, s_part, d_out);
 #pragma unroll
@@ -384,287 +389,55 @@ __global__ __launch_bounds__(kHB) void gspat_r_fp32(HoloArgs a) {
         for (int y = 0; y < BS; ++y) R[(rb + x) * MP + cb + y] = acc[x][y];
 }
 
-// ---------------------------------------------------------------- kernel A (MFMA)
-// v_mfma_f32_16x16x32_f16: lane l supplies A[l % 16][8 (l / 16) + v] and
-// B[8 (l / 16) + v][l % 16], v = 0..7; it receives D[4 (l / 16) + e][l % 16].
-// With both operands taken as rows of H, D is a 16 x 16 block of H_a H_b^T.
-// SPLIT: H = hi + lo with both halves in FP16 (lo = H - hi rounded again) and
-// R = hi hi^H + hi lo^H + lo hi^H, FP32 accumulation; H is first scaled by a
-// per-instance power of two s ~ M / 2 (undone exactly on output) so that lo
-// stays in the normal FP16 range.
-// MP = 64: at most 168 registers (accumulators included), so that 3 waves
-// share a SIMD and 3 workgroups a CU (LDS allows 3); at the 196 the compiler
-// otherwise takes, 2 workgroups per CU ran the 2052 workgroups of a
-// 513-instance bucket in 5 rounds, the last with 4 of them.
-template <int SRC, int MP, bool SPLIT>
-__global__ __launch_bounds__(kHB) __attribute__((amdgpu_waves_per_eu(MP == 64 ? 3 : 1))) void gspat_r_mfma(
-        HoloArgs a) {
-    static_assert(MP >= 32, "one 16x16 tile per wave at least");
-    constexpr int T = MP / 16;
-    constexpr int TPW = T * T / 4;  // tiles per wave
-    constexpr int HS = kKC + 8;     // row stride in halves (80 bytes)
-    constexpr int NH = SPLIT ? 2 : 1;
-    __shared__ std::conditional_t<SRC == kT4010A1, CellsLds, float> s_cells_mem;
-    CellsLds* s_cells = reinterpret_cast<CellsLds*>(&s_cells_mem);
-    __shared__ Foci<MP> f;
-    __shared__ float s_part[8][kKC];
-    __shared__ __attribute__((aligned(16))) _Float16 s_hr[NH][MP * HS];
-    __shared__ __attribute__((aligned(16))) _Float16 s_hi[NH][MP * HS];
-    __shared__ SrcBlock s_src;
-    const int t = threadIdx.x;
-    const int w = t >> 6, l = t & 63;
-    const uint32_t k = blockIdx.x / a.split, part = blockIdx.x % a.split, b = a.inst[k];
-    const uint32_t n_chunks = a.n_packed / kKC;
-    const uint32_t c_begin = part * n_chunks / a.split, c_end = (part + 1) * n_chunks / a.split;
-    uint32_t M, base;
-    load_instance<SRC, MP>(a, b, f, s_cells, M, base);
-    __syncthreads();
-    // s = 2^floor(log2(M)) / 2 (>= 1) for SPLIT, 1 otherwise
-    const float scale = SPLIT ? fmaxf(1.0f, exp2f(floorf(log2f((float)M)) - 1.0f)) : 1.0f;
-    // R = H H^H: Re R += Hr Hr^T + Hi Hi^T, Im R += Hi Hr^T + (-Hr) Hi^T, the
-    // sign flip applied to the FP16 A operand (exact)
-    floatx4 accRe[TPW], accIm[TPW];
-#pragma unroll
-    for (int s = 0; s < TPW; ++s) {
-        accRe[s] = floatx4{0.f, 0.f, 0.f, 0.f};
-        accIm[s] = accRe[s];
-    }
-    float* d_out = a.d + (uint64_t)k * a.n_packed;
-    float4 pre[3];
-    src_block_fetch(a, c_begin, pre);
-    for (uint32_t chunk = c_begin; chunk < c_end; ++chunk) {
-        const uint32_t j0 = chunk * kKC, in_blk = (chunk - c_begin) % kSB;
-        if (in_blk == 0) {
-            // every wave is past its last read of the previous block (the
-            // previous chunk's tile barrier)
-            src_block_put(s_src, pre);
-            lds_sync();
-            if (chunk + kSB < c_end) src_block_fetch(a, chunk + kSB, pre);
-        }
-        const SrcF sc = src_from_block(s_src, in_blk);
-        float2 g[MP / 8];
-        g_tile<SRC, MP, 2>(sc, f, s_cells, M, g);
-        store_g_tile<MP>(a, k, j0, M, g);
-        const float inv = column_inv<MP>(g, j0, a.n, s_part, d_out) * scale;
-#pragma unroll
-        for (int r = 0; r < MP / 8; ++r) {
-            const int i = (t >> 5) + 8 * r;
-            const float hre = g[r].x * inv, him = g[r].y * inv;
-            const _Float16 rh = (_Float16)hre, ih = (_Float16)him;
-            s_hr[0][i * HS + (t & 31)] = rh;
-            s_hi[0][i * HS + (t & 31)] = ih;
-            if (SPLIT) {
-                s_hr[NH - 1][i * HS + (t & 31)] = (_Float16)(hre - (float)rh);
-                s_hi[NH - 1][i * HS + (t & 31)] = (_Float16)(him - (float)ih);
-            }
-        }
-        // the tile barrier: s_hr / s_hi are written again only after the next
-        // chunk's column barrier, which every wave reaches after its MFMAs
-        lds_sync();
-        const int kb = (l >> 4) * 8;
-#pragma unroll
-        for (int s = 0; s < TPW; ++s) {
-            const int tau = w * TPW + s;
-            const int arow = (tau / T) * 16 + (l & 15);
-            const int brow = (tau % T) * 16 + (l & 15);
-            const half8 ar = *reinterpret_cast<const half8*>(&s_hr[0][arow * HS + kb]);
-            const half8 ai = *reinterpret_cast<const half8*>(&s_hi[0][arow * HS + kb]);
-            const half8 br = *reinterpret_cast<const half8*>(&s_hr[0][brow * HS + kb]);
-            const half8 bi = *reinterpret_cast<const half8*>(&s_hi[0][brow * HS + kb]);
-            const half8 nar = -ar;
-            if (SPLIT) {
-                const half8 arl = *reinterpret_cast<const half8*>(&s_hr[NH - 1][arow * HS + kb]);
-                const half8 ail = *reinterpret_cast<const half8*>(&s_hi[NH - 1][arow * HS + kb]);
-                const half8 brl = *reinterpret_cast<const half8*>(&s_hr[NH - 1][brow * HS + kb]);
-                const half8 bil = *reinterpret_cast<const half8*>(&s_hi[NH - 1][brow * HS + kb]);
-                const half8 narl = -arl;
-                // cross terms first: they are small, the hi products dominate the sum
-                accRe[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ar, brl, accRe[s], 0, 0, 0);
-                accRe[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(arl, br, accRe[s], 0, 0, 0);
-                accRe[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ai, bil, accRe[s], 0, 0, 0);
-                accRe[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ail, bi, accRe[s], 0, 0, 0);
-                accIm[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ai, brl, accIm[s], 0, 0, 0);
-                accIm[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ail, br, accIm[s], 0, 0, 0);
-                accIm[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(nar, bil, accIm[s], 0, 0, 0);
-                accIm[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(narl, bi, accIm[s], 0, 0, 0);
-            }
-            accRe[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ar, br, accRe[s], 0, 0, 0);
-            accRe[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ai, bi, accRe[s], 0, 0, 0);
-            accIm[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ai, br, accIm[s], 0, 0, 0);
-            accIm[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(nar, bi, accIm[s], 0, 0, 0);
-        }
-    }
-    const float unscale = 1.0f / (scale * scale);  // exact: scale is a power of two
-    float2* R = a.R + (uint64_t)blockIdx.x * MP * MP;  // partial `part` of instance k
-#pragma unroll
-    for (int s = 0; s < TPW; ++s) {
-        const int tau = w * TPW + s;
-        const int col = (tau % T) * 16 + (l & 15);
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int row = (tau / T) * 16 + 4 * (l >> 4) + e;
-            R[row * MP + col] = make_float2(accRe[s][e] * unscale, accIm[s][e] * unscale);
-        }
-    }
-}
-
-// ---------------------------------------------------------------- MIXED: kernels A1, A2
-// Exponent s_j of column j's FP16 copy G_ij 2^s_j: s_j = 8 - floor(log2 |amp_j|),
-// so that |G_ij 2^s_j| ~ 256 D / r (r in mm) sits in the FP16 normal range;
-// 0 for a zero amplitude (the column is 0).
-__device__ __forceinline__ int gen16_scale(float la) {
-    return (la > -64.f && la < 64.f) ? 8 - (int)floorf(la) : 0;
-}
-
-__device__ __forceinline__ uint32_t pack_h2(float re, float im) {
-    return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)re) |
-           ((uint32_t)__builtin_bit_cast(uint16_t, (_Float16)im) << 16);
-}
-
+// ---------------------------------------------------------------- FP32 mode: kernels A1, A2
 // A1: G of every instance of the bucket, one thread per source column j and
-// the instance's foci in turn (four at a time through the staged term),
-// stored for rows < M as FP16 G_ij 2^s_j (MIXED, 4 bytes) or FP32 G_ij (F32:
-// FP32 mode, 8 bytes, the layout kernel C reads), non-temporal, and d_j =
-// sum_i |G_ij| over i in order (FP32, from each term's modulus).
-template <int SRC, int MP, bool F32>
+// the instance's foci in turn (four at a time through the exact-difference
+// term), stored for rows < M as FP32 G_ij (8 bytes, the layout kernel C
+// reads), non-temporal, and d_j = sum_i |G_ij| over i in order (FP32, from
+// each term's modulus).
+template <int SRC, int MP>
 __global__ __launch_bounds__(kHB) void gspat_gen(HoloArgs a) {
     __shared__ std::conditional_t<SRC == kT4010A1, CellsLds, float> s_cells_mem;
     CellsLds* s_cells = reinterpret_cast<CellsLds*>(&s_cells_mem);
-    __shared__ float4 s_f[MP];
+    __shared__ float4 s_fh[MP], s_fl[MP];  // the foci split as by split3
     const uint32_t k = blockIdx.y, b = a.inst[k];
     const uint32_t base = a.offsets[b], M = a.offsets[b + 1] - base;
     for (int i = threadIdx.x; i < MP; i += kHB) {
         const uint64_t q = 3 * (uint64_t)(base + i);
-        s_f[i] = i < (int)M ? make_float4(a.foci[q], a.foci[q + 1], a.foci[q + 2], 0.f) : make_float4(0.f, 0.f, 1.f, 0.f);
+        const Split3 p = i < (int)M ? split3(a.foci[q], a.foci[q + 1], a.foci[q + 2]) : split3(0.f, 0.f, 1.f);
+        s_fh[i] = make_float4(p.xh, p.yh, p.zh, 0.f);
+        s_fl[i] = make_float4(p.xl, p.yl, p.zl, 0.f);
     }
     if (SRC == kT4010A1) stage_cells_lds(a.cells, s_cells);
     __syncthreads();
     const uint32_t j = blockIdx.x * kHB + threadIdx.x;
     if (j >= a.n_packed) return;
-    SrcF src = load_staged(a.staged, j);
-    const int sc = F32 ? 0 : gen16_scale(src.la);
-    // the scale rides on the amplitude's exponent: every term comes out as
-    // G_ij 2^s_j, and d_j is scaled back exactly at the end
-    src.la += (float)sc;
-    uint32_t* G16 = reinterpret_cast<uint32_t*>(a.G) + (uint64_t)k * MP * a.n_packed + j;
+    const SrcF src = load_staged(a.staged, j);
+    const Split3 ss = split3(src.x, src.y, src.z);
     float2* G32 = a.G + (uint64_t)k * MP * a.n_packed + j;
     float d = 0.f;
     for (uint32_t i = 0; i < M; i += 4) {
         // i + 3 < MP (i is a multiple of 4 below M <= MP, MP a multiple of
         // 16); rows from M on are padding foci, evaluated and not stored
-        float px[4], py[4], pz[4], re[4], im[4], am[4];
+        Split3 pt[4];
+        float re[4], im[4], am[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const float4 f = s_f[i + u];
-            px[u] = f.x;
-            py[u] = f.y;
-            pz[u] = f.z;
+            const float4 h = s_fh[i + u], lo = s_fl[i + u];
+            pt[u] = {h.x, h.y, h.z, lo.x, lo.y, lo.z};
             re[u] = im[u] = 0.f;
         }
-        add_terms_staged<SRC, 4>(src, px, py, pz, s_cells->p, s_cells->q, re, im, nullptr, am);
+        add_terms_exact<SRC, 4>(src, ss, pt, s_cells->p, s_cells->q, re, im, nullptr, am);
 #pragma unroll
         for (int u = 0; u < 4; ++u)
             if (i + u < M) {
                 d += fabsf(am[u]);
-                if constexpr (F32) {
-                    float2* g = G32 + (uint64_t)(i + u) * a.n_packed;
-                    __builtin_nontemporal_store(re[u], &g->x);
-                    __builtin_nontemporal_store(im[u], &g->y);
-                } else {
-                    __builtin_nontemporal_store(pack_h2(re[u], im[u]), G16 + (uint64_t)(i + u) * a.n_packed);
-                }
+                float2* g = G32 + (uint64_t)(i + u) * a.n_packed;
+                __builtin_nontemporal_store(re[u], &g->x);
+                __builtin_nontemporal_store(im[u], &g->y);
             }
     }
-    a.d[(uint64_t)k * a.n_packed + j] = ldexpf(d, -sc);
-}
-
-// A2: R = H H^H over the split's source chunk range from A1's copy, H_ij =
-// G_ij / d_j = FP16(G_ij 2^s_j) 2^-s_j / d_j rounded to FP16 in LDS, then the
-// same MFMA tiles as kernel A (Re R += Hr Hr^T + Hi Hi^T, Im R += Hi Hr^T +
-// (-Hr) Hi^T). The tile is double-buffered by chunk parity: a buffer is written
-// again two chunks later, after a barrier every wave reaches after its MFMAs
-// on it. Each thread's words of the next chunk are loaded during this one.
-template <int MP>
-__global__ __launch_bounds__(kHB) __attribute__((amdgpu_waves_per_eu(MP == 64 ? 3 : 1))) void gspat_r16(HoloArgs a) {
-    static_assert(MP >= 32, "one 16x16 tile per wave at least");
-    constexpr int T = MP / 16;
-    constexpr int TPW = T * T / 4;
-    constexpr int HS = kKC + 8;
-    __shared__ __attribute__((aligned(16))) _Float16 s_hr[2][MP * HS];
-    __shared__ __attribute__((aligned(16))) _Float16 s_hi[2][MP * HS];
-    const int t = threadIdx.x;
-    const int w = t >> 6, l = t & 63;
-    const uint32_t k = blockIdx.x / a.split, part = blockIdx.x % a.split, b = a.inst[k];
-    const uint32_t n_chunks = a.n_packed / kKC;
-    const uint32_t c_begin = part * n_chunks / a.split, c_end = (part + 1) * n_chunks / a.split;
-    const uint32_t M = a.offsets[b + 1] - a.offsets[b];
-    const uint32_t* G16 = reinterpret_cast<const uint32_t*>(a.G) + (uint64_t)k * MP * a.n_packed + (t & 31);
-    const float* dk = a.d + (uint64_t)k * a.n_packed + (t & 31);
-    floatx4 accRe[TPW], accIm[TPW];
-#pragma unroll
-    for (int s = 0; s < TPW; ++s) {
-        accRe[s] = floatx4{0.f, 0.f, 0.f, 0.f};
-        accIm[s] = accRe[s];
-    }
-    // the thread's words (rows (t >> 5) + 8 r, column j0 + t % 32), d_j and la_j
-    uint32_t nx[MP / 8];
-    float nd = 0.f, nla = 0.f;
-    auto fetch = [&](uint32_t j0) {
-#pragma unroll
-        for (int r = 0; r < MP / 8; ++r) {
-            const int i = (t >> 5) + 8 * r;
-            nx[r] = i < (int)M ? __builtin_nontemporal_load(G16 + (uint64_t)i * a.n_packed + j0) : 0u;
-        }
-        nd = dk[j0];
-        nla = a.staged[3 * (uint64_t)(j0 + (t & 31))].w;
-    };
-    if (c_begin < c_end) fetch(c_begin * kKC);
-    for (uint32_t chunk = c_begin; chunk < c_end; ++chunk) {
-        const uint32_t j0 = chunk * kKC;
-        const int pb = (chunk - c_begin) & 1;
-        uint32_t x[MP / 8];
-#pragma unroll
-        for (int r = 0; r < MP / 8; ++r) x[r] = nx[r];
-        const float dj = nd, laj = nla;
-        if (chunk + 1 < c_end) fetch(j0 + kKC);
-        const float f = (j0 + (t & 31) < a.n) ? ldexpf(1.0f / dj, -gen16_scale(laj)) : 0.f;
-#pragma unroll
-        for (int r = 0; r < MP / 8; ++r) {
-            const int i = (t >> 5) + 8 * r;
-            const float hre = (float)__builtin_bit_cast(_Float16, (uint16_t)(x[r] & 0xffffu)) * f;
-            const float him = (float)__builtin_bit_cast(_Float16, (uint16_t)(x[r] >> 16)) * f;
-            s_hr[pb][i * HS + (t & 31)] = (_Float16)hre;
-            s_hi[pb][i * HS + (t & 31)] = (_Float16)him;
-        }
-        lds_sync();
-        const int kb = (l >> 4) * 8;
-#pragma unroll
-        for (int s = 0; s < TPW; ++s) {
-            const int tau = w * TPW + s;
-            const int arow = (tau / T) * 16 + (l & 15);
-            const int brow = (tau % T) * 16 + (l & 15);
-            const half8 ar = *reinterpret_cast<const half8*>(&s_hr[pb][arow * HS + kb]);
-            const half8 ai = *reinterpret_cast<const half8*>(&s_hi[pb][arow * HS + kb]);
-            const half8 br = *reinterpret_cast<const half8*>(&s_hr[pb][brow * HS + kb]);
-            const half8 bi = *reinterpret_cast<const half8*>(&s_hi[pb][brow * HS + kb]);
-            const half8 nar = -ar;
-            accRe[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ar, br, accRe[s], 0, 0, 0);
-            accRe[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ai, bi, accRe[s], 0, 0, 0);
-            accIm[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ai, br, accIm[s], 0, 0, 0);
-            accIm[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(nar, bi, accIm[s], 0, 0, 0);
-        }
-    }
-    float2* R = a.R + (uint64_t)blockIdx.x * MP * MP;  // partial `part` of instance k
-#pragma unroll
-    for (int s = 0; s < TPW; ++s) {
-        const int tau = w * TPW + s;
-        const int col = (tau % T) * 16 + (l & 15);
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int row = (tau / T) * 16 + 4 * (l >> 4) + e;
-            R[row * MP + col] = make_float2(accRe[s][e], accIm[s][e]);
-        }
-    }
+    a.d[(uint64_t)k * a.n_packed + j] = d;
 }
 
 // A2 for FP32 mode (MP >= 32): R = H H^H over the split's source chunk range
@@ -866,6 +639,481 @@ __global__ __launch_bounds__(kHB) void gspat_iter(HoloArgs a, uint32_t iters, in
     }
 }
 
+// ---------------------------------------------------------------- kernel A, fused (MFMA modes)
+// One workgroup of T = MP / 16 waves per instance and source-chunk range. Wave
+// w owns the foci 16 w .. 16 w + 15 (row tile w): lane l evaluates the terms
+// of focus 16 w + l % 16 at the 8 sources 8 (l / 16) .. + 7 of each chunk of
+// 32, which is where v_mfma_f32_16x16x32_f16 expects a row of H in its A and B
+// operands. Per chunk:
+//   1. the 8 terms (sources staged in LDS, one chunk ahead); a tile whose
+//      rows are all padding (>= M) skips them;
+//   2. d_j = sum_i |G_ij|: the 16 rows of the wave by DPP (every lane of the
+//      row group ends with the same bits), then the T waves' partials through
+//      LDS in wave order;
+//   3. H = s G / d_j (s = 2^floor(log2 M) / 2, a power of two that keeps the
+//      low halves in the FP16 normal range) split into FP16 hi + lo: the
+//      wave's own operands stay in registers, and are written to LDS for the
+//      other waves; G (FP32) or H hi (FP16) is stored for kernel C;
+//   4. R += hi hi^H + hi lo^H + lo hi^H on MFMA with FP32 accumulation, over
+//      the wave's tiles (w, (w + d) mod T): d = 0 .. (T - 1) / 2, plus d = T / 2
+//      for w < T / 2, so each unordered pair of row tiles is formed once, with
+//      the wave's own registers as the A operand. A wave with one tile fewer
+//      forms a duplicate (never written) so every wave runs the same
+//      straight-line MFMA sequence.
+// R (the partial of this source range, times 1 / s^2) is written in full: each
+// tile and, off the diagonal, its conjugate transpose.
+template <int T>
+struct FusedTiles {
+    static constexpr int kMax = 1 + (T - 1) / 2 + (T % 2 == 0 ? 1 : 0);
+};
+
+__device__ __forceinline__ uint32_t pack_h2_bits(_Float16 re, _Float16 im) {
+    return (uint32_t)__builtin_bit_cast(uint16_t, re) | ((uint32_t)__builtin_bit_cast(uint16_t, im) << 16);
+}
+
+template <int SRC, int MP, bool G32>
+__global__ __launch_bounds__(4 * MP) void gspat_fused(HoloArgs a) {
+    constexpr int T = MP / 16;
+    constexpr int NTW = FusedTiles<T>::kMax;
+    constexpr int RS = kKC + 8;  // LDS row stride in halves (80 bytes)
+    __shared__ std::conditional_t<SRC == kT4010A1, CellsLds, float> s_cells_mem;
+    CellsLds* s_cells = reinterpret_cast<CellsLds*>(&s_cells_mem);
+    __shared__ float4 s_src[2][3][kKC];                            // staged sources of a chunk, by float4 index
+    __shared__ __attribute__((aligned(16))) float s_part[T][kKC];  // per-wave column sums of |G|
+    __shared__ __attribute__((aligned(16))) _Float16 s_b[4][MP * RS];  // hi re, hi im, lo re, lo im
+    const int t = threadIdx.x;
+    const int w = t >> 6, l = t & 63, r16 = l & 15, g = l >> 4;
+    const uint32_t k = blockIdx.x / a.split, part = blockIdx.x % a.split, b = a.inst[k];
+    const uint32_t n_chunks = a.n_packed / kKC;
+    const uint32_t c_begin = part * n_chunks / a.split, c_end = (part + 1) * n_chunks / a.split;
+    const uint32_t base = a.offsets[b], M = a.offsets[b + 1] - base;
+    if (SRC == kT4010A1) {
+        const float4* gc = reinterpret_cast<const float4*>(a.cells);
+        for (int j = t; j < kDirCells; j += 4 * MP) {
+            s_cells->p[j] = gc[2 * j];
+            s_cells->q[j] = gc[2 * j + 1];
+        }
+    }
+    // the lane's focus (rows >= M are padding: weight 0)
+    const int row = 16 * w + r16;
+    float px = 0.f, py = 0.f, pz = 1.f, wt = 0.f;
+    if (row < (int)M) {
+        const uint64_t q = 3 * (uint64_t)(base + row);
+        px = a.foci[q];
+        py = a.foci[q + 1];
+        pz = a.foci[q + 2];
+        wt = 1.f;
+    }
+    const Split3 pf = split3(px, py, pz);
+    const bool live = 16 * w < (int)M;  // wave-uniform
+    const float scale = fmaxf(1.0f, exp2f(floorf(log2f((float)M)) - 1.0f));
+    // the chunk's staged sources: thread t < 3 kKC loads float4 t of the chunk
+    constexpr int kStageT = 3 * kKC;
+    float4 pre = make_float4(0.f, 0.f, 0.f, 0.f);
+    auto fetch_src = [&](uint32_t chunk) {
+        if (t < kStageT) pre = a.staged[(uint64_t)chunk * kKC * 3 + t];
+    };
+    auto put_src = [&](int buf) {
+        if (t < kStageT) s_src[buf][t % 3][t / 3] = pre;
+    };
+    if (c_begin < c_end) {
+        fetch_src(c_begin);
+        put_src(0);
+        if (c_begin + 1 < c_end) fetch_src(c_begin + 1);
+    }
+    __syncthreads();
+    // tiles of this wave: J = (w + d) mod T
+    int tJ[NTW];
+#pragma unroll
+    for (int s = 0; s < NTW; ++s) {
+        int d = s;
+        if (s == NTW - 1 && T % 2 == 0 && w >= T / 2) d = 0;  // the duplicate slot
+        tJ[s] = (w + d) % T;
+    }
+    const bool has_last = !(T % 2 == 0 && w >= T / 2);
+    floatx4 accRe[NTW], accIm[NTW];
+#pragma unroll
+    for (int s = 0; s < NTW; ++s) {
+        accRe[s] = floatx4{0.f, 0.f, 0.f, 0.f};
+        accIm[s] = accRe[s];
+    }
+    float* d_out = a.d + (uint64_t)k * a.n_packed;
+    for (uint32_t chunk = c_begin; chunk < c_end; ++chunk) {
+        const uint32_t j0 = chunk * kKC;
+        const int buf = (chunk - c_begin) & 1;
+        // 1. terms of the lane's focus at sources j0 + 8 g + v
+        float re[8], im[8], am[8];
+#pragma unroll
+        for (int v = 0; v < 8; ++v) re[v] = im[v] = am[v] = 0.f;
+        if (live) {
+#pragma unroll
+            for (int v = 0; v < 8; ++v) {
+                const int c = 8 * g + v;
+                const float4 q0 = s_src[buf][0][c], q1 = s_src[buf][1][c], q2 = s_src[buf][2][c];
+                const SrcF sv = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
+                add_terms_exact<SRC, 1>(sv, split3(q0.x, q0.y, q0.z), &pf, s_cells->p, s_cells->q, &re[v], &im[v], &wt,
+                                        &am[v]);
+            }
+        }
+        // 2. column sums over the wave's 16 rows, then over the waves
+        float cs[8];
+#pragma unroll
+        for (int v = 0; v < 8; ++v) cs[v] = row_group_sum<16>(am[v]);
+        if (r16 == 0) {
+            *reinterpret_cast<float4*>(&s_part[w][8 * g]) = make_float4(cs[0], cs[1], cs[2], cs[3]);
+            *reinterpret_cast<float4*>(&s_part[w][8 * g + 4]) = make_float4(cs[4], cs[5], cs[6], cs[7]);
+        }
+        lds_sync();
+        // every wave is past its reads of s_src[buf ^ 1] (chunk - 1) and of s_b
+        if (chunk + 1 < c_end) {
+            put_src(buf ^ 1);
+            if (chunk + 2 < c_end) fetch_src(chunk + 2);
+        }
+        float d[8];
+#pragma unroll
+        for (int v = 0; v < 8; ++v) d[v] = 0.f;
+#pragma unroll
+        for (int ww = 0; ww < T; ++ww) {
+            const float4 p0 = *reinterpret_cast<const float4*>(&s_part[ww][8 * g]);
+            const float4 p1 = *reinterpret_cast<const float4*>(&s_part[ww][8 * g + 4]);
+            d[0] += p0.x;
+            d[1] += p0.y;
+            d[2] += p0.z;
+            d[3] += p0.w;
+            d[4] += p1.x;
+            d[5] += p1.y;
+            d[6] += p1.z;
+            d[7] += p1.w;
+        }
+        if (w == 0 && r16 == 0) {
+            float4* dp = reinterpret_cast<float4*>(d_out + j0 + 8 * g);
+            dp[0] = make_float4(d[0], d[1], d[2], d[3]);
+            dp[1] = make_float4(d[4], d[5], d[6], d[7]);
+        }
+        // 3. H = s G / d_j split into FP16 hi + lo (a real column with d = 0
+        // gives NaN, as conj(G) / 0 does in the reference)
+        half8 ahr, ahi, alr, ali;
+#pragma unroll
+        for (int v = 0; v < 8; ++v) {
+            // 1 / d from v_rcp_f32 and one Newton step (within an ulp)
+            float r = __builtin_amdgcn_rcpf(d[v]);
+            r = fmaf(r, fmaf(-d[v], r, 1.0f), r);
+            const float inv = (j0 + 8 * g + v < a.n) ? scale * r : 0.f;
+            const float hr = re[v] * inv, hi = im[v] * inv;
+            const _Float16 hr16 = (_Float16)hr, hi16 = (_Float16)hi;
+            ahr[v] = hr16;
+            ahi[v] = hi16;
+            alr[v] = (_Float16)(hr - (float)hr16);
+            ali[v] = (_Float16)(hi - (float)hi16);
+        }
+        {
+            const int o = row * RS + 8 * g;
+            *reinterpret_cast<half8*>(&s_b[0][o]) = ahr;
+            *reinterpret_cast<half8*>(&s_b[1][o]) = ahi;
+            *reinterpret_cast<half8*>(&s_b[2][o]) = alr;
+            *reinterpret_cast<half8*>(&s_b[3][o]) = ali;
+        }
+        // kernel C's copy of the tile, rows < M: FP32 G, or the hi half of H
+        if (row < (int)M) {
+            if constexpr (G32) {
+                floatx4* gp = reinterpret_cast<floatx4*>(a.G + ((uint64_t)k * MP + row) * a.n_packed + j0 + 8 * g);
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    __builtin_nontemporal_store(floatx4{re[2 * v], im[2 * v], re[2 * v + 1], im[2 * v + 1]}, gp + v);
+            } else {
+                uintx4* hp = reinterpret_cast<uintx4*>(reinterpret_cast<uint32_t*>(a.G) +
+                                                       ((uint64_t)k * MP + row) * a.n_packed + j0 + 8 * g);
+                __builtin_nontemporal_store(uintx4{pack_h2_bits(ahr[0], ahi[0]), pack_h2_bits(ahr[1], ahi[1]),
+                                                   pack_h2_bits(ahr[2], ahi[2]), pack_h2_bits(ahr[3], ahi[3])},
+                                            hp);
+                __builtin_nontemporal_store(uintx4{pack_h2_bits(ahr[4], ahi[4]), pack_h2_bits(ahr[5], ahi[5]),
+                                                   pack_h2_bits(ahr[6], ahi[6]), pack_h2_bits(ahr[7], ahi[7])},
+                                            hp + 1);
+            }
+        }
+        lds_sync();
+        // 4. R tiles: A = this wave's rows (registers), B = rows of tile J
+        const half8 nhr = -ahr, nlr = -alr;
+#pragma unroll
+        for (int s = 0; s < NTW; ++s) {
+            half8 bhr = ahr, bhi = ahi, blr = alr, bli = ali;
+            if (tJ[s] != w) {
+                const int o = (16 * tJ[s] + r16) * RS + 8 * g;
+                bhr = *reinterpret_cast<const half8*>(&s_b[0][o]);
+                bhi = *reinterpret_cast<const half8*>(&s_b[1][o]);
+                blr = *reinterpret_cast<const half8*>(&s_b[2][o]);
+                bli = *reinterpret_cast<const half8*>(&s_b[3][o]);
+            }
+            // the small cross products first
+            accRe[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahr, blr, accRe[s], 0, 0, 0);
+            accRe[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(alr, bhr, accRe[s], 0, 0, 0);
+            accRe[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi, bli, accRe[s], 0, 0, 0);
+            accRe[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ali, bhi, accRe[s], 0, 0, 0);
+            accIm[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi, blr, accIm[s], 0, 0, 0);
+            accIm[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ali, bhr, accIm[s], 0, 0, 0);
+            accIm[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(nhr, bli, accIm[s], 0, 0, 0);
+            accIm[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(nlr, bhi, accIm[s], 0, 0, 0);
+            accRe[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahr, bhr, accRe[s], 0, 0, 0);
+            accRe[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi, bhi, accRe[s], 0, 0, 0);
+            accIm[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi, bhr, accIm[s], 0, 0, 0);
+            accIm[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(nhr, bhi, accIm[s], 0, 0, 0);
+        }
+    }
+    // lane l holds rows 4 (l / 16) + e, column l % 16 of each tile
+    const float unscale = 1.0f / (scale * scale);  // exact: scale is a power of two
+    float2* R = a.R + (uint64_t)blockIdx.x * MP * MP;
+#pragma unroll
+    for (int s = 0; s < NTW; ++s) {
+        if (s == NTW - 1 && !has_last) break;
+        const int J = tJ[s];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int ri = 16 * w + 4 * g + e, cj = 16 * J + r16;
+            const float2 v = make_float2(accRe[s][e] * unscale, accIm[s][e] * unscale);
+            R[ri * MP + cj] = v;
+            if (J != w) R[cj * MP + ri] = make_float2(v.x, -v.y);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- kernel A, one wave per instance row set
+typedef _Float16 half4 __attribute__((ext_vector_type(4)));
+
+// D += A B on the 16 x 16 MFMA whose K matches the operand width: 32
+// (v_mfma_f32_16x16x32_f16, 8 halves per lane) or 16 (v_mfma_f32_16x16x16_f16,
+// 4 halves). Lane l supplies A[l % 16][K / 4 (l / 16) + v] and B[K / 4 (l / 16) +
+// v][l % 16] and receives D[4 (l / 16) + e][l % 16] in both.
+__device__ __forceinline__ floatx4 mfma16(half8 a, half8 b, floatx4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ floatx4 mfma16(half4 a, half4 b, floatx4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
+}
+
+// For MP <= 64 (T = MP / 16 <= 4 row tiles): a workgroup of 4 waves takes one
+// instance and one source-chunk range; wave w handles the chunks c_begin + w,
+// c_begin + w + 4, ... of the range, each for all MP foci, so the column sums
+// d_j and every MFMA operand stay inside the wave. A chunk of 32 sources runs as
+// 32 / K steps of K = 4 SPL columns (SPL = 8 sources per lane, K = 32, up to
+// two row tiles; SPL = 4, K = 16, for four, which keeps the wave within 256
+// registers). Lane l evaluates focus 16 t + l % 16 (t < T) at the sources
+// SPL (l / 16) .. + SPL - 1 of the step: each staged source is read once and
+// used for the T foci. Per step:
+//   1. the SPL T terms (tiles whose rows are all padding skip them);
+//   2. d_j: the lane's T rows in order, then the 16 rows of the lane group by
+//      DPP (every lane of the group ends with the same bits);
+//   3. kernel C's copy (FP32 G, or the hi half of H) for rows < M;
+//   4. H = s G / d_j (s = 2^floor(log2 M) / 2) split into FP16 hi + lo;
+//   5. R(I, J) += hi_I hi_J^H + hi_I lo_J^H + lo_I hi_J^H on MFMA (FP32
+//      accumulation) for the upper-triangle tiles I <= J with live rows.
+// Sources are staged in LDS kSB chunks at a time, loaded a block ahead and
+// written after the block (one barrier per block). At the end the four waves'
+// partial R are summed in LDS in wave order, and the workgroup writes R (times
+// 1 / s^2) with the lower tiles as conjugate transposes.
+template <int SRC, int MP, bool G32>
+__global__ __launch_bounds__(kHB) void gspat_wave(HoloArgs a) {
+    constexpr int T = MP / 16;
+    constexpr int NU = T * (T + 1) / 2;  // upper-triangle tiles
+    constexpr int SPL = T >= 4 ? 4 : 8;  // sources per lane per step
+    constexpr int KS = 4 * SPL;          // columns per step
+    using hvec = std::conditional_t<SPL == 8, half8, half4>;
+    static_assert(T >= 1 && T <= 4, "one wave holds at most 4 row tiles");
+    __shared__ std::conditional_t<SRC == kT4010A1, CellsLds, float> s_cells_mem;
+    CellsLds* s_cells = reinterpret_cast<CellsLds*>(&s_cells_mem);
+    constexpr size_t kSrcBytes = 2 * sizeof(SrcBlock), kRBytes = (size_t)MP * MP * sizeof(float2);
+    __shared__ __attribute__((aligned(16))) unsigned char s_buf[kSrcBytes > kRBytes ? kSrcBytes : kRBytes];
+    SrcBlock* s_src = reinterpret_cast<SrcBlock*>(s_buf);
+    float2* s_R = reinterpret_cast<float2*>(s_buf);
+    const int t = threadIdx.x;
+    const int w = t >> 6, l = t & 63, r16 = l & 15, g = l >> 4;
+    const uint32_t k = blockIdx.x / a.split, part = blockIdx.x % a.split, b = a.inst[k];
+    const uint32_t n_chunks = a.n_packed / kKC;
+    const uint32_t c_begin = part * n_chunks / a.split, c_end = (part + 1) * n_chunks / a.split;
+    const uint32_t base = a.offsets[b], M = a.offsets[b + 1] - base;
+    if (SRC == kT4010A1) stage_cells_lds(a.cells, s_cells);
+    Split3 pf[T];
+    float wt[T];
+    bool live[T];
+#pragma unroll
+    for (int q = 0; q < T; ++q) {
+        const int row = 16 * q + r16;
+        live[q] = 16 * q < (int)M;  // wave-uniform
+        float px = 0.f, py = 0.f, pz = 1.f;
+        wt[q] = 0.f;
+        if (row < (int)M) {
+            const uint64_t o = 3 * (uint64_t)(base + row);
+            px = a.foci[o];
+            py = a.foci[o + 1];
+            pz = a.foci[o + 2];
+            wt[q] = 1.f;
+        }
+        pf[q] = split3(px, py, pz);
+    }
+    const float scale = fmaxf(1.0f, exp2f(floorf(log2f((float)M)) - 1.0f));
+    floatx4 accRe[NU], accIm[NU];
+#pragma unroll
+    for (int s = 0; s < NU; ++s) {
+        accRe[s] = floatx4{0.f, 0.f, 0.f, 0.f};
+        accIm[s] = accRe[s];
+    }
+    float* d_out = a.d + (uint64_t)k * a.n_packed;
+    float4 pre[3];
+    if (c_begin < c_end) {
+        src_block_fetch(a, c_begin, pre);
+        src_block_put(s_src[0], pre);
+        if (c_begin + kSB < c_end) src_block_fetch(a, c_begin + kSB, pre);
+    }
+    __syncthreads();
+    for (uint32_t c0 = c_begin; c0 < c_end; c0 += kSB) {
+        const int bb = ((c0 - c_begin) / kSB) & 1;
+        for (uint32_t cb = w; cb < kSB && c0 + cb < c_end; cb += 4) {
+#pragma unroll
+            for (int h = 0; h < kKC / KS; ++h) {
+                const uint32_t jb = (c0 + cb) * kKC + h * KS + SPL * g;  // the lane's first column
+                // 1. terms; cs: the lane's rows in order
+                float re[T][SPL], im[T][SPL], cs[SPL];
+#pragma unroll
+                for (int v = 0; v < SPL; ++v) {
+                    cs[v] = 0.f;
+                    const int j = cb * kKC + h * KS + SPL * g + v;
+                    const float4 q0 = s_src[bb].v[0][j], q1 = s_src[bb].v[1][j], q2 = s_src[bb].v[2][j];
+                    const SrcF sv = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
+                    const Split3 ss = split3(q0.x, q0.y, q0.z);
+#pragma unroll
+                    for (int q = 0; q < T; ++q) {
+                        re[q][v] = im[q][v] = 0.f;
+                        if (live[q]) {
+                            float am;
+                            add_terms_exact<SRC, 1>(sv, ss, &pf[q], s_cells->p, s_cells->q, &re[q][v], &im[q][v],
+                                                    &wt[q], &am);
+                            cs[v] += am;
+                        }
+                    }
+                }
+                // 2. d_j over the 16 lanes of the group
+                float d[SPL];
+#pragma unroll
+                for (int v = 0; v < SPL; ++v) d[v] = row_group_sum<16>(cs[v]);
+                if (r16 == 0) {
+                    floatx4* dp = reinterpret_cast<floatx4*>(d_out + jb);
+#pragma unroll
+                    for (int v = 0; v < SPL / 4; ++v) dp[v] = floatx4{d[4 * v], d[4 * v + 1], d[4 * v + 2], d[4 * v + 3]};
+                }
+                // 3. kernel C's copy, rows < M (FP32 G here; H16 after the split)
+                if constexpr (G32) {
+#pragma unroll
+                    for (int q = 0; q < T; ++q) {
+                        const int row = 16 * q + r16;
+                        if (row < (int)M) {
+                            floatx4* gp = reinterpret_cast<floatx4*>(a.G + ((uint64_t)k * MP + row) * a.n_packed + jb);
+#pragma unroll
+                            for (int v = 0; v < SPL / 2; ++v)
+                                __builtin_nontemporal_store(
+                                    floatx4{re[q][2 * v], im[q][2 * v], re[q][2 * v + 1], im[q][2 * v + 1]}, gp + v);
+                        }
+                    }
+                }
+                // 4. H = s G / d_j in FP16 hi + lo (a real column with d = 0
+                // gives NaN, as conj(G) / 0 does in the reference)
+                float inv[SPL];
+#pragma unroll
+                for (int v = 0; v < SPL; ++v) {
+                    float r = __builtin_amdgcn_rcpf(d[v]);
+                    r = fmaf(r, fmaf(-d[v], r, 1.0f), r);
+                    inv[v] = (jb + v < a.n) ? scale * r : 0.f;
+                }
+                hvec hr[T], hi[T], lr[T], li[T];
+#pragma unroll
+                for (int q = 0; q < T; ++q) {
+#pragma unroll
+                    for (int v = 0; v < SPL; ++v) {
+                        const float xr = re[q][v] * inv[v], xi = im[q][v] * inv[v];
+                        const _Float16 rh = (_Float16)xr, ih = (_Float16)xi;
+                        hr[q][v] = rh;
+                        hi[q][v] = ih;
+                        lr[q][v] = (_Float16)(xr - (float)rh);
+                        li[q][v] = (_Float16)(xi - (float)ih);
+                    }
+                    if constexpr (!G32) {
+                        const int row = 16 * q + r16;
+                        if (row < (int)M) {
+                            uintx4* hp = reinterpret_cast<uintx4*>(reinterpret_cast<uint32_t*>(a.G) +
+                                                                   ((uint64_t)k * MP + row) * a.n_packed + jb);
+#pragma unroll
+                            for (int v = 0; v < SPL / 4; ++v)
+                                __builtin_nontemporal_store(
+                                    uintx4{pack_h2_bits(hr[q][4 * v], hi[q][4 * v]),
+                                           pack_h2_bits(hr[q][4 * v + 1], hi[q][4 * v + 1]),
+                                           pack_h2_bits(hr[q][4 * v + 2], hi[q][4 * v + 2]),
+                                           pack_h2_bits(hr[q][4 * v + 3], hi[q][4 * v + 3])},
+                                    hp + v);
+                        }
+                    }
+                }
+                // 5. the upper-triangle tiles with live rows
+                int s = 0;
+#pragma unroll
+                for (int I = 0; I < T; ++I) {
+                    const hvec nhr = -hr[I], nlr = -lr[I];
+#pragma unroll
+                    for (int J = I; J < T; ++J, ++s) {
+                        if (!(live[I] && live[J])) continue;
+                        accRe[s] = mfma16(hr[I], lr[J], accRe[s]);
+                        accRe[s] = mfma16(lr[I], hr[J], accRe[s]);
+                        accRe[s] = mfma16(hi[I], li[J], accRe[s]);
+                        accRe[s] = mfma16(li[I], hi[J], accRe[s]);
+                        accIm[s] = mfma16(hi[I], lr[J], accIm[s]);
+                        accIm[s] = mfma16(li[I], hr[J], accIm[s]);
+                        accIm[s] = mfma16(nhr, li[J], accIm[s]);
+                        accIm[s] = mfma16(nlr, hi[J], accIm[s]);
+                        accRe[s] = mfma16(hr[I], hr[J], accRe[s]);
+                        accRe[s] = mfma16(hi[I], hi[J], accRe[s]);
+                        accIm[s] = mfma16(hi[I], hr[J], accIm[s]);
+                        accIm[s] = mfma16(nhr, hi[J], accIm[s]);
+                    }
+                }
+            }
+        }
+        // every wave is past its reads of this block: stage the next one
+        if (c0 + kSB < c_end) {
+            src_block_put(s_src[bb ^ 1], pre);
+            if (c0 + 2 * kSB < c_end) src_block_fetch(a, c0 + 2 * kSB, pre);
+        }
+        lds_sync();
+    }
+    // the four waves' partials summed in LDS in wave order; lane l holds rows
+    // 4 (l / 16) + e, column l % 16 of each tile
+    for (int ww = 0; ww < 4; ++ww) {
+        if (w == ww) {
+            int s = 0;
+#pragma unroll
+            for (int I = 0; I < T; ++I)
+#pragma unroll
+                for (int J = I; J < T; ++J, ++s)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const int ri = 16 * I + 4 * g + e, cj = 16 * J + r16;
+                        float2 v = make_float2(accRe[s][e], accIm[s][e]);
+                        if (ww > 0) {
+                            const float2 o = s_R[ri * MP + cj];
+                            v = make_float2(o.x + v.x, o.y + v.y);
+                        }
+                        s_R[ri * MP + cj] = v;
+                    }
+        }
+        lds_sync();
+    }
+    const float unscale = 1.0f / (scale * scale);  // exact: scale is a power of two
+    float2* R = a.R + (uint64_t)blockIdx.x * MP * MP;
+    for (int e = t; e < MP * MP; e += kHB) {
+        const int ri = e / MP, cj = e % MP;
+        // the upper triangle of tiles holds the sums; lower tiles by symmetry
+        const bool up = ri / 16 <= cj / 16;
+        const float2 v = up ? s_R[e] : s_R[cj * MP + ri];
+        R[e] = make_float2(v.x * unscale, up ? v.y * unscale : -v.y * unscale);
+    }
+}
+
 // ---------------------------------------------------------------- kernel C
 // q = B p over a range of source chunks: a.split workgroups per instance,
 // each with the chunk range kernel A's workgroup of the same index had.
@@ -906,6 +1154,8 @@ __global__ __launch_bounds__(kHB) void gspat_q(HoloArgs a, int closing, int pass
         }
     }
     lds_sync();
+    // the fused kernel A's scale of H (a power of two)
+    const float hscale = fmaxf(1.0f, exp2f(floorf(log2f((float)M)) - 1.0f));
 
     float2 gam[MP / 8];
 #pragma unroll
@@ -919,12 +1169,13 @@ __global__ __launch_bounds__(kHB) void gspat_q(HoloArgs a, int closing, int pass
     const __amdgpu_buffer_rsrc_t gr = __builtin_amdgcn_make_buffer_rsrc(
         reinterpret_cast<char*>(a.G) + (uint64_t)k * MP * a.n_packed * kElem, (short)0,
         (int)(M * a.n_packed * kElem), 0x00020000);
-    // one chunk of columns from its G tile (FP32, H16 words widened), d_j and la_j
-    auto run_chunk = [&](const float2 (&g)[MP / 8], float dj, float laj, uint32_t chunk) {
+    // one chunk of columns from its G tile (FP32, H16 words widened) and d_j
+    auto run_chunk = [&](const float2 (&g)[MP / 8], float dj, uint32_t chunk) {
         const uint32_t j0 = chunk * kKC;
         const int pb = (chunk - c_begin) & 1, slot = (chunk - c_begin) % 8;
         // q_j = w_j sum_i conj(G_ij) p_i: this row group's partial, then the 8
-        // groups in a fixed order; w_j = 1 / d_j^2 (FP32 G) or 2^-s_j / d_j^2 (H16)
+        // groups in a fixed order; w_j = 1 / d_j^2 (FP32 G) or 1 / (s d_j) (H16,
+        // the copy holds H = s G / d_j)
         float2 pq = make_float2(0.f, 0.f);
 #pragma unroll
         for (int r = 0; r < MP / 8; ++r) {
@@ -941,8 +1192,7 @@ __global__ __launch_bounds__(kHB) void gspat_q(HoloArgs a, int closing, int pass
                 q.x += s_qp[pb][q8][t].x;
                 q.y += s_qp[pb][q8][t].y;
             }
-            const float w = (j0 + t < a.n) ? (H16 ? ldexpf(1.0f / (dj * dj), -gen16_scale(laj)) : 1.0f / (dj * dj))
-                                           : 0.f;
+            const float w = (j0 + t < a.n) ? (H16 ? 1.0f / (hscale * dj) : 1.0f / (dj * dj)) : 0.f;
             q = make_float2(q.x * w, q.y * w);
             if (pass == kQGamma)
                 s_q[pb][t] = q;
@@ -989,9 +1239,9 @@ __global__ __launch_bounds__(kHB) void gspat_q(HoloArgs a, int closing, int pass
                 const uint32_t chunk = c0 + u;
                 float2 g[MP / 8];
                 chunk_g<MP, H16>(cq[u], g);
-                const float dj = cq[u].d, laj = cq[u].la;
+                const float dj = cq[u].d;
                 load_chunk<MP, H16>(a, gr, k, clamp_chunk(chunk + PD) * kKC, M, cq[u]);
-                if (chunk < c_end) run_chunk(g, dj, laj, chunk);  // wave-uniform
+                if (chunk < c_end) run_chunk(g, dj, chunk);  // wave-uniform
             }
         }
     } else {
@@ -1000,9 +1250,9 @@ __global__ __launch_bounds__(kHB) void gspat_q(HoloArgs a, int closing, int pass
         for (uint32_t chunk = c_begin; chunk < c_end; ++chunk) {
             float2 g[MP / 8];
             chunk_g<MP, H16>(cq, g);
-            const float dj = cq.d, laj = cq.la;
+            const float dj = cq.d;
             if (chunk + 1 < c_end) load_chunk<MP, H16>(a, gr, k, (chunk + 1) * kKC, M, cq);
-            run_chunk(g, dj, laj, chunk);
+            run_chunk(g, dj, chunk);
         }
     }
     if (pass == kQGamma) {
@@ -1364,8 +1614,8 @@ size_t gspat_workspace_bound(uint32_t n_packed, uint32_t batch) {
            al(B * mp * sp * sizeof(float2)) + al(B * np * sizeof(float));
 }
 
-static int padded_m(uint32_t m, bool mfma) {
-    if (m <= 16) return mfma ? 32 : 16;
+static int padded_m(uint32_t m) {
+    if (m <= 16) return 16;
     if (m <= 32) return 32;
     if (m <= 64) return 64;
     if (m <= 128) return 128;
@@ -1381,9 +1631,9 @@ static hipError_t launch_gspat_bucket(HoloArgs a, int src, int precision, uint32
         if constexpr (MP >= 32) {
             // A1 (FP32 G, one thread per source column) then A2 on FP32 MFMA
             if (src == kT4010A1)
-                hipLaunchKernelGGL((gspat_gen<kT4010A1, MP, true>), grid_g, blk, 0, st, a);
+                hipLaunchKernelGGL((gspat_gen<kT4010A1, MP>), grid_g, blk, 0, st, a);
             else
-                hipLaunchKernelGGL((gspat_gen<kSphere, MP, true>), grid_g, blk, 0, st, a);
+                hipLaunchKernelGGL((gspat_gen<kSphere, MP>), grid_g, blk, 0, st, a);
             hipLaunchKernelGGL((gspat_r32<MP>), grid_a, blk, 0, st, a);
         } else if (src == kT4010A1) {
             hipLaunchKernelGGL((gspat_r_fp32<kT4010A1, MP>), grid_a, blk, 0, st, a);
@@ -1391,27 +1641,34 @@ static hipError_t launch_gspat_bucket(HoloArgs a, int src, int precision, uint32
             hipLaunchKernelGGL((gspat_r_fp32<kSphere, MP>), grid_a, blk, 0, st, a);
         }
     } else {
-        if constexpr (MP >= 32) {
-            if (precision == AFH_HOLO_MIXED) {
-                // A1 (G, one thread per source column) then A2 (R from A1's FP16 copy)
-                if (src == kT4010A1)
-                    hipLaunchKernelGGL((gspat_gen<kT4010A1, MP, false>), grid_g, blk, 0, st, a);
+        // kernel A: G, d_j and the split-FP16 MFMA R in one pass; MIXED keeps
+        // the FP16 H for kernel C, the other modes FP32 G
+        const bool g32 = precision != AFH_HOLO_MIXED;
+        if constexpr (MP <= 64) {
+            if (src == kT4010A1) {
+                if (g32)
+                    hipLaunchKernelGGL((gspat_wave<kT4010A1, MP, true>), grid_a, blk, 0, st, a);
                 else
-                    hipLaunchKernelGGL((gspat_gen<kSphere, MP, false>), grid_g, blk, 0, st, a);
-                hipLaunchKernelGGL((gspat_r16<MP>), grid_a, blk, 0, st, a);
-            } else if (src == kT4010A1) {
-                if (precision == AFH_HOLO_MIXED_SPLIT)
-                    hipLaunchKernelGGL((gspat_r_mfma<kT4010A1, MP, true>), grid_a, blk, 0, st, a);
-                else
-                    hipLaunchKernelGGL((gspat_r_mfma<kT4010A1, MP, false>), grid_a, blk, 0, st, a);
+                    hipLaunchKernelGGL((gspat_wave<kT4010A1, MP, false>), grid_a, blk, 0, st, a);
             } else {
-                if (precision == AFH_HOLO_MIXED_SPLIT)
-                    hipLaunchKernelGGL((gspat_r_mfma<kSphere, MP, true>), grid_a, blk, 0, st, a);
+                if (g32)
+                    hipLaunchKernelGGL((gspat_wave<kSphere, MP, true>), grid_a, blk, 0, st, a);
                 else
-                    hipLaunchKernelGGL((gspat_r_mfma<kSphere, MP, false>), grid_a, blk, 0, st, a);
+                    hipLaunchKernelGGL((gspat_wave<kSphere, MP, false>), grid_a, blk, 0, st, a);
             }
         } else {
-            return hipErrorInvalidValue;
+            const dim3 blk_f(4 * MP);
+            if (src == kT4010A1) {
+                if (g32)
+                    hipLaunchKernelGGL((gspat_fused<kT4010A1, MP, true>), grid_a, blk_f, 0, st, a);
+                else
+                    hipLaunchKernelGGL((gspat_fused<kT4010A1, MP, false>), grid_a, blk_f, 0, st, a);
+            } else {
+                if (g32)
+                    hipLaunchKernelGGL((gspat_fused<kSphere, MP, true>), grid_a, blk_f, 0, st, a);
+                else
+                    hipLaunchKernelGGL((gspat_fused<kSphere, MP, false>), grid_a, blk_f, 0, st, a);
+            }
         }
     }
     const int closing = precision == AFH_HOLO_MIXED_CLOSING ? 1 : 0;
@@ -1470,7 +1727,7 @@ constexpr int kBucketMP[4] = {16, 32, 64, 128};
 
 static int bucket_of(const HoloBatch& hb, uint32_t b) {
     const uint32_t m = hb.offsets_host[b + 1] - hb.offsets_host[b];
-    const int mp = padded_m(m, hb.precision != AFH_HOLO_FP32 && hb.algo == kAlgoGSPAT);
+    const int mp = padded_m(m);
     for (int q = 0; q < 4; ++q)
         if (kBucketMP[q] == mp) return q;
     return -1;
@@ -1746,7 +2003,7 @@ static char* holo_scratch(hipStream_t st) {
     build_dir_cells(host);
     WordUploader up(st, p);
     const uint32_t* words = reinterpret_cast<const uint32_t*>(host);
-    for (size_t i = 0; i < sizeof(host) / sizeof(uint32_t); ++i) up.push(words[i]);
+    for (size_t i = 0; i < kDirCells * sizeof(DirCell) / sizeof(uint32_t); ++i) up.push(words[i]);
     up.flush();
     // the table must be in place before a call on another stream reads it
     if (!up.ok || !hip_ok(hipStreamSynchronize(st), "hipStreamSynchronize")) return nullptr;
