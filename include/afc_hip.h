@@ -225,12 +225,15 @@ int32_t AFH_CreateContext(void** out);
 void AFH_DestroyContext(void* ctx);
 
 enum { AFH_ALGO_GSPAT = 0, AFH_ALGO_GS = 1, AFH_ALGO_NAIVE = 2 };
-/* GS-PAT precision: FP32 forms R = G diag(1/d^2) G^H on the VALU; MIXED forms
- * it from an FP16 copy of G scaled per column by 1/d (|entries| <= 1) on MFMA
- * with FP32 accumulation; MIXED_CLOSING additionally redoes the last R p
- * product as G (B p) in FP32 before q = B p; MIXED_SPLIT carries each FP16
- * entry as a hi + lo pair and forms hi hi^H + hi lo^H + lo hi^H on MFMA
- * (near-FP32 R). GS and Naive are FP32 only. */
+/* GS-PAT precision. FP32 forms R = G diag(1/d^2) G^H in FP32 (FP32 MFMA, or
+ * the VALU for at most 16 foci) and q = B p from FP32 G. The mixed modes form
+ * R in one pass with G on FP16 MFMA with FP32 accumulation, each entry of
+ * H = s G / d (per-column scaling, |H| <= s) carried as an FP16 hi + lo pair
+ * (hi hi^H + hi lo^H + lo hi^H, near-FP32 R); MIXED then takes q = B p from
+ * the FP16 hi copy of H, MIXED_SPLIT from FP32 G, and MIXED_CLOSING redoes the
+ * last R p product as G (B p) in FP32 before q = B p. Every GS-PAT mode takes
+ * the source-focus distance from the exact coordinate difference. GS and
+ * Naive are FP32 only. */
 enum { AFH_HOLO_FP32 = 0, AFH_HOLO_MIXED = 1, AFH_HOLO_MIXED_CLOSING = 2, AFH_HOLO_MIXED_SPLIT = 3 };
 
 /* Solves `batch` instances. packed_sources_dev: AFH_PackSources layout
