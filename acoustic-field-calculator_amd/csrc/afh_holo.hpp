@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include <mutex>
+#include <vector>
 
 #include "afh_host.hpp"
 
@@ -55,18 +56,31 @@ struct HoloBatch {
 };
 
 // Stream + grow-only device workspace; calls on one context are serialized.
+// A small batch runs its padded-focus buckets side by side on the context's
+// auxiliary streams (created on first use, with cu_mask when one is set),
+// forked from and joined back into the caller's stream by events.
 struct Context {
     std::mutex mu;
     hipStream_t stream = nullptr;
     DevBuf work, io;
     HostBuf staging, io_host;
     hipEvent_t upload_done = nullptr;
+    std::vector<uint32_t> cu_mask;  // CU mask of the auxiliary streams (empty: all CUs)
+    hipStream_t aux[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t aux_done[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t fork = nullptr;
     bool ensure_stream() {
         if (stream) return true;
         return hip_ok(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
     }
+    bool ensure_aux();
     bool reserve_staging(size_t bytes);
     ~Context() {
+        for (int i = 0; i < 4; ++i) {
+            if (aux[i]) (void)hipStreamDestroy(aux[i]);
+            if (aux_done[i]) (void)hipEventDestroy(aux_done[i]);
+        }
+        if (fork) (void)hipEventDestroy(fork);
         if (upload_done) (void)hipEventDestroy(upload_done);
         if (stream) (void)hipStreamDestroy(stream);
         work.release();

@@ -1797,9 +1797,31 @@ static void greedy_candidate(uint32_t q, uint32_t div, float2& c, float& ph) {
     ph = atan2f(c.y, c.x);
 }
 
+bool Context::ensure_aux() {
+    if (fork) return true;
+    for (int i = 0; i < 4; ++i) {
+        bool made = false;
+        if (!cu_mask.empty()) {
+            made = hipExtStreamCreateWithCUMask(&aux[i], (uint32_t)cu_mask.size(), cu_mask.data()) == hipSuccess;
+            if (!made) (void)hipGetLastError();
+        }
+        if (!made && !hip_ok(hipStreamCreateWithFlags(&aux[i], hipStreamNonBlocking), "hipStreamCreate")) return false;
+        if (!hip_ok(hipEventCreateWithFlags(&aux_done[i], hipEventDisableTiming), "hipEventCreate")) return false;
+    }
+    return hip_ok(hipEventCreateWithFlags(&fork, hipEventDisableTiming), "hipEventCreate");
+}
+
+// Batches up to this many instances run their buckets side by side (they are
+// independent); larger ones fill the GPU bucket by bucket, where a second
+// stream only queued behind the first and added cross-stream waits.
+constexpr uint32_t kForkMaxBatch = 256;
+
 // Launches the solver on a workspace whose offsets, instance lists and Greedy
-// candidates are already on the device at wbase (laid out by `pl`).
-static int32_t run_holo(const HoloBatch& hb, const HoloPlan& pl, char* wbase, const DirCell* cells, hipStream_t st) {
+// candidates are already on the device at wbase (laid out by `pl`). With a
+// context (ctx not null), a small GS-PAT or GS batch runs its buckets on the
+// context's auxiliary streams.
+static int32_t run_holo(const HoloBatch& hb, const HoloPlan& pl, char* wbase, const DirCell* cells, hipStream_t st,
+                        Context* ctx = nullptr) {
     const uint32_t B = hb.batch, np = hb.n_packed;
     HoloArgs a;
     memset(&a, 0, sizeof(a));
@@ -1821,11 +1843,20 @@ static int32_t run_holo(const HoloBatch& hb, const HoloPlan& pl, char* wbase, co
     hipLaunchKernelGGL(stage_holo_sources, dim3((np + kHB - 1) / kHB), dim3(kHB), 0, st, hb.sources, np,
                        reinterpret_cast<float4*>(wbase + pl.o_stg));
     if (!hip_ok(hipGetLastError(), "holo staging launch")) return AFH_ERR_HIP;
+    int buckets = 0;
+    for (int q = 0; q < 4; ++q) buckets += pl.nb[q] > 0;
+    const bool fork = ctx && buckets > 1 && B <= kForkMaxBatch && (hb.algo == kAlgoGSPAT || hb.algo == kAlgoGS);
+    if (fork && (!ctx->ensure_aux() || !hip_ok(hipEventRecord(ctx->fork, st), "hipEventRecord"))) return AFH_ERR_HIP;
+    const hipStream_t caller = st;
     size_t list_pos = 0, r_pos = 0, p_pos = 0, g_pos = 0, q_pos = 0, d_pos = 0;
     for (int q = 0; q < 4; ++q) {
         const uint32_t nb = pl.nb[q];
         const int mp = kBucketMP[q];
         if (nb == 0) continue;
+        if (fork) {
+            st = ctx->aux[q];
+            if (!hip_ok(hipStreamWaitEvent(st, ctx->fork, 0), "hipStreamWaitEvent")) return AFH_ERR_HIP;
+        }
         a.inst = reinterpret_cast<const uint32_t*>(wbase + pl.o_list) + list_pos;
         a.R = reinterpret_cast<float2*>(wbase + pl.o_r) + r_pos;
         a.p = reinterpret_cast<float2*>(wbase + pl.o_p) + p_pos;
@@ -1864,7 +1895,11 @@ static int32_t run_holo(const HoloBatch& hb, const HoloPlan& pl, char* wbase, co
         }
         if (!hip_ok(e, "holo kernel launch")) return AFH_ERR_HIP;
         list_pos += nb;
+        if (fork && (!hip_ok(hipEventRecord(ctx->aux_done[q], st), "hipEventRecord") ||
+                     !hip_ok(hipStreamWaitEvent(caller, ctx->aux_done[q], 0), "hipStreamWaitEvent")))
+            return AFH_ERR_HIP;
     }
+    st = caller;
     if (hb.algo == kAlgoGreedy) {
         const float2* cand = reinterpret_cast<const float2*>(wbase + pl.o_cand);
         const float* cph = reinterpret_cast<const float*>(wbase + pl.o_cand + (size_t)pl.div * sizeof(float2));
@@ -1925,7 +1960,7 @@ int32_t holo_solve(Context* ctx, const HoloBatch& hb, hipStream_t st) {
     if (!ctx->upload_done && !hip_ok(hipEventCreateWithFlags(&ctx->upload_done, hipEventDisableTiming), "event"))
         return AFH_ERR_HIP;
     if (!hip_ok(hipEventRecord(ctx->upload_done, st), "hipEventRecord")) return AFH_ERR_HIP;
-    return run_holo(hb, pl, wbase, dir_cells_device(), st);
+    return run_holo(hb, pl, wbase, dir_cells_device(), st, ctx);
 }
 
 // ---------------------------------------------------------------- allocation-free entry point

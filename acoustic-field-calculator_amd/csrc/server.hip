@@ -149,6 +149,7 @@ struct Server {
     // pointers requests took at submission, which a push_back does not move
     std::vector<std::unique_ptr<SrvArray>> arrays;
     std::mutex mu;
+    std::mutex add_mu;  // serializes AFS_AddArray (an array's id picks its GS-PAT context)
     std::condition_variable cv_submit, cv_done;
     std::deque<Request*> pending;
     std::unordered_map<uint64_t, Request*> by_ticket;
@@ -185,8 +186,11 @@ struct Server {
         for (auto& t : copiers)
             if (t.joinable()) t.join();
         for (auto s : streams) (void)hipStreamSynchronize(s);
+        for (auto s : gextra) (void)hipStreamSynchronize(s);
         ctxs.clear();
+        gextra_ctx.clear();
         for (auto s : streams) (void)hipStreamDestroy(s);
+        for (auto s : gextra) (void)hipStreamDestroy(s);
         for (auto& a : arrays)
             if (a->packed) (void)hipFree(a->packed);
         if (dev_base) (void)hipFree(dev_base);
@@ -194,11 +198,24 @@ struct Server {
         for (auto& kv : by_ticket) delete kv.second;  // the worker drained `pending` first
     }
 
-    // Stream 0 carries the GS-PAT batches (on its own CU partition when the
-    // configuration reserves one), so a solve never waits behind a large field
-    // evaluation; a field request goes to the field stream with the least
-    // queued work (all streams when there is only one).
-    hipStream_t gspat_stream() { return streams[0]; }
+    // GS-PAT batches run on stream 0 and kGspatExtra more streams of their own
+    // (on the CU partition when the configuration reserves one), so a solve
+    // never waits behind a large field evaluation and the batches of different
+    // arrays run side by side: array a on GS-PAT stream a mod (1 + kGspatExtra),
+    // each with its own context (workspace). A field request goes to the field
+    // stream with the least queued work (all streams when there is only one).
+    static constexpr int kGspatExtra = 2;
+    std::vector<hipStream_t> gextra;
+    std::vector<std::unique_ptr<Context>> gextra_ctx;
+    int gspat_slots() const { return 1 + (int)gextra.size(); }
+    hipStream_t gspat_stream(uint32_t array_id) {
+        const int i = (int)(array_id % gspat_slots());
+        return i == 0 ? streams[0] : gextra[i - 1];
+    }
+    Context* gspat_ctx(uint32_t array_id) {
+        const int i = (int)(array_id % gspat_slots());
+        return i == 0 ? ctxs[0].get() : gextra_ctx[i - 1].get();
+    }
     int field_stream_index() {
         if (streams.size() == 1) return 0;
         int best = 1;
@@ -305,8 +322,8 @@ struct Server {
             dev.release(f.dev_off, f.dev_bytes);
             return false;
         }
-        hipStream_t st = gspat_stream();
-        Context* ctx = ctxs[stream_index(st)].get();
+        hipStream_t st = gspat_stream(array_id);
+        Context* ctx = gspat_ctx(array_id);
         char* d = dev_base + f.dev_off;
         char* h = pin_base + f.pin_off;
         std::vector<uint32_t> offs(cnt + 1, 0);
@@ -488,28 +505,44 @@ int32_t AFS_Create(const AFS_Config* cfg, void** out) {
     int n_cu = 0;
     (void)hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, s->device);
     const bool part = cfg->gspat_cus > 0 && cfg->n_streams > 1 && n_cu > (int)cfg->gspat_cus && n_cu <= 1024;
-    for (uint32_t i = 0; i < cfg->n_streams; ++i) {
-        hipStream_t st = nullptr;
+    std::vector<uint32_t> gmask;  // the GS-PAT partition (empty: all CUs)
+    auto make_stream = [&](bool gspat, hipStream_t* st) {
         bool made = false;
         if (part) {
             std::vector<uint32_t> mask((n_cu + 31) / 32, 0u);
             for (int c = 0; c < n_cu; ++c) {
-                const bool mine = (i == 0) ? c < (int)cfg->gspat_cus : c >= (int)cfg->gspat_cus;
+                const bool mine = gspat ? c < (int)cfg->gspat_cus : c >= (int)cfg->gspat_cus;
                 if (mine) mask[c / 32] |= 1u << (c % 32);
             }
-            made = hipExtStreamCreateWithCUMask(&st, (uint32_t)mask.size(), mask.data()) == hipSuccess;
+            made = hipExtStreamCreateWithCUMask(st, (uint32_t)mask.size(), mask.data()) == hipSuccess;
             if (!made) (void)hipGetLastError();
+            if (made && gspat) gmask = mask;
         }
-        if (!made && !hip_ok(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "AFS_Create: stream"))
-            return AFH_ERR_HIP;
+        return made || hip_ok(hipStreamCreateWithFlags(st, hipStreamNonBlocking), "AFS_Create: stream");
+    };
+    for (uint32_t i = 0; i < cfg->n_streams; ++i) {
+        hipStream_t st = nullptr;
+        if (!make_stream(i == 0, &st)) return AFH_ERR_HIP;
         s->streams.push_back(st);
         s->outstanding.push_back(0.0);
         s->ctxs.emplace_back(new Context());
     }
+    for (int i = 0; i < Server::kGspatExtra; ++i) {
+        hipStream_t st = nullptr;
+        if (!make_stream(true, &st)) return AFH_ERR_HIP;
+        s->gextra.push_back(st);
+        s->gextra_ctx.emplace_back(new Context());
+    }
+    s->ctxs[0]->cu_mask = gmask;
+    for (auto& c : s->gextra_ctx) c->cu_mask = gmask;
+    // the auxiliary streams of small batches now, not on the first fork while serving
+    for (int i = 0; i < s->gspat_slots(); ++i)
+        if (!s->gspat_ctx(i)->ensure_aux()) return AFH_ERR_HIP;
     // the GS-PAT staging for a full batch is reserved now rather than grown
     // while serving; the workspace, which depends on the array size, when an
     // array is added
-    if (!s->ctxs[0]->staging.reserve((2 * (size_t)cfg->max_gspat_batch + 2) * 4)) return AFH_ERR_HIP;
+    for (int i = 0; i < s->gspat_slots(); ++i)
+        if (!s->gspat_ctx(i)->staging.reserve((2 * (size_t)cfg->max_gspat_batch + 2) * 4)) return AFH_ERR_HIP;
     if (!dir_cells_device()) {
         set_error("AFS_Create: directivity table");
         return AFH_ERR_HIP;
@@ -532,6 +565,7 @@ int32_t AFS_AddArray(void* srv, const void* sources, const float* wavenums, cons
         set_error("AFS_AddArray: bad argument");
         return AFH_ERR_ARG;
     }
+    std::lock_guard<std::mutex> add_lock(s->add_mu);
     SrvArray A;
     A.source_type = source_type;
     A.n = (uint32_t)n;
@@ -548,11 +582,16 @@ int32_t AFS_AddArray(void* srv, const void* sources, const float* wavenums, cons
         // the GS-PAT workspace for a full batch of 128-focus instances on this
         // array, reserved now rather than grown while serving (growing frees the
         // old block, so work queued on the GS-PAT stream must finish first)
-        Context* ctx = s->ctxs[0].get();
+        uint32_t next_id;
+        {
+            std::lock_guard<std::mutex> g(s->mu);
+            next_id = (uint32_t)s->arrays.size();
+        }
+        Context* ctx = s->gspat_ctx(next_id);
         std::lock_guard<std::mutex> g(ctx->mu);
         const size_t need = gspat_workspace_bound(A.np, s->cfg.max_gspat_batch);
         if (need > ctx->work.cap &&
-            (!hip_ok(hipStreamSynchronize(s->streams[0]), "AFS_AddArray: hipStreamSynchronize") ||
+            (!hip_ok(hipStreamSynchronize(s->gspat_stream(next_id)), "AFS_AddArray: hipStreamSynchronize") ||
              !ctx->work.reserve(need))) {
             (void)hipFree(A.packed);
             return AFH_ERR_HIP;

@@ -108,6 +108,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-qps", default="500,1000,2000,3000")
     ap.add_argument("--serve-seconds", type=float, default=1.5)
+    ap.add_argument("--serve-gspat-qps", default="10000,40000,80000,120000,160000,240000,320000")
     ap.add_argument("--serve-gspat-cus", type=int, default=32, help="CUs reserved for GS-PAT batches while serving")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device-to-device copy (reference for HBM byte counters)")
@@ -232,6 +233,8 @@ def main():
     if not args.no_serve:
         result["serve"] = bench_serve(D, [float(q) for q in args.serve_qps.split(",")], args.serve_seconds,
                                       args.serve_gspat_cus)
+        result["serve_gspat"] = bench_serve_gspat(afc, D, [float(q) for q in args.serve_gspat_qps.split(",")],
+                                                  args.serve_seconds)
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline(src, k, a, area["order"], cnt, o, area["res"])
@@ -336,6 +339,113 @@ def bench_serve(D, qps_list, seconds, gspat_cus=32):
     out["max_qps_p99_le_5ms"] = max((r["qps_offered"] for r in ok), default=None)
     okg = [r for r in out["runs"] if r["gspat_p99_ms"] <= 5.0]
     out["max_qps_gspat_p99_le_5ms"] = max((r["qps_offered"] for r in okg), default=None)
+    return out
+
+
+def bench_serve_gspat(afc, D, qps_list, seconds, precisions=(("mixed", 1), ("fp32", 0)), plain_calls=200):
+    """Served GS-PAT alone: Poisson arrivals of GS-PAT requests (M ~ U[2, 128],
+    foci over each array, targets U[0.5, 1]) on the 249-, 996- and
+    3984-transducer arrays chosen uniformly, at a ramp of offered rates until
+    the p99 latency passes 5 ms; the same ramp with the server in FP32 mode;
+    and the plain path, AFO_GSPAT one request per call, on requests drawn from
+    the same distribution."""
+    from ctypes import byref, c_double, c_uint64, c_void_p
+    from afc import workloads as W
+    from afc.server import Server
+
+    rng = np.random.default_rng(11)
+    arrs = []
+    for tiles in ((1, 1), (2, 2), (4, 4)):
+        src, k, a = W.array_rows(tiles, phase_seed=249 * tiles[0] * tiles[1])
+        arrs.append((src, k, a, src[:, 0:3].max(0)))
+    # a pool of request specs, reused cyclically: array, foci, targets
+    pool = []
+    for _ in range(2048):
+        aid = int(rng.integers(0, 3))
+        m = int(rng.integers(2, 129))
+        ext = arrs[aid][3]
+        foci = np.ascontiguousarray(np.stack([rng.uniform(0, ext[0], m), rng.uniform(0, ext[1], m),
+                                              rng.uniform(100, 250, m)], 1).astype(np.float32))
+        amps = np.ascontiguousarray(rng.uniform(0.5, 1.0, m).astype(np.float32))
+        pool.append((aid, m, foci, amps))
+    ring = 4096
+    outs = [np.empty((ring, 2, a[0].shape[0]), np.float32) for a in arrs]
+    dll = afc.dll()
+    out = {"workload": "GS-PAT requests only: Poisson arrivals, M ~ U[2,128], arrays 249/996/3984 uniform, "
+                       "100 iterations; p99 over the requests of each rate", "runs": {}}
+    for name, prec in precisions:
+        runs = []
+        for qps in qps_list:
+            srv = Server(hbm_budget=2 << 30, pinned_budget=1 << 30, n_streams=2, max_gspat_batch=256,
+                         gspat_precision=prec, gspat_cus=0)
+            ids = [srv.add_array(1, src, k, a) for src, k, a, _ in arrs]
+            for aid, (src, k, a, ext) in enumerate(arrs):
+                for m in (4, 20, 40, 100):
+                    f = pool[0][2][:1].repeat(m, 0) + np.arange(m, dtype=np.float32)[:, None]
+                    srv.wait(srv.submit_gspat(ids[aid], f, np.ones(m, np.float32), np.empty(len(src), np.float32),
+                                              np.empty(len(src), np.float32)))
+            srv.stats(reset=True)
+            nreq = max(1, int(qps * seconds))
+            arrivals = np.cumsum(rng.exponential(1.0 / qps, nreq))
+            args = []
+            for i in range(nreq):
+                aid, m, foci, amps = pool[i % len(pool)]
+                buf = outs[aid][i % ring]
+                args.append((ids[aid], foci.ctypes.data, amps.ctypes.data, m, buf[0].ctypes.data, buf[1].ctypes.data))
+            tk = c_uint64()
+            ref = byref(tk)
+            h = srv.handle
+            submit = dll.AFS_SubmitGSPAT
+            tickets = []
+            t0 = time.perf_counter()
+            for i in range(nreq):
+                target = t0 + arrivals[i]
+                while time.perf_counter() < target:
+                    pass
+                aid_, fp, ap, m, oa, op = args[i]
+                if submit(h, aid_, fp, ap, m, oa, op, ref) != 0:
+                    raise RuntimeError("AFS_SubmitGSPAT failed")
+                tickets.append(tk.value)
+            t_sub = time.perf_counter() - t0
+            lat = c_double()
+            for t in tickets:
+                dll.AFS_Wait(h, t, byref(lat))
+            elapsed = time.perf_counter() - t0
+            p50, p99, done = srv.stats(kind=1)
+            srv.close()
+            run = {"qps_offered": qps, "requests": nreq, "submitted_per_s": round(nreq / t_sub, 1),
+                   "served_solves_per_s": round(nreq / elapsed, 1), "p50_ms": round(p50, 3), "p99_ms": round(p99, 3)}
+            runs.append(run)
+            log(f"serve gspat {name} qps {qps}: {run}")
+            if p99 > 5.0 or nreq / t_sub < 0.9 * qps:
+                break
+        ok = [r for r in runs if r["p99_ms"] <= 5.0]
+        out["runs"][name] = runs
+        out[f"{name}_max_solves_per_s_p99_le_5ms"] = max((r["served_solves_per_s"] for r in ok), default=None)
+    # the plain path: AFO_GSPAT, one request per call, the same request distribution
+    systems = []
+    for src, k, a, _ in arrs:
+        s = afc.UniformSystem(W.TEMP_340)
+        for row in src:
+            s.add_wave_source(afc.T4010A1(row[0:3], row[3:6], row[6], row[7], row[8]))
+        systems.append(s)
+    lat = []
+    for i in range(plain_calls):
+        aid, m, foci, amps = pool[i]
+        sa = systems[aid].source_array()
+        sa[:, 6] = 1.0
+        t0 = time.perf_counter()
+        afc.Optimizer.gs_pat(systems[aid], foci, amps)
+        lat.append(time.perf_counter() - t0)
+    out["plain_afo_gspat"] = {"calls": plain_calls, "solves_per_s": round(plain_calls / sum(lat), 1),
+                              "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                              "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3)}
+    best = out.get("mixed_max_solves_per_s_p99_le_5ms")
+    if best:
+        out["served_mixed_vs_plain_afo_gspat"] = round(best / out["plain_afo_gspat"]["solves_per_s"], 2)
+        if out.get("fp32_max_solves_per_s_p99_le_5ms"):
+            out["served_mixed_vs_served_fp32"] = round(best / out["fp32_max_solves_per_s_p99_le_5ms"], 2)
+    log(f"serve gspat: {json.dumps({k: v for k, v in out.items() if k != 'runs'})}")
     return out
 
 
