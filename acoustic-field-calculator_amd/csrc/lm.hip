@@ -589,8 +589,8 @@ __global__ __launch_bounds__(kLB) void lm_solve(LmArgs a) {
     }
     double F, gmax, amax;
     linearize(F, gmax, amax);
-    if (a.algo == kNlsGN) {
-        // gauss_newton.rs:49-84
+    // Gauss-Newton, gauss_newton.rs:49-84 (LM runs in lm_wide)
+    {
         bool found = gmax <= EPS_1;
         uint32_t it = 0;
         for (; it < a.k_max; ++it) {
@@ -632,83 +632,526 @@ __global__ __launch_bounds__(kLB) void lm_solve(LmArgs a) {
         if (t == 0 && a.out_iters) a.out_iters[b] = it;
         return;
     }
-    // the Gram matrix must survive rejected steps: keep a copy of its lower
-    // triangle in the upper triangle of s_K (mirrored), the factor goes low
-    for (uint32_t e = t; e < D * D; e += kLB) {
-        const uint32_t p = e / D, q = e % D;
-        if (q > p) s_K[p * kLmMaxD + q] = s_K[q * kLmMaxD + p];
+}
+
+// ---------------------------------------------------------------- LM, wide
+// The LM iteration (lm.rs:40-105) on a workgroup of kLW threads (8 waves)
+// per instance. The chain of one iteration is latency-bound (a few hundred
+// FP64 flops per thread between barriers), so every step is spread over all
+// waves: the columns l of x, g, h and e^{ix} over the threads (one or two
+// per thread for n <= 2 kLW, their loads in flight together); the focus rows
+// of the focal field over the waves; the Gram tiles over the waves. W = [Re C;
+// Im C] (C = B diag(e^{ix})) is never stored: g, h and the Gram matrix form
+// it from G and e^{ix}, so that an instance's working set is G and a few
+// n-vectors (it stays in L2 for the 996-source array, where G and W did not).
+// Besides:
+//   - g = W^T v is formed in the same pass over G as diag(A) (for mu0, first
+//     linearization only: |e^{ix}| = 1, so it never changes);
+//   - v comes from the focal field of the accepted point, which the
+//     acceptance test has just computed;
+//   - x / x_new and e^{ix} / e^{ix_new} swap roles on acceptance;
+//   - the Gram matrix goes to the lower triangle of s_K and its diagonal to
+//     s_gd, where it survives rejected steps; the L D L^T solve reads
+//     mu I + Gram from there and writes L to its own array.
+// Every reduction has a fixed order; the iteration count, stopping rules and
+// mu/nu updates are the reference's.
+constexpr int kLW = 512;
+constexpr int kLWaves = kLW / 64;
+constexpr int kLmPart = kLmMaxD * (kLmMaxD + 1);  // doubles of the second Gram staging buffer
+
+// Fixed-order workgroup sums (maxima) of NV values: lanes by xor steps, then
+// the wave results by xor steps within each 16-lane row (lane q of a row
+// reads wave q's value, lanes past the last wave the identity). Every thread
+// gets the same bits.
+template <int NV, bool MAX>
+__device__ __forceinline__ void lw_reduce(double* v, double* red) {
+    static_assert(kLWaves <= 16, "one wave result per lane of a DPP row");
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = MAX ? wave_max_f64(v[k]) : wave_sum_f64(v[k]);
+    const int w = threadIdx.x >> 6, q = threadIdx.x & 15;
+    lds_barrier();
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) red[k * kLWaves + w] = v[k];
+    lds_barrier();
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        double u = q < kLWaves ? red[k * kLWaves + q] : (MAX ? -INFINITY : 0.0);
+        u = MAX ? fmax(u, xor_lane_f64<8>(u)) : u + xor_lane_f64<8>(u);
+        u = MAX ? fmax(u, xor_lane_f64<4>(u)) : u + xor_lane_f64<4>(u);
+        u = MAX ? fmax(u, xor_lane_f64<2>(u)) : u + xor_lane_f64<2>(u);
+        u = MAX ? fmax(u, xor_lane_f64<1>(u)) : u + xor_lane_f64<1>(u);
+        v[k] = u;
+    }
+}
+
+// s_i = sum_l B_il E_l: the rows i = w, w + kLWaves, ... of wave w together
+// (one load of E_j for all of them), lanes over the columns, each row summed
+// in column order and reduced over the lanes.
+__device__ inline void lw_field(const double2* __restrict__ G, const double2* __restrict__ E, uint32_t N, uint32_t M,
+                                const float* amps, double2* s_out) {
+    constexpr int R = (kLmMaxM + kLWaves - 1) / kLWaves;
+    const uint32_t w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const uint32_t nr = M > w ? (M - w + kLWaves - 1) / kLWaves : 0;  // rows of this wave
+    double re[R], im[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) re[r] = im[r] = 0.0;
+#pragma unroll 2
+    for (uint32_t j = l; j < N; j += 64) {
+        const double2 e = E[j];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if ((uint32_t)r < nr) {
+                const double2 gv = G[(uint64_t)(w + r * kLWaves) * N + j];
+                re[r] += gv.x * e.x - gv.y * e.y;
+                im[r] += gv.x * e.y + gv.y * e.x;
+            }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        if ((uint32_t)r < nr) {
+            const uint32_t i = w + r * kLWaves;
+            const double sr = wave_sum_f64(re[r]), si = wave_sum_f64(im[r]);
+            if (l == 0) {
+                const double2 e = E[N + i];  // the -diag(amps) block of B: column N + i
+                const double am = (double)amps[i];
+                s_out[i] = make_double2(sr - am * e.x, si - am * e.y);
+            }
+        }
+}
+
+// Gram matrix W W^T for W = [Re C; Im C], C = B diag(E) (D = 2M rows, n =
+// N + M columns), with FP64 MFMA (v_mfma_f64_16x16x4f64 as in gram_mfma),
+// without W in memory. The N columns of G go through LDS in chunks of cw
+// (64 for M <= 16, 32 above, 2 elements of C per thread): all threads form the chunk's W (thread per element of C, G read
+// coalesced along the columns) into one of two buffers while the waves run
+// the MFMAs of the previous chunk on the other, one barrier per chunk; the
+// next chunk's G and E are loaded before those MFMAs. Wave w accumulates
+// unit u = w (and u = w + kLWaves while u < ntiles): lower-triangle tile
+// u % ntiles over the 4-column blocks kb = u / ntiles (mod S) of every
+// chunk, S = the number of units of that tile. The units' tiles are summed
+// in unit order, then the last M columns (-diag(amps), one nonzero C_i,N+i
+// each) are added to the entries they touch: (i, i), (M + i, M + i) and
+// (M + i, i). Writes the off-diagonal entries to the lower triangle
+// (sK[p * kLmMaxD + q], q < p) and the diagonal to sgd. buf0 and buf1 hold
+// kLmMaxD x kGramWS doubles each.
+constexpr int kGramWS = 65;  // LDS row stride of a chunk (rows spread over the banks)
+constexpr int kGramPer = 2;  // elements of C per thread and chunk
+
+__device__ __attribute__((noinline)) void lw_gram(const double2* __restrict__ G, const double2* __restrict__ E, uint32_t N, uint32_t M,
+                        const float* amps, double* sK, double* sgd, double* buf0, double* buf1) {
+    const int t = threadIdx.x, w = t >> 6, l = t & 63;
+    const uint32_t D = 2 * M;
+    const int T = (int)(D + 15) / 16, ntiles = T * (T + 1) / 2;
+    const int units = ntiles > kLWaves ? ntiles : kLWaves;
+    const uint32_t cw = M <= 16 ? 64 : 32;  // M x cw = kGramPer x kLW
+    const uint32_t nchunks = (N + cw - 1) / cw;
+    const uint32_t kq = l >> 4;
+    // chunk staging: element e = t + kLW j of the M x kGramCW block of C
+    double2 pg[kGramPer], pe[kGramPer];
+    auto fetch = [&](uint32_t c) {
+#pragma unroll
+        for (int j = 0; j < kGramPer; ++j) {
+            const uint32_t e = t + kLW * j, i = e / cw, k = c * cw + e % cw;
+            const bool in = i < M && k < N;
+            pg[j] = in ? G[(uint64_t)i * N + k] : make_double2(0.0, 0.0);
+            pe[j] = in ? E[k] : make_double2(0.0, 0.0);
+        }
+    };
+    auto store = [&](double* buf) {
+#pragma unroll
+        for (int j = 0; j < kGramPer; ++j) {
+            const uint32_t e = t + kLW * j, i = e / cw, k = e % cw;
+            if (i < M) {
+                buf[i * kGramWS + k] = pg[j].x * pe[j].x - pg[j].y * pe[j].y;
+                buf[(M + i) * kGramWS + k] = pg[j].x * pe[j].y + pg[j].y * pe[j].x;
+            }
+        }
+    };
+    // the (at most two) units of this wave
+    int ut[2], uk[2], us[2], uI[2], uJ[2];
+    bool uv[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int u = w + q * kLWaves;
+        uv[q] = u < units;
+        ut[q] = u % ntiles;
+        uk[q] = u / ntiles;
+        us[q] = (units - 1 - ut[q]) / ntiles + 1;
+        int I = 0, J = ut[q];
+        while (J > I) {
+            J -= I + 1;
+            ++I;
+        }
+        uI[q] = I;
+        uJ[q] = J;
+    }
+    doublex4 acc[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+    fetch(0);
+    store(buf0);
+    lds_barrier();
+    for (uint32_t c = 0; c < nchunks; ++c) {
+        double* cur = (c & 1) ? buf1 : buf0;
+        double* nxt = (c & 1) ? buf0 : buf1;
+        if (c + 1 < nchunks) fetch(c + 1);
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            if (!uv[q]) continue;
+            const uint32_t ra = 16 * uI[q] + (l & 15), rb = 16 * uJ[q] + (l & 15);
+            const bool va = ra < D, vb = rb < D;
+            const double* pa = cur + (va ? ra : 0) * kGramWS + kq;
+            const double* pb = cur + (vb ? rb : 0) * kGramWS + kq;
+            for (int kb = uk[q]; kb < (int)cw / 4; kb += us[q]) {
+                const double a0 = va ? pa[4 * kb] : 0.0, b0 = vb ? pb[4 * kb] : 0.0;
+                acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[q], 0, 0, 0);
+            }
+        }
+        if (c + 1 < nchunks) store(nxt);
+        lds_barrier();
+    }
+    // partial tiles into buf0 (free after the last barrier), unit u at u * 256
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        if (!uv[q]) continue;
+        const int u = w + q * kLWaves;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) buf0[u * 256 + (kq + 4 * e) * 16 + (l & 15)] = acc[q][e];
+    }
+    lds_barrier();
+    for (int e = t; e < ntiles * 256; e += kLW) {
+        const int tile = e >> 8, pl = (e >> 4) & 15, ql = e & 15;
+        int I = 0, J = tile;
+        while (J > I) {
+            J -= I + 1;
+            ++I;
+        }
+        const uint32_t p = 16 * I + pl, q = 16 * J + ql;
+        if (p >= D || q > p) continue;
+        double v = buf0[tile * 256 + pl * 16 + ql];
+        for (int u = tile + ntiles; u < units; u += ntiles) v += buf0[u * 256 + pl * 16 + ql];
+        if (p == q || p == q + M) {
+            const uint32_t i0 = q < M ? q : q - M;
+            const double am = (double)amps[i0];
+            const double2 ee = E[N + i0];
+            const double cr = -am * ee.x, ci = -am * ee.y;
+            v += p != q ? ci * cr : (p < M ? cr * cr : ci * ci);
+        }
+        if (p == q)
+            sgd[p] = v;
+        else
+            sK[p * kLmMaxD + q] = v;
+    }
+}
+
+// 1 / x to FP64 rounding: v_rcp_f64 and two Newton steps.
+__device__ __forceinline__ double rcp_f64(double x) {
+    double y = __builtin_amdgcn_rcp(x);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const double e = fma(-x, y, 1.0);
+        y = fma(y, e, y);
+    }
+    return y;
+}
+
+// One wave solves (mu I + Gram) y = v (D <= DP <= 32; the Gram matrix in the
+// lower triangle of sK and its diagonal in sgd) by K = L diag(d) L^T with
+// unit lower L: lane i keeps row i in registers (rows i >= D are identity
+// rows, so the padded system is block-diagonal), the pivot and the column
+// entries of other lanes by v_readlane. Column k scales by 1/d_k (v_rcp_f64
+// and two Newton steps); the updates read the unscaled column, so they do
+// not wait for the reciprocal, and are not masked by lane: a lane's entries
+// right of its diagonal receive values that nothing reads. Branch-free:
+// every lane loads its whole row of sK (128-bit reads, fenced in groups of
+// four by an empty asm so that the compiler neither moves them into a
+// branch nor waits for each one) and selects, and writes all of its row of
+// L to sL (row stride kLmMaxD), from where the backward substitution reads
+// its column. Then
+// L z = v, L^T y = z / d. Returns false when a pivot is not positive (or
+// NaN).
+template <int DP>
+__device__ __attribute__((noinline)) bool chol_lm_wave(const double* sK, const double* sgd, double mu,
+                                                       const double* sv, double* sy, double* sL, uint32_t D) {
+    // the lane index through an empty asm keeps the per-lane masks (i < j,
+    // i == k, ...) from being hoisted into long-lived SGPR pairs
+    int i = threadIdx.x;  // < 64, the whole wave active
+    __asm__ volatile("" : "+v"(i));
+    const bool real = i < (int)D;
+    const double di = sgd[i] + mu;
+    typedef double d2v __attribute__((ext_vector_type(2)));
+    const d2v* row = reinterpret_cast<const d2v*>(sK + i * kLmMaxD);
+    d2v kv[DP / 2];
+#pragma unroll
+    for (int h = 0; h < DP / 2; ++h) kv[h] = row[h];
+#pragma unroll
+    for (int g = 0; g < DP / 8; ++g)
+        __asm__ volatile("" : "+v"(kv[4 * g]), "+v"(kv[4 * g + 1]), "+v"(kv[4 * g + 2]), "+v"(kv[4 * g + 3]));
+    double r[DP];
+#pragma unroll
+    for (int j = 0; j < DP; ++j) {
+        const double kij = kv[j / 2][j % 2];
+        r[j] = real ? (j < i ? kij : (j == i ? di : 0.0)) : (j == i ? 1.0 : 0.0);
+    }
+    bool ok = true;
+    double dinv = 1.0;
+#pragma unroll
+    for (int k = 0; k < DP; ++k) {
+        const double dkk = readlane_f64(r[k], k);
+        ok = ok && dkk > 0.0;
+        const double inv = rcp_f64(dkk);
+        dinv = i == k ? inv : dinv;
+        const double lik = r[k] * inv;
+#pragma unroll
+        for (int j = k + 1; j < DP; ++j) r[j] = fma(-lik, readlane_f64(r[k], j), r[j]);
+        r[k] = i == k ? dkk : lik;
+    }
+#pragma unroll
+    for (int j = 0; j < DP; ++j) sL[i * kLmMaxD + j] = r[j];
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    // L z = v (a lane's acc is final when its own step reads it)
+    double acc = real ? sv[i] : 0.0, z = 0.0;
+#pragma unroll
+    for (int j = 0; j < DP; ++j) {
+        const double zj = readlane_f64(acc, j);
+        z = i == j ? zj : z;
+        acc = fma(-r[j], zj, acc);
+    }
+    // L^T y = z / d with column i of L (L_ji, j > i) from sL
+    double lc[DP];
+#pragma unroll
+    for (int j = 0; j < DP; ++j) lc[j] = sL[j * kLmMaxD + i];
+    acc = z * dinv;
+    double y = 0.0;
+#pragma unroll
+    for (int j = DP - 1; j >= 0; --j) {
+        const double yj = readlane_f64(acc, j);
+        y = i == j ? yj : y;
+        acc = fma(-lc[j], yj, acc);
+    }
+    if (real) sy[i] = y;
+    return ok;
+}
+
+// The same L diag(d) L^T solve for 32 < D <= 64 with the rows in LDS (work,
+// row stride kLmMaxD + 1, so that a column spreads over the banks) and the
+// column broadcast through col.
+__device__ __attribute__((noinline)) bool chol_lm_wave_lds(const double* sK, const double* sgd, double mu,
+                                                           const double* sv, double* sy, double* col, double* work,
+                                                           uint32_t D) {
+    constexpr int DP = kLmMaxD, WS = kLmMaxD + 1;
+    int i = threadIdx.x;  // < 64
+    __asm__ volatile("" : "+v"(i));  // as in chol_lm_wave
+    const bool real = i < (int)D;
+    for (int j = 0; j < DP; ++j)
+        work[i * WS + j] = real ? (j < i ? sK[i * kLmMaxD + j] : (j == i ? sgd[i] + mu : 0.0)) : (j == i ? 1.0 : 0.0);
+    bool ok = true;
+    double dinv = 1.0;
+    for (int k = 0; k < DP; ++k) {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        const double dkk = work[k * WS + k];
+        ok = ok && dkk > 0.0;
+        const double inv = rcp_f64(dkk);
+        dinv = i == k ? inv : dinv;
+        const double rk = work[i * WS + k];
+        col[i] = rk;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        const double lik = rk * inv;
+        for (int j = k + 1; j < DP; ++j) work[i * WS + j] = fma(-lik, col[j], work[i * WS + j]);
+        work[i * WS + k] = i == k ? dkk : lik;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    double acc = real ? sv[i] : 0.0, z = 0.0;
+    for (int j = 0; j < DP; ++j) {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        if (i == j) col[j] = acc;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        const double zj = col[j];
+        z = i == j ? zj : z;
+        acc = fma(-work[i * WS + j], zj, acc);
+    }
+    acc = z * dinv;
+    double y = 0.0;
+    for (int j = DP - 1; j >= 0; --j) {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        if (i == j) col[j] = acc;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        const double yj = col[j];
+        y = i == j ? yj : y;
+        acc = fma(-work[j * WS + i], yj, acc);
+    }
+    if (real) sy[i] = y;
+    return ok;
+}
+
+// A workgroup-uniform pointer, read from the first lane (so it lives in
+// SGPRs) and passed through an empty asm: addresses derived from it are
+// recomputed where they are used instead of being hoisted out of the
+// iteration loop, where dozens of them would stay live across every step.
+template <typename T>
+__device__ __forceinline__ T* fresh(T* p) {
+    const uint64_t u = reinterpret_cast<uint64_t>(p);
+    uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)u);
+    uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(u >> 32));
+    __asm__ volatile("" : "+s"(lo), "+s"(hi));
+    return reinterpret_cast<T*>(((uint64_t)hi << 32) | lo);
+}
+
+__global__ __launch_bounds__(kLW) void lm_wide(LmArgs a) {
+    __shared__ double s_K[kLmMaxD * kLmMaxD];
+    __shared__ double s_gd[kLmMaxD];
+    __shared__ double s_v[kLmMaxD];
+    __shared__ double s_y[kLmMaxD];
+    __shared__ double2 s_s[kLmMaxM];
+    __shared__ double s_red[3 * kLWaves];
+    __shared__ double s_col[kLmMaxD];
+    __shared__ double s_work[kLmMaxD * (kLmMaxD + 1)];
+    __shared__ double s_part[kLmPart];
+    __shared__ int s_ok;
+    const int t = threadIdx.x;
+    const uint32_t b = blockIdx.x;
+    const uint32_t base = a.offsets[b], M = a.offsets[b + 1] - base;
+    const uint32_t N = a.n, n = N + M, D = 2 * M;
+    const float* amps = a.amps + base;
+    const LmScratch s = lm_scratch(a.scratch + b * a.scratch_stride, N, M);
+    double* x = s.x;
+    double* xn = s.xn;
+    double2* E = s.E;
+    double2* En = s.En;
+    const double EPS_1 = 1e-8, EPS_2 = 1e-8, TAU = 1e-3;
+
+    // From the focal field s of the current point (s_s): v = [Im s; -Re s],
+    // g = W^T v with W = [Re C; Im C], C = B diag(E), and the Gram matrix
+    // W W^T (lower triangle of s_K, diagonal in s_gd). Returns max_l g_l and,
+    // when asked, max_l diag(A)_l = max_l sum_i |B_il|^2.
+    auto linearize = [&](bool want_amax, double& gmax, double& amax) {
+        if (t < (int)M) {
+            s_v[t] = s_s[t].y;
+            s_v[M + t] = -s_s[t].x;
+        }
+        lds_barrier();
+        const double2* __restrict__ Gp = fresh(s.G);
+        double* __restrict__ gp = fresh(s.g);
+        double mx[2] = {-INFINITY, -INFINITY};
+        for (uint32_t l = t; l < n; l += kLW) {
+            const double2 e = E[l];
+            double gr = 0.0, gi = 0.0, colsq = 0.0;
+#pragma unroll 8
+            for (uint32_t i = 0; i < M; ++i) {
+                double2 bv;
+                if (l < N)
+                    bv = Gp[(uint64_t)i * N + l];
+                else
+                    bv = make_double2(l - N == i ? -(double)amps[i] : 0.0, 0.0);
+                const double cr = bv.x * e.x - bv.y * e.y, ci = bv.x * e.y + bv.y * e.x;
+                gr = fma(cr, s_v[i], gr);
+                gi = fma(ci, s_v[M + i], gi);
+                if (want_amax) colsq += cr * cr + ci * ci;
+            }
+            const double gl = gr + gi;
+            gp[l] = gl;
+            mx[0] = fmax(mx[0], gl);
+            mx[1] = fmax(mx[1], colsq);
+        }
+        lw_reduce<2, true>(mx, s_red);
+        gmax = mx[0];
+        amax = mx[1];
+        lw_gram(Gp, E, N, M, amps, s_K, s_gd, s_work, s_part);
+        lds_barrier();
+    };
+
+    for (uint32_t l = t; l < n; l += kLW) {
+        x[l] = 0.0;
+        E[l] = make_double2(1.0, 0.0);
     }
     __syncthreads();
+    lw_field(s.G, E, N, M, amps, s_s);
+    lds_barrier();
+    double F = 0.0;
+    for (uint32_t i = 0; i < M; ++i) F += s_s[i].x * s_s[i].x + s_s[i].y * s_s[i].y;
+    double gmax, amax;
+    linearize(true, gmax, amax);
     double mu = TAU * amax, nu = 2.0;
     bool found = gmax <= EPS_1;
     uint32_t it = 0;
     for (; it < a.k_max; ++it) {
         if (found) break;
-        // K = Gram + mu I into the lower triangle (from the mirrored copy)
-        for (uint32_t e = t; e < D * D; e += kLB) {
-            const uint32_t p = e / D, q = e % D;
-            if (q < p) s_K[p * kLmMaxD + q] = s_K[q * kLmMaxD + p];
-            if (q == p) s_K[p * kLmMaxD + p] = s_gd[p] + mu;
-        }
-        __syncthreads();
-        // Cholesky K = L L^T and K y = v by wave 0 (the other waves wait)
+        x = fresh(x);
+        xn = fresh(xn);
+        E = fresh(E);
+        En = fresh(En);
+        const double* __restrict__ gm = fresh(s.g);
+        // (mu I + W W^T) y = v by wave 0
         if (t < 64) {
             bool okw;
             if (D <= 16)
-                okw = chol_solve_wave<16, 0>(s_K, s_v, s_y, s_col, D);
+                okw = chol_lm_wave<16>(s_K, s_gd, mu, s_v, s_y, s_work, D);
             else if (D <= 32)
-                okw = chol_solve_wave<32, 0>(s_K, s_v, s_y, s_col, D);
+                okw = chol_lm_wave<32>(s_K, s_gd, mu, s_v, s_y, s_work, D);
             else
-                okw = chol_solve_wave_lds<0>(s_K, s_v, s_y, s_col, s_work, D);
+                okw = chol_lm_wave_lds(s_K, s_gd, mu, s_v, s_y, s_col, s_work, D);
             if (t == 0) s_ok = okw ? 1 : 0;
         }
-        __syncthreads();
+        lds_barrier();
         if (!s_ok) break;  // the reference stops when its solve fails (lm.rs:72-77)
-        // h = -W^T y; |h|, |x|, L = h.(mu h - g) / 2, x_new, e^{i x_new}
-        double hh = 0.0, xx = 0.0, lp = 0.0;
-        for (uint32_t l = t; l < n; l += kLB) {
-            double acc = 0.0;
-#pragma unroll 8
-            for (uint32_t q = 0; q < D; ++q) acc += s.W[(uint64_t)q * n + l] * s_y[q];
+        // h = -W^T y; |h|^2, |x|^2, h.(mu h - g); x_new, e^{i x_new}
+        double sums[3] = {0.0, 0.0, 0.0};
+        auto step = [&](uint32_t l, double acc) {
             const double hl = -acc;
-            s.h[l] = hl;
-            const double xl = s.x[l];
-            hh += hl * hl;
-            xx += xl * xl;
-            lp += hl * (mu * hl - s.g[l]);
-            const double xn = xl + hl;
-            s.xn[l] = xn;
+            const double xl = x[l];
+            sums[0] += hl * hl;
+            sums[1] += xl * xl;
+            sums[2] += hl * (mu * hl - gm[l]);
+            const double xv = xl + hl;
+            xn[l] = xv;
             double sn, cs;
-            sincos(xn, &sn, &cs);
-            s.En[l] = make_double2(cs, sn);
+            sincos(xv, &sn, &cs);
+            En[l] = make_double2(cs, sn);
+        };
+        // h_l = -(W^T y)_l = -sum_i (Re C_il y_i + Im C_il y_{M+i}), C_il = B_il E_l from G;
+        // columns l0 and l0 + kLW together, so that both columns' loads are in flight
+        const double2* __restrict__ Gh = fresh(s.G);
+        for (uint32_t l0 = t; l0 < n; l0 += 2 * kLW) {
+            const uint32_t l1 = l0 + kLW;
+            const bool two = l1 < n;
+            const uint32_t l1c = two ? l1 : l0;
+            const double2 e0 = E[l0], e1 = E[l1c];
+            double hr0 = 0.0, hi0 = 0.0, hr1 = 0.0, hi1 = 0.0;
+#pragma unroll 4
+            for (uint32_t i = 0; i < M; ++i) {
+                const double yr = s_y[i], yi = s_y[M + i];
+                const double am = -(double)amps[i];
+                const double2 b0 = l0 < N ? Gh[(uint64_t)i * N + l0] : make_double2(l0 - N == i ? am : 0.0, 0.0);
+                const double2 b1 = l1c < N ? Gh[(uint64_t)i * N + l1c] : make_double2(l1c - N == i ? am : 0.0, 0.0);
+                hr0 = fma(b0.x * e0.x - b0.y * e0.y, yr, hr0);
+                hi0 = fma(b0.x * e0.y + b0.y * e0.x, yi, hi0);
+                hr1 = fma(b1.x * e1.x - b1.y * e1.y, yr, hr1);
+                hi1 = fma(b1.x * e1.y + b1.y * e1.x, yi, hi1);
+            }
+            step(l0, hr0 + hi0);
+            if (two) step(l1, hr1 + hi1);
         }
-        const double hnorm = sqrt(block_sum(hh, s_red));
-        const double xnorm = sqrt(block_sum(xx, s_red));
-        const double L0 = 0.5 * block_sum(lp, s_red);
+        lw_reduce<3, false>(sums, s_red);
+        const double hnorm = sqrt(sums[0]), xnorm = sqrt(sums[1]), L0 = 0.5 * sums[2];
         if (hnorm <= EPS_2 * (xnorm + EPS_2)) {
             found = true;
             continue;
         }
-        field_at_foci(s, s.En, N, M, amps, s_s);
-        __syncthreads();
+        __syncthreads();  // e^{i x_new} before every wave reads it
+        lw_field(fresh(s.G), En, N, M, amps, s_s);
+        lds_barrier();
         double Fn = 0.0;
         for (uint32_t i = 0; i < M; ++i) Fn += s_s[i].x * s_s[i].x + s_s[i].y * s_s[i].y;
         const double rho = (F - Fn) / L0;
         F = Fn;  // assigned on rejection too (lm.rs:85)
         if (rho > 0.0) {
-            __syncthreads();
-            for (uint32_t l = t; l < n; l += kLB) {
-                s.x[l] = s.xn[l];
-                s.E[l] = s.En[l];
-            }
-            __syncthreads();
-            double Fdummy, am2;
-            linearize(Fdummy, gmax, am2);
-            for (uint32_t e = t; e < D * D; e += kLB) {
-                const uint32_t p = e / D, q = e % D;
-                if (q > p) s_K[p * kLmMaxD + q] = s_K[q * kLmMaxD + p];
-            }
-            __syncthreads();
+            double* tx = x;
+            x = xn;
+            xn = tx;
+            double2* te = E;
+            E = En;
+            En = te;
+            double am2;
+            linearize(false, gmax, am2);
             found = gmax <= EPS_1;
             const double c = 2.0 * rho - 1.0;
             mu *= fmax(1.0 / 3.0, 1.0 - c * c * c);
@@ -718,7 +1161,7 @@ __global__ __launch_bounds__(kLB) void lm_solve(LmArgs a) {
             nu *= 2.0;
         }
     }
-    for (uint32_t j = t; j < N; j += kLB) a.out_phase[(uint64_t)b * N + j] = (float)s.x[j];
+    for (uint32_t j = t; j < N; j += kLW) a.out_phase[(uint64_t)b * N + j] = (float)x[j];
     if (t == 0 && a.out_iters) a.out_iters[b] = it;
 }
 
@@ -1219,6 +1662,8 @@ int32_t lm_solve_batch(Context* ctx, const float* sources, uint32_t n, uint32_t 
         hipLaunchKernelGGL((gd_solve<16, 2>), dim3(batch), dim3(kLB), 0, st, a);
     else if (algo == kNlsGD && n <= kLB)
         hipLaunchKernelGGL((gd_solve<32, 1>), dim3(batch), dim3(kLB), 0, st, a);
+    else if (algo == kNlsLM)
+        hipLaunchKernelGGL(lm_wide, dim3(batch), dim3(kLW), 0, st, a);
     else
         hipLaunchKernelGGL(lm_solve, dim3(batch), dim3(kLB), 0, st, a);
     if (!hip_ok(hipGetLastError(), "LM kernel launch")) return AFH_ERR_HIP;
