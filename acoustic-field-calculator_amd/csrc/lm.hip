@@ -1035,7 +1035,7 @@ __global__ __launch_bounds__(kLW) void lm_wide(LmArgs a) {
         for (uint32_t l = t; l < n; l += kLW) {
             const double2 e = E[l];
             double gr = 0.0, gi = 0.0, colsq = 0.0;
-#pragma unroll 8
+#pragma unroll 16
             for (uint32_t i = 0; i < M; ++i) {
                 double2 bv;
                 if (l < N)
@@ -1107,28 +1107,38 @@ __global__ __launch_bounds__(kLW) void lm_wide(LmArgs a) {
             sincos(xv, &sn, &cs);
             En[l] = make_double2(cs, sn);
         };
-        // h_l = -(W^T y)_l = -sum_i (Re C_il y_i + Im C_il y_{M+i}), C_il = B_il E_l from G;
-        // columns l0 and l0 + kLW together, so that both columns' loads are in flight
+        // h_l = -(W^T y)_l = -sum_i (Re C_il y_i + Im C_il y_{M+i}), C_il = B_il E_l from G
         const double2* __restrict__ Gh = fresh(s.G);
-        for (uint32_t l0 = t; l0 < n; l0 += 2 * kLW) {
-            const uint32_t l1 = l0 + kLW;
-            const bool two = l1 < n;
-            const uint32_t l1c = two ? l1 : l0;
-            const double2 e0 = E[l0], e1 = E[l1c];
-            double hr0 = 0.0, hi0 = 0.0, hr1 = 0.0, hi1 = 0.0;
-#pragma unroll 4
-            for (uint32_t i = 0; i < M; ++i) {
-                const double yr = s_y[i], yi = s_y[M + i];
-                const double am = -(double)amps[i];
-                const double2 b0 = l0 < N ? Gh[(uint64_t)i * N + l0] : make_double2(l0 - N == i ? am : 0.0, 0.0);
-                const double2 b1 = l1c < N ? Gh[(uint64_t)i * N + l1c] : make_double2(l1c - N == i ? am : 0.0, 0.0);
-                hr0 = fma(b0.x * e0.x - b0.y * e0.y, yr, hr0);
-                hi0 = fma(b0.x * e0.y + b0.y * e0.x, yi, hi0);
-                hr1 = fma(b1.x * e1.x - b1.y * e1.y, yr, hr1);
-                hi1 = fma(b1.x * e1.y + b1.y * e1.x, yi, hi1);
+        auto hcol = [&](uint32_t l, double2 e, uint32_t i, double& hr, double& hi) {
+            const double2 bv = l < N ? Gh[(uint64_t)i * N + l] : make_double2(l - N == i ? -(double)amps[i] : 0.0, 0.0);
+            hr = fma(bv.x * e.x - bv.y * e.y, s_y[i], hr);
+            hi = fma(bv.x * e.y + bv.y * e.x, s_y[M + i], hi);
+        };
+        if (n <= (uint32_t)kLW) {
+            // one column per thread: all of its loads in flight at once
+            if ((uint32_t)t < n) {
+                const double2 e0 = E[t];
+                double hr0 = 0.0, hi0 = 0.0;
+#pragma unroll 8
+                for (uint32_t i = 0; i < M; ++i) hcol(t, e0, i, hr0, hi0);
+                step(t, hr0 + hi0);
             }
-            step(l0, hr0 + hi0);
-            if (two) step(l1, hr1 + hi1);
+        } else {
+            // columns l0 and l0 + kLW together, so that both columns' loads are in flight
+            for (uint32_t l0 = t; l0 < n; l0 += 2 * kLW) {
+                const uint32_t l1 = l0 + kLW;
+                const bool two = l1 < n;
+                const uint32_t l1c = two ? l1 : l0;
+                const double2 e0 = E[l0], e1 = E[l1c];
+                double hr0 = 0.0, hi0 = 0.0, hr1 = 0.0, hi1 = 0.0;
+#pragma unroll 4
+                for (uint32_t i = 0; i < M; ++i) {
+                    hcol(l0, e0, i, hr0, hi0);
+                    hcol(l1c, e1, i, hr1, hi1);
+                }
+                step(l0, hr0 + hi0);
+                if (two) step(l1, hr1 + hi1);
+            }
         }
         lw_reduce<3, false>(sums, s_red);
         const double hnorm = sqrt(sums[0]), xnorm = sqrt(sums[1]), L0 = 0.5 * sums[2];
