@@ -723,7 +723,7 @@ __device__ inline void lw_field(const double2* __restrict__ G, const double2* __
 // Gram matrix W W^T for W = [Re C; Im C], C = B diag(E) (D = 2M rows, n =
 // N + M columns), with FP64 MFMA (v_mfma_f64_16x16x4f64 as in gram_mfma),
 // without W in memory. The N columns of G go through LDS in chunks of cw
-// (64 for M <= 16, 32 above, 2 elements of C per thread): all threads form the chunk's W (thread per element of C, G read
+// (128 for M <= 16, 64 above, 4 elements of C per thread): all threads form the chunk's W (thread per element of C, G read
 // coalesced along the columns) into one of two buffers while the waves run
 // the MFMAs of the previous chunk on the other, one barrier per chunk; the
 // next chunk's G and E are loaded before those MFMAs. Wave w accumulates
@@ -734,9 +734,8 @@ __device__ inline void lw_field(const double2* __restrict__ G, const double2* __
 // each) are added to the entries they touch: (i, i), (M + i, M + i) and
 // (M + i, i). Writes the off-diagonal entries to the lower triangle
 // (sK[p * kLmMaxD + q], q < p) and the diagonal to sgd. buf0 and buf1 hold
-// kLmMaxD x kGramWS doubles each.
-constexpr int kGramWS = 65;  // LDS row stride of a chunk (rows spread over the banks)
-constexpr int kGramPer = 2;  // elements of C per thread and chunk
+// kLmMaxD (kLmMaxD + 1) doubles each (D rows of cw + 1).
+constexpr int kGramPer = 4;  // elements of C per thread and chunk
 
 __device__ __attribute__((noinline)) void lw_gram(const double2* __restrict__ G, const double2* __restrict__ E, uint32_t N, uint32_t M,
                         const float* amps, double* sK, double* sgd, double* buf0, double* buf1) {
@@ -744,7 +743,8 @@ __device__ __attribute__((noinline)) void lw_gram(const double2* __restrict__ G,
     const uint32_t D = 2 * M;
     const int T = (int)(D + 15) / 16, ntiles = T * (T + 1) / 2;
     const int units = ntiles > kLWaves ? ntiles : kLWaves;
-    const uint32_t cw = M <= 16 ? 64 : 32;  // M x cw = kGramPer x kLW
+    const uint32_t cw = M <= 16 ? 128 : 64;  // M x cw <= kGramPer x kLW
+    const uint32_t ws = cw + 1;              // LDS row stride (rows spread over the banks)
     const uint32_t nchunks = (N + cw - 1) / cw;
     const uint32_t kq = l >> 4;
     // chunk staging: element e = t + kLW j of the M x kGramCW block of C
@@ -763,8 +763,8 @@ __device__ __attribute__((noinline)) void lw_gram(const double2* __restrict__ G,
         for (int j = 0; j < kGramPer; ++j) {
             const uint32_t e = t + kLW * j, i = e / cw, k = e % cw;
             if (i < M) {
-                buf[i * kGramWS + k] = pg[j].x * pe[j].x - pg[j].y * pe[j].y;
-                buf[(M + i) * kGramWS + k] = pg[j].x * pe[j].y + pg[j].y * pe[j].x;
+                buf[i * ws + k] = pg[j].x * pe[j].x - pg[j].y * pe[j].y;
+                buf[(M + i) * ws + k] = pg[j].x * pe[j].y + pg[j].y * pe[j].x;
             }
         }
     };
@@ -799,8 +799,8 @@ __device__ __attribute__((noinline)) void lw_gram(const double2* __restrict__ G,
             if (!uv[q]) continue;
             const uint32_t ra = 16 * uI[q] + (l & 15), rb = 16 * uJ[q] + (l & 15);
             const bool va = ra < D, vb = rb < D;
-            const double* pa = cur + (va ? ra : 0) * kGramWS + kq;
-            const double* pb = cur + (vb ? rb : 0) * kGramWS + kq;
+            const double* pa = cur + (va ? ra : 0) * ws + kq;
+            const double* pb = cur + (vb ? rb : 0) * ws + kq;
             for (int kb = uk[q]; kb < (int)cw / 4; kb += us[q]) {
                 const double a0 = va ? pa[4 * kb] : 0.0, b0 = vb ? pb[4 * kb] : 0.0;
                 acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[q], 0, 0, 0);
