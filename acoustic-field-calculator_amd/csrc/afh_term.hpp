@@ -230,35 +230,6 @@ __device__ __forceinline__ SrcF stage_fast(const Src& s) {
     return f;
 }
 
-// One staged source's term at one point: (re, im) = a (cos phi, sin phi).
-// Same quantities as term_value's FAST mode, arranged so that every VALU
-// operand is a VGPR or an inline constant:
-//   a   = D(u) / r * 2^(la - alpha log2e r);
-//   phi = 2 pi (fract(r kh) + [err(r kh) + r kl + ph + r_lo kh]), with
-//         r_lo kh = (d2 - r^2) (kh / 2) / r; sin / cos take revolutions
-//         (argument in [0, 2)).
-template <int SRC>
-__device__ __forceinline__ void term_staged(const SrcF& s, float px, float py, float pz, const DirCell* cells,
-                                            float& a, float& cs, float& sn) {
-    const float dx = px - s.x;
-    const float dy = py - s.y;
-    const float dz = pz - s.z;
-    const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-    const float rinv = __builtin_amdgcn_rsqf(d2);
-    const float r = d2 * rinv;
-    float w = rinv;
-    if (SRC == kT4010A1) w *= directivity_cells(fmaf(s.nx, dx, fmaf(s.ny, dy, s.nz * dz)) * rinv, cells);
-    a = w * __builtin_amdgcn_exp2f(fmaf(s.nal, r, s.la));
-    const float e = fmaf(-r, r, d2);
-    const float t_hi = r * s.kh;
-    const float err = fmaf(r, s.kh, -t_hi);
-    float lo = fmaf(r, s.kl, s.ph);
-    lo = fmaf(e, rinv * s.kh2, lo);
-    const float u = __builtin_amdgcn_fractf(t_hi) + (lo + err);
-    sn = __builtin_amdgcn_sinf(u);
-    cs = __builtin_amdgcn_cosf(u);
-}
-
 // floor(u) as an integer in one instruction (v_cvt_flr_i32_f32; NaN gives 0).
 __device__ __forceinline__ int floor_to_int(float u) {
     int j;
@@ -267,11 +238,17 @@ __device__ __forceinline__ int floor_to_int(float u) {
 }
 
 // The terms of one staged source at NP points (coordinates px[k], py[k],
-// pz[k]), added to (re[k], im[k]). Same arithmetic as term_staged (up to the
-// order of the two products that form a), arranged in stages over the
-// points: distances and cell lookups for every point first, so the LDS reads
-// of the cell table are in flight while the exponentials, phases and sin /
-// cos are computed. The table is read as two float4 arrays (cp = p0 p1 p2 k,
+// pz[k]), added to (re[k], im[k]): (re, im) += a (cos phi, sin phi) with
+//   a   = D(u) / r * 2^(la - alpha log2e r),  u = n . (p - s) / r (scaled);
+//   phi = 2 pi (fract(r kh) + [err(r kh) + r kl + ph + r_lo kh]), with
+//         r_lo kh = (d2 - r^2) (kh / 2) / r; sin / cos take revolutions
+//         (argument in [0, 2)),
+// every VALU operand a VGPR or an inline constant. d2 and n . (p - s) sum the
+// x and y parts first and add the z part last, so that add_terms_zrun (which
+// shares the x and y parts between points of one column) gives the same
+// bits. Arranged in stages over the points: distances and cell lookups for
+// every point first, so the LDS reads of the cell table are in flight while
+// the exponentials, phases and sin / cos are computed. The table is read as two float4 arrays (cp = p0 p1 p2 k,
 // cq = q0 q1 q2 s of each DirCell), 16 bytes per read.
 // With a non-null weight, the amplitude of point k is also multiplied by
 // weight[k] (0 or 1 masks padding rows without a select); with a non-null
@@ -287,10 +264,10 @@ __device__ __forceinline__ void add_terms_staged(const SrcF& s, const float* px,
         const float dx = px[k] - s.x;
         const float dy = py[k] - s.y;
         const float dz = pz[k] - s.z;
-        d2[k] = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+        d2[k] = fmaf(dz, dz, fmaf(dx, dx, dy * dy));
         rinv[k] = __builtin_amdgcn_rsqf(d2[k]);
         if (SRC == kT4010A1) {
-            const float u = fmaf(s.nx, dx, fmaf(s.ny, dy, s.nz * dz)) * rinv[k];
+            const float u = fmaf(s.nz, dz, fmaf(s.nx, dx, s.ny * dy)) * rinv[k];
             t[k] = __builtin_amdgcn_fractf(u);
             int j = floor_to_int(u) + 2;
             j = j < 0 ? 0 : (j > kDirCells - 1 ? kDirCells - 1 : j);
@@ -323,6 +300,61 @@ __device__ __forceinline__ void add_terms_staged(const SrcF& s, const float* px,
         }
         if (weight) a *= weight[k];
         if (amp_out) amp_out[k] = a;
+        re[k] = fmaf(a, cs[k], re[k]);
+        im[k] = fmaf(a, sn[k], im[k]);
+    }
+}
+
+// add_terms_staged for NP points that share x and y (a run along z, heights
+// pz[k]): dx, dy, dx^2 + dy^2 and the x / y part of n . (p - s) once per
+// source instead of once per point, the rest per point exactly as in
+// add_terms_staged (same operations, same order, same bits).
+template <int SRC, int NP>
+__device__ __forceinline__ void add_terms_zrun(const SrcF& s, float px, float py, const float* pz, const float4* cp,
+                                               const float4* cq, float* re, float* im) {
+    const float dx = px - s.x;
+    const float dy = py - s.y;
+    const float sxy = fmaf(dx, dx, dy * dy);
+    const float nxy = fmaf(s.nx, dx, s.ny * dy);
+    float d2[NP], rinv[NP], t[NP];
+    float4 P[NP], Q[NP];
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        const float dz = pz[k] - s.z;
+        d2[k] = fmaf(dz, dz, sxy);
+        rinv[k] = __builtin_amdgcn_rsqf(d2[k]);
+        if (SRC == kT4010A1) {
+            const float u = fmaf(s.nz, dz, nxy) * rinv[k];
+            t[k] = __builtin_amdgcn_fractf(u);
+            int j = floor_to_int(u) + 2;
+            j = j < 0 ? 0 : (j > kDirCells - 1 ? kDirCells - 1 : j);
+            P[k] = cp[j];
+            Q[k] = cq[j];
+        }
+    }
+    float w[NP], sn[NP], cs[NP];
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        const float r = d2[k] * rinv[k];
+        w[k] = rinv[k] * __builtin_amdgcn_exp2f(fmaf(s.nal, r, s.la));
+        const float e = fmaf(-r, r, d2[k]);
+        const float t_hi = r * s.kh;
+        const float err = fmaf(r, s.kh, -t_hi);
+        float lo = fmaf(r, s.kl, s.ph);
+        lo = fmaf(e, rinv[k] * s.kh2, lo);
+        const float u = __builtin_amdgcn_fractf(t_hi) + (lo + err);
+        sn[k] = __builtin_amdgcn_sinf(u);
+        cs[k] = __builtin_amdgcn_cosf(u);
+    }
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        float a = w[k];
+        if (SRC == kT4010A1) {
+            const float p = fmaf(fmaf(P[k].z, t[k], P[k].y), t[k], P[k].x);
+            const float q = fmaf(fmaf(Q[k].z, t[k], Q[k].y), t[k], Q[k].x);
+            const float step = fminf(fmaxf(fmaf(t[k], Q[k].w, P[k].w), 0.0f), 1.0f);
+            a *= fmaf(step, q, p);
+        }
         re[k] = fmaf(a, cs[k], re[k]);
         im[k] = fmaf(a, sn[k], im[k]);
     }
@@ -406,15 +438,6 @@ __device__ __forceinline__ void add_terms_exact(const SrcF& s, const Split3& ss,
         re[k] = fmaf(a, cs[k], re[k]);
         im[k] = fmaf(a, sn[k], im[k]);
     }
-}
-
-template <int SRC>
-__device__ __forceinline__ void add_term_staged(const SrcF& s, float px, float py, float pz, const DirCell* cells,
-                                                float& re, float& im) {
-    float a, cs, sn;
-    term_staged<SRC>(s, px, py, pz, cells, a, cs, sn);
-    re = fmaf(a, cs, re);
-    im = fmaf(a, sn, im);
 }
 
 // Adds one source's term at one point to (re, im).

@@ -8,7 +8,9 @@
 // Layout and schedule (gfx950):
 //   * workgroup = 256 lanes (4 wave64); each lane owns PPL observation points
 //     (PPL = 1) strided by 256 so loads of scatter points and all output
-//     stores coalesce;
+//     stores coalesce; on grids whose slowest axis is z a lane owns a run of
+//     2 points along z instead (field_zrun_kernel), which share the x / y
+//     parts of every term;
 //   * grid points are generated in registers from (origin, resolution,
 //     counts), with the reference's own f32 rounding (origin + n * res, no
 //     contraction), so no point array is read from HBM;
@@ -268,6 +270,95 @@ __global__ __launch_bounds__(kBlock) void field_kernel(FieldArgs a) {
     }
 }
 
+// Grid variant for grids whose slowest axis is z (FAST mode): lane
+// (n0, n1, q) owns the ZR points n2 = ZR q + k of one column, which share x
+// and y, so add_terms_zrun forms dx, dy, dx^2 + dy^2 and the x / y part of
+// n . (p - s) once per source for all ZR of them (and each source's LDS
+// operands are read once per ZR terms). Consecutive lanes take
+// consecutive n0 (coalesced stores, one store per point run row). Same
+// per-term arithmetic, source order and compensated sums as field_kernel:
+// every point gets the bits it gets there.
+template <int SRC, int OUT, int ZR>
+__global__ __launch_bounds__(kBlock) void field_zrun_kernel(FieldArgs a) {
+    constexpr int kCells = SRC == kT4010A1 ? kDirCells : 1;
+    __shared__ float4 s_src[kTile * 3];
+    __shared__ float4 s_cp[kCells], s_cq[kCells];
+    const int tid = threadIdx.x;
+    if (SRC == kT4010A1) stage_cells(a.cells, s_cp, s_cq);
+
+    const uint32_t cols = a.count0 * a.count1;
+    const uint32_t count2 = a.n_points / cols;
+    const uint32_t runs = (count2 + ZR - 1) / ZR;
+    const uint64_t total = (uint64_t)cols * runs;
+    const uint64_t g = (uint64_t)blockIdx.x * kBlock + tid;
+    const uint64_t gc = g < total ? g : total - 1;  // tail lanes recompute the last run
+    const uint32_t n0 = (uint32_t)(gc % a.count0);
+    const uint64_t rest = gc / a.count0;
+    const uint32_t n1 = (uint32_t)(rest % a.count1);
+    const uint32_t q = (uint32_t)(rest / a.count1);
+    float px, py, pz[ZR];
+    {
+#pragma clang fp contract(off)
+        // grid_area.rs: origin[axis] + (n as Float * resolution), two roundings
+        float p[3] = {a.origin[0], a.origin[1], a.origin[2]};
+        p[a.ax1] = a.origin[a.ax1] + (float)n1 * a.res;
+        p[a.ax0] = a.origin[a.ax0] + (float)n0 * a.res;
+        px = p[0];
+        py = p[1];
+#pragma unroll
+        for (int k = 0; k < ZR; ++k) {
+            const uint32_t n2 = min(q * ZR + k, count2 - 1);
+            pz[k] = a.origin[2] + (float)n2 * a.res;
+        }
+    }
+    float sr[ZR], si[ZR], cr[ZR], ci[ZR];
+#pragma unroll
+    for (int k = 0; k < ZR; ++k) sr[k] = si[k] = cr[k] = ci[k] = 0.0f;
+
+    const float4* __restrict__ gsrc = reinterpret_cast<const float4*>(a.sources);
+    for (uint32_t t0 = 0; t0 < a.n_sources; t0 += kTile) {
+        const uint32_t cnt = min((uint32_t)kTile, a.n_sources - t0);
+        __syncthreads();
+        stage_tile<kFast>(gsrc + (uint64_t)t0 * 3, cnt, s_src);
+        __syncthreads();
+        for (uint32_t j0 = 0; j0 < cnt; j0 += kChunk) {
+            float tr[ZR], ti[ZR];
+#pragma unroll
+            for (int k = 0; k < ZR; ++k) tr[k] = ti[k] = 0.0f;
+#pragma unroll 2
+            for (int j = 0; j < kChunk; ++j) {
+                const float4 q0 = s_src[(j0 + j) * 3];
+                const float4 q1 = s_src[(j0 + j) * 3 + 1];
+                const float4 q2 = s_src[(j0 + j) * 3 + 2];
+                const SrcF sf = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
+                add_terms_zrun<SRC, ZR>(sf, px, py, pz, s_cp, s_cq, tr, ti);
+            }
+#pragma unroll
+            for (int k = 0; k < ZR; ++k) {
+                two_sum_acc(sr[k], cr[k], tr[k]);
+                two_sum_acc(si[k], ci[k], ti[k]);
+            }
+        }
+    }
+
+#pragma unroll
+    for (int k = 0; k < ZR; ++k) {
+        const uint32_t n2 = q * ZR + k;
+        if (g >= total || n2 >= count2) continue;
+        const uint32_t idx = n0 + a.count0 * (n1 + a.count1 * n2);
+        const float re = sr[k] + cr[k];
+        const float im = si[k] + ci[k];
+        if (OUT == kComplex) {
+            reinterpret_cast<float2*>(a.out)[idx] = make_float2(re, im);
+        } else {
+#pragma clang fp contract(off)
+            // pressure_field.rs:35-37 norm_sqr().sqrt(); power_field.rs:35-37 norm_sqr()
+            const float ns = re * re + im * im;
+            a.out[idx] = (OUT == kAbs) ? sqrtf(ns) : ns;
+        }
+    }
+}
+
 // Source-split variant for small point sets: each wave owns PPW points and its
 // 64 lanes split the sources (lane l takes sources l, l + 64, ... of each LDS
 // tile); per-lane sums are TwoSum-compensated per tile and the 64 partial
@@ -380,6 +471,24 @@ static hipError_t launch4(const FieldArgs& a, hipStream_t stream) {
         const uint32_t blocks = (uint32_t)(((uint64_t)a.n_points + per_block - 1) / per_block);
         hipLaunchKernelGGL((field_split_kernel<SRC, PTS, OUT, PREC, PPW>), dim3(blocks), dim3(kBlock), 0, stream, a);
         return hipGetLastError();
+    }
+    if (PTS == kGrid && PREC == kFast && a.ax2 == 2) {
+        // z runs of 2 points per lane (AFH_FIELD_ZRUN=1 turns them off, for
+        // measurements). Measured on MI355X, 3984 sources x 256 x 256 x 128:
+        // 2 points (63 VGPRs, 8 waves per SIMD) 6.5% faster than one point per
+        // lane; 4 points (94-128 VGPRs, 4-5 waves) gained less or nothing
+        static const bool zrun = [] {
+            const char* e = getenv("AFH_FIELD_ZRUN");
+            return !(e && atoi(e) == 1);
+        }();
+        if (zrun) {
+            constexpr int ZR = 2;
+            const uint32_t cols = a.count0 * a.count1;
+            const uint64_t total = (uint64_t)cols * ((a.n_points / cols + ZR - 1) / ZR);
+            const uint32_t blocks = (uint32_t)((total + kBlock - 1) / kBlock);
+            hipLaunchKernelGGL((field_zrun_kernel<SRC, OUT, ZR>), dim3(blocks), dim3(kBlock), 0, stream, a);
+            return hipGetLastError();
+        }
     }
     // one point per lane: few VGPRs, 8 waves per SIMD
     return launch_ppl<SRC, PTS, OUT, PREC, 1>(a, stream);
