@@ -1,4 +1,6 @@
-"""Builds libafc_hip.so (gfx950) in-tree from csrc/*.hip with hipcc.
+"""Builds libafc_hip.so (gfx950) in-tree from csrc/*.hip with hipcc, and
+libafc_issue.so from measure/issue_rate.hip (the VALU issue-rate calibration
+that bench.py --calibrate-issue runs).
 
 Usage: python acoustic-field-calculator_amd/build.py [-j N]
 """
@@ -15,6 +17,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libafc_hip.so")
+ISSUE_SRC = os.path.join(HERE, "measure", "issue_rate.hip")
+ISSUE_LIB = os.path.join(HERE, "libafc_issue.so")
 ARCH = os.environ.get("AFH_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -48,6 +52,12 @@ def build(jobs: int = 8) -> str:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("link failed")
     os.replace(tmp, LIB)
+    tmp = ISSUE_LIB + ".tmp"
+    r = subprocess.run([HIPCC, *FLAGS, "-shared", ISSUE_SRC, "-o", tmp], capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("hipcc failed on measure/issue_rate.hip")
+    os.replace(tmp, ISSUE_LIB)
     return LIB
 
 

@@ -32,10 +32,68 @@ FLOP_PER_TERM = 38  # T4010A1 term, counted from uniform_medium.rs:111-119 and t
 # a 256 MiB device copy reads)
 FIELD_TRAFFIC_BYTES = 68.8e6
 PEAK_FP32 = 157.3  # TFLOP/s, MI355X vector FP32
+# Static VALU mix of one term of the timed field kernel (field_zrun_kernel,
+# T4010A1, complex output), per source-point term of its inner loop: wave64
+# VALU instructions, of which transcendental (rsq, exp, sin, cos) and FMAs with
+# three VGPR sources (v_fma_f32, which issue at half rate on gfx950). Read from
+# the kernel's gfx950 assembly (145 VALU, 16 transcendental, 36 three-source
+# FMAs per 2 sources x 2 points) and checked against rocprofv3 SQ_INSTS_VALU /
+# SQ_INSTS_VALU_TRANS_F32 per dispatch (profiles/field_kernel_3984src_256x256x128.md).
+FIELD_VALU_PER_TERM = 36.25
+FIELD_TRANS_PER_TERM = 4.0
+FIELD_FMA3_PER_TERM = 9.0
+PEAK_ENGINE_GHZ = 2.4  # MI355X peak engine clock
+ISSUE_LIB = os.path.join(ROOT, "acoustic-field-calculator_amd", "libafc_issue.so")
+ISSUE_KINDS = {"fma": 0, "mul": 1, "sin": 2, "rsq": 3, "exp": 4, "mix_9fma_1sin": 5, "fmac_vop2": 6,
+               "fma_const": 7, "mul_vop3": 8}
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def issue_rate(kinds=("fma", "mul", "sin"), waves_per_simd=8, reps=4000):
+    """SIMD cycles per wave64 VALU instruction of each class, measured live by
+    the calibration kernels of libafc_issue.so (measure/issue_rate.hip): a long
+    stream of one instruction class on 16 independent chains per lane,
+    waves_per_simd waves on every SIMD; cycles = kernel time (HIP events) x
+    shader clock (s_memtime / s_memrealtime of the median wave) x SIMDs /
+    wave instructions."""
+    import ctypes
+
+    if not os.path.exists(ISSUE_LIB):
+        raise ImportError(f"{ISSUE_LIB} not found: build it with `python acoustic-field-calculator_amd/build.py`")
+    lib = ctypes.CDLL(ISSUE_LIB)
+    fn = lib.afi_issue_rate
+    fn.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    fn.restype = ctypes.c_int
+    out = (ctypes.c_double * 5)()
+    res = {}
+    for name in kinds:
+        rc = fn(ISSUE_KINDS[name], waves_per_simd, reps, out)
+        if rc != 0:
+            raise RuntimeError(f"issue-rate calibration {name} failed ({rc})")
+        res[name] = {"cycles_per_instr": round(out[3], 3), "cycles_per_instr_wave_clock": round(out[4], 3),
+                     "clock_ghz": round(out[1], 3), "kernel_ms": round(out[0], 3), "wave_instructions": out[2]}
+    return res
+
+
+def field_issue_model(cal, terms_per_s):
+    """Issue-slot ceiling of the field kernel: its static VALU mix per term
+    priced with the measured per-class costs (three-source FMA, two-operand
+    VALU, transcendental), at the peak engine clock, and the kernel's fraction
+    of it."""
+    c3, c2, ct = (cal[k]["cycles_per_instr"] for k in ("fma", "mul", "sin"))
+    n2 = FIELD_VALU_PER_TERM - FIELD_TRANS_PER_TERM - FIELD_FMA3_PER_TERM
+    cyc = FIELD_FMA3_PER_TERM * c3 + n2 * c2 + FIELD_TRANS_PER_TERM * ct
+    simds = 1024  # 256 CUs x 4
+    ceiling = simds * PEAK_ENGINE_GHZ * 1e9 / cyc * 64
+    return {"valu_per_term": FIELD_VALU_PER_TERM, "fma3_per_term": FIELD_FMA3_PER_TERM,
+            "trans_per_term": FIELD_TRANS_PER_TERM, "cycles_fma3": c3, "cycles_2op": c2, "cycles_trans": ct,
+            "cycles_per_wave_term": round(cyc, 2), "clock_ghz": PEAK_ENGINE_GHZ,
+            "ceiling_gsrc_pt_per_s": round(ceiling / 1e9, 1), "frac_of_issue_ceiling": round(terms_per_s / ceiling, 4),
+            "source": "per-class cycles measured live (bench.py issue_rate, 8 waves per SIMD); VALU mix per term "
+                      "from the kernel's assembly, checked with rocprofv3 counters"}
 
 
 def cpu_threads():
@@ -112,7 +170,16 @@ def main():
     ap.add_argument("--serve-gspat-cus", type=int, default=32, help="CUs reserved for GS-PAT batches while serving")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device-to-device copy (reference for HBM byte counters)")
+    ap.add_argument("--calibrate-issue", action="store_true",
+                    help="only measure SIMD cycles per wave64 VALU instruction of each class (fma, mul, sin, rsq, "
+                         "exp, 9 fma : 1 sin) at 4 and 8 waves per SIMD")
     args = ap.parse_args()
+    if args.calibrate_issue:
+        import torch  # noqa: F401  (one HIP runtime per process: torch's, loaded first)
+
+        out = {"calibrate_issue": {f"waves_per_simd_{w}": issue_rate(tuple(ISSUE_KINDS), w) for w in (4, 8)}}
+        print(json.dumps(out), flush=True)
+        return
     if args.calibrate_copy:
         import torch
         x = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
@@ -198,6 +265,11 @@ def main():
     log(f"rel-L2 vs FP64 on 512 sampled points: {err:.3e}")
 
     achieved = terms * FLOP_PER_TERM / (kern_ms * 1e-3) / 1e12
+    issue = field_issue_model(issue_rate(), terms / (kern_ms * 1e-3))
+    log(f"issue model: {issue['cycles_fma3']} / {issue['cycles_2op']} / {issue['cycles_trans']} cycles per "
+        f"3-source FMA / 2-operand / transcendental, {issue['cycles_per_wave_term']} per wave-term at "
+        f"{issue['clock_ghz']} GHz -> ceiling {issue['ceiling_gsrc_pt_per_s']} Gsrc*pt/s, "
+        f"kernel at {issue['frac_of_issue_ceiling']:.3f} of it")
     result = {
         "metric": "field Gsrc*pt/s (%VALU peak); GS-PAT solves/s + p50/p99 latency",
         "value": round(value, 3),
@@ -218,7 +290,8 @@ def main():
                      "traffic_source": "rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 per dispatch "
                                        "(profiles/field_kernel_3984src_256x256x128.md)",
                      "algorithmic_bytes": n_pts * 8 + packed.nbytes,
-                     "flop_per_term": FLOP_PER_TERM, "kernel_ms": round(kern_ms, 4)},
+                     "flop_per_term": FLOP_PER_TERM, "kernel_ms": round(kern_ms, 4),
+                     "issue_model": issue},
         "accuracy": {"rel_l2_vs_fp64": err, "sample": "512 random points of the timed output"},
         "pcie_inclusive": {"ms": round(with_copy_ms, 4), "gsrc_pt_per_s": round(terms / with_copy_ms / 1e6, 1),
                            "note": "launch + copy of the 8 B/point field into pinned host memory"},
