@@ -15,18 +15,23 @@
 //
 // GS-PAT formulation here: H_ij = G_ij / d_j (|H_ij| <= 1, the per-column
 // scaling), so R = H H^H. Three kernels per bucket of instances with the same
-// padded focus count MP:
-//   A  generates G column chunks in registers (never stored), reduces d_j in
-//      LDS and accumulates R -- in FP32 on the VALU (precision FP32), or on
-//      MFMA v_mfma_f32_16x16x32_f16 with FP32 accumulation from an FP16 copy
-//      of H (MIXED, MIXED_CLOSING) or from H split into two FP16 halves with
-//      three products hi hi^H + hi lo^H + lo hi^H (MIXED_SPLIT);
+// padded focus count MP (16, 32, 64, 128):
+//   A  generates G in registers from the exact coordinate difference of
+//      source and focus (add_terms_exact), reduces d_j and accumulates R:
+//        - mixed modes: H = s G / d_j split into FP16 hi + lo, R += hi hi^H +
+//          hi lo^H + lo hi^H on v_mfma_f32_16x16x32_f16 with FP32
+//          accumulation, fused with the generation (gspat_wave for MP <= 64:
+//          each wave takes every fourth chunk of the range for all MP foci;
+//          gspat_fused for MP = 128: one wave per 16 foci);
+//        - FP32: gspat_gen stores FP32 G and d_j, gspat_r32 forms R on
+//          v_mfma_f32_16x16x4f32 (MP >= 32); gspat_r_fp32 on the VALU (MP = 16);
+//      and keeps kernel C's copy: FP16 hi of H (MIXED) or FP32 G (the others),
+//      rows < M only;
 //   B  runs the iterations with R held in registers, one workgroup per
-//      instance, reductions by fixed-order lane shuffles;
-//   C  computes q = B p in FP32 from the G tiles and column sums d_j that A
-//      stored (rows < M only; reading them back costs less than generating G
-//      again); in MIXED_CLOSING a first pass redoes the last gamma = R p
-//      exactly as G (B p) in FP32 (the closing FP32 iteration).
+//      instance, reductions by fixed-order lane shuffles (gspat_iter);
+//   C  computes q = B p in FP32 from that copy and d_j (reading it back costs
+//      less than generating G again); in MIXED_CLOSING a first pass redoes the
+//      last gamma = R p exactly as G (B p) in FP32 (gspat_q).
 // A and C run a.split workgroups per instance over ranges of source chunks;
 // A's partial R and C's partial gamma are summed in a fixed order.
 // Every reduction is per instance and in a fixed order, so an instance's result
