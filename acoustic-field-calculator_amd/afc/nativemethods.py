@@ -85,7 +85,7 @@ class T4010A1(Structure):
 class AFS_Config(Structure):
     _fields_ = [("hbm_budget_bytes", c_uint64), ("pinned_budget_bytes", c_uint64), ("n_streams", c_uint32),
                 ("max_gspat_batch", c_uint32), ("gspat_precision", ctypes.c_int32), ("gspat_iterations", c_uint32),
-                ("gspat_cus", c_uint32)]
+                ("gspat_cus", c_uint32), ("gspat_graphs", c_uint32)]
 
 
 class AFH_PointSet(Structure):
@@ -191,6 +191,7 @@ def _declare(d):
     _sig(d, "AFS_Wait", [c_void_p, c_uint64, POINTER(ctypes.c_double)], ctypes.c_int32)
     _sig(d, "AFS_Stats", [c_void_p, ctypes.c_int32, POINTER(ctypes.c_double), POINTER(ctypes.c_double),
                           POINTER(c_uint64), ctypes.c_int32], ctypes.c_int32)
+    _sig(d, "AFS_GraphCounters", [c_void_p, POINTER(c_uint64), POINTER(c_uint64)], ctypes.c_int32)
     _sig(d, "AFH_DirectivityT4010A1", [c_void_p, c_uint64, ctypes.c_int32, c_void_p], ctypes.c_int32)
     _sig(d, "AFH_PackSources", [c_void_p, c_void_p, c_void_p, c_uint64, ctypes.c_int32, c_void_p], ctypes.c_int32)
     _sig(d, "AFH_FieldEval", [c_void_p, c_uint64, ctypes.c_int32, POINTER(AFH_PointSet), ctypes.c_int32,

@@ -24,9 +24,9 @@ def _err(what, rc):
 
 class Server:
     def __init__(self, hbm_budget=1 << 30, pinned_budget=256 << 20, n_streams=4, max_gspat_batch=256,
-                 gspat_precision=3, gspat_iterations=100, gspat_cus=0):
+                 gspat_precision=3, gspat_iterations=100, gspat_cus=0, gspat_graphs=False):
         cfg = AFS_Config(hbm_budget, pinned_budget, n_streams, max_gspat_batch, gspat_precision, gspat_iterations,
-                         gspat_cus)
+                         gspat_cus, 1 if gspat_graphs else 0)
         self.handle = c_void_p()
         _err("AFS_Create", dll().AFS_Create(byref(cfg), byref(self.handle)))
         self._keep = {}
@@ -77,6 +77,12 @@ class Server:
         self._keep.pop(ticket, None)
         _err("AFS_Wait", rc)
         return lat.value
+
+    def graph_counters(self):
+        """(graph launches, graph builds) of the GS-PAT bucket chains."""
+        n, b = c_uint64(), c_uint64()
+        _err("AFS_GraphCounters", dll().AFS_GraphCounters(self.handle, byref(n), byref(b)))
+        return n.value, b.value
 
     def stats(self, kind=-1, reset=False):
         """(p50 ms, p99 ms, completed) of field (0), GS-PAT (1) or all (-1) requests."""

@@ -4,8 +4,10 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <string.h>
 
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "afh_host.hpp"
@@ -55,6 +57,54 @@ struct HoloBatch {
     float* out_phase;
 };
 
+// One kernel launch kept as a record: the kernel, its launch shape and a copy
+// of its arguments (at most 4, stored in `blob` at offsets `off`).
+struct KLaunch {
+    const void* fn = nullptr;
+    dim3 grid, block;
+    int nargs = 0;
+    uint16_t off[4] = {0, 0, 0, 0};
+    alignas(16) unsigned char blob[256];
+};
+
+// Launches that run one after another on one stream (at most 4).
+struct KChain {
+    KLaunch k[4];
+    int n = 0;
+    template <class... A>
+    void add(const void* fn, dim3 grid, dim3 block, const A&... args) {
+        static_assert(sizeof...(A) >= 1 && sizeof...(A) <= 4, "one to four kernel arguments");
+        KLaunch& l = k[n++];
+        l.fn = fn;
+        l.grid = grid;
+        l.block = block;
+        l.nargs = 0;
+        size_t pos = 0;
+        auto put = [&](const auto& v) {
+            using T = std::decay_t<decltype(v)>;
+            pos = (pos + alignof(T) - 1) & ~(alignof(T) - 1);
+            memcpy(l.blob + pos, &v, sizeof(T));
+            l.off[l.nargs++] = (uint16_t)pos;
+            pos += sizeof(T);
+        };
+        (put(args), ...);
+    }
+};
+
+// A KChain instantiated as a hipGraph (a linear chain of kernel nodes). Later
+// chains of the same kernels and block shapes reuse it with their own grids
+// and arguments set on its nodes, once its previous launch has completed.
+struct ChainGraph {
+    const void* fn[4] = {nullptr, nullptr, nullptr, nullptr};
+    dim3 block[4];
+    int n = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipGraphNode_t node[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t done = nullptr;  // recorded after each launch
+    uint64_t last_use = 0;
+};
+
 // Stream + grow-only device workspace; calls on one context are serialized.
 // A small batch runs its padded-focus buckets side by side on the context's
 // auxiliary streams (created on first use, with cu_mask when one is set),
@@ -69,13 +119,27 @@ struct Context {
     hipStream_t aux[4] = {nullptr, nullptr, nullptr, nullptr};
     hipEvent_t aux_done[4] = {nullptr, nullptr, nullptr, nullptr};
     hipEvent_t fork = nullptr;
+    // GS-PAT bucket chains issued as hipGraphs (off: launch by launch)
+    bool graphs = false;
+    std::vector<ChainGraph> chain_graphs;
+    uint64_t graph_clock = 0, graph_launches = 0, graph_builds = 0;
     bool ensure_stream() {
         if (stream) return true;
         return hip_ok(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
     }
     bool ensure_aux();
     bool reserve_staging(size_t bytes);
+    void drop_graph(ChainGraph& g) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        if (g.graph) (void)hipGraphDestroy(g.graph);
+        if (g.done) (void)hipEventDestroy(g.done);
+        g = ChainGraph();
+    }
     ~Context() {
+        for (auto& g : chain_graphs) {
+            if (g.done) (void)hipEventSynchronize(g.done);
+            drop_graph(g);
+        }
         for (int i = 0; i < 4; ++i) {
             if (aux[i]) (void)hipStreamDestroy(aux[i]);
             if (aux_done[i]) (void)hipEventDestroy(aux_done[i]);

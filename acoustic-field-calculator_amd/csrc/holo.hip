@@ -1627,63 +1627,169 @@ static int padded_m(uint32_t m) {
     return 0;
 }
 
+// A kernel's host-side handle, as hipLaunchKernel and graph nodes take it.
+template <class F>
+static const void* kfn(F* f) {
+    return reinterpret_cast<const void*>(f);
+}
+
+// Launch by launch on `st`.
+static hipError_t issue_direct(const KChain& ch, hipStream_t st) {
+    for (int i = 0; i < ch.n; ++i) {
+        const KLaunch& l = ch.k[i];
+        void* args[4];
+        for (int j = 0; j < l.nargs; ++j) args[j] = const_cast<unsigned char*>(l.blob) + l.off[j];
+        const hipError_t e = hipLaunchKernel(l.fn, l.grid, l.block, args, 0, st);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+// Cached chain graphs per context; the least recently used idle one makes
+// room for a new shape.
+constexpr size_t kChainGraphs = 32;
+
+static bool same_shape(const ChainGraph& g, const KChain& ch) {
+    if (g.n != ch.n) return false;
+    for (int i = 0; i < ch.n; ++i)
+        if (g.fn[i] != ch.k[i].fn || g.block[i].x != ch.k[i].block.x || g.block[i].y != ch.k[i].block.y ||
+            g.block[i].z != ch.k[i].block.z)
+            return false;
+    return true;
+}
+
+// Node parameters of launch l; args receives the argument pointers (the
+// runtime copies the values when the node is added or set).
+static hipKernelNodeParams node_params(const KLaunch& l, void* (&args)[4]) {
+    for (int j = 0; j < l.nargs; ++j) args[j] = const_cast<unsigned char*>(l.blob) + l.off[j];
+    hipKernelNodeParams p;
+    memset(&p, 0, sizeof(p));
+    p.func = const_cast<void*>(l.fn);
+    p.gridDim = l.grid;
+    p.blockDim = l.block;
+    p.sharedMemBytes = 0;
+    p.kernelParams = args;
+    p.extra = nullptr;
+    return p;
+}
+
+static hipError_t build_chain_graph(const KChain& ch, ChainGraph& g) {
+    g.n = ch.n;
+    hipError_t e = hipGraphCreate(&g.graph, 0);
+    for (int i = 0; e == hipSuccess && i < ch.n; ++i) {
+        g.fn[i] = ch.k[i].fn;
+        g.block[i] = ch.k[i].block;
+        void* args[4];
+        const hipKernelNodeParams p = node_params(ch.k[i], args);
+        e = hipGraphAddKernelNode(&g.node[i], g.graph, i ? &g.node[i - 1] : nullptr, i ? 1 : 0, &p);
+    }
+    if (e == hipSuccess) e = hipGraphInstantiateWithFlags(&g.exec, g.graph, 0);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&g.done, hipEventDisableTiming);
+    return e;
+}
+
+// The chain as one graph launch on `st`: a cached graph of the same shape whose
+// last launch has completed gets this chain's grids and arguments, otherwise a
+// new graph is built (and cached). Without an idle graph to replace in a full
+// cache, the chain is launched directly.
+static hipError_t issue_graph(Context* ctx, const KChain& ch, hipStream_t st) {
+    ChainGraph* use = nullptr;
+    for (auto& g : ctx->chain_graphs) {
+        if (!same_shape(g, ch)) continue;
+        const hipError_t q = hipEventQuery(g.done);
+        if (q == hipErrorNotReady) continue;
+        if (q != hipSuccess) return q;
+        bool ok = true;
+        for (int i = 0; ok && i < ch.n; ++i) {
+            void* args[4];
+            const hipKernelNodeParams p = node_params(ch.k[i], args);
+            ok = hipGraphExecKernelNodeSetParams(g.exec, g.node[i], &p) == hipSuccess;
+        }
+        if (!ok) {
+            // a graph whose nodes refuse the new parameters is rebuilt
+            (void)hipGetLastError();
+            ctx->drop_graph(g);
+            continue;
+        }
+        use = &g;
+        break;
+    }
+    if (!use) {
+        // a free slot: a dropped entry, room in the cache, or the least recently
+        // used idle graph
+        ChainGraph* slot = nullptr;
+        for (auto& g : ctx->chain_graphs)
+            if (!g.exec) {
+                slot = &g;
+                break;
+            }
+        if (!slot && ctx->chain_graphs.size() < kChainGraphs) {
+            ctx->chain_graphs.emplace_back();
+            slot = &ctx->chain_graphs.back();
+        }
+        if (!slot) {
+            for (auto& g : ctx->chain_graphs)
+                if (hipEventQuery(g.done) == hipSuccess && (!slot || g.last_use < slot->last_use)) slot = &g;
+            (void)hipGetLastError();
+            if (!slot) return issue_direct(ch, st);
+            ctx->drop_graph(*slot);
+        }
+        const hipError_t e = build_chain_graph(ch, *slot);
+        if (e != hipSuccess) {
+            ctx->drop_graph(*slot);
+            return e;
+        }
+        ++ctx->graph_builds;
+        use = slot;
+    }
+    hipError_t e = hipGraphLaunch(use->exec, st);
+    if (e == hipSuccess) e = hipEventRecord(use->done, st);
+    use->last_use = ++ctx->graph_clock;
+    ++ctx->graph_launches;
+    return e;
+}
+
+static hipError_t issue_chain(Context* ctx, const KChain& ch, hipStream_t st) {
+    return ctx && ctx->graphs ? issue_graph(ctx, ch, st) : issue_direct(ch, st);
+}
+
+// The kernel chain of one GS-PAT bucket (A, B, C: at most four launches), kept
+// as records so that it can be issued on a stream launch by launch or as one
+// hipGraph (KChain, issue_chain).
 template <int MP>
-static hipError_t launch_gspat_bucket(HoloArgs a, int src, int precision, uint32_t nb, uint32_t iters,
-                                      hipStream_t st) {
+static void record_gspat_bucket(KChain& ch, const HoloArgs& a, int src, int precision, uint32_t nb, uint32_t iters) {
     const dim3 grid(nb), grid_a(nb * a.split), blk(kHB);
     const dim3 grid_g(a.n_packed / kHB + (a.n_packed % kHB ? 1 : 0), nb);
+    const bool t4 = src == kT4010A1;
     if (precision == AFH_HOLO_FP32) {
         if constexpr (MP >= 32) {
             // A1 (FP32 G, one thread per source column) then A2 on FP32 MFMA
-            if (src == kT4010A1)
-                hipLaunchKernelGGL((gspat_gen<kT4010A1, MP>), grid_g, blk, 0, st, a);
-            else
-                hipLaunchKernelGGL((gspat_gen<kSphere, MP>), grid_g, blk, 0, st, a);
-            hipLaunchKernelGGL((gspat_r32<MP>), grid_a, blk, 0, st, a);
-        } else if (src == kT4010A1) {
-            hipLaunchKernelGGL((gspat_r_fp32<kT4010A1, MP>), grid_a, blk, 0, st, a);
+            ch.add(t4 ? kfn(gspat_gen<kT4010A1, MP>) : kfn(gspat_gen<kSphere, MP>), grid_g, blk, a);
+            ch.add(kfn(gspat_r32<MP>), grid_a, blk, a);
         } else {
-            hipLaunchKernelGGL((gspat_r_fp32<kSphere, MP>), grid_a, blk, 0, st, a);
+            ch.add(t4 ? kfn(gspat_r_fp32<kT4010A1, MP>) : kfn(gspat_r_fp32<kSphere, MP>), grid_a, blk, a);
         }
     } else {
         // kernel A: G, d_j and the split-FP16 MFMA R in one pass; MIXED keeps
         // the FP16 H for kernel C, the other modes FP32 G
         const bool g32 = precision != AFH_HOLO_MIXED;
         if constexpr (MP <= 64) {
-            if (src == kT4010A1) {
-                if (g32)
-                    hipLaunchKernelGGL((gspat_wave<kT4010A1, MP, true>), grid_a, blk, 0, st, a);
-                else
-                    hipLaunchKernelGGL((gspat_wave<kT4010A1, MP, false>), grid_a, blk, 0, st, a);
-            } else {
-                if (g32)
-                    hipLaunchKernelGGL((gspat_wave<kSphere, MP, true>), grid_a, blk, 0, st, a);
-                else
-                    hipLaunchKernelGGL((gspat_wave<kSphere, MP, false>), grid_a, blk, 0, st, a);
-            }
+            const void* fn = t4 ? (g32 ? kfn(gspat_wave<kT4010A1, MP, true>) : kfn(gspat_wave<kT4010A1, MP, false>))
+                                : (g32 ? kfn(gspat_wave<kSphere, MP, true>) : kfn(gspat_wave<kSphere, MP, false>));
+            ch.add(fn, grid_a, blk, a);
         } else {
-            const dim3 blk_f(4 * MP);
-            if (src == kT4010A1) {
-                if (g32)
-                    hipLaunchKernelGGL((gspat_fused<kT4010A1, MP, true>), grid_a, blk_f, 0, st, a);
-                else
-                    hipLaunchKernelGGL((gspat_fused<kT4010A1, MP, false>), grid_a, blk_f, 0, st, a);
-            } else {
-                if (g32)
-                    hipLaunchKernelGGL((gspat_fused<kSphere, MP, true>), grid_a, blk_f, 0, st, a);
-                else
-                    hipLaunchKernelGGL((gspat_fused<kSphere, MP, false>), grid_a, blk_f, 0, st, a);
-            }
+            const void* fn = t4 ? (g32 ? kfn(gspat_fused<kT4010A1, MP, true>) : kfn(gspat_fused<kT4010A1, MP, false>))
+                                : (g32 ? kfn(gspat_fused<kSphere, MP, true>) : kfn(gspat_fused<kSphere, MP, false>));
+            ch.add(fn, grid_a, dim3(4 * MP), a);
         }
     }
     const int closing = precision == AFH_HOLO_MIXED_CLOSING ? 1 : 0;
-    hipLaunchKernelGGL((gspat_iter<MP>), grid, blk, 0, st, a, iters, closing);
-    if (closing) hipLaunchKernelGGL((gspat_q<MP, false>), grid_a, blk, 0, st, a, closing, (int)kQGamma);
+    ch.add(kfn(gspat_iter<MP>), grid, blk, a, iters, closing);
+    if (closing) ch.add(kfn(gspat_q<MP, false>), grid_a, blk, a, closing, (int)kQGamma);
     if (precision == AFH_HOLO_MIXED)
-        hipLaunchKernelGGL((gspat_q<MP, true>), grid_a, blk, 0, st, a, closing, (int)kQFinal);
+        ch.add(kfn(gspat_q<MP, true>), grid_a, blk, a, closing, (int)kQFinal);
     else
-        hipLaunchKernelGGL((gspat_q<MP, false>), grid_a, blk, 0, st, a, closing, (int)kQFinal);
-    return hipGetLastError();
+        ch.add(kfn(gspat_q<MP, false>), grid_a, blk, a, closing, (int)kQFinal);
 }
 
 template <int MP>
@@ -1876,12 +1982,14 @@ static int32_t run_holo(const HoloBatch& hb, const HoloPlan& pl, char* wbase, co
                 set_error("holo: GS-PAT supports at most 2^32 / (8 * 128) padded sources");
                 return AFH_ERR_UNSUPPORTED;
             }
+            KChain ch;
             switch (mp) {
-                case 16: e = launch_gspat_bucket<16>(a, hb.source_type, hb.precision, nb, hb.iterations, st); break;
-                case 32: e = launch_gspat_bucket<32>(a, hb.source_type, hb.precision, nb, hb.iterations, st); break;
-                case 64: e = launch_gspat_bucket<64>(a, hb.source_type, hb.precision, nb, hb.iterations, st); break;
-                default: e = launch_gspat_bucket<128>(a, hb.source_type, hb.precision, nb, hb.iterations, st); break;
+                case 16: record_gspat_bucket<16>(ch, a, hb.source_type, hb.precision, nb, hb.iterations); break;
+                case 32: record_gspat_bucket<32>(ch, a, hb.source_type, hb.precision, nb, hb.iterations); break;
+                case 64: record_gspat_bucket<64>(ch, a, hb.source_type, hb.precision, nb, hb.iterations); break;
+                default: record_gspat_bucket<128>(ch, a, hb.source_type, hb.precision, nb, hb.iterations); break;
             }
+            e = issue_chain(ctx, ch, st);
             r_pos += (size_t)nb * mp * mp * a.split;
             p_pos += (size_t)nb * mp;
             g_pos += (size_t)nb * mp * np;

@@ -135,6 +135,7 @@ struct Flight {
     int stream = 0;
     double work = 0;  // source-point terms, for the stream choice
     bool gspat = false;
+    int slot = 0;  // GS-PAT stream slot
 };
 
 struct Server {
@@ -169,7 +170,6 @@ struct Server {
     std::thread worker;
     int next_stream = 0;
     int device = 0;
-    int gspat_inflight = 0;  // GS-PAT batches on the GPU
 
     ~Server() {
         {
@@ -205,15 +205,17 @@ struct Server {
     // each with its own context (workspace). A field request goes to the field
     // stream with the least queued work (all streams when there is only one).
     static constexpr int kGspatExtra = 2;
+    int gspat_inflight[1 + kGspatExtra] = {};  // GS-PAT batches on the GPU, per GS-PAT stream
     std::vector<hipStream_t> gextra;
     std::vector<std::unique_ptr<Context>> gextra_ctx;
     int gspat_slots() const { return 1 + (int)gextra.size(); }
+    int gspat_slot(uint32_t array_id) const { return (int)(array_id % gspat_slots()); }
     hipStream_t gspat_stream(uint32_t array_id) {
-        const int i = (int)(array_id % gspat_slots());
+        const int i = gspat_slot(array_id);
         return i == 0 ? streams[0] : gextra[i - 1];
     }
     Context* gspat_ctx(uint32_t array_id) {
-        const int i = (int)(array_id % gspat_slots());
+        const int i = gspat_slot(array_id);
         return i == 0 ? ctxs[0].get() : gextra_ctx[i - 1].get();
     }
     int field_stream_index() {
@@ -232,16 +234,17 @@ struct Server {
     // Launches what fits, in arrival order: field requests one launch each (a
     // field request that does not fit holds back the later ones, so large
     // requests are not starved); GS-PAT requests grouped per array into batched
-    // launches of at most max_gspat_batch instances. New GS-PAT batches start
-    // only when the previous ones have completed: requests that arrive in the
-    // meantime wait and form the next, larger batch (a batch costs a few
-    // latency-bound launches whatever its size, so under load this raises the
-    // solve rate; at low load a request still starts at once).
+    // launches of at most max_gspat_batch instances. A new GS-PAT batch on a
+    // GS-PAT stream starts only when that stream's previous batch has
+    // completed: requests that arrive in the meantime wait and form the next,
+    // larger batch (a batch costs a few latency-bound launches whatever its
+    // size, so under load this raises the solve rate; at low load a request
+    // still starts at once), while the other streams' arrays go ahead.
     void dispatch(const std::vector<Request*>& todo, std::vector<Request*>& admitted) {
         std::map<uint32_t, std::vector<Request*>> groups;
         for (Request* r : todo) {
             if (r->kind == kReqGSPAT) {
-                if (gspat_inflight == 0) groups[r->array_id].push_back(r);
+                if (gspat_inflight[gspat_slot(r->array_id)] == 0) groups[r->array_id].push_back(r);
                 continue;
             }
             if (!launch_field(r)) break;
@@ -369,7 +372,8 @@ struct Server {
         }
         if (e != hipSuccess || rc != AFH_OK) (void)hipStreamSynchronize(st);
         f.gspat = true;
-        ++gspat_inflight;
+        f.slot = gspat_slot(array_id);
+        ++gspat_inflight[gspat_slot(array_id)];
         flights.push_back(f);
         return true;
     }
@@ -399,7 +403,7 @@ struct Server {
                 jobs.push_back({r, hold});
             }
             if (it->ev) (void)hipEventDestroy(it->ev);
-            if (it->gspat) --gspat_inflight;
+            if (it->gspat) --gspat_inflight[it->slot];
             outstanding[it->stream] -= it->work;
             dev.release(it->dev_off, it->dev_bytes);
             it = flights.erase(it);
@@ -484,7 +488,7 @@ int32_t AFS_Create(const AFS_Config* cfg, void** out) {
     clear_error();
     if (!cfg || !out || cfg->n_streams == 0 || cfg->n_streams > 16 || cfg->max_gspat_batch == 0 ||
         cfg->hbm_budget_bytes < 4096 || cfg->pinned_budget_bytes < 4096 ||
-        cfg->gspat_precision < AFH_HOLO_FP32 || cfg->gspat_precision > AFH_HOLO_MIXED_SPLIT) {
+        cfg->gspat_precision < AFH_HOLO_FP32 || cfg->gspat_precision > AFH_HOLO_MIXED_SPLIT || cfg->gspat_graphs > 1) {
         set_error("AFS_Create: bad configuration");
         return AFH_ERR_ARG;
     }
@@ -535,6 +539,7 @@ int32_t AFS_Create(const AFS_Config* cfg, void** out) {
     }
     s->ctxs[0]->cu_mask = gmask;
     for (auto& c : s->gextra_ctx) c->cu_mask = gmask;
+    for (int i = 0; i < s->gspat_slots(); ++i) s->gspat_ctx(i)->graphs = cfg->gspat_graphs != 0;
     // the auxiliary streams of small batches now, not on the first fork while serving
     for (int i = 0; i < s->gspat_slots(); ++i)
         if (!s->gspat_ctx(i)->ensure_aux()) return AFH_ERR_HIP;
@@ -741,6 +746,19 @@ int32_t AFS_Stats(void* srv, int32_t kind, double* p50_ms, double* p99_ms, uint6
         s->latencies[0].clear();
         s->latencies[1].clear();
         s->completed = 0;
+    }
+    return AFH_OK;
+}
+
+int32_t AFS_GraphCounters(void* srv, uint64_t* launches, uint64_t* builds) {
+    Server* s = static_cast<Server*>(srv);
+    if (!s || !launches || !builds) return AFH_ERR_ARG;
+    *launches = *builds = 0;
+    for (int i = 0; i < s->gspat_slots(); ++i) {
+        Context* c = s->gspat_ctx(i);
+        std::lock_guard<std::mutex> g(c->mu);
+        *launches += c->graph_launches;
+        *builds += c->graph_builds;
     }
     return AFH_OK;
 }

@@ -415,13 +415,16 @@ def bench_serve(D, qps_list, seconds, gspat_cus=32):
     return out
 
 
-def bench_serve_gspat(afc, D, qps_list, seconds, precisions=(("mixed", 1), ("fp32", 0)), plain_calls=200):
+def bench_serve_gspat(afc, D, qps_list, seconds,
+                      precisions=(("mixed", 1, False), ("mixed_graphs", 1, True), ("fp32", 0, False)),
+                      plain_calls=200):
     """Served GS-PAT alone: Poisson arrivals of GS-PAT requests (M ~ U[2, 128],
     foci over each array, targets U[0.5, 1]) on the 249-, 996- and
     3984-transducer arrays chosen uniformly, at a ramp of offered rates until
-    the p99 latency passes 5 ms; the same ramp with the server in FP32 mode;
-    and the plain path, AFO_GSPAT one request per call, on requests drawn from
-    the same distribution."""
+    the p99 latency passes 5 ms; the same ramp with each bucket's kernel chain
+    launched as a hipGraph, and with the server in FP32 mode; and the plain
+    path, AFO_GSPAT one request per call, on requests drawn from the same
+    distribution."""
     from ctypes import byref, c_double, c_uint64, c_void_p
     from afc import workloads as W
     from afc.server import Server
@@ -446,11 +449,11 @@ def bench_serve_gspat(afc, D, qps_list, seconds, precisions=(("mixed", 1), ("fp3
     dll = afc.dll()
     out = {"workload": "GS-PAT requests only: Poisson arrivals, M ~ U[2,128], arrays 249/996/3984 uniform, "
                        "100 iterations; p99 over the requests of each rate", "runs": {}}
-    for name, prec in precisions:
+    for name, prec, graphs in precisions:
         runs = []
         for qps in qps_list:
             srv = Server(hbm_budget=2 << 30, pinned_budget=1 << 30, n_streams=2, max_gspat_batch=256,
-                         gspat_precision=prec, gspat_cus=0)
+                         gspat_precision=prec, gspat_cus=0, gspat_graphs=graphs)
             ids = [srv.add_array(1, src, k, a) for src, k, a, _ in arrs]
             for aid, (src, k, a, ext) in enumerate(arrs):
                 for m in (4, 20, 40, 100):
@@ -485,9 +488,12 @@ def bench_serve_gspat(afc, D, qps_list, seconds, precisions=(("mixed", 1), ("fp3
                 dll.AFS_Wait(h, t, byref(lat))
             elapsed = time.perf_counter() - t0
             p50, p99, done = srv.stats(kind=1)
+            g_launch, g_build = srv.graph_counters()
             srv.close()
             run = {"qps_offered": qps, "requests": nreq, "submitted_per_s": round(nreq / t_sub, 1),
                    "served_solves_per_s": round(nreq / elapsed, 1), "p50_ms": round(p50, 3), "p99_ms": round(p99, 3)}
+            if graphs:
+                run["graph_launches"], run["graph_builds"] = g_launch, g_build
             runs.append(run)
             log(f"serve gspat {name} qps {qps}: {run}")
             if p99 > 5.0 or nreq / t_sub < 0.9 * qps:

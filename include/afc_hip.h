@@ -333,6 +333,10 @@ typedef struct {
     int32_t gspat_precision;       /* AFH_HOLO_* */
     uint32_t gspat_iterations;     /* 100 in the reference */
     uint32_t gspat_cus;            /* CUs reserved for GS-PAT batches (0: shared) */
+    uint32_t gspat_graphs;         /* 1: each GS-PAT bucket's kernel chain is one
+                                      hipGraph launch (graphs cached per shape,
+                                      grids and arguments set per batch); 0:
+                                      launched kernel by kernel */
 } AFS_Config;
 
 int32_t AFS_Create(const AFS_Config* cfg, void** out);
@@ -357,6 +361,9 @@ int32_t AFS_Wait(void* srv, uint64_t ticket, double* latency_ms);
 /* Latency percentiles over the requests completed since the last reset, of
  * one kind (0 field, 1 GS-PAT) or of both (-1). */
 int32_t AFS_Stats(void* srv, int32_t kind, double* p50_ms, double* p99_ms, uint64_t* completed, int32_t reset);
+/* GS-PAT graph launches so far and how many of them had to build (instantiate)
+ * a graph rather than reuse a cached one (both 0 with gspat_graphs = 0). */
+int32_t AFS_GraphCounters(void* srv, uint64_t* launches, uint64_t* builds);
 
 /* T4010A1 directivity as a function of cos(theta), evaluated on the host with
  * exactly the formulas the kernels use in each precision mode (cell table

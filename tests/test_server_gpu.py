@@ -185,3 +185,65 @@ def test_mix_equals_alone_3984_and_large_grids(afc, oracle, torch_cuda):
     log(f"  10^6-point field on the 3984 array: rel-L2 vs FP64 at 2048 points {err:.2e}")
     assert err <= 5e-6
     srv.close()
+
+
+@pytest.mark.parametrize("precision", [1, 0, 2])
+def test_graph_launch_equals_direct(afc, oracle, torch_cuda, precision):
+    """GS-PAT batches whose bucket chains run as cached hipGraphs (grids and
+    arguments set per batch) give the same bits as kernel-by-kernel launches:
+    every request of three rounds (alone, a concurrent mix that forms batches
+    of other shapes, alone again) equals the direct-launch server's result
+    alone, and later batches reuse the graphs that earlier ones built."""
+    from afc.server import Server
+    arrs = arrays(oracle, ((1, 1), (2, 2), (4, 4)))
+    rng = np.random.default_rng(11 + precision)
+    reqs = []
+    for i in range(30):
+        aid = i % 3
+        m = int(rng.choice([2, 9, 16, 17, 31, 33, 64, 65, 100, 128]))
+        span = (170, 130) if aid == 0 else ((360, 280) if aid == 1 else (730, 570))
+        foci = np.stack([rng.uniform(0, span[0], m), rng.uniform(0, span[1], m), rng.uniform(100, 250, m)],
+                        1).astype(np.float32)
+        reqs.append(("gspat", aid, foci, rng.uniform(0.5, 1.0, m).astype(np.float32)))
+    log(f"case: graph vs direct GS-PAT launches, precision {precision}, {len(reqs)} requests on 249/996/3984")
+    cfg = dict(hbm_budget=256 << 20, pinned_budget=64 << 20, n_streams=4, max_gspat_batch=64, gspat_cus=32,
+               gspat_precision=precision)
+    direct = Server(**cfg)
+    ids = [direct.add_array(1, *a) for a in arrs]
+    arr_n = [a[0].shape[0] for a in arrs]
+    want = []
+    for r in reqs:
+        t, bufs = submit(direct, ids, arr_n, r)
+        direct.wait(t)
+        want.append(bufs)
+    assert direct.graph_counters() == (0, 0)
+    direct.close()
+    srv = Server(gspat_graphs=True, **cfg)
+    ids = [srv.add_array(1, *a) for a in arrs]
+    for rnd in range(3):
+        got = [None] * len(reqs)
+        if rnd == 1:
+            tickets = [None] * len(reqs)
+
+            def worker(idx):
+                for i in idx:
+                    tickets[i], got[i] = submit(srv, ids, arr_n, reqs[i])
+
+            th = [threading.Thread(target=worker, args=(list(range(j, len(reqs), 3)),)) for j in range(3)]
+            for t in th:
+                t.start()
+            for t in th:
+                t.join()
+            for t in tickets:
+                srv.wait(t)
+        else:
+            for i, r in enumerate(reqs):
+                t, got[i] = submit(srv, ids, arr_n, r)
+                srv.wait(t)
+        for i in range(len(reqs)):
+            for x, y in zip(want[i], got[i]):
+                assert np.array_equal(x, y), (rnd, i, reqs[i][2].shape[0])
+    launches, builds = srv.graph_counters()
+    log(f"  graph launches {launches}, builds {builds}")
+    assert builds >= 1 and launches > 2 * builds
+    srv.close()
