@@ -532,7 +532,7 @@ extern "C" int32_t AFH_DirectivityT4010A1(const float* cos_theta, uint64_t n, in
     static DirCell cells[kDirCells];
     static std::once_flag once;
     std::call_once(once, [] { build_dir_cells(cells); });
-    if ((!cos_theta || !out) && n) return AFH_ERR_ARG;
+    if (((!cos_theta || !out) && n) || (mode != AFH_PREC_FAST && mode != AFH_PREC_ACCURATE)) return AFH_ERR_ARG;
     for (uint64_t i = 0; i < n; ++i)
         out[i] = mode == AFH_PREC_ACCURATE ? directivity_band(cos_theta[i], h_dir_tab)
                                            : directivity_table(cos_theta[i], cells);

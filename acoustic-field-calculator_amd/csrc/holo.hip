@@ -2046,11 +2046,13 @@ int32_t holo_solve(Context* ctx, const HoloBatch& hb, hipStream_t st) {
     if (rc != AFH_OK) return rc;
     const size_t off_bytes = (B + 1) * sizeof(uint32_t), list_bytes = B * sizeof(uint32_t);
     const size_t cand_bytes = (size_t)pl.div * (sizeof(float2) + sizeof(float));
+    // the candidates' float2 table starts 16-byte aligned in the staging buffer
+    const size_t cand_off = (off_bytes + list_bytes + 15) & ~(size_t)15;
     // growing the workspace frees the old block: work queued on this stream
     // may still use it
     if (pl.total > ctx->work.cap && !hip_ok(hipStreamSynchronize(st), "hipStreamSynchronize")) return AFH_ERR_HIP;
     if (!ctx->work.reserve(pl.total)) return AFH_ERR_HIP;
-    if (!ctx->reserve_staging(off_bytes + list_bytes + cand_bytes)) return AFH_ERR_HIP;
+    if (!ctx->reserve_staging(cand_off + cand_bytes)) return AFH_ERR_HIP;
     uint32_t* stage = static_cast<uint32_t*>(ctx->staging.ptr);
     memcpy(stage, hb.offsets_host, off_bytes);
     {
@@ -2063,7 +2065,7 @@ int32_t holo_solve(Context* ctx, const HoloBatch& hb, hipStream_t st) {
                 "H2D lists"))
         return AFH_ERR_HIP;
     if (pl.div) {
-        char* cs = reinterpret_cast<char*>(stage) + off_bytes + list_bytes;
+        char* cs = reinterpret_cast<char*>(stage) + cand_off;
         float2* c2 = reinterpret_cast<float2*>(cs);
         float* cph = reinterpret_cast<float*>(cs + (size_t)pl.div * sizeof(float2));
         for (uint32_t q = 0; q < pl.div; ++q) greedy_candidate(q, pl.div, c2[q], cph[q]);

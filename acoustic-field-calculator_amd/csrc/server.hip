@@ -547,7 +547,7 @@ int32_t AFS_Create(const AFS_Config* cfg, void** out) {
     // while serving; the workspace, which depends on the array size, when an
     // array is added
     for (int i = 0; i < s->gspat_slots(); ++i)
-        if (!s->gspat_ctx(i)->staging.reserve((2 * (size_t)cfg->max_gspat_batch + 2) * 4)) return AFH_ERR_HIP;
+        if (!s->gspat_ctx(i)->staging.reserve((2 * (size_t)cfg->max_gspat_batch + 2) * 4 + 16)) return AFH_ERR_HIP;
     if (!dir_cells_device()) {
         set_error("AFS_Create: directivity table");
         return AFH_ERR_HIP;
