@@ -112,6 +112,8 @@ struct Request {
     double t_submit = 0, t_done = 0;
     std::atomic<int> state{kPending};  // written by the dispatcher and copy threads, read by AFS_Wait
     int32_t status = AFH_OK;
+    bool direct = false;               // field result DMA'd straight into `out` (pinned caller memory)
+    std::atomic<int> parts_left{0};    // copy jobs of this request still to run
 };
 
 // A completed launch's pinned staging, released once the results of all its
@@ -125,7 +127,12 @@ struct PinHold {
 struct CopyJob {
     Request* r = nullptr;
     std::shared_ptr<PinHold> hold;
+    size_t off = 0, bytes = 0;  // the part of the result this job copies
+    bool second = false;        // GS-PAT: the phases (out2) rather than the amplitudes
 };
+
+// Results larger than this are copied by several copy threads in parts.
+constexpr size_t kCopyPart = 1 << 20;
 
 // One launch in flight: a field request or a GS-PAT batch.
 struct Flight {
@@ -271,9 +278,12 @@ struct Server {
         const size_t pts_b = r->scatter ? r->n_points * 3 * sizeof(float) : 0;
         Flight f;
         f.dev_bytes = Arena::round(out_b) + Arena::round(pts_b);  // out | points
-        f.pin_bytes = out_b + pts_b;
+        // pinned staging: the result (unless it goes straight to the caller's
+        // pinned buffer), then the scatter points
+        const size_t stage_out = r->direct ? 0 : out_b;
+        f.pin_bytes = stage_out + pts_b;
         if (!dev.alloc(f.dev_bytes, &f.dev_off)) return false;
-        if (!pin_alloc(f.pin_bytes, &f.pin_off)) {
+        if (f.pin_bytes && !pin_alloc(f.pin_bytes, &f.pin_off)) {
             dev.release(f.dev_off, f.dev_bytes);
             return false;
         }
@@ -285,8 +295,8 @@ struct Server {
         char* h = pin_base + f.pin_off;
         const float* pts_dev = nullptr;
         if (r->scatter) {
-            memcpy(h + out_b, r->points.data(), pts_b);
-            (void)hipMemcpyAsync(d + Arena::round(out_b), h + out_b, pts_b, hipMemcpyHostToDevice, st);
+            memcpy(h + stage_out, r->points.data(), pts_b);
+            (void)hipMemcpyAsync(d + Arena::round(out_b), h + stage_out, pts_b, hipMemcpyHostToDevice, st);
             pts_dev = reinterpret_cast<const float*>(d + Arena::round(out_b));
         }
         FieldArgs a = field_args_for(r->scatter ? nullptr : &r->grid, pts_dev, r->n_points);
@@ -294,7 +304,7 @@ struct Server {
         a.n_sources = A.np;
         a.out = reinterpret_cast<float*>(d);
         hipError_t e = afh::launch_field(a, A.source_type, r->scatter ? kScatter : kGrid, r->field_type, kFast, st);
-        if (e == hipSuccess) e = hipMemcpyAsync(h, d, out_b, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(r->direct ? static_cast<void*>(r->out) : h, d, out_b, hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&f.ev, hipEventDisableTiming);
         if (e == hipSuccess) e = hipEventRecord(f.ev, st);
         r->out_bytes = out_b;
@@ -397,11 +407,26 @@ struct Server {
             auto hold = std::make_shared<PinHold>();
             hold->off = it->pin_off;
             hold->bytes = it->pin_bytes;
-            hold->left = (int)it->reqs.size();
+            const size_t first = jobs.size();
             for (Request* r : it->reqs) {
                 if (q != hipSuccess && r->status == AFH_OK) r->status = AFH_ERR_HIP;
-                jobs.push_back({r, hold});
+                // a large field result is copied in parts by several copy
+                // threads; a GS-PAT result (amplitudes and phases), a result
+                // already in the caller's memory or a failed request is one job
+                const bool parts = r->status == AFH_OK && r->kind == kReqField && !r->direct;
+                const size_t bytes = parts ? r->out_bytes : 0;
+                const size_t n = parts ? std::max<size_t>(1, (bytes + kCopyPart - 1) / kCopyPart) : 1;
+                r->parts_left = (int)n;
+                for (size_t p = 0; p < n; ++p) {
+                    CopyJob j;
+                    j.r = r;
+                    j.hold = hold;
+                    j.off = p * kCopyPart;
+                    j.bytes = parts ? std::min(kCopyPart, bytes - j.off) : 0;
+                    jobs.push_back(j);
+                }
             }
+            hold->left = (int)(jobs.size() - first);
             if (it->ev) (void)hipEventDestroy(it->ev);
             if (it->gspat) --gspat_inflight[it->slot];
             outstanding[it->stream] -= it->work;
@@ -432,14 +457,20 @@ struct Server {
             }
             Request* r = j.r;
             if (r->status == AFH_OK) {
-                memcpy(r->out, pin_base + r->pin_off, r->out_bytes);
-                if (r->kind == kReqGSPAT) memcpy(r->out2, pin_base + r->pin_off2, r->out_bytes);
+                if (r->kind == kReqGSPAT) {
+                    memcpy(r->out, pin_base + r->pin_off, r->out_bytes);
+                    memcpy(r->out2, pin_base + r->pin_off2, r->out_bytes);
+                } else if (j.bytes) {
+                    memcpy(reinterpret_cast<char*>(r->out) + j.off, pin_base + r->pin_off + j.off, j.bytes);
+                }
             }
-            const double t = now_ms();
-            if (--j.hold->left == 0) {
+            if (--j.hold->left == 0 && j.hold->bytes) {
                 std::lock_guard<std::mutex> g(pin_mu);
                 pin.release(j.hold->off, j.hold->bytes);
             }
+            // the request completes with its last part
+            if (--r->parts_left > 0) continue;
+            const double t = now_ms();
             {
                 std::lock_guard<std::mutex> g(mu);
                 r->t_done = t;
@@ -608,6 +639,26 @@ int32_t AFS_AddArray(void* srv, const void* sources, const float* wavenums, cons
     return AFH_OK;
 }
 
+// True when [p, p + bytes) lies inside one pinned host allocation (HIP
+// page-locked memory), which the DMA engine can write directly.
+static bool pinned_range(const void* p, size_t bytes) {
+    hipPointerAttribute_t at;
+    if (hipPointerGetAttributes(&at, p) != hipSuccess) {
+        (void)hipGetLastError();  // pageable memory is not an error here
+        return false;
+    }
+    if (at.type != hipMemoryTypeHost || !at.devicePointer) return false;
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    if (hipMemGetAddressRange(&base, &size, at.devicePointer) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    const char* b = static_cast<const char*>(base);
+    const char* d = static_cast<const char*>(at.devicePointer);
+    return d >= b && (size_t)(d - b) <= size && bytes <= size - (size_t)(d - b);
+}
+
 static int32_t enqueue(Server* s, Request* r, uint64_t* ticket) {
     {
         std::lock_guard<std::mutex> g(s->mu);
@@ -674,6 +725,7 @@ int32_t AFS_SubmitField(void* srv, uint32_t array_id, const AFH_PointSet* pts, i
         set_error("AFS_SubmitField: request does not fit the HBM / pinned budget (or has no points)");
         return AFH_ERR_BUDGET;
     }
+    r->direct = pinned_range(out_host, r->n_points * per * sizeof(float));
     return enqueue(s, r.release(), ticket);
 }
 

@@ -165,9 +165,12 @@ def main():
     ap.add_argument("--no-greedy", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-qps", default="500,1000,2000,3000")
-    ap.add_argument("--serve-seconds", type=float, default=1.5)
+    ap.add_argument("--serve-seconds", type=float, default=4.0, help="Poisson arrivals per rate, mixed stream")
+    ap.add_argument("--serve-gspat-seconds", type=float, default=1.5, help="Poisson arrivals per rate, GS-PAT ramp")
     ap.add_argument("--serve-gspat-qps", default="10000,40000,80000,120000,160000,240000,320000")
     ap.add_argument("--serve-gspat-cus", type=int, default=32, help="CUs reserved for GS-PAT batches while serving")
+    ap.add_argument("--serve-pageable", action="store_true",
+                    help="mixed stream: field results into pageable buffers (default: pinned, DMA'd directly)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device-to-device copy (reference for HBM byte counters)")
     ap.add_argument("--calibrate-issue", action="store_true",
@@ -305,16 +308,16 @@ def main():
         result["greedy"] = bench_greedy(torch, D, O)
     if not args.no_serve:
         result["serve"] = bench_serve(D, [float(q) for q in args.serve_qps.split(",")], args.serve_seconds,
-                                      args.serve_gspat_cus)
+                                      args.serve_gspat_cus, pinned=not args.serve_pageable)
         result["serve_gspat"] = bench_serve_gspat(afc, D, [float(q) for q in args.serve_gspat_qps.split(",")],
-                                                  args.serve_seconds)
+                                                  args.serve_gspat_seconds)
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline(src, k, a, area["order"], cnt, o, area["res"])
     print(json.dumps(result), flush=True)
 
 
-def bench_serve(D, qps_list, seconds, gspat_cus=32):
+def bench_serve(D, qps_list, seconds, gspat_cus=32, pinned=True):
     """Served mixed stream: Poisson arrivals at a fixed QPS, 70 % field
     evaluations (grids of 1e4..1e6 points, log-uniform, 1 mm, complex output)
     and 30 % GS-PAT solves (2..128 foci), each on one of the 249-, 996- or
@@ -332,7 +335,20 @@ def bench_serve(D, qps_list, seconds, gspat_cus=32):
         ext = src[:, 0:3].max(0)
         arrs.append((src, k, a, ext))
     out = {"workload": "Poisson arrivals; 70% field (1e4..1e6 pts log-uniform, complex) / 30% GS-PAT (M~U[2,128], "
-                       "MIXED_SPLIT); arrays 249/996/3984 uniform", "gspat_cus": gspat_cus, "runs": []}
+                       "MIXED_SPLIT); arrays 249/996/3984 uniform", "gspat_cus": gspat_cus,
+           "field_results": "pinned client buffers" if pinned else "pageable client buffers", "runs": []}
+    # result buffers from a ring, touched once beforehand, as a serving client
+    # reuses them (a fresh np.empty per request would add its first-touch page
+    # faults to the server's copy of every result); 256 slots cover the
+    # requests in flight (tens at the highest rate)
+    ring = 256
+    if pinned:
+        # page-locked buffers: the server DMAs field results into them directly
+        import torch
+        fbufs = torch.ones((ring, 2 * 1010 * 1000), dtype=torch.float32, pin_memory=True).numpy()
+    else:
+        fbufs = np.ones((ring, 2 * 1010 * 1000), np.float32)  # grids drawn below have <= 1.0033e6 points
+    gbufs = [np.ones((ring, 2, a[0].shape[0]), np.float32) for a in arrs]
     for qps in qps_list:
         srv = Server(hbm_budget=4 << 30, pinned_budget=1 << 30, n_streams=4, max_gspat_batch=256,
                      gspat_precision=D.HOLO_MIXED_SPLIT, gspat_cus=gspat_cus)
@@ -387,13 +403,13 @@ def bench_serve(D, qps_list, seconds, gspat_cus=32):
             if sp[0] == "f":
                 _, aid, cnt, origin = sp
                 ps = D.grid_pointset((0, 1, 2), cnt, origin, 1.0)
-                buf = np.empty(cnt[0] * cnt[1] * cnt[2] * 2, np.float32)
+                buf = fbufs[i % ring, :cnt[0] * cnt[1] * cnt[2] * 2]
                 q.put(srv.submit_field(ids[aid], ps, 2, buf))
                 terms += cnt[0] * cnt[1] * cnt[2] * arrs[aid][0].shape[0]
             else:
                 _, aid, foci, amps = sp
-                n = arrs[aid][0].shape[0]
-                q.put(srv.submit_gspat(ids[aid], foci, amps, np.empty(n, np.float32), np.empty(n, np.float32)))
+                gb = gbufs[aid][i % ring]
+                q.put(srv.submit_gspat(ids[aid], foci, amps, gb[0], gb[1]))
                 solves += 1
         q.put(None)
         th.join()

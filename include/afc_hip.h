@@ -347,7 +347,10 @@ int32_t AFS_AddArray(void* srv, const void* sources, const float* wavenums, cons
                      int32_t source_type, uint32_t* array_id);
 /* Field of a registered array at a point set (grid, or scatter with HOST
  * points), FAST precision, written to out_host when the request completes
- * (1 or 2 floats per point as AFC_FIELD_*). */
+ * (1 or 2 floats per point as AFC_FIELD_*). When out_host lies in pinned
+ * (page-locked) host memory the result is DMA'd there directly; otherwise it
+ * is staged in the pinned arena and copied (in parts, by several threads,
+ * for results above 1 MiB). */
 int32_t AFS_SubmitField(void* srv, uint32_t array_id, const AFH_PointSet* pts, int32_t field_type, float* out_host,
                         uint64_t* ticket);
 /* GS-PAT on a registered array: m (1..128) foci (host, 3 floats each) and
