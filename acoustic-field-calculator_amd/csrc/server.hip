@@ -142,7 +142,7 @@ struct Flight {
     int stream = 0;
     double work = 0;  // source-point terms, for the stream choice
     bool gspat = false;
-    int slot = 0;  // GS-PAT stream slot
+    int slot = 0;  // GS-PAT lane
 };
 
 struct Server {
@@ -205,25 +205,35 @@ struct Server {
         for (auto& kv : by_ticket) delete kv.second;  // the worker drained `pending` first
     }
 
-    // GS-PAT batches run on stream 0 and kGspatExtra more streams of their own
-    // (on the CU partition when the configuration reserves one), so a solve
-    // never waits behind a large field evaluation and the batches of different
-    // arrays run side by side: array a on GS-PAT stream a mod (1 + kGspatExtra),
-    // each with its own context (workspace). A field request goes to the field
-    // stream with the least queued work (all streams when there is only one).
-    static constexpr int kGspatExtra = 2;
-    int gspat_inflight[1 + kGspatExtra] = {};  // GS-PAT batches on the GPU, per GS-PAT stream
-    std::vector<hipStream_t> gextra;
+    // GS-PAT batches run on lanes, each a stream with its own context
+    // (workspace): lane 0 is stream 0, the others are streams of their own (on
+    // the CU partition when the configuration reserves one), so a solve never
+    // waits behind a large field evaluation. Array a owns lanes_per_array lanes
+    // from lanes_per_array * a on (mod the lane count): a batch of the array
+    // starts on an idle one, so with two lanes its next batch runs beside the
+    // previous one, and the batches of different arrays run side by side.
+    // Without a CU partition there are 6 lanes, 2 per array; with one, 3 lanes,
+    // 1 per array: every CU-masked stream (a lane and its 4 auxiliary streams)
+    // holds a hardware queue of its own, and 6 lanes' 30 queues beside the
+    // field streams' were measured to stall the field requests for
+    // milliseconds. A field request goes to the field stream with the least
+    // queued work (all streams when there is only one).
+    static constexpr int kGspatLanes = 6;
+    int gspat_inflight[kGspatLanes] = {};  // GS-PAT batches on the GPU, per lane
+    int lanes_per_array = 2;
+    std::vector<hipStream_t> gextra;       // lanes 1 .. gspat_lanes() - 1
     std::vector<std::unique_ptr<Context>> gextra_ctx;
-    int gspat_slots() const { return 1 + (int)gextra.size(); }
-    int gspat_slot(uint32_t array_id) const { return (int)(array_id % gspat_slots()); }
-    hipStream_t gspat_stream(uint32_t array_id) {
-        const int i = gspat_slot(array_id);
-        return i == 0 ? streams[0] : gextra[i - 1];
+    int gspat_lanes() const { return 1 + (int)gextra.size(); }
+    int array_lane(uint32_t array_id, int k) const {
+        return (int)(((uint64_t)lanes_per_array * array_id + (uint64_t)(k % lanes_per_array)) % gspat_lanes());
     }
-    Context* gspat_ctx(uint32_t array_id) {
-        const int i = gspat_slot(array_id);
-        return i == 0 ? ctxs[0].get() : gextra_ctx[i - 1].get();
+    hipStream_t lane_stream(int lane) { return lane == 0 ? streams[0] : gextra[lane - 1]; }
+    Context* lane_ctx(int lane) { return lane == 0 ? ctxs[0].get() : gextra_ctx[lane - 1].get(); }
+    // an idle lane of the array, or -1
+    int idle_lane(uint32_t array_id) const {
+        for (int k = 0; k < lanes_per_array; ++k)
+            if (gspat_inflight[array_lane(array_id, k)] == 0) return array_lane(array_id, k);
+        return -1;
     }
     int field_stream_index() {
         if (streams.size() == 1) return 0;
@@ -251,7 +261,7 @@ struct Server {
         std::map<uint32_t, std::vector<Request*>> groups;
         for (Request* r : todo) {
             if (r->kind == kReqGSPAT) {
-                if (gspat_inflight[gspat_slot(r->array_id)] == 0) groups[r->array_id].push_back(r);
+                if (idle_lane(r->array_id) >= 0) groups[r->array_id].push_back(r);
                 continue;
             }
             if (!launch_field(r)) break;
@@ -262,8 +272,10 @@ struct Server {
             for (size_t start = 0; start < rs.size();) {
                 // the largest batch that fits the arena now (one request always
                 // fits an empty arena: checked at submission)
+                const int lane = idle_lane(kv.first);
+                if (lane < 0) break;  // both lanes busy: the rest waits for the next batch
                 size_t cnt = std::min<size_t>(rs.size() - start, cfg.max_gspat_batch);
-                while (cnt > 0 && !launch_gspat(kv.first, &rs[start], cnt)) cnt /= 2;
+                while (cnt > 0 && !launch_gspat(kv.first, lane, &rs[start], cnt)) cnt /= 2;
                 if (cnt == 0) break;
                 admitted.insert(admitted.end(), rs.begin() + start, rs.begin() + start + cnt);
                 start += cnt;
@@ -319,7 +331,7 @@ struct Server {
         return true;
     }
 
-    bool launch_gspat(uint32_t array_id, Request** rs, size_t cnt) {
+    bool launch_gspat(uint32_t array_id, int lane, Request** rs, size_t cnt) {
         const SrvArray& A = *rs[0]->arr;  // every request of the group is on array_id
         size_t sum_m = 0;
         for (size_t i = 0; i < cnt; ++i) sum_m += rs[i]->m;
@@ -335,8 +347,8 @@ struct Server {
             dev.release(f.dev_off, f.dev_bytes);
             return false;
         }
-        hipStream_t st = gspat_stream(array_id);
-        Context* ctx = gspat_ctx(array_id);
+        hipStream_t st = lane_stream(lane);
+        Context* ctx = lane_ctx(lane);
         char* d = dev_base + f.dev_off;
         char* h = pin_base + f.pin_off;
         std::vector<uint32_t> offs(cnt + 1, 0);
@@ -382,8 +394,8 @@ struct Server {
         }
         if (e != hipSuccess || rc != AFH_OK) (void)hipStreamSynchronize(st);
         f.gspat = true;
-        f.slot = gspat_slot(array_id);
-        ++gspat_inflight[gspat_slot(array_id)];
+        f.slot = lane;
+        ++gspat_inflight[lane];
         flights.push_back(f);
         return true;
     }
@@ -562,7 +574,9 @@ int32_t AFS_Create(const AFS_Config* cfg, void** out) {
         s->outstanding.push_back(0.0);
         s->ctxs.emplace_back(new Context());
     }
-    for (int i = 0; i < Server::kGspatExtra; ++i) {
+    s->lanes_per_array = part ? 1 : 2;
+    const int lanes = part ? Server::kGspatLanes / 2 : Server::kGspatLanes;
+    for (int i = 1; i < lanes; ++i) {
         hipStream_t st = nullptr;
         if (!make_stream(true, &st)) return AFH_ERR_HIP;
         s->gextra.push_back(st);
@@ -570,15 +584,15 @@ int32_t AFS_Create(const AFS_Config* cfg, void** out) {
     }
     s->ctxs[0]->cu_mask = gmask;
     for (auto& c : s->gextra_ctx) c->cu_mask = gmask;
-    for (int i = 0; i < s->gspat_slots(); ++i) s->gspat_ctx(i)->graphs = cfg->gspat_graphs != 0;
+    for (int i = 0; i < s->gspat_lanes(); ++i) s->lane_ctx(i)->graphs = cfg->gspat_graphs != 0;
     // the auxiliary streams of small batches now, not on the first fork while serving
-    for (int i = 0; i < s->gspat_slots(); ++i)
-        if (!s->gspat_ctx(i)->ensure_aux()) return AFH_ERR_HIP;
+    for (int i = 0; i < s->gspat_lanes(); ++i)
+        if (!s->lane_ctx(i)->ensure_aux()) return AFH_ERR_HIP;
     // the GS-PAT staging for a full batch is reserved now rather than grown
     // while serving; the workspace, which depends on the array size, when an
     // array is added
-    for (int i = 0; i < s->gspat_slots(); ++i)
-        if (!s->gspat_ctx(i)->staging.reserve((2 * (size_t)cfg->max_gspat_batch + 2) * 4 + 16)) return AFH_ERR_HIP;
+    for (int i = 0; i < s->gspat_lanes(); ++i)
+        if (!s->lane_ctx(i)->staging.reserve((2 * (size_t)cfg->max_gspat_batch + 2) * 4 + 16)) return AFH_ERR_HIP;
     if (!dir_cells_device()) {
         set_error("AFS_Create: directivity table");
         return AFH_ERR_HIP;
@@ -623,14 +637,17 @@ int32_t AFS_AddArray(void* srv, const void* sources, const float* wavenums, cons
             std::lock_guard<std::mutex> g(s->mu);
             next_id = (uint32_t)s->arrays.size();
         }
-        Context* ctx = s->gspat_ctx(next_id);
-        std::lock_guard<std::mutex> g(ctx->mu);
         const size_t need = gspat_workspace_bound(A.np, s->cfg.max_gspat_batch);
-        if (need > ctx->work.cap &&
-            (!hip_ok(hipStreamSynchronize(s->gspat_stream(next_id)), "AFS_AddArray: hipStreamSynchronize") ||
-             !ctx->work.reserve(need))) {
-            (void)hipFree(A.packed);
-            return AFH_ERR_HIP;
+        for (int k = 0; k < s->lanes_per_array; ++k) {
+            const int lane = s->array_lane(next_id, k);
+            Context* ctx = s->lane_ctx(lane);
+            std::lock_guard<std::mutex> g(ctx->mu);
+            if (need > ctx->work.cap &&
+                (!hip_ok(hipStreamSynchronize(s->lane_stream(lane)), "AFS_AddArray: hipStreamSynchronize") ||
+                 !ctx->work.reserve(need))) {
+                (void)hipFree(A.packed);
+                return AFH_ERR_HIP;
+            }
         }
     }
     std::lock_guard<std::mutex> g(s->mu);
@@ -806,8 +823,8 @@ int32_t AFS_GraphCounters(void* srv, uint64_t* launches, uint64_t* builds) {
     Server* s = static_cast<Server*>(srv);
     if (!s || !launches || !builds) return AFH_ERR_ARG;
     *launches = *builds = 0;
-    for (int i = 0; i < s->gspat_slots(); ++i) {
-        Context* c = s->gspat_ctx(i);
+    for (int i = 0; i < s->gspat_lanes(); ++i) {
+        Context* c = s->lane_ctx(i);
         std::lock_guard<std::mutex> g(c->mu);
         *launches += c->graph_launches;
         *builds += c->graph_builds;
