@@ -914,9 +914,12 @@ __device__ __forceinline__ floatx4 mfma16(half4 a, half4 b, floatx4 c) {
 // Sources are staged in LDS kSB chunks at a time, loaded a block ahead and
 // written after the block (one barrier per block). At the end the four waves'
 // partial R are summed in LDS in wave order, and the workgroup writes R (times
-// 1 / s^2) with the lower tiles as conjugate transposes.
+// 1 / s^2) with the lower tiles as conjugate transposes. Occupancy: at most 32
+// foci the kernel fits 3 waves per SIMD (168 registers, the accumulators in
+// VGPRs) with no spills; for 64 foci that bound spills and runs slower, so it
+// keeps 2.
 template <int SRC, int MP, bool G32>
-__global__ __launch_bounds__(kHB) void gspat_wave(HoloArgs a) {
+__global__ __launch_bounds__(kHB, MP <= 32 ? 3 : 2) void gspat_wave(HoloArgs a) {
     constexpr int T = MP / 16;
     constexpr int NU = T * (T + 1) / 2;  // upper-triangle tiles
     constexpr int SPL = T >= 4 ? 4 : 8;  // sources per lane per step
