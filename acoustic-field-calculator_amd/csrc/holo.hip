@@ -20,8 +20,9 @@
 //      source and focus (add_terms_exact), reduces d_j and accumulates R:
 //        - mixed modes: H = s G / d_j split into FP16 hi + lo, R += hi hi^H +
 //          hi lo^H + lo hi^H on v_mfma_f32_16x16x32_f16 with FP32
-//          accumulation, fused with the generation (gspat_wave for MP <= 64:
+//          accumulation, fused with the generation (gspat_wave for MP <= 32:
 //          each wave takes every fourth chunk of the range for all MP foci;
+//          gspat_pair for MP = 64: two waves share a chunk, 32 foci each;
 //          gspat_fused for MP = 128: one wave per 16 foci);
 //        - FP32: gspat_gen stores FP32 G and d_j, gspat_r32 forms R on
 //          v_mfma_f32_16x16x4f32 (MP >= 32); gspat_r_fp32 on the VALU (MP = 16);
@@ -882,28 +883,20 @@ __global__ __launch_bounds__(4 * MP) void gspat_fused(HoloArgs a) {
 }
 
 // ---------------------------------------------------------------- kernel A, one wave per instance row set
-typedef _Float16 half4 __attribute__((ext_vector_type(4)));
-
-// D += A B on the 16 x 16 MFMA whose K matches the operand width: 32
-// (v_mfma_f32_16x16x32_f16, 8 halves per lane) or 16 (v_mfma_f32_16x16x16_f16,
-// 4 halves). Lane l supplies A[l % 16][K / 4 (l / 16) + v] and B[K / 4 (l / 16) +
-// v][l % 16] and receives D[4 (l / 16) + e][l % 16] in both.
+// D += A B on v_mfma_f32_16x16x32_f16 (K = 32, 8 halves per lane): lane l
+// supplies A[l % 16][8 (l / 16) + v] and B[8 (l / 16) + v][l % 16] and
+// receives D[4 (l / 16) + e][l % 16].
 __device__ __forceinline__ floatx4 mfma16(half8 a, half8 b, floatx4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
 }
-__device__ __forceinline__ floatx4 mfma16(half4 a, half4 b, floatx4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
-}
 
-// For MP <= 64 (T = MP / 16 <= 4 row tiles): a workgroup of 4 waves takes one
+// For MP <= 32 (T = MP / 16 <= 2 row tiles): a workgroup of 4 waves takes one
 // instance and one source-chunk range; wave w handles the chunks c_begin + w,
 // c_begin + w + 4, ... of the range, each for all MP foci, so the column sums
-// d_j and every MFMA operand stay inside the wave. A chunk of 32 sources runs as
-// 32 / K steps of K = 4 SPL columns (SPL = 8 sources per lane, K = 32, up to
-// two row tiles; SPL = 4, K = 16, for four, which keeps the wave within 256
-// registers). Lane l evaluates focus 16 t + l % 16 (t < T) at the sources
-// SPL (l / 16) .. + SPL - 1 of the step: each staged source is read once and
-// used for the T foci. Per step:
+// d_j and every MFMA operand stay inside the wave. A chunk of 32 sources is one
+// K = 32 step (SPL = 8 sources per lane). Lane l evaluates focus 16 t + l % 16
+// (t < T) at the sources SPL (l / 16) .. + SPL - 1 of the chunk: each staged
+// source is read once and used for the T foci. Per step:
 //   1. the SPL T terms (tiles whose rows are all padding skip them);
 //   2. d_j: the lane's T rows in order, then the 16 rows of the lane group by
 //      DPP (every lane of the group ends with the same bits);
@@ -914,18 +907,17 @@ __device__ __forceinline__ floatx4 mfma16(half4 a, half4 b, floatx4 c) {
 // Sources are staged in LDS kSB chunks at a time, loaded a block ahead and
 // written after the block (one barrier per block). At the end the four waves'
 // partial R are summed in LDS in wave order, and the workgroup writes R (times
-// 1 / s^2) with the lower tiles as conjugate transposes. Occupancy: at most 32
-// foci the kernel fits 3 waves per SIMD (168 registers, the accumulators in
-// VGPRs) with no spills; for 64 foci that bound spills and runs slower, so it
-// keeps 2.
+// 1 / s^2) with the lower tiles as conjugate transposes. Occupancy: 3 waves
+// per SIMD (at most 168 registers, the accumulators in VGPRs, no spills).
+// 64 foci run on wave pairs (gspat_pair).
 template <int SRC, int MP, bool G32>
-__global__ __launch_bounds__(kHB, MP <= 32 ? 3 : 2) void gspat_wave(HoloArgs a) {
+__global__ __launch_bounds__(kHB, 3) void gspat_wave(HoloArgs a) {
     constexpr int T = MP / 16;
     constexpr int NU = T * (T + 1) / 2;  // upper-triangle tiles
-    constexpr int SPL = T >= 4 ? 4 : 8;  // sources per lane per step
+    constexpr int SPL = 8;               // sources per lane per step
     constexpr int KS = 4 * SPL;          // columns per step
-    using hvec = std::conditional_t<SPL == 8, half8, half4>;
-    static_assert(T >= 1 && T <= 4, "one wave holds at most 4 row tiles");
+    using hvec = half8;
+    static_assert(T >= 1 && T <= 2, "one wave holds at most 2 row tiles");
     __shared__ std::conditional_t<SRC == kT4010A1, CellsLds, float> s_cells_mem;
     CellsLds* s_cells = reinterpret_cast<CellsLds*>(&s_cells_mem);
     constexpr size_t kSrcBytes = 2 * sizeof(SrcBlock), kRBytes = (size_t)MP * MP * sizeof(float2);
@@ -1116,6 +1108,261 @@ __global__ __launch_bounds__(kHB, MP <= 32 ? 3 : 2) void gspat_wave(HoloArgs a) 
     for (int e = t; e < MP * MP; e += kHB) {
         const int ri = e / MP, cj = e % MP;
         // the upper triangle of tiles holds the sums; lower tiles by symmetry
+        const bool up = ri / 16 <= cj / 16;
+        const float2 v = up ? s_R[e] : s_R[cj * MP + ri];
+        R[e] = make_float2(v.x * unscale, up ? v.y * unscale : -v.y * unscale);
+    }
+}
+
+// ---------------------------------------------------------------- kernel A, wave pairs (64 foci)
+// Kernel A for MP = 64 on pairs of waves: the two waves of a pair take the same
+// chunk of 32 sources, each for its own 32 foci (row tiles 2 hw and 2 hw + 1,
+// hw the wave's half of the pair), as gspat_wave lays out one wave for 32 foci
+// (K = 32 MFMA steps, 8 sources per lane). Per chunk:
+//   1. the 16 terms of each lane (2 row tiles x 8 sources);
+//   2. the pair's column sums d_j from the two waves' partials over their 32
+//      rows, exchanged through LDS (both waves add them in the same order);
+//   3. H = s G / d_j split into FP16 hi + lo, kernel C's copy stored;
+//   4. the operands the partner needs exchanged through LDS (hw = 0 sends tile
+//      1, hw = 1 tiles 2 and 3), then 5 of the 10 upper-triangle tiles per
+//      wave: hw = 0 (0,0) (0,1) (1,1) (0,2) (0,3), hw = 1 (2,2) (2,3) (3,3)
+//      (1,2) (1,3).
+// Half the accumulators and operands of the one-wave layout per wave: 3 waves
+// per SIMD (53 KB of LDS per workgroup of 2 pairs, the source block single-
+// buffered). The two pairs take alternate chunks of the workgroup's range and
+// run in lockstep (two barriers per chunk); their partial R are summed in
+// LDS in pair order at the end. Kernel C's copy is written with plain stores:
+// the FP32 copy with non-temporal ones (16-byte pieces at a 64-byte stride per
+// row and instruction) took 40% longer.
+template <int SRC, bool G32>
+__global__ __launch_bounds__(kHB, 3) void gspat_pair(HoloArgs a) {
+    constexpr int MP = 64, SPL = 8;
+    static_assert(4 * SPL == kKC, "one K = 32 step per chunk");
+    __shared__ std::conditional_t<SRC == kT4010A1, CellsLds, float> s_cells_mem;
+    CellsLds* s_cells = reinterpret_cast<CellsLds*>(&s_cells_mem);
+    struct Xch {
+        SrcBlock src;
+        uintx4 op[2][3][4][64];  // [pair][slot: tile 1, 2, 3][hr hi lr li][lane]
+        float dpart[2][2][kKC];  // [pair][hw][column]
+    };
+    constexpr size_t kXBytes = sizeof(Xch), kRBytes = (size_t)MP * MP * sizeof(float2);
+    __shared__ __attribute__((aligned(16))) unsigned char s_buf[kXBytes > kRBytes ? kXBytes : kRBytes];
+    Xch* X = reinterpret_cast<Xch*>(s_buf);
+    float2* s_R = reinterpret_cast<float2*>(s_buf);
+    const int t = threadIdx.x;
+    const int w = t >> 6, l = t & 63, r16 = l & 15, g = l >> 4;
+    const int pr = w >> 1, hw = w & 1;
+    const uint32_t k = blockIdx.x / a.split, part = blockIdx.x % a.split, b = a.inst[k];
+    const uint32_t n_chunks = a.n_packed / kKC;
+    const uint32_t c_begin = part * n_chunks / a.split, c_end = (part + 1) * n_chunks / a.split;
+    const uint32_t base = a.offsets[b], M = a.offsets[b + 1] - base;
+    if (SRC == kT4010A1) stage_cells_lds(a.cells, s_cells);
+    // own row tiles 2 hw + q, q < 2
+    Split3 pf[2];
+    float wt[2];
+    bool live[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int tl = 2 * hw + q, row = 16 * tl + r16;
+        live[q] = 16 * tl < (int)M;  // wave-uniform
+        float px = 0.f, py = 0.f, pz = 1.f;
+        wt[q] = 0.f;
+        if (row < (int)M) {
+            const uint64_t o = 3 * (uint64_t)(base + row);
+            px = a.foci[o];
+            py = a.foci[o + 1];
+            pz = a.foci[o + 2];
+            wt[q] = 1.f;
+        }
+        pf[q] = split3(px, py, pz);
+    }
+    const bool live1 = 16 < (int)M, live2 = 32 < (int)M, live3 = 48 < (int)M;  // tile 0 is always live
+    const float scale = fmaxf(1.0f, exp2f(floorf(log2f((float)M)) - 1.0f));
+    // accumulators: slots 0..2 the wave's own tiles (q, q') = (0,0) (0,1) (1,1),
+    // slots 3, 4 the cross tiles
+    floatx4 accRe[5], accIm[5];
+#pragma unroll
+    for (int s = 0; s < 5; ++s) {
+        accRe[s] = floatx4{0.f, 0.f, 0.f, 0.f};
+        accIm[s] = accRe[s];
+    }
+    float* d_out = a.d + (uint64_t)k * a.n_packed;
+    float4 pre[3];
+    if (c_begin < c_end) {
+        src_block_fetch(a, c_begin, pre);
+        src_block_put(X->src, pre);
+    }
+    __syncthreads();
+    for (uint32_t c0 = c_begin; c0 < c_end; c0 += kSB) {
+        if (c0 + kSB < c_end) src_block_fetch(a, c0 + kSB, pre);
+        for (int rd = 0; rd < kSB / 2; ++rd) {
+            const int cb = 2 * rd + pr;  // the pair's chunk within the block
+            const bool act = c0 + cb < c_end;  // pair-uniform
+            const uint32_t jb = (c0 + cb) * kKC + SPL * g;  // the lane's first column
+            half8 hr[2], hi[2], lr[2], li[2];
+            if (act) {
+                // 1. terms; cs: the lane's rows in order
+                float re[2][SPL], im[2][SPL], cs[SPL];
+#pragma unroll
+                for (int v = 0; v < SPL; ++v) {
+                    cs[v] = 0.f;
+                    const int j = cb * kKC + SPL * g + v;
+                    const float4 q0 = X->src.v[0][j], q1 = X->src.v[1][j], q2 = X->src.v[2][j];
+                    const SrcF sv = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
+                    const Split3 ss = split3(q0.x, q0.y, q0.z);
+#pragma unroll
+                    for (int q = 0; q < 2; ++q) {
+                        re[q][v] = im[q][v] = 0.f;
+                        if (live[q]) {
+                            float am;
+                            add_terms_exact<SRC, 1>(sv, ss, &pf[q], s_cells->p, s_cells->q, &re[q][v], &im[q][v],
+                                                    &wt[q], &am);
+                            cs[v] += am;
+                        }
+                    }
+                }
+                // 2. the wave's partial column sums over its 32 rows
+#pragma unroll
+                for (int v = 0; v < SPL; ++v) cs[v] = row_group_sum<16>(cs[v]);
+                if (r16 == 0) {
+                    floatx4* dp = reinterpret_cast<floatx4*>(&X->dpart[pr][hw][SPL * g]);
+                    dp[0] = floatx4{cs[0], cs[1], cs[2], cs[3]};
+                    dp[1] = floatx4{cs[4], cs[5], cs[6], cs[7]};
+                }
+                lds_sync();
+                float d[SPL];
+#pragma unroll
+                for (int v = 0; v < SPL; ++v) d[v] = X->dpart[pr][0][SPL * g + v] + X->dpart[pr][1][SPL * g + v];
+                if (hw == 0 && r16 == 0) {
+                    floatx4* dg = reinterpret_cast<floatx4*>(d_out + jb);
+                    dg[0] = floatx4{d[0], d[1], d[2], d[3]};
+                    dg[1] = floatx4{d[4], d[5], d[6], d[7]};
+                }
+                // 3. H = s G / d_j in FP16 hi + lo; kernel C's copy
+                float inv[SPL];
+#pragma unroll
+                for (int v = 0; v < SPL; ++v) {
+                    float r = __builtin_amdgcn_rcpf(d[v]);
+                    r = fmaf(r, fmaf(-d[v], r, 1.0f), r);
+                    inv[v] = (jb + v < a.n) ? scale * r : 0.f;
+                }
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
+#pragma unroll
+                    for (int v = 0; v < SPL; ++v) {
+                        const float xr = re[q][v] * inv[v], xi = im[q][v] * inv[v];
+                        const _Float16 rh = (_Float16)xr, ih = (_Float16)xi;
+                        hr[q][v] = rh;
+                        hi[q][v] = ih;
+                        lr[q][v] = (_Float16)(xr - (float)rh);
+                        li[q][v] = (_Float16)(xi - (float)ih);
+                    }
+                    const int row = 16 * (2 * hw + q) + r16;
+                    if (row < (int)M) {
+                        if constexpr (G32) {
+                            floatx4* gp = reinterpret_cast<floatx4*>(a.G + ((uint64_t)k * MP + row) * a.n_packed + jb);
+#pragma unroll
+                            for (int v = 0; v < SPL / 2; ++v)
+                                gp[v] = floatx4{re[q][2 * v], im[q][2 * v], re[q][2 * v + 1], im[q][2 * v + 1]};
+                        } else {
+                            uintx4* hp = reinterpret_cast<uintx4*>(reinterpret_cast<uint32_t*>(a.G) +
+                                                                   ((uint64_t)k * MP + row) * a.n_packed + jb);
+#pragma unroll
+                            for (int v = 0; v < SPL / 4; ++v)
+                                hp[v] = uintx4{pack_h2_bits(hr[q][4 * v], hi[q][4 * v]),
+                                               pack_h2_bits(hr[q][4 * v + 1], hi[q][4 * v + 1]),
+                                               pack_h2_bits(hr[q][4 * v + 2], hi[q][4 * v + 2]),
+                                               pack_h2_bits(hr[q][4 * v + 3], hi[q][4 * v + 3])};
+                        }
+                    }
+                }
+                // 4. the operands the partner needs: hw = 0 sends tile 1 (slot
+                // 0), hw = 1 tiles 2 and 3 (slots 1, 2)
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    if (hw == 0 && q == 0) continue;
+                    const int slot = hw == 0 ? 0 : 1 + q;
+                    X->op[pr][slot][0][l] = __builtin_bit_cast(uintx4, hr[q]);
+                    X->op[pr][slot][1][l] = __builtin_bit_cast(uintx4, hi[q]);
+                    X->op[pr][slot][2][l] = __builtin_bit_cast(uintx4, lr[q]);
+                    X->op[pr][slot][3][l] = __builtin_bit_cast(uintx4, li[q]);
+                }
+            } else {
+                lds_sync();  // the d exchange of the other pair
+            }
+            lds_sync();
+            if (act) {
+                // R(I, J) += H_I H_J^H in split FP16: the 12 products of
+                // gspat_wave, with hr / hi / lr / li the real and imaginary
+                // hi and lo halves
+                auto tile = [&](int s, const half8& ahr, const half8& ahi, const half8& alr, const half8& ali,
+                                const half8& bhr, const half8& bhi, const half8& blr, const half8& bli) {
+                    const half8 nhr = -ahr, nlr = -alr;
+                    accRe[s] = mfma16(ahr, blr, accRe[s]);
+                    accRe[s] = mfma16(alr, bhr, accRe[s]);
+                    accRe[s] = mfma16(ahi, bli, accRe[s]);
+                    accRe[s] = mfma16(ali, bhi, accRe[s]);
+                    accIm[s] = mfma16(ahi, blr, accIm[s]);
+                    accIm[s] = mfma16(ali, bhr, accIm[s]);
+                    accIm[s] = mfma16(nhr, bli, accIm[s]);
+                    accIm[s] = mfma16(nlr, bhi, accIm[s]);
+                    accRe[s] = mfma16(ahr, bhr, accRe[s]);
+                    accRe[s] = mfma16(ahi, bhi, accRe[s]);
+                    accIm[s] = mfma16(ahi, bhr, accIm[s]);
+                    accIm[s] = mfma16(nhr, bhi, accIm[s]);
+                };
+                auto xop = [&](int slot, int c) { return __builtin_bit_cast(half8, X->op[pr][slot][c][l]); };
+                if (hw == 0) {
+                    // own tiles 0, 1; partner tiles 2, 3
+                    tile(0, hr[0], hi[0], lr[0], li[0], hr[0], hi[0], lr[0], li[0]);
+                    if (live1) {
+                        tile(1, hr[0], hi[0], lr[0], li[0], hr[1], hi[1], lr[1], li[1]);
+                        tile(2, hr[1], hi[1], lr[1], li[1], hr[1], hi[1], lr[1], li[1]);
+                    }
+                    if (live2) tile(3, hr[0], hi[0], lr[0], li[0], xop(1, 0), xop(1, 1), xop(1, 2), xop(1, 3));
+                    if (live3) tile(4, hr[0], hi[0], lr[0], li[0], xop(2, 0), xop(2, 1), xop(2, 2), xop(2, 3));
+                } else {
+                    // own tiles 2, 3; partner tile 1
+                    if (live2) tile(0, hr[0], hi[0], lr[0], li[0], hr[0], hi[0], lr[0], li[0]);
+                    if (live3) {
+                        tile(1, hr[0], hi[0], lr[0], li[0], hr[1], hi[1], lr[1], li[1]);
+                        tile(2, hr[1], hi[1], lr[1], li[1], hr[1], hi[1], lr[1], li[1]);
+                    }
+                    if (live2) tile(3, xop(0, 0), xop(0, 1), xop(0, 2), xop(0, 3), hr[0], hi[0], lr[0], li[0]);
+                    if (live3) tile(4, xop(0, 0), xop(0, 1), xop(0, 2), xop(0, 3), hr[1], hi[1], lr[1], li[1]);
+                }
+            }
+        }
+        // every wave is past its reads of this block: stage the next one
+        lds_sync();
+        if (c0 + kSB < c_end) src_block_put(X->src, pre);
+        lds_sync();
+    }
+    // the two pairs' partials summed in LDS in pair order; lane l holds rows
+    // 4 (l / 16) + e, column l % 16 of each tile
+    const int tI[5] = {hw ? 2 : 0, hw ? 2 : 0, hw ? 3 : 1, hw ? 1 : 0, hw ? 1 : 0};
+    const int tJ[5] = {hw ? 2 : 0, hw ? 3 : 1, hw ? 3 : 1, hw ? 2 : 2, hw ? 3 : 3};
+    for (int pp = 0; pp < 2; ++pp) {
+        if (pr == pp) {
+#pragma unroll
+            for (int s = 0; s < 5; ++s)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int ri = 16 * tI[s] + 4 * g + e, cj = 16 * tJ[s] + r16;
+                    float2 v = make_float2(accRe[s][e], accIm[s][e]);
+                    if (pp > 0) {
+                        const float2 o = s_R[ri * MP + cj];
+                        v = make_float2(o.x + v.x, o.y + v.y);
+                    }
+                    s_R[ri * MP + cj] = v;
+                }
+        }
+        lds_sync();
+    }
+    const float unscale = 1.0f / (scale * scale);  // exact: scale is a power of two
+    float2* R = a.R + (uint64_t)blockIdx.x * MP * MP;
+    for (int e = t; e < MP * MP; e += kHB) {
+        const int ri = e / MP, cj = e % MP;
         const bool up = ri / 16 <= cj / 16;
         const float2 v = up ? s_R[e] : s_R[cj * MP + ri];
         R[e] = make_float2(v.x * unscale, up ? v.y * unscale : -v.y * unscale);
@@ -1776,7 +2023,11 @@ static void record_gspat_bucket(KChain& ch, const HoloArgs& a, int src, int prec
         // kernel A: G, d_j and the split-FP16 MFMA R in one pass; MIXED keeps
         // the FP16 H for kernel C, the other modes FP32 G
         const bool g32 = precision != AFH_HOLO_MIXED;
-        if constexpr (MP <= 64) {
+        if constexpr (MP == 64) {
+            const void* fn = t4 ? (g32 ? kfn(gspat_pair<kT4010A1, true>) : kfn(gspat_pair<kT4010A1, false>))
+                                : (g32 ? kfn(gspat_pair<kSphere, true>) : kfn(gspat_pair<kSphere, false>));
+            ch.add(fn, grid_a, blk, a);
+        } else if constexpr (MP <= 32) {
             const void* fn = t4 ? (g32 ? kfn(gspat_wave<kT4010A1, MP, true>) : kfn(gspat_wave<kT4010A1, MP, false>))
                                 : (g32 ? kfn(gspat_wave<kSphere, MP, true>) : kfn(gspat_wave<kSphere, MP, false>));
             ch.add(fn, grid_a, blk, a);
