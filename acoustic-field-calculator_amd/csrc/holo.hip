@@ -673,8 +673,44 @@ struct FusedTiles {
     static constexpr int kMax = 1 + (T - 1) / 2 + (T % 2 == 0 ? 1 : 0);
 };
 
-__device__ __forceinline__ uint32_t pack_h2_bits(_Float16 re, _Float16 im) {
-    return (uint32_t)__builtin_bit_cast(uint16_t, re) | ((uint32_t)__builtin_bit_cast(uint16_t, im) << 16);
+typedef float floatx2 __attribute__((ext_vector_type(2)));
+typedef _Float16 halfx2 __attribute__((ext_vector_type(2)));
+
+// x[v] f[v] (v < 8) split into FP16 hi + lo halves, two halves a word in v
+// order (the half8 operand layout of mfma16): hi = fp16(x f), lo = fp16(x f
+// - hi), x f - hi formed exactly by one fused multiply-add and rounded once.
+// hi + lo carries x f to ~22 bits. Two values per v_cvt_pk_f16_f32, lo by
+// v_fma_mix{lo,hi}_f16 straight into the packed word.
+__device__ __forceinline__ void split_h8(const float (&x)[8], const float (&f)[8], uintx4& hi, uintx4& lo) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const float p0 = x[2 * i] * f[2 * i], p1 = x[2 * i + 1] * f[2 * i + 1];
+        const halfx2 h = __builtin_convertvector((floatx2){p0, p1}, halfx2);
+        const halfx2 r = {(_Float16)fmaf(x[2 * i], f[2 * i], -(float)h.x),
+                          (_Float16)fmaf(x[2 * i + 1], f[2 * i + 1], -(float)h.y)};
+        hi[i] = __builtin_bit_cast(uint32_t, h);
+        lo[i] = __builtin_bit_cast(uint32_t, r);
+    }
+}
+
+// Kernel C's FP16 copy of 8 elements, (re, im) halves a word, from the packed
+// real and imaginary hi words: element 2 i is the low halves of word i,
+// element 2 i + 1 the high halves. Words 4 h .. 4 h + 3 of the copy.
+__device__ __forceinline__ uintx4 interleave_h8(const uintx4& re, const uintx4& im, int h) {
+    const uint32_t r0 = re[2 * h], r1 = re[2 * h + 1], i0 = im[2 * h], i1 = im[2 * h + 1];
+    return uintx4{(r0 & 0xffffu) | (i0 << 16), (r0 >> 16) | (i0 & 0xffff0000u), (r1 & 0xffffu) | (i1 << 16),
+                  (r1 >> 16) | (i1 & 0xffff0000u)};
+}
+
+// The sign bits of the 8 FP16 halves of an operand flipped (exact negation).
+__device__ __forceinline__ uintx4 neg_h8(const uintx4& x) { return x ^ 0x80008000u; }
+
+// D += A B on v_mfma_f32_16x16x32_f16 (K = 32, 8 halves per lane): lane l
+// supplies A[l % 16][8 (l / 16) + v] and B[8 (l / 16) + v][l % 16] and
+// receives D[4 (l / 16) + e][l % 16].
+__device__ __forceinline__ floatx4 mfma16(const uintx4& a, const uintx4& b, floatx4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(half8, a), __builtin_bit_cast(half8, b), c, 0,
+                                                   0, 0);
 }
 
 template <int SRC, int MP, bool G32>
@@ -798,26 +834,23 @@ __global__ __launch_bounds__(4 * MP) void gspat_fused(HoloArgs a) {
         }
         // 3. H = s G / d_j split into FP16 hi + lo (a real column with d = 0
         // gives NaN, as conj(G) / 0 does in the reference)
-        half8 ahr, ahi, alr, ali;
+        float inv[8];
 #pragma unroll
         for (int v = 0; v < 8; ++v) {
             // 1 / d from v_rcp_f32 and one Newton step (within an ulp)
             float r = __builtin_amdgcn_rcpf(d[v]);
             r = fmaf(r, fmaf(-d[v], r, 1.0f), r);
-            const float inv = (j0 + 8 * g + v < a.n) ? scale * r : 0.f;
-            const float hr = re[v] * inv, hi = im[v] * inv;
-            const _Float16 hr16 = (_Float16)hr, hi16 = (_Float16)hi;
-            ahr[v] = hr16;
-            ahi[v] = hi16;
-            alr[v] = (_Float16)(hr - (float)hr16);
-            ali[v] = (_Float16)(hi - (float)hi16);
+            inv[v] = (j0 + 8 * g + v < a.n) ? scale * r : 0.f;
         }
+        uintx4 ahr, ahi, alr, ali;
+        split_h8(re, inv, ahr, alr);
+        split_h8(im, inv, ahi, ali);
         {
             const int o = row * RS + 8 * g;
-            *reinterpret_cast<half8*>(&s_b[0][o]) = ahr;
-            *reinterpret_cast<half8*>(&s_b[1][o]) = ahi;
-            *reinterpret_cast<half8*>(&s_b[2][o]) = alr;
-            *reinterpret_cast<half8*>(&s_b[3][o]) = ali;
+            *reinterpret_cast<uintx4*>(&s_b[0][o]) = ahr;
+            *reinterpret_cast<uintx4*>(&s_b[1][o]) = ahi;
+            *reinterpret_cast<uintx4*>(&s_b[2][o]) = alr;
+            *reinterpret_cast<uintx4*>(&s_b[3][o]) = ali;
         }
         // kernel C's copy of the tile, rows < M: FP32 G, or the hi half of H
         if (row < (int)M) {
@@ -829,40 +862,36 @@ __global__ __launch_bounds__(4 * MP) void gspat_fused(HoloArgs a) {
             } else {
                 uintx4* hp = reinterpret_cast<uintx4*>(reinterpret_cast<uint32_t*>(a.G) +
                                                        ((uint64_t)k * MP + row) * a.n_packed + j0 + 8 * g);
-                __builtin_nontemporal_store(uintx4{pack_h2_bits(ahr[0], ahi[0]), pack_h2_bits(ahr[1], ahi[1]),
-                                                   pack_h2_bits(ahr[2], ahi[2]), pack_h2_bits(ahr[3], ahi[3])},
-                                            hp);
-                __builtin_nontemporal_store(uintx4{pack_h2_bits(ahr[4], ahi[4]), pack_h2_bits(ahr[5], ahi[5]),
-                                                   pack_h2_bits(ahr[6], ahi[6]), pack_h2_bits(ahr[7], ahi[7])},
-                                            hp + 1);
+                __builtin_nontemporal_store(interleave_h8(ahr, ahi, 0), hp);
+                __builtin_nontemporal_store(interleave_h8(ahr, ahi, 1), hp + 1);
             }
         }
         lds_sync();
         // 4. R tiles: A = this wave's rows (registers), B = rows of tile J
-        const half8 nhr = -ahr, nlr = -alr;
+        const uintx4 nhr = neg_h8(ahr), nlr = neg_h8(alr);
 #pragma unroll
         for (int s = 0; s < NTW; ++s) {
-            half8 bhr = ahr, bhi = ahi, blr = alr, bli = ali;
+            uintx4 bhr = ahr, bhi = ahi, blr = alr, bli = ali;
             if (tJ[s] != w) {
                 const int o = (16 * tJ[s] + r16) * RS + 8 * g;
-                bhr = *reinterpret_cast<const half8*>(&s_b[0][o]);
-                bhi = *reinterpret_cast<const half8*>(&s_b[1][o]);
-                blr = *reinterpret_cast<const half8*>(&s_b[2][o]);
-                bli = *reinterpret_cast<const half8*>(&s_b[3][o]);
+                bhr = *reinterpret_cast<const uintx4*>(&s_b[0][o]);
+                bhi = *reinterpret_cast<const uintx4*>(&s_b[1][o]);
+                blr = *reinterpret_cast<const uintx4*>(&s_b[2][o]);
+                bli = *reinterpret_cast<const uintx4*>(&s_b[3][o]);
             }
             // the small cross products first
-            accRe[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahr, blr, accRe[s], 0, 0, 0);
-            accRe[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(alr, bhr, accRe[s], 0, 0, 0);
-            accRe[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi, bli, accRe[s], 0, 0, 0);
-            accRe[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ali, bhi, accRe[s], 0, 0, 0);
-            accIm[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi, blr, accIm[s], 0, 0, 0);
-            accIm[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ali, bhr, accIm[s], 0, 0, 0);
-            accIm[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(nhr, bli, accIm[s], 0, 0, 0);
-            accIm[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(nlr, bhi, accIm[s], 0, 0, 0);
-            accRe[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahr, bhr, accRe[s], 0, 0, 0);
-            accRe[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi, bhi, accRe[s], 0, 0, 0);
-            accIm[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi, bhr, accIm[s], 0, 0, 0);
-            accIm[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(nhr, bhi, accIm[s], 0, 0, 0);
+            accRe[s] = mfma16(ahr, blr, accRe[s]);
+            accRe[s] = mfma16(alr, bhr, accRe[s]);
+            accRe[s] = mfma16(ahi, bli, accRe[s]);
+            accRe[s] = mfma16(ali, bhi, accRe[s]);
+            accIm[s] = mfma16(ahi, blr, accIm[s]);
+            accIm[s] = mfma16(ali, bhr, accIm[s]);
+            accIm[s] = mfma16(nhr, bli, accIm[s]);
+            accIm[s] = mfma16(nlr, bhi, accIm[s]);
+            accRe[s] = mfma16(ahr, bhr, accRe[s]);
+            accRe[s] = mfma16(ahi, bhi, accRe[s]);
+            accIm[s] = mfma16(ahi, bhr, accIm[s]);
+            accIm[s] = mfma16(nhr, bhi, accIm[s]);
         }
     }
     // lane l holds rows 4 (l / 16) + e, column l % 16 of each tile
@@ -883,12 +912,6 @@ __global__ __launch_bounds__(4 * MP) void gspat_fused(HoloArgs a) {
 }
 
 // ---------------------------------------------------------------- kernel A, one wave per instance row set
-// D += A B on v_mfma_f32_16x16x32_f16 (K = 32, 8 halves per lane): lane l
-// supplies A[l % 16][8 (l / 16) + v] and B[8 (l / 16) + v][l % 16] and
-// receives D[4 (l / 16) + e][l % 16].
-__device__ __forceinline__ floatx4 mfma16(half8 a, half8 b, floatx4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
-}
 
 // For MP <= 32 (T = MP / 16 <= 2 row tiles): a workgroup of 4 waves takes one
 // instance and one source-chunk range; wave w handles the chunks c_begin + w,
@@ -916,7 +939,6 @@ __global__ __launch_bounds__(kHB, 3) void gspat_wave(HoloArgs a) {
     constexpr int NU = T * (T + 1) / 2;  // upper-triangle tiles
     constexpr int SPL = 8;               // sources per lane per step
     constexpr int KS = 4 * SPL;          // columns per step
-    using hvec = half8;
     static_assert(T >= 1 && T <= 2, "one wave holds at most 2 row tiles");
     __shared__ std::conditional_t<SRC == kT4010A1, CellsLds, float> s_cells_mem;
     CellsLds* s_cells = reinterpret_cast<CellsLds*>(&s_cells_mem);
@@ -1022,18 +1044,11 @@ __global__ __launch_bounds__(kHB, 3) void gspat_wave(HoloArgs a) {
                     r = fmaf(r, fmaf(-d[v], r, 1.0f), r);
                     inv[v] = (jb + v < a.n) ? scale * r : 0.f;
                 }
-                hvec hr[T], hi[T], lr[T], li[T];
+                uintx4 hr[T], hi[T], lr[T], li[T];
 #pragma unroll
                 for (int q = 0; q < T; ++q) {
-#pragma unroll
-                    for (int v = 0; v < SPL; ++v) {
-                        const float xr = re[q][v] * inv[v], xi = im[q][v] * inv[v];
-                        const _Float16 rh = (_Float16)xr, ih = (_Float16)xi;
-                        hr[q][v] = rh;
-                        hi[q][v] = ih;
-                        lr[q][v] = (_Float16)(xr - (float)rh);
-                        li[q][v] = (_Float16)(xi - (float)ih);
-                    }
+                    split_h8(re[q], inv, hr[q], lr[q]);
+                    split_h8(im[q], inv, hi[q], li[q]);
                     if constexpr (!G32) {
                         const int row = 16 * q + r16;
                         if (row < (int)M) {
@@ -1041,12 +1056,7 @@ __global__ __launch_bounds__(kHB, 3) void gspat_wave(HoloArgs a) {
                                                                    ((uint64_t)k * MP + row) * a.n_packed + jb);
 #pragma unroll
                             for (int v = 0; v < SPL / 4; ++v)
-                                __builtin_nontemporal_store(
-                                    uintx4{pack_h2_bits(hr[q][4 * v], hi[q][4 * v]),
-                                           pack_h2_bits(hr[q][4 * v + 1], hi[q][4 * v + 1]),
-                                           pack_h2_bits(hr[q][4 * v + 2], hi[q][4 * v + 2]),
-                                           pack_h2_bits(hr[q][4 * v + 3], hi[q][4 * v + 3])},
-                                    hp + v);
+                                __builtin_nontemporal_store(interleave_h8(hr[q], hi[q], v), hp + v);
                         }
                     }
                 }
@@ -1054,7 +1064,7 @@ __global__ __launch_bounds__(kHB, 3) void gspat_wave(HoloArgs a) {
                 int s = 0;
 #pragma unroll
                 for (int I = 0; I < T; ++I) {
-                    const hvec nhr = -hr[I], nlr = -lr[I];
+                    const uintx4 nhr = neg_h8(hr[I]), nlr = neg_h8(lr[I]);
 #pragma unroll
                     for (int J = I; J < T; ++J, ++s) {
                         if (!(live[I] && live[J])) continue;
@@ -1199,7 +1209,7 @@ __global__ __launch_bounds__(kHB, 3) void gspat_pair(HoloArgs a) {
             const int cb = 2 * rd + pr;  // the pair's chunk within the block
             const bool act = c0 + cb < c_end;  // pair-uniform
             const uint32_t jb = (c0 + cb) * kKC + SPL * g;  // the lane's first column
-            half8 hr[2], hi[2], lr[2], li[2];
+            uintx4 hr[2], hi[2], lr[2], li[2];
             if (act) {
                 // 1. terms; cs: the lane's rows in order
                 float re[2][SPL], im[2][SPL], cs[SPL];
@@ -1248,15 +1258,8 @@ __global__ __launch_bounds__(kHB, 3) void gspat_pair(HoloArgs a) {
                 }
 #pragma unroll
                 for (int q = 0; q < 2; ++q) {
-#pragma unroll
-                    for (int v = 0; v < SPL; ++v) {
-                        const float xr = re[q][v] * inv[v], xi = im[q][v] * inv[v];
-                        const _Float16 rh = (_Float16)xr, ih = (_Float16)xi;
-                        hr[q][v] = rh;
-                        hi[q][v] = ih;
-                        lr[q][v] = (_Float16)(xr - (float)rh);
-                        li[q][v] = (_Float16)(xi - (float)ih);
-                    }
+                    split_h8(re[q], inv, hr[q], lr[q]);
+                    split_h8(im[q], inv, hi[q], li[q]);
                     const int row = 16 * (2 * hw + q) + r16;
                     if (row < (int)M) {
                         if constexpr (G32) {
@@ -1268,11 +1271,7 @@ __global__ __launch_bounds__(kHB, 3) void gspat_pair(HoloArgs a) {
                             uintx4* hp = reinterpret_cast<uintx4*>(reinterpret_cast<uint32_t*>(a.G) +
                                                                    ((uint64_t)k * MP + row) * a.n_packed + jb);
 #pragma unroll
-                            for (int v = 0; v < SPL / 4; ++v)
-                                hp[v] = uintx4{pack_h2_bits(hr[q][4 * v], hi[q][4 * v]),
-                                               pack_h2_bits(hr[q][4 * v + 1], hi[q][4 * v + 1]),
-                                               pack_h2_bits(hr[q][4 * v + 2], hi[q][4 * v + 2]),
-                                               pack_h2_bits(hr[q][4 * v + 3], hi[q][4 * v + 3])};
+                            for (int v = 0; v < SPL / 4; ++v) hp[v] = interleave_h8(hr[q], hi[q], v);
                         }
                     }
                 }
@@ -1282,10 +1281,10 @@ __global__ __launch_bounds__(kHB, 3) void gspat_pair(HoloArgs a) {
                 for (int q = 0; q < 2; ++q) {
                     if (hw == 0 && q == 0) continue;
                     const int slot = hw == 0 ? 0 : 1 + q;
-                    X->op[pr][slot][0][l] = __builtin_bit_cast(uintx4, hr[q]);
-                    X->op[pr][slot][1][l] = __builtin_bit_cast(uintx4, hi[q]);
-                    X->op[pr][slot][2][l] = __builtin_bit_cast(uintx4, lr[q]);
-                    X->op[pr][slot][3][l] = __builtin_bit_cast(uintx4, li[q]);
+                    X->op[pr][slot][0][l] = hr[q];
+                    X->op[pr][slot][1][l] = hi[q];
+                    X->op[pr][slot][2][l] = lr[q];
+                    X->op[pr][slot][3][l] = li[q];
                 }
             } else {
                 lds_sync();  // the d exchange of the other pair
@@ -1294,42 +1293,76 @@ __global__ __launch_bounds__(kHB, 3) void gspat_pair(HoloArgs a) {
             if (act) {
                 // R(I, J) += H_I H_J^H in split FP16: the 12 products of
                 // gspat_wave, with hr / hi / lr / li the real and imaginary
-                // hi and lo halves
-                auto tile = [&](int s, const half8& ahr, const half8& ahi, const half8& alr, const half8& ali,
-                                const half8& bhr, const half8& bhi, const half8& blr, const half8& bli) {
-                    const half8 nhr = -ahr, nlr = -alr;
+                // hi and lo halves. Im R = Hi_I Hr_J^T - Hr_I Hi_J^T: the
+                // three negative products take (m1a, m1b), (m2a, m2b),
+                // (m3a, m3b), the minus on A's real halves (wave 0, whose A
+                // tiles are its own) or on B's imaginary halves (wave 1, whose
+                // B tiles are its own), so each wave negates only its own two
+                // tiles, once per chunk
+                auto tile = [&](int s, const uintx4& ahr, const uintx4& ahi, const uintx4& alr, const uintx4& ali,
+                                const uintx4& bhr, const uintx4& bhi, const uintx4& blr, const uintx4& bli,
+                                const uintx4& m1a, const uintx4& m1b, const uintx4& m2a, const uintx4& m2b,
+                                const uintx4& m3a, const uintx4& m3b) {
                     accRe[s] = mfma16(ahr, blr, accRe[s]);
                     accRe[s] = mfma16(alr, bhr, accRe[s]);
                     accRe[s] = mfma16(ahi, bli, accRe[s]);
                     accRe[s] = mfma16(ali, bhi, accRe[s]);
                     accIm[s] = mfma16(ahi, blr, accIm[s]);
                     accIm[s] = mfma16(ali, bhr, accIm[s]);
-                    accIm[s] = mfma16(nhr, bli, accIm[s]);
-                    accIm[s] = mfma16(nlr, bhi, accIm[s]);
+                    accIm[s] = mfma16(m1a, m1b, accIm[s]);  // -hr_I li_J
+                    accIm[s] = mfma16(m2a, m2b, accIm[s]);  // -lr_I hi_J
                     accRe[s] = mfma16(ahr, bhr, accRe[s]);
                     accRe[s] = mfma16(ahi, bhi, accRe[s]);
                     accIm[s] = mfma16(ahi, bhr, accIm[s]);
-                    accIm[s] = mfma16(nhr, bhi, accIm[s]);
+                    accIm[s] = mfma16(m3a, m3b, accIm[s]);  // -hr_I hi_J
                 };
-                auto xop = [&](int slot, int c) { return __builtin_bit_cast(half8, X->op[pr][slot][c][l]); };
+                auto xop = [&](int slot, int c) { return X->op[pr][slot][c][l]; };
                 if (hw == 0) {
-                    // own tiles 0, 1; partner tiles 2, 3
-                    tile(0, hr[0], hi[0], lr[0], li[0], hr[0], hi[0], lr[0], li[0]);
+                    // own tiles 0, 1; partner tiles 2, 3; the minus on A
+                    uintx4 nhr[2], nlr[2];
+#pragma unroll
+                    for (int q = 0; q < 2; ++q) {
+                        nhr[q] = neg_h8(hr[q]);
+                        nlr[q] = neg_h8(lr[q]);
+                    }
+                    tile(0, hr[0], hi[0], lr[0], li[0], hr[0], hi[0], lr[0], li[0], nhr[0], li[0], nlr[0], hi[0],
+                         nhr[0], hi[0]);
                     if (live1) {
-                        tile(1, hr[0], hi[0], lr[0], li[0], hr[1], hi[1], lr[1], li[1]);
-                        tile(2, hr[1], hi[1], lr[1], li[1], hr[1], hi[1], lr[1], li[1]);
+                        tile(1, hr[0], hi[0], lr[0], li[0], hr[1], hi[1], lr[1], li[1], nhr[0], li[1], nlr[0], hi[1],
+                             nhr[0], hi[1]);
+                        tile(2, hr[1], hi[1], lr[1], li[1], hr[1], hi[1], lr[1], li[1], nhr[1], li[1], nlr[1], hi[1],
+                             nhr[1], hi[1]);
                     }
-                    if (live2) tile(3, hr[0], hi[0], lr[0], li[0], xop(1, 0), xop(1, 1), xop(1, 2), xop(1, 3));
-                    if (live3) tile(4, hr[0], hi[0], lr[0], li[0], xop(2, 0), xop(2, 1), xop(2, 2), xop(2, 3));
-                } else {
-                    // own tiles 2, 3; partner tile 1
-                    if (live2) tile(0, hr[0], hi[0], lr[0], li[0], hr[0], hi[0], lr[0], li[0]);
+                    if (live2) {
+                        const uintx4 bhr = xop(1, 0), bhi = xop(1, 1), blr = xop(1, 2), bli = xop(1, 3);
+                        tile(3, hr[0], hi[0], lr[0], li[0], bhr, bhi, blr, bli, nhr[0], bli, nlr[0], bhi, nhr[0], bhi);
+                    }
                     if (live3) {
-                        tile(1, hr[0], hi[0], lr[0], li[0], hr[1], hi[1], lr[1], li[1]);
-                        tile(2, hr[1], hi[1], lr[1], li[1], hr[1], hi[1], lr[1], li[1]);
+                        const uintx4 bhr = xop(2, 0), bhi = xop(2, 1), blr = xop(2, 2), bli = xop(2, 3);
+                        tile(4, hr[0], hi[0], lr[0], li[0], bhr, bhi, blr, bli, nhr[0], bli, nlr[0], bhi, nhr[0], bhi);
                     }
-                    if (live2) tile(3, xop(0, 0), xop(0, 1), xop(0, 2), xop(0, 3), hr[0], hi[0], lr[0], li[0]);
-                    if (live3) tile(4, xop(0, 0), xop(0, 1), xop(0, 2), xop(0, 3), hr[1], hi[1], lr[1], li[1]);
+                } else {
+                    // own tiles 2, 3; partner tile 1; the minus on B
+                    uintx4 nhi[2], nli[2];
+#pragma unroll
+                    for (int q = 0; q < 2; ++q) {
+                        nhi[q] = neg_h8(hi[q]);
+                        nli[q] = neg_h8(li[q]);
+                    }
+                    if (live2)
+                        tile(0, hr[0], hi[0], lr[0], li[0], hr[0], hi[0], lr[0], li[0], hr[0], nli[0], lr[0], nhi[0],
+                             hr[0], nhi[0]);
+                    if (live3) {
+                        tile(1, hr[0], hi[0], lr[0], li[0], hr[1], hi[1], lr[1], li[1], hr[0], nli[1], lr[0], nhi[1],
+                             hr[0], nhi[1]);
+                        tile(2, hr[1], hi[1], lr[1], li[1], hr[1], hi[1], lr[1], li[1], hr[1], nli[1], lr[1], nhi[1],
+                             hr[1], nhi[1]);
+                    }
+                    const uintx4 ahr = xop(0, 0), ahi = xop(0, 1), alr = xop(0, 2), ali = xop(0, 3);
+                    if (live2)
+                        tile(3, ahr, ahi, alr, ali, hr[0], hi[0], lr[0], li[0], ahr, nli[0], alr, nhi[0], ahr, nhi[0]);
+                    if (live3)
+                        tile(4, ahr, ahi, alr, ali, hr[1], hi[1], lr[1], li[1], ahr, nli[1], alr, nhi[1], ahr, nhi[1]);
                 }
             }
         }
