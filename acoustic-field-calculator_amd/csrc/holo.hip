@@ -858,12 +858,12 @@ __global__ __launch_bounds__(4 * MP) void gspat_fused(HoloArgs a) {
                 floatx4* gp = reinterpret_cast<floatx4*>(a.G + ((uint64_t)k * MP + row) * a.n_packed + j0 + 8 * g);
 #pragma unroll
                 for (int v = 0; v < 4; ++v)
-                    __builtin_nontemporal_store(floatx4{re[2 * v], im[2 * v], re[2 * v + 1], im[2 * v + 1]}, gp + v);
+                    gp[v] = floatx4{re[2 * v], im[2 * v], re[2 * v + 1], im[2 * v + 1]};
             } else {
                 uintx4* hp = reinterpret_cast<uintx4*>(reinterpret_cast<uint32_t*>(a.G) +
                                                        ((uint64_t)k * MP + row) * a.n_packed + j0 + 8 * g);
-                __builtin_nontemporal_store(interleave_h8(ahr, ahi, 0), hp);
-                __builtin_nontemporal_store(interleave_h8(ahr, ahi, 1), hp + 1);
+                hp[0] = interleave_h8(ahr, ahi, 0);
+                hp[1] = interleave_h8(ahr, ahi, 1);
             }
         }
         lds_sync();
@@ -912,6 +912,93 @@ __global__ __launch_bounds__(4 * MP) void gspat_fused(HoloArgs a) {
 }
 
 // ---------------------------------------------------------------- kernel A, one wave per instance row set
+// The terms of a lane's first NL rows (points pf[0 .. NL - 1]; rows NL .. NQ - 1
+// are zero) at the staged sources j0 .. j0 + 7 of sb: re / im[q][v], and cs[v]
+// the sum of the amplitudes over q in order. NL is a template constant so the
+// 8 NL terms form one straight-line block in which the LDS reads and the long
+// per-term chains overlap: the NL rows of a source in one staged evaluation,
+// sources in pairs; with SERIAL (kernels that also keep FP32 G, where the
+// longer live ranges spill) each term on its own and one source at a time.
+template <int SRC, int NQ, int NL, bool SERIAL>
+__device__ __forceinline__ void lane_terms(const SrcBlock& sb, int j0, const Split3 (&pf)[NQ], const float (&wt)[NQ],
+                                           const CellsLds* cells, float (&re)[NQ][8], float (&im)[NQ][8],
+                                           float (&cs)[8]) {
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+        float rr[NL > 0 ? NL : 1], ii[NL > 0 ? NL : 1], am[NL > 0 ? NL : 1];
+        if constexpr (NL > 0) {
+            const int j = j0 + v;
+            const float4 q0 = sb.v[0][j], q1 = sb.v[1][j], q2 = sb.v[2][j];
+            const SrcF sv = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
+#pragma unroll
+            for (int q = 0; q < NL; ++q) rr[q] = ii[q] = 0.f;
+            if constexpr (SERIAL) {
+                const Split3 ss = split3(q0.x, q0.y, q0.z);
+#pragma unroll
+                for (int q = 0; q < NL; ++q)
+                    add_terms_exact<SRC, 1>(sv, ss, &pf[q], cells->p, cells->q, &rr[q], &ii[q], &wt[q], &am[q]);
+            } else {
+                add_terms_exact<SRC, NL>(sv, split3(q0.x, q0.y, q0.z), pf, cells->p, cells->q, rr, ii, wt, am);
+            }
+        }
+        cs[v] = 0.f;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            if (q < NL) {
+                re[q][v] = rr[q];
+                im[q][v] = ii[q];
+                cs[v] += am[q];
+            } else {
+                re[q][v] = im[q][v] = 0.f;
+            }
+        }
+        if (NL > 1 && (v + 1) % (SERIAL ? 1 : 2) == 0) __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+// lane_terms with the live rows tested term by term (each term its own
+// block): the shortest live ranges, for the wave-pair kernel that keeps FP32 G.
+template <int SRC, int NQ>
+__device__ __forceinline__ void lane_terms_each(int nl, const SrcBlock& sb, int j0, const Split3 (&pf)[NQ],
+                                                const float (&wt)[NQ], const CellsLds* cells, float (&re)[NQ][8],
+                                                float (&im)[NQ][8], float (&cs)[8]) {
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+        cs[v] = 0.f;
+        const int j = j0 + v;
+        const float4 q0 = sb.v[0][j], q1 = sb.v[1][j], q2 = sb.v[2][j];
+        const SrcF sv = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
+        const Split3 ss = split3(q0.x, q0.y, q0.z);
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            re[q][v] = im[q][v] = 0.f;
+            if (q < nl) {
+                float am;
+                add_terms_exact<SRC, 1>(sv, ss, &pf[q], cells->p, cells->q, &re[q][v], &im[q][v], &wt[q], &am);
+                cs[v] += am;
+            }
+        }
+    }
+}
+
+// lane_terms for the nl (wave-uniform) live rows of the lane's NQ <= 2.
+template <int SRC, int NQ, bool SERIAL>
+__device__ __forceinline__ void lane_terms_live(int nl, const SrcBlock& sb, int j0, const Split3 (&pf)[NQ],
+                                                const float (&wt)[NQ], const CellsLds* cells, float (&re)[NQ][8],
+                                                float (&im)[NQ][8], float (&cs)[8]) {
+    static_assert(NQ == 1 || NQ == 2, "one or two rows per lane");
+    if constexpr (NQ == 2) {
+        if (nl >= 2) {
+            lane_terms<SRC, 2, 2, SERIAL>(sb, j0, pf, wt, cells, re, im, cs);
+            return;
+        }
+    }
+    if (nl >= 1)
+        lane_terms<SRC, NQ, 1, SERIAL>(sb, j0, pf, wt, cells, re, im, cs);
+    else
+        lane_terms<SRC, NQ, 0, SERIAL>(sb, j0, pf, wt, cells, re, im, cs);
+}
+
 
 // For MP <= 32 (T = MP / 16 <= 2 row tiles): a workgroup of 4 waves takes one
 // instance and one source-chunk range; wave w handles the chunks c_begin + w,
@@ -947,7 +1034,7 @@ __global__ __launch_bounds__(kHB, 3) void gspat_wave(HoloArgs a) {
     SrcBlock* s_src = reinterpret_cast<SrcBlock*>(s_buf);
     float2* s_R = reinterpret_cast<float2*>(s_buf);
     const int t = threadIdx.x;
-    const int w = t >> 6, l = t & 63, r16 = l & 15, g = l >> 4;
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6), l = t & 63, r16 = l & 15, g = l >> 4;
     const uint32_t k = blockIdx.x / a.split, part = blockIdx.x % a.split, b = a.inst[k];
     const uint32_t n_chunks = a.n_packed / kKC;
     const uint32_t c_begin = part * n_chunks / a.split, c_end = (part + 1) * n_chunks / a.split;
@@ -971,6 +1058,9 @@ __global__ __launch_bounds__(kHB, 3) void gspat_wave(HoloArgs a) {
         }
         pf[q] = split3(px, py, pz);
     }
+    int n_live = 0;  // live rows tiles are a prefix
+#pragma unroll
+    for (int q = 0; q < T; ++q) n_live += live[q] ? 1 : 0;
     const float scale = fmaxf(1.0f, exp2f(floorf(log2f((float)M)) - 1.0f));
     floatx4 accRe[NU], accIm[NU];
 #pragma unroll
@@ -994,24 +1084,7 @@ __global__ __launch_bounds__(kHB, 3) void gspat_wave(HoloArgs a) {
                 const uint32_t jb = (c0 + cb) * kKC + h * KS + SPL * g;  // the lane's first column
                 // 1. terms; cs: the lane's rows in order
                 float re[T][SPL], im[T][SPL], cs[SPL];
-#pragma unroll
-                for (int v = 0; v < SPL; ++v) {
-                    cs[v] = 0.f;
-                    const int j = cb * kKC + h * KS + SPL * g + v;
-                    const float4 q0 = s_src[bb].v[0][j], q1 = s_src[bb].v[1][j], q2 = s_src[bb].v[2][j];
-                    const SrcF sv = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
-                    const Split3 ss = split3(q0.x, q0.y, q0.z);
-#pragma unroll
-                    for (int q = 0; q < T; ++q) {
-                        re[q][v] = im[q][v] = 0.f;
-                        if (live[q]) {
-                            float am;
-                            add_terms_exact<SRC, 1>(sv, ss, &pf[q], s_cells->p, s_cells->q, &re[q][v], &im[q][v],
-                                                    &wt[q], &am);
-                            cs[v] += am;
-                        }
-                    }
-                }
+                lane_terms_live<SRC, T, G32>(n_live, s_src[bb], cb * kKC + h * KS + SPL * g, pf, wt, s_cells, re, im, cs);
                 // 2. d_j over the 16 lanes of the group
                 float d[SPL];
 #pragma unroll
@@ -1030,8 +1103,7 @@ __global__ __launch_bounds__(kHB, 3) void gspat_wave(HoloArgs a) {
                             floatx4* gp = reinterpret_cast<floatx4*>(a.G + ((uint64_t)k * MP + row) * a.n_packed + jb);
 #pragma unroll
                             for (int v = 0; v < SPL / 2; ++v)
-                                __builtin_nontemporal_store(
-                                    floatx4{re[q][2 * v], im[q][2 * v], re[q][2 * v + 1], im[q][2 * v + 1]}, gp + v);
+                                gp[v] = floatx4{re[q][2 * v], im[q][2 * v], re[q][2 * v + 1], im[q][2 * v + 1]};
                         }
                     }
                 }
@@ -1055,8 +1127,7 @@ __global__ __launch_bounds__(kHB, 3) void gspat_wave(HoloArgs a) {
                             uintx4* hp = reinterpret_cast<uintx4*>(reinterpret_cast<uint32_t*>(a.G) +
                                                                    ((uint64_t)k * MP + row) * a.n_packed + jb);
 #pragma unroll
-                            for (int v = 0; v < SPL / 4; ++v)
-                                __builtin_nontemporal_store(interleave_h8(hr[q], hi[q], v), hp + v);
+                            for (int v = 0; v < SPL / 4; ++v) hp[v] = interleave_h8(hr[q], hi[q], v);
                         }
                     }
                 }
@@ -1186,6 +1257,7 @@ __global__ __launch_bounds__(kHB, 3) void gspat_pair(HoloArgs a) {
         }
         pf[q] = split3(px, py, pz);
     }
+    const int n_live = (live[0] ? 1 : 0) + (live[1] ? 1 : 0);  // live tiles are a prefix
     const bool live1 = 16 < (int)M, live2 = 32 < (int)M, live3 = 48 < (int)M;  // tile 0 is always live
     const float scale = fmaxf(1.0f, exp2f(floorf(log2f((float)M)) - 1.0f));
     // accumulators: slots 0..2 the wave's own tiles (q, q') = (0,0) (0,1) (1,1),
@@ -1213,24 +1285,10 @@ __global__ __launch_bounds__(kHB, 3) void gspat_pair(HoloArgs a) {
             if (act) {
                 // 1. terms; cs: the lane's rows in order
                 float re[2][SPL], im[2][SPL], cs[SPL];
-#pragma unroll
-                for (int v = 0; v < SPL; ++v) {
-                    cs[v] = 0.f;
-                    const int j = cb * kKC + SPL * g + v;
-                    const float4 q0 = X->src.v[0][j], q1 = X->src.v[1][j], q2 = X->src.v[2][j];
-                    const SrcF sv = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
-                    const Split3 ss = split3(q0.x, q0.y, q0.z);
-#pragma unroll
-                    for (int q = 0; q < 2; ++q) {
-                        re[q][v] = im[q][v] = 0.f;
-                        if (live[q]) {
-                            float am;
-                            add_terms_exact<SRC, 1>(sv, ss, &pf[q], s_cells->p, s_cells->q, &re[q][v], &im[q][v],
-                                                    &wt[q], &am);
-                            cs[v] += am;
-                        }
-                    }
-                }
+                if constexpr (G32)
+                    lane_terms_each<SRC, 2>(n_live, X->src, cb * kKC + SPL * g, pf, wt, s_cells, re, im, cs);
+                else
+                    lane_terms_live<SRC, 2, false>(n_live, X->src, cb * kKC + SPL * g, pf, wt, s_cells, re, im, cs);
                 // 2. the wave's partial column sums over its 32 rows
 #pragma unroll
                 for (int v = 0; v < SPL; ++v) cs[v] = row_group_sum<16>(cs[v]);
