@@ -53,6 +53,7 @@ namespace afh {
 
 constexpr int kHB = 256;  // threads per workgroup
 constexpr int kKC = 32;   // G columns per chunk
+constexpr int kMaxMP = 128;  // largest padded focus count (bucket)
 
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
@@ -585,15 +586,16 @@ __device__ __forceinline__ float row_group_sum(float v) {
 
 // gamma = R p0; iters x {p = gamma / |gamma| p0; gamma = R p}. Writes the
 // last p (closing) or p = gamma / |gamma|^2 p0^2 to a.p.
+// Workgroup blk of the bucket (one instance) with the p vectors in s_p.
 template <int MP>
-__global__ __launch_bounds__(kHB) void gspat_iter(HoloArgs a, uint32_t iters, int closing) {
+__device__ __forceinline__ void gspat_iter_run(const HoloArgs& a, uint32_t blk, uint32_t iters, int closing,
+                                               float2 (*s_p)[kMaxMP]) {
     constexpr int TPR = kHB / MP;  // lanes per row (consecutive, same wave)
     constexpr int CPT = MP / TPR;  // columns per lane
     static_assert(TPR >= 1 && TPR <= 64, "row group within a wave");
-    __shared__ float2 s_p[2][MP];
     const int t = threadIdx.x;
     const int row = t / TPR, h = t % TPR;
-    const uint32_t k = blockIdx.x, b = a.inst[k];
+    const uint32_t k = blk, b = a.inst[k];
     const uint32_t base = a.offsets[b], M = a.offsets[b + 1] - base;
     // R = sum of kernel A's partials over source chunk ranges, in order
     const float2* R = a.R + (uint64_t)k * a.split * MP * MP + (uint64_t)row * MP + h * CPT;
@@ -643,6 +645,12 @@ __global__ __launch_bounds__(kHB) void gspat_iter(HoloArgs a, uint32_t iters, in
         }
         a.p[(uint64_t)k * MP + row] = out;
     }
+}
+
+template <int MP>
+__global__ __launch_bounds__(kHB) void gspat_iter(HoloArgs a, uint32_t iters, int closing) {
+    __shared__ float2 s_p[2][kMaxMP];
+    gspat_iter_run<MP>(a, blockIdx.x, iters, closing, s_p);
 }
 
 // ---------------------------------------------------------------- kernel A, fused (MFMA modes)
@@ -1469,18 +1477,26 @@ __global__ __launch_bounds__(kHB, 3) void gspat_pair(HoloArgs a) {
 //   gamma the sum of the partials in order; q_j -> amplitude and phase.
 enum { kQGamma = 0, kQFinal = 1 };
 
+// Kernel C's LDS. s_qp and s_q are double-buffered by chunk parity: a buffer
+// is written again two chunks later, after a barrier that every reader of its
+// previous contents has reached.
+struct QLds {
+    float2 qp[2][8][kKC];
+    float2 q[2][kKC];
+    float2 qall[8][kKC];  // kQFinal: q of up to 8 chunks, written out together
+    float2 p[kMaxMP];
+};
+
+// Workgroup blk of the bucket (instance blk / split, chunk range blk % split).
 template <int MP, bool H16>
-__global__ __launch_bounds__(kHB) void gspat_q(HoloArgs a, int closing, int pass) {
-    // double-buffered by chunk parity: a buffer is written again two chunks
-    // later, after a barrier that every reader of its previous contents has
-    // reached
-    __shared__ float2 s_qp[2][8][kKC];
-    __shared__ float2 s_q[2][kKC];
-    __shared__ float2 s_qall[8][kKC];  // kQFinal: q of up to 8 chunks, written out together
-    __shared__ float2 s_p[MP];
+__device__ __forceinline__ void gspat_q_run(const HoloArgs& a, uint32_t blk, int closing, int pass, QLds& L) {
+    auto& s_qp = L.qp;
+    auto& s_q = L.q;
+    auto& s_qall = L.qall;
+    auto& s_p = L.p;
     const int t = threadIdx.x;
     const int grp = t >> 5, jj = t & 31;
-    const uint32_t k = blockIdx.x / a.split, part = blockIdx.x % a.split, b = a.inst[k];
+    const uint32_t k = blk / a.split, part = blk % a.split, b = a.inst[k];
     const uint32_t n_chunks = a.n_packed / kKC;
     const uint32_t c_begin = part * n_chunks / a.split, c_end = (part + 1) * n_chunks / a.split;
     const uint32_t base = a.offsets[b], M = a.offsets[b + 1] - base;
@@ -1613,8 +1629,77 @@ __global__ __launch_bounds__(kHB) void gspat_q(HoloArgs a, int closing, int pass
         }
         if (jj == 0)
 #pragma unroll
-            for (int r = 0; r < MP / 8; ++r) a.gam[(uint64_t)blockIdx.x * MP + grp + 8 * r] = gam[r];
+            for (int r = 0; r < MP / 8; ++r) a.gam[(uint64_t)blk * MP + grp + 8 * r] = gam[r];
     }
+}
+
+template <int MP, bool H16>
+__global__ __launch_bounds__(kHB) void gspat_q(HoloArgs a, int closing, int pass) {
+    __shared__ QLds L;
+    gspat_q_run<MP, H16>(a, blockIdx.x, closing, pass, L);
+}
+
+// Kernels B and C of every bucket of a batch in one launch each: bucket u
+// takes the workgroups first[u] .. first[u + 1] - 1 (instances for B,
+// instances times split for C), buckets in the order listed.
+struct BucketSet {
+    uint32_t n;
+    uint32_t mp[4];
+    uint32_t first[5];
+    const uint32_t* inst[4];
+    float2* R[4];
+    float2* p[4];
+    float2* G[4];
+    float* d[4];
+    float2* gam[4];
+};
+
+// The bucket of workgroup blockIdx.x: its arguments in a, its workgroup index
+// within the bucket in blk; returns the bucket's padded focus count.
+__device__ __forceinline__ uint32_t set_bucket(const BucketSet& s, HoloArgs& a, uint32_t& blk) {
+    uint32_t u = 0;
+    blk = blockIdx.x;
+    while (u + 1 < s.n && blk >= s.first[u + 1]) ++u;
+    blk -= s.first[u];
+    a.inst = s.inst[u];
+    a.R = s.R[u];
+    a.p = s.p[u];
+    a.G = s.G[u];
+    a.d = s.d[u];
+    a.gam = s.gam[u];
+    return s.mp[u];
+}
+
+// MAXMP: the largest bucket of the set (64 or 128); the register use is the
+// largest bucket's.
+template <int MAXMP>
+__global__ __launch_bounds__(kHB) void gspat_iter_set(HoloArgs a, BucketSet s, uint32_t iters, int closing) {
+    __shared__ float2 s_p[2][kMaxMP];
+    uint32_t blk;
+    const uint32_t mp = set_bucket(s, a, blk);
+    if (mp == 16)
+        gspat_iter_run<16>(a, blk, iters, closing, s_p);
+    else if (mp == 32)
+        gspat_iter_run<32>(a, blk, iters, closing, s_p);
+    else if (MAXMP == 64 || mp == 64)
+        gspat_iter_run<64>(a, blk, iters, closing, s_p);
+    else if constexpr (MAXMP == 128)
+        gspat_iter_run<128>(a, blk, iters, closing, s_p);
+}
+
+template <int MAXMP, bool H16>
+__global__ __launch_bounds__(kHB) void gspat_q_set(HoloArgs a, BucketSet s, int closing, int pass) {
+    __shared__ QLds L;
+    uint32_t blk;
+    const uint32_t mp = set_bucket(s, a, blk);
+    if (mp == 16)
+        gspat_q_run<16, H16>(a, blk, closing, pass, L);
+    else if (mp == 32)
+        gspat_q_run<32, H16>(a, blk, closing, pass, L);
+    else if (MAXMP == 64 || mp == 64)
+        gspat_q_run<64, H16>(a, blk, closing, pass, L);
+    else if constexpr (MAXMP == 128)
+        gspat_q_run<128, H16>(a, blk, closing, pass, L);
 }
 
 // ---------------------------------------------------------------- GS
@@ -2094,11 +2179,13 @@ static hipError_t issue_chain(Context* ctx, const KChain& ch, hipStream_t st) {
     return ctx && ctx->graphs ? issue_graph(ctx, ch, st) : issue_direct(ch, st);
 }
 
-// The kernel chain of one GS-PAT bucket (A, B, C: at most four launches), kept
-// as records so that it can be issued on a stream launch by launch or as one
-// hipGraph (KChain, issue_chain).
+// The kernel chains of one GS-PAT bucket, kept as records so that each can be
+// issued on a stream launch by launch or as one hipGraph (KChain,
+// issue_chain): ch the kernel A launches (one or two), tail the iterations
+// and the final pass (two or three launches).
 template <int MP>
-static void record_gspat_bucket(KChain& ch, const HoloArgs& a, int src, int precision, uint32_t nb, uint32_t iters) {
+static void record_gspat_bucket(KChain& ch, KChain& tail, const HoloArgs& a, int src, int precision, uint32_t nb,
+                                uint32_t iters) {
     const dim3 grid(nb), grid_a(nb * a.split), blk(kHB);
     const dim3 grid_g(a.n_packed / kHB + (a.n_packed % kHB ? 1 : 0), nb);
     const bool t4 = src == kT4010A1;
@@ -2129,12 +2216,12 @@ static void record_gspat_bucket(KChain& ch, const HoloArgs& a, int src, int prec
         }
     }
     const int closing = precision == AFH_HOLO_MIXED_CLOSING ? 1 : 0;
-    ch.add(kfn(gspat_iter<MP>), grid, blk, a, iters, closing);
-    if (closing) ch.add(kfn(gspat_q<MP, false>), grid_a, blk, a, closing, (int)kQGamma);
+    tail.add(kfn(gspat_iter<MP>), grid, blk, a, iters, closing);
+    if (closing) tail.add(kfn(gspat_q<MP, false>), grid_a, blk, a, closing, (int)kQGamma);
     if (precision == AFH_HOLO_MIXED)
-        ch.add(kfn(gspat_q<MP, true>), grid_a, blk, a, closing, (int)kQFinal);
+        tail.add(kfn(gspat_q<MP, true>), grid_a, blk, a, closing, (int)kQFinal);
     else
-        ch.add(kfn(gspat_q<MP, false>), grid_a, blk, a, closing, (int)kQFinal);
+        tail.add(kfn(gspat_q<MP, false>), grid_a, blk, a, closing, (int)kQFinal);
 }
 
 template <int MP>
@@ -2268,8 +2355,10 @@ bool Context::ensure_aux() {
 }
 
 // Batches up to this many instances run their buckets side by side (they are
-// independent); larger ones fill the GPU bucket by bucket, where a second
-// stream only queued behind the first and added cross-stream waits.
+// independent). Larger GS-PAT batches run the buckets' kernel A in turn, each
+// filling the GPU, then kernels B and C once for all buckets
+// (gspat_iter_set, gspat_q_set): one launch each instead of one per bucket,
+// the small buckets' latency-bound iterations under the large bucket's.
 constexpr uint32_t kForkMaxBatch = 256;
 
 // Launches the solver on a workspace whose offsets, instance lists and Greedy
@@ -2302,10 +2391,39 @@ static int32_t run_holo(const HoloBatch& hb, const HoloPlan& pl, char* wbase, co
     int buckets = 0;
     for (int q = 0; q < 4; ++q) buckets += pl.nb[q] > 0;
     const bool fork = ctx && buckets > 1 && B <= kForkMaxBatch && (hb.algo == kAlgoGSPAT || hb.algo == kAlgoGS);
+    const bool merged = buckets > 1 && B > kForkMaxBatch && hb.algo == kAlgoGSPAT;
     if (fork && (!ctx->ensure_aux() || !hip_ok(hipEventRecord(ctx->fork, st), "hipEventRecord"))) return AFH_ERR_HIP;
+    BucketSet set_b, set_c;  // merged: the buckets' workgroup ranges for kernels B and C
+    memset(&set_b, 0, sizeof(set_b));
+    memset(&set_c, 0, sizeof(set_c));
     const hipStream_t caller = st;
-    size_t list_pos = 0, r_pos = 0, p_pos = 0, g_pos = 0, q_pos = 0, d_pos = 0;
-    for (int q = 0; q < 4; ++q) {
+    // each bucket's workspace ranges, in bucket order
+    size_t list_at[4], r_at[4], p_at[4], g_at[4], q_at[4], d_at[4];
+    {
+        size_t list_pos = 0, r_pos = 0, p_pos = 0, g_pos = 0, q_pos = 0, d_pos = 0;
+        for (int q = 0; q < 4; ++q) {
+            const size_t nb = pl.nb[q], mp = kBucketMP[q];
+            list_at[q] = list_pos;
+            r_at[q] = r_pos;
+            p_at[q] = p_pos;
+            g_at[q] = g_pos;
+            q_at[q] = q_pos;
+            d_at[q] = d_pos;
+            list_pos += nb;
+            if (hb.algo == kAlgoGSPAT) {
+                r_pos += nb * mp * mp * a.split;
+                p_pos += nb * mp;
+                g_pos += nb * mp * np;
+                d_pos += nb * np;
+                q_pos += nb * mp * a.split;
+            } else if (hb.algo == kAlgoGS) {
+                g_pos += nb * mp * np;
+                q_pos += nb * np;
+            }
+        }
+    }
+    for (int qi = 0; qi < 4; ++qi) {
+        const int q = merged ? 3 - qi : qi;  // merged: largest buckets first
         const uint32_t nb = pl.nb[q];
         const int mp = kBucketMP[q];
         if (nb == 0) continue;
@@ -2313,13 +2431,13 @@ static int32_t run_holo(const HoloBatch& hb, const HoloPlan& pl, char* wbase, co
             st = ctx->aux[q];
             if (!hip_ok(hipStreamWaitEvent(st, ctx->fork, 0), "hipStreamWaitEvent")) return AFH_ERR_HIP;
         }
-        a.inst = reinterpret_cast<const uint32_t*>(wbase + pl.o_list) + list_pos;
-        a.R = reinterpret_cast<float2*>(wbase + pl.o_r) + r_pos;
-        a.p = reinterpret_cast<float2*>(wbase + pl.o_p) + p_pos;
-        a.G = reinterpret_cast<float2*>(wbase + pl.o_g) + g_pos;
-        a.q = reinterpret_cast<float2*>(wbase + pl.o_q) + q_pos;
-        a.d = reinterpret_cast<float*>(wbase + pl.o_d) + d_pos;
-        a.gam = reinterpret_cast<float2*>(wbase + pl.o_q) + q_pos;
+        a.inst = reinterpret_cast<const uint32_t*>(wbase + pl.o_list) + list_at[q];
+        a.R = reinterpret_cast<float2*>(wbase + pl.o_r) + r_at[q];
+        a.p = reinterpret_cast<float2*>(wbase + pl.o_p) + p_at[q];
+        a.G = reinterpret_cast<float2*>(wbase + pl.o_g) + g_at[q];
+        a.q = reinterpret_cast<float2*>(wbase + pl.o_q) + q_at[q];
+        a.d = reinterpret_cast<float*>(wbase + pl.o_d) + d_at[q];
+        a.gam = reinterpret_cast<float2*>(wbase + pl.o_q) + q_at[q];
         hipError_t e = hipSuccess;
         if (hb.algo == kAlgoGSPAT) {
             // kernel C reads an instance's G through 32-bit buffer offsets
@@ -2327,19 +2445,31 @@ static int32_t run_holo(const HoloBatch& hb, const HoloPlan& pl, char* wbase, co
                 set_error("holo: GS-PAT supports at most 2^32 / (8 * 128) padded sources");
                 return AFH_ERR_UNSUPPORTED;
             }
-            KChain ch;
+            KChain ch, tail;
             switch (mp) {
-                case 16: record_gspat_bucket<16>(ch, a, hb.source_type, hb.precision, nb, hb.iterations); break;
-                case 32: record_gspat_bucket<32>(ch, a, hb.source_type, hb.precision, nb, hb.iterations); break;
-                case 64: record_gspat_bucket<64>(ch, a, hb.source_type, hb.precision, nb, hb.iterations); break;
-                default: record_gspat_bucket<128>(ch, a, hb.source_type, hb.precision, nb, hb.iterations); break;
+                case 16: record_gspat_bucket<16>(ch, tail, a, hb.source_type, hb.precision, nb, hb.iterations); break;
+                case 32: record_gspat_bucket<32>(ch, tail, a, hb.source_type, hb.precision, nb, hb.iterations); break;
+                case 64: record_gspat_bucket<64>(ch, tail, a, hb.source_type, hb.precision, nb, hb.iterations); break;
+                default:
+                    record_gspat_bucket<128>(ch, tail, a, hb.source_type, hb.precision, nb, hb.iterations);
+                    break;
             }
             e = issue_chain(ctx, ch, st);
-            r_pos += (size_t)nb * mp * mp * a.split;
-            p_pos += (size_t)nb * mp;
-            g_pos += (size_t)nb * mp * np;
-            d_pos += (size_t)nb * np;
-            q_pos += (size_t)nb * mp * a.split;
+            if (merged) {
+                for (BucketSet* bs : {&set_b, &set_c}) {
+                    const uint32_t u = bs->n++;
+                    bs->mp[u] = (uint32_t)mp;
+                    bs->first[u + 1] = bs->first[u] + (bs == &set_b ? nb : nb * a.split);
+                    bs->inst[u] = a.inst;
+                    bs->R[u] = a.R;
+                    bs->p[u] = a.p;
+                    bs->G[u] = a.G;
+                    bs->d[u] = a.d;
+                    bs->gam[u] = a.gam;
+                }
+            } else if (e == hipSuccess) {
+                e = issue_chain(ctx, tail, st);
+            }
         } else if (hb.algo == kAlgoGS) {
             const uint32_t gstride = (uint32_t)(mp * np);
             switch (mp) {
@@ -2348,14 +2478,37 @@ static int32_t run_holo(const HoloBatch& hb, const HoloPlan& pl, char* wbase, co
                 case 64: e = launch_gs_bucket<64>(a, hb.source_type, nb, hb.iterations, gstride, st); break;
                 default: e = launch_gs_bucket<128>(a, hb.source_type, nb, hb.iterations, gstride, st); break;
             }
-            g_pos += (size_t)nb * mp * np;
-            q_pos += (size_t)nb * np;
         }
         if (!hip_ok(e, "holo kernel launch")) return AFH_ERR_HIP;
-        list_pos += nb;
         if (fork && (!hip_ok(hipEventRecord(ctx->aux_done[q], st), "hipEventRecord") ||
                      !hip_ok(hipStreamWaitEvent(caller, ctx->aux_done[q], 0), "hipStreamWaitEvent")))
             return AFH_ERR_HIP;
+    }
+    if (merged) {
+        // kernels B and C of every bucket, as record_gspat_bucket's tail
+        const int closing = hb.precision == AFH_HOLO_MIXED_CLOSING ? 1 : 0;
+        const bool big = set_b.mp[0] == 128;  // the largest bucket is listed first
+        const dim3 gb(set_b.first[set_b.n]), gc(set_c.first[set_c.n]), blk(kHB);
+        const bool h16 = hb.precision == AFH_HOLO_MIXED;
+        if (big)
+            hipLaunchKernelGGL(gspat_iter_set<128>, gb, blk, 0, st, a, set_b, hb.iterations, closing);
+        else
+            hipLaunchKernelGGL(gspat_iter_set<64>, gb, blk, 0, st, a, set_b, hb.iterations, closing);
+        if (closing) {
+            if (big)
+                hipLaunchKernelGGL((gspat_q_set<128, false>), gc, blk, 0, st, a, set_c, closing, (int)kQGamma);
+            else
+                hipLaunchKernelGGL((gspat_q_set<64, false>), gc, blk, 0, st, a, set_c, closing, (int)kQGamma);
+        }
+        if (big && h16)
+            hipLaunchKernelGGL((gspat_q_set<128, true>), gc, blk, 0, st, a, set_c, closing, (int)kQFinal);
+        else if (big)
+            hipLaunchKernelGGL((gspat_q_set<128, false>), gc, blk, 0, st, a, set_c, closing, (int)kQFinal);
+        else if (h16)
+            hipLaunchKernelGGL((gspat_q_set<64, true>), gc, blk, 0, st, a, set_c, closing, (int)kQFinal);
+        else
+            hipLaunchKernelGGL((gspat_q_set<64, false>), gc, blk, 0, st, a, set_c, closing, (int)kQFinal);
+        if (!hip_ok(hipGetLastError(), "holo kernel launch")) return AFH_ERR_HIP;
     }
     st = caller;
     if (hb.algo == kAlgoGreedy) {
