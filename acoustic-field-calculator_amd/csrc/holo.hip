@@ -722,7 +722,7 @@ __device__ __forceinline__ floatx4 mfma16(const uintx4& a, const uintx4& b, floa
 }
 
 template <int SRC, int MP, bool G32>
-__global__ __launch_bounds__(4 * MP) void gspat_fused(HoloArgs a) {
+__global__ __launch_bounds__(4 * MP, 4) void gspat_fused(HoloArgs a) {
     constexpr int T = MP / 16;
     constexpr int NTW = FusedTiles<T>::kMax;
     constexpr int RS = kKC + 8;  // LDS row stride in halves (80 bytes)
@@ -1670,9 +1670,7 @@ __device__ __forceinline__ uint32_t set_bucket(const BucketSet& s, HoloArgs& a, 
     return s.mp[u];
 }
 
-// MAXMP: the largest bucket of the set (64 or 128); the register use is the
-// largest bucket's.
-template <int MAXMP>
+// Buckets of MP <= 64 only (one thread count, the register use of MP = 64).
 __global__ __launch_bounds__(kHB) void gspat_iter_set(HoloArgs a, BucketSet s, uint32_t iters, int closing) {
     __shared__ float2 s_p[2][kMaxMP];
     uint32_t blk;
@@ -1681,13 +1679,11 @@ __global__ __launch_bounds__(kHB) void gspat_iter_set(HoloArgs a, BucketSet s, u
         gspat_iter_run<16>(a, blk, iters, closing, s_p);
     else if (mp == 32)
         gspat_iter_run<32>(a, blk, iters, closing, s_p);
-    else if (MAXMP == 64 || mp == 64)
+    else
         gspat_iter_run<64>(a, blk, iters, closing, s_p);
-    else if constexpr (MAXMP == 128)
-        gspat_iter_run<128>(a, blk, iters, closing, s_p);
 }
 
-template <int MAXMP, bool H16>
+template <bool H16>
 __global__ __launch_bounds__(kHB) void gspat_q_set(HoloArgs a, BucketSet s, int closing, int pass) {
     __shared__ QLds L;
     uint32_t blk;
@@ -1696,10 +1692,8 @@ __global__ __launch_bounds__(kHB) void gspat_q_set(HoloArgs a, BucketSet s, int 
         gspat_q_run<16, H16>(a, blk, closing, pass, L);
     else if (mp == 32)
         gspat_q_run<32, H16>(a, blk, closing, pass, L);
-    else if (MAXMP == 64 || mp == 64)
+    else
         gspat_q_run<64, H16>(a, blk, closing, pass, L);
-    else if constexpr (MAXMP == 128)
-        gspat_q_run<128, H16>(a, blk, closing, pass, L);
 }
 
 // ---------------------------------------------------------------- GS
@@ -2356,9 +2350,11 @@ bool Context::ensure_aux() {
 
 // Batches up to this many instances run their buckets side by side (they are
 // independent). Larger GS-PAT batches run the buckets' kernel A in turn, each
-// filling the GPU, then kernels B and C once for all buckets
+// filling the GPU, then kernels B and C once for all buckets up to MP = 64
 // (gspat_iter_set, gspat_q_set): one launch each instead of one per bucket,
-// the small buckets' latency-bound iterations under the large bucket's.
+// the small buckets' latency-bound iterations under the large bucket's. The
+// MP = 128 bucket keeps its per-bucket chain: its kernels B and C hold 166-168
+// VGPRs, which in a merged launch would set every bucket's occupancy.
 constexpr uint32_t kForkMaxBatch = 256;
 
 // Launches the solver on a workspace whose offsets, instance lists and Greedy
@@ -2391,7 +2387,9 @@ static int32_t run_holo(const HoloBatch& hb, const HoloPlan& pl, char* wbase, co
     int buckets = 0;
     for (int q = 0; q < 4; ++q) buckets += pl.nb[q] > 0;
     const bool fork = ctx && buckets > 1 && B <= kForkMaxBatch && (hb.algo == kAlgoGSPAT || hb.algo == kAlgoGS);
-    const bool merged = buckets > 1 && B > kForkMaxBatch && hb.algo == kAlgoGSPAT;
+    int small_buckets = 0;  // buckets that kernels B and C of a merged batch take together
+    for (int q = 0; q < 4; ++q) small_buckets += pl.nb[q] > 0 && kBucketMP[q] <= 64;
+    const bool merged = small_buckets > 1 && B > kForkMaxBatch && hb.algo == kAlgoGSPAT;
     if (fork && (!ctx->ensure_aux() || !hip_ok(hipEventRecord(ctx->fork, st), "hipEventRecord"))) return AFH_ERR_HIP;
     BucketSet set_b, set_c;  // merged: the buckets' workgroup ranges for kernels B and C
     memset(&set_b, 0, sizeof(set_b));
@@ -2455,7 +2453,7 @@ static int32_t run_holo(const HoloBatch& hb, const HoloPlan& pl, char* wbase, co
                     break;
             }
             e = issue_chain(ctx, ch, st);
-            if (merged) {
+            if (merged && mp <= 64) {
                 for (BucketSet* bs : {&set_b, &set_c}) {
                     const uint32_t u = bs->n++;
                     bs->mp[u] = (uint32_t)mp;
@@ -2487,27 +2485,13 @@ static int32_t run_holo(const HoloBatch& hb, const HoloPlan& pl, char* wbase, co
     if (merged) {
         // kernels B and C of every bucket, as record_gspat_bucket's tail
         const int closing = hb.precision == AFH_HOLO_MIXED_CLOSING ? 1 : 0;
-        const bool big = set_b.mp[0] == 128;  // the largest bucket is listed first
         const dim3 gb(set_b.first[set_b.n]), gc(set_c.first[set_c.n]), blk(kHB);
-        const bool h16 = hb.precision == AFH_HOLO_MIXED;
-        if (big)
-            hipLaunchKernelGGL(gspat_iter_set<128>, gb, blk, 0, st, a, set_b, hb.iterations, closing);
+        hipLaunchKernelGGL(gspat_iter_set, gb, blk, 0, st, a, set_b, hb.iterations, closing);
+        if (closing) hipLaunchKernelGGL(gspat_q_set<false>, gc, blk, 0, st, a, set_c, closing, (int)kQGamma);
+        if (hb.precision == AFH_HOLO_MIXED)
+            hipLaunchKernelGGL(gspat_q_set<true>, gc, blk, 0, st, a, set_c, closing, (int)kQFinal);
         else
-            hipLaunchKernelGGL(gspat_iter_set<64>, gb, blk, 0, st, a, set_b, hb.iterations, closing);
-        if (closing) {
-            if (big)
-                hipLaunchKernelGGL((gspat_q_set<128, false>), gc, blk, 0, st, a, set_c, closing, (int)kQGamma);
-            else
-                hipLaunchKernelGGL((gspat_q_set<64, false>), gc, blk, 0, st, a, set_c, closing, (int)kQGamma);
-        }
-        if (big && h16)
-            hipLaunchKernelGGL((gspat_q_set<128, true>), gc, blk, 0, st, a, set_c, closing, (int)kQFinal);
-        else if (big)
-            hipLaunchKernelGGL((gspat_q_set<128, false>), gc, blk, 0, st, a, set_c, closing, (int)kQFinal);
-        else if (h16)
-            hipLaunchKernelGGL((gspat_q_set<64, true>), gc, blk, 0, st, a, set_c, closing, (int)kQFinal);
-        else
-            hipLaunchKernelGGL((gspat_q_set<64, false>), gc, blk, 0, st, a, set_c, closing, (int)kQFinal);
+            hipLaunchKernelGGL(gspat_q_set<false>, gc, blk, 0, st, a, set_c, closing, (int)kQFinal);
         if (!hip_ok(hipGetLastError(), "holo kernel launch")) return AFH_ERR_HIP;
     }
     st = caller;
