@@ -106,7 +106,7 @@ struct ChainGraph {
 };
 
 // Stream + grow-only device workspace; calls on one context are serialized.
-// A small batch runs its padded-focus buckets side by side on the context's
+// A small GS batch runs its padded-focus buckets side by side on the context's
 // auxiliary streams (created on first use, with cu_mask when one is set),
 // forked from and joined back into the caller's stream by events.
 struct Context {

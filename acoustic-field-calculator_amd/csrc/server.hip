@@ -213,9 +213,9 @@ struct Server {
     // starts on an idle one, so with two lanes its next batch runs beside the
     // previous one, and the batches of different arrays run side by side.
     // Without a CU partition there are 6 lanes, 2 per array; with one, 3 lanes,
-    // 1 per array: every CU-masked stream (a lane and its 4 auxiliary streams)
-    // holds a hardware queue of its own, and 6 lanes' 30 queues beside the
-    // field streams' were measured to stall the field requests for
+    // 1 per array: every CU-masked stream holds a hardware queue of its own,
+    // and when each lane also had 4 auxiliary streams, 6 lanes' 30 queues
+    // beside the field streams' were measured to stall the field requests for
     // milliseconds. A field request goes to the field stream with the least
     // queued work (all streams when there is only one).
     static constexpr int kGspatLanes = 6;
@@ -494,27 +494,38 @@ struct Server {
         }
     }
 
+    // The worker. Submissions go to `pending` under mu; the worker moves them
+    // to its own backlog (in arrival order) and works on that without the
+    // lock, so a submitter never waits behind a scan of the queued requests.
     void run() {
         (void)hipSetDevice(device);
+        std::deque<Request*> backlog;  // submitted and not yet launched, in arrival order
         std::vector<Request*> todo;
         for (;;) {
             {
                 std::unique_lock<std::mutex> g(mu);
-                if (pending.empty() && flights.empty()) {
+                if (pending.empty() && backlog.empty() && flights.empty()) {
                     cv_submit.wait_for(g, std::chrono::milliseconds(2), [&] { return stop || !pending.empty(); });
                 }
-                if (stop && pending.empty() && flights.empty()) return;
-                todo.assign(pending.begin(), pending.end());
+                if (stop && pending.empty() && backlog.empty() && flights.empty()) return;
+                if (backlog.empty()) {
+                    backlog.swap(pending);
+                } else {
+                    backlog.insert(backlog.end(), pending.begin(), pending.end());
+                    pending.clear();
+                }
             }
             std::vector<Request*> admitted;
-            if (!todo.empty()) dispatch(todo, admitted);
+            if (!backlog.empty()) {
+                todo.assign(backlog.begin(), backlog.end());
+                dispatch(todo, admitted);
+            }
             if (!admitted.empty()) {
                 // admitted requests are no longer kPending; keep the rest in order
-                std::lock_guard<std::mutex> g(mu);
                 std::deque<Request*> keep;
-                for (Request* r : pending)
+                for (Request* r : backlog)
                     if (r->state == kPending) keep.push_back(r);
-                pending.swap(keep);
+                backlog.swap(keep);
             }
             if (reap() == 0 && admitted.empty()) std::this_thread::sleep_for(std::chrono::microseconds(20));
         }
@@ -585,9 +596,6 @@ int32_t AFS_Create(const AFS_Config* cfg, void** out) {
     s->ctxs[0]->cu_mask = gmask;
     for (auto& c : s->gextra_ctx) c->cu_mask = gmask;
     for (int i = 0; i < s->gspat_lanes(); ++i) s->lane_ctx(i)->graphs = cfg->gspat_graphs != 0;
-    // the auxiliary streams of small batches now, not on the first fork while serving
-    for (int i = 0; i < s->gspat_lanes(); ++i)
-        if (!s->lane_ctx(i)->ensure_aux()) return AFH_ERR_HIP;
     // the GS-PAT staging for a full batch is reserved now rather than grown
     // while serving; the workspace, which depends on the array size, when an
     // array is added
