@@ -188,6 +188,8 @@ def _declare(d):
                                 POINTER(c_uint64)], ctypes.c_int32)
     _sig(d, "AFS_SubmitGSPAT", [c_void_p, c_uint32, c_void_p, c_void_p, c_uint32, c_void_p, c_void_p,
                                 POINTER(c_uint64)], ctypes.c_int32)
+    _sig(d, "AFS_SubmitGSPATBatch", [c_void_p, c_uint32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                     c_void_p], ctypes.c_int32)
     _sig(d, "AFS_Wait", [c_void_p, c_uint64, POINTER(ctypes.c_double)], ctypes.c_int32)
     _sig(d, "AFS_Stats", [c_void_p, ctypes.c_int32, POINTER(ctypes.c_double), POINTER(ctypes.c_double),
                           POINTER(c_uint64), ctypes.c_int32], ctypes.c_int32)

@@ -71,6 +71,28 @@ class Server:
         self._keep[t.value] = (out_amp, out_phase)
         return t.value
 
+    def submit_gspat_batch(self, array_ids, foci, amps, out_amp, out_phase) -> list:
+        """Several GS-PAT requests at once (AFS_SubmitGSPATBatch): request i is
+        (array_ids[i], foci[i], amps[i]) with results in out_amp[i] /
+        out_phase[i]. Returns the tickets in order."""
+        n = len(array_ids)
+        fs = [np.ascontiguousarray(f, np.float32) for f in foci]
+        as_ = [np.ascontiguousarray(a, np.float32) for a in amps]
+        ids = np.ascontiguousarray(array_ids, np.uint32)
+        m = np.array([a.shape[0] for a in as_], np.uint32)
+        ptr = lambda xs: np.array([x.ctypes.data for x in xs], np.uint64)
+        pf, pa, po, pp = ptr(fs), ptr(as_), ptr(out_amp), ptr(out_phase)
+        tickets = np.zeros(n, np.uint64)
+        rc = dll().AFS_SubmitGSPATBatch(self.handle, n, ids.ctypes.data_as(c_void_p), pf.ctypes.data_as(c_void_p),
+                                        pa.ctypes.data_as(c_void_p), m.ctypes.data_as(c_void_p),
+                                        po.ctypes.data_as(c_void_p), pp.ctypes.data_as(c_void_p),
+                                        tickets.ctypes.data_as(c_void_p))
+        _err("AFS_SubmitGSPATBatch", rc)
+        out = [int(t) for t in tickets]
+        for t, oa, op in zip(out, out_amp, out_phase):
+            self._keep[t] = (oa, op)
+        return out
+
     def wait(self, ticket: int) -> float:
         lat = c_double()
         rc = dll().AFS_Wait(self.handle, ticket, byref(lat))

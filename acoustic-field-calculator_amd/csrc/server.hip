@@ -788,6 +788,63 @@ int32_t AFS_SubmitGSPAT(void* srv, uint32_t array_id, const float* foci, const f
     return enqueue(s, r.release(), ticket);
 }
 
+int32_t AFS_SubmitGSPATBatch(void* srv, uint32_t count, const uint32_t* array_ids, const float* const* foci,
+                             const float* const* amps, const uint32_t* m, float* const* out_amp,
+                             float* const* out_phase, uint64_t* tickets) {
+    clear_error();
+    Server* s = static_cast<Server*>(srv);
+    if (!s || (count > 0 && (!array_ids || !foci || !amps || !m || !out_amp || !out_phase || !tickets))) {
+        set_error("AFS_SubmitGSPATBatch: bad argument");
+        return AFH_ERR_ARG;
+    }
+    std::vector<std::unique_ptr<Request>> rs(count);
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        for (uint32_t i = 0; i < count; ++i) {
+            if (!foci[i] || !amps[i] || !out_amp[i] || !out_phase[i] || m[i] == 0 || m[i] > 128) {
+                set_error("AFS_SubmitGSPATBatch: bad argument in request " + std::to_string(i) + " (1 to 128 foci)");
+                return AFH_ERR_ARG;
+            }
+            if (array_ids[i] >= s->arrays.size()) {
+                set_error("AFS_SubmitGSPATBatch: unknown array in request " + std::to_string(i));
+                return AFH_ERR_ARG;
+            }
+            rs[i].reset(new Request());
+            rs[i]->arr = s->arrays[array_ids[i]].get();
+        }
+    }
+    for (uint32_t i = 0; i < count; ++i) {
+        Request* r = rs[i].get();
+        const uint64_t need = (uint64_t)m[i] * 16 + 2ull * r->arr->n * 4 + 1024;
+        if (need > s->cfg.hbm_budget_bytes || need > s->cfg.pinned_budget_bytes) {
+            set_error("AFS_SubmitGSPATBatch: request " + std::to_string(i) + " does not fit the budget");
+            return AFH_ERR_BUDGET;
+        }
+        r->kind = kReqGSPAT;
+        r->array_id = array_ids[i];
+        r->m = m[i];
+        r->foci.assign(foci[i], foci[i] + 3 * (size_t)m[i]);
+        r->amps.assign(amps[i], amps[i] + m[i]);
+        r->out = out_amp[i];
+        r->out2 = out_phase[i];
+    }
+    {
+        // one lock and one wake-up for the whole batch, in the given order
+        std::lock_guard<std::mutex> g(s->mu);
+        const double t = now_ms();
+        for (uint32_t i = 0; i < count; ++i) {
+            Request* r = rs[i].release();
+            r->ticket = s->next_ticket++;
+            r->t_submit = t;
+            s->by_ticket[r->ticket] = r;
+            s->pending.push_back(r);
+            tickets[i] = r->ticket;
+        }
+    }
+    if (count > 0) s->cv_submit.notify_one();
+    return AFH_OK;
+}
+
 int32_t AFS_Wait(void* srv, uint64_t ticket, double* latency_ms) {
     clear_error();
     Server* s = static_cast<Server*>(srv);

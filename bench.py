@@ -168,7 +168,7 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=4.0, help="Poisson arrivals per rate, mixed stream")
     ap.add_argument("--serve-gspat-seconds", type=float, default=1.5, help="Poisson arrivals per rate, GS-PAT ramp")
     ap.add_argument("--serve-gspat-qps",
-                    default="10000,40000,80000,120000,160000,200000,240000,280000,320000,400000,500000,640000")
+                    default="10000,40000,80000,160000,240000,320000,400000,500000,640000,800000,1000000")
     ap.add_argument("--serve-gspat-3984-qps", default="10000,20000,40000,60000,80000,100000,120000,160000,200000")
     ap.add_argument("--serve-gspat-cus", type=int, default=32, help="CUs reserved for GS-PAT batches while serving")
     ap.add_argument("--serve-pageable", action="store_true",
@@ -313,8 +313,7 @@ def main():
                                       args.serve_gspat_cus, pinned=not args.serve_pageable)
         result["serve_gspat"] = bench_serve_gspat(afc, D, [float(q) for q in args.serve_gspat_qps.split(",")],
                                                   args.serve_gspat_seconds)
-        # the largest array alone: its knee is within the submitter's reach
-        # (~470K submissions/s), so the precision modes' capacities show
+        # the largest array alone: the most work per request
         result["serve_gspat_3984"] = bench_serve_gspat(
             afc, D, [float(q) for q in args.serve_gspat_3984_qps.split(",")], args.serve_gspat_seconds,
             precisions=(("mixed", 1, False), ("fp32", 0, False)), arrays=(2,))
@@ -445,7 +444,9 @@ def bench_serve_gspat(afc, D, qps_list, seconds,
     foci over each array, targets U[0.5, 1]) on the 249-, 996- and
     3984-transducer arrays (those listed in `arrays`) chosen uniformly, at a
     ramp of offered rates until p99 passes 20 ms or the submitter falls behind
-    (the highest rate with p99 <= 5 ms is reported); the same ramp with each
+    (the highest rate with p99 <= 5 ms is reported; the requests that have
+    arrived are submitted together with AFS_SubmitGSPATBatch, pass by pass);
+    the same ramp with each
     bucket's kernel chain launched as a hipGraph, and with the server in FP32
     mode; and the plain path, AFO_GSPAT one request per call, on requests drawn
     from the same distribution."""
@@ -471,6 +472,12 @@ def bench_serve_gspat(afc, D, qps_list, seconds,
     ring = 4096
     outs = [np.empty((ring, 2, a[0].shape[0]), np.float32) for a in arrs]
     dll = afc.dll()
+    # the pool and the output ring as raw addresses
+    pool_aid = np.array([p_[0] for p_ in pool], np.int64)
+    pool_m = np.array([p_[1] for p_ in pool], np.uint32)
+    pool_foci = np.array([p_[2].ctypes.data for p_ in pool], np.uint64)
+    pool_amps = np.array([p_[3].ctypes.data for p_ in pool], np.uint64)
+    out_ptr = np.array([[(o[i, 0].ctypes.data, o[i, 1].ctypes.data) for i in range(ring)] for o in outs], np.uint64)
     names = "/".join(("249", "996", "3984")[i] for i in arrays)
     out = {"workload": f"GS-PAT requests only: Poisson arrivals, M ~ U[2,128], arrays {names} uniform, "
                        "100 iterations; p99 over the requests of each rate", "runs": {}}
@@ -488,28 +495,37 @@ def bench_serve_gspat(afc, D, qps_list, seconds,
             srv.stats(reset=True)
             nreq = max(1, int(qps * seconds))
             arrivals = np.cumsum(rng.exponential(1.0 / qps, nreq))
-            args = []
-            for i in range(nreq):
-                aid, m, foci, amps = pool[i % len(pool)]
-                buf = outs[aid][i % ring]
-                args.append((ids[aid], foci.ctypes.data, amps.ctypes.data, m, buf[0].ctypes.data, buf[1].ctypes.data))
-            tk = c_uint64()
-            ref = byref(tk)
+            # per request: array, foci, targets, output buffers as raw
+            # addresses (AFS_SubmitGSPATBatch takes arrays of them)
+            sel = np.arange(nreq) % len(pool)
+            which = pool_aid[sel]
+            a_id = np.array(ids, np.uint32)[which]
+            a_m = pool_m[sel]
+            a_foci = pool_foci[sel]
+            a_amps = pool_amps[sel]
+            slot = np.arange(nreq) % ring
+            a_oa = np.ascontiguousarray(out_ptr[which, slot, 0])
+            a_op = np.ascontiguousarray(out_ptr[which, slot, 1])
+            tickets = np.zeros(nreq, np.uint64)
+            base = [x.ctypes.data for x in (a_id, a_foci, a_amps, a_m, a_oa, a_op, tickets)]
             h = srv.handle
-            submit = dll.AFS_SubmitGSPAT
-            tickets = []
+            submit = dll.AFS_SubmitGSPATBatch
+            i = 0
             t0 = time.perf_counter()
-            for i in range(nreq):
-                target = t0 + arrivals[i]
-                while time.perf_counter() < target:
-                    pass
-                aid_, fp, ap, m, oa, op = args[i]
-                if submit(h, aid_, fp, ap, m, oa, op, ref) != 0:
-                    raise RuntimeError("AFS_SubmitGSPAT failed")
-                tickets.append(tk.value)
+            # every pass submits the requests that have arrived by now, in
+            # one call: the Poisson schedule is kept to the pass time (a few
+            # microseconds), and one Python call per request (which topped
+            # out near 470K submissions/s) does not bound the offered rate
+            while i < nreq:
+                j = int(np.searchsorted(arrivals, time.perf_counter() - t0, side="right"))
+                if j > i:
+                    if submit(h, j - i, base[0] + 4 * i, base[1] + 8 * i, base[2] + 8 * i, base[3] + 4 * i,
+                              base[4] + 8 * i, base[5] + 8 * i, base[6] + 8 * i) != 0:
+                        raise RuntimeError("AFS_SubmitGSPATBatch failed")
+                    i = j
             t_sub = time.perf_counter() - t0
             lat = c_double()
-            for t in tickets:
+            for t in tickets.tolist():
                 dll.AFS_Wait(h, t, byref(lat))
             elapsed = time.perf_counter() - t0
             p50, p99, done = srv.stats(kind=1)

@@ -358,6 +358,15 @@ int32_t AFS_SubmitField(void* srv, uint32_t array_id, const AFH_PointSet* pts, i
  * out_phase (n floats each) when the request completes. */
 int32_t AFS_SubmitGSPAT(void* srv, uint32_t array_id, const float* foci, const float* amps, uint32_t m,
                         float* out_amp, float* out_phase, uint64_t* ticket);
+/* count GS-PAT requests at once (request i as AFS_SubmitGSPAT with
+ * array_ids[i], foci[i], amps[i], m[i], out_amp[i], out_phase[i]; its ticket
+ * in tickets[i]), queued in order under one lock with one wake-up of the
+ * worker. All or nothing: an invalid request rejects the batch before any is
+ * queued. No counterpart in the reference (its optimizers are called one
+ * problem at a time); for clients that hold several requests at once. */
+int32_t AFS_SubmitGSPATBatch(void* srv, uint32_t count, const uint32_t* array_ids, const float* const* foci,
+                             const float* const* amps, const uint32_t* m, float* const* out_amp,
+                             float* const* out_phase, uint64_t* tickets);
 /* Blocks until the request completes; returns its status and latency
  * (submit to result in the caller's buffer). Each ticket is waited once. */
 int32_t AFS_Wait(void* srv, uint64_t ticket, double* latency_ms);

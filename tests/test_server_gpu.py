@@ -286,3 +286,51 @@ def test_pinned_and_pageable_results_agree(afc, oracle, torch_cuda):
             got = pin.numpy()
             assert np.all(np.isfinite(got)) and np.array_equal(got, page)
     srv.close()
+
+
+def test_batch_submit_equals_single(afc, oracle, torch_cuda):
+    """AFS_SubmitGSPATBatch: requests queued at once give the same bits as the
+    same requests submitted and waited one by one, and a batch holding an
+    invalid request is rejected before any of it is queued."""
+    from afc.server import Server
+    arrs = arrays(oracle, ((1, 1), (2, 2), (4, 4)))
+    rng = np.random.default_rng(23)
+    ids_l, foci_l, amps_l = [], [], []
+    for i in range(24):
+        aid = i % 3
+        m = int(rng.choice([2, 16, 17, 33, 64, 65, 128]))
+        span = (170, 130) if aid == 0 else ((360, 280) if aid == 1 else (730, 570))
+        foci_l.append(np.stack([rng.uniform(0, span[0], m), rng.uniform(0, span[1], m), rng.uniform(100, 250, m)],
+                               1).astype(np.float32))
+        amps_l.append(rng.uniform(0.5, 1.0, m).astype(np.float32))
+        ids_l.append(aid)
+    log(f"case: batched vs single GS-PAT submission, {len(ids_l)} requests on 249/996/3984")
+    srv = Server(hbm_budget=256 << 20, pinned_budget=64 << 20, n_streams=2, max_gspat_batch=64, gspat_cus=0,
+                 gspat_precision=1)
+    ids = [srv.add_array(1, *a) for a in arrs]
+    arr_n = [a[0].shape[0] for a in arrs]
+    want = []
+    for aid, f, a in zip(ids_l, foci_l, amps_l):
+        oa, op = np.empty(arr_n[aid], np.float32), np.empty(arr_n[aid], np.float32)
+        srv.wait(srv.submit_gspat(ids[aid], f, a, oa, op))
+        want.append((oa, op))
+    got_a = [np.empty(arr_n[aid], np.float32) for aid in ids_l]
+    got_p = [np.empty(arr_n[aid], np.float32) for aid in ids_l]
+    tickets = srv.submit_gspat_batch([ids[aid] for aid in ids_l], foci_l, amps_l, got_a, got_p)
+    assert len(tickets) == len(ids_l) and tickets == sorted(tickets)
+    for t in tickets:
+        srv.wait(t)
+    for i in range(len(ids_l)):
+        assert np.array_equal(want[i][0], got_a[i]) and np.array_equal(want[i][1], got_p[i]), i
+    # one request with 129 foci: nothing of the batch is queued
+    bad_f = foci_l[:2] + [np.zeros((129, 3), np.float32)]
+    bad_a = amps_l[:2] + [np.ones(129, np.float32)]
+    from afc import AFCError
+    with pytest.raises(AFCError):
+        srv.submit_gspat_batch([ids[0], ids[1], ids[2]], bad_f, bad_a, got_a[:3], got_p[:3])
+    assert srv.stats(kind=1)[2] >= len(ids_l)
+    done_before = srv.stats(kind=1)[2]
+    t = srv.submit_gspat(ids[0], foci_l[0], amps_l[0], got_a[0], got_p[0])
+    srv.wait(t)
+    assert srv.stats(kind=1)[2] == done_before + 1
+    srv.close()
