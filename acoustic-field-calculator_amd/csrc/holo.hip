@@ -2023,17 +2023,22 @@ __global__ __launch_bounds__(kHB) void greedy_kernel(HoloArgs a, const float2* _
 // instance is computed the same way in any batch. One workgroup per instance
 // leaves most of the chip idle or waiting on a tail round (R's chunk loop is
 // serial, with four barriers per chunk).
-static uint32_t gspat_split(uint32_t n_packed) {
+// Kernel A / C workgroups per instance (source-chunk ranges) of a bucket: a
+// function of the array and the bucket only, so an instance's result does
+// not depend on its batch. MP = 128 takes twice the ranges (at most 8): its
+// workgroups of 8 waves are the longest, and a small batch of them (as a
+// server forms) is otherwise latency-bound on their chunk loop.
+static uint32_t gspat_split(uint32_t n_packed, int mp) {
     const uint32_t chunks = n_packed / kKC;
     uint32_t s = 1;
     while (s < 8 && 2 * s * 16 <= chunks) s *= 2;
-    return s;
+    return mp == 128 ? std::min<uint32_t>(8, 2 * s) : s;
 }
 
 size_t gspat_workspace_bound(uint32_t n_packed, uint32_t batch) {
     // holo_solve's layout with every instance in the MP = 128 bucket
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    const size_t B = batch, np = n_packed, mp = 128, sp = gspat_split(n_packed);
+    const size_t B = batch, np = n_packed, mp = 128, sp = gspat_split(n_packed, 128);
     return al((B + 1) * sizeof(uint32_t)) + al(B * sizeof(uint32_t)) + al(np * AFH_SOURCE_STRIDE * sizeof(float)) +
            al(B * mp * mp * sp * sizeof(float2)) + al(B * mp * sizeof(float2)) + al(B * mp * np * sizeof(float2)) +
            al(B * mp * sp * sizeof(float2)) + al(B * np * sizeof(float));
@@ -2289,11 +2294,11 @@ static int32_t plan_holo(const HoloBatch& hb, HoloPlan& pl) {
     for (int q = 0; q < 4; ++q) {
         const size_t nb = pl.nb[q], mp = kBucketMP[q];
         if (hb.algo == kAlgoGSPAT) {
-            r_elems += nb * mp * mp * gspat_split(np);
+            r_elems += nb * mp * mp * gspat_split(np, (int)mp);
             p_elems += nb * mp;
             g_elems += nb * mp * np;
             d_elems += nb * np;
-            q_elems += nb * mp * gspat_split(np);  // partial gammas
+            q_elems += nb * mp * gspat_split(np, (int)mp);  // partial gammas
         } else if (hb.algo == kAlgoGS) {
             g_elems += nb * mp * np;
             q_elems += nb * np;
@@ -2381,7 +2386,7 @@ static int32_t run_holo(const HoloBatch& hb, const HoloPlan& pl, char* wbase, co
     a.cells = cells;
     a.out_amp = hb.out_amp;
     a.out_phase = hb.out_phase;
-    a.split = gspat_split(np);
+    a.split = gspat_split(np, 64);  // per GS-PAT bucket below
     if (!a.cells) {
         set_error("holo: directivity table allocation failed");
         return AFH_ERR_HIP;
@@ -2414,11 +2419,11 @@ static int32_t run_holo(const HoloBatch& hb, const HoloPlan& pl, char* wbase, co
             d_at[q] = d_pos;
             list_pos += nb;
             if (hb.algo == kAlgoGSPAT) {
-                r_pos += nb * mp * mp * a.split;
+                r_pos += nb * mp * mp * gspat_split(np, (int)mp);
                 p_pos += nb * mp;
                 g_pos += nb * mp * np;
                 d_pos += nb * np;
-                q_pos += nb * mp * a.split;
+                q_pos += nb * mp * gspat_split(np, (int)mp);
             } else if (hb.algo == kAlgoGS) {
                 g_pos += nb * mp * np;
                 q_pos += nb * np;
@@ -2443,6 +2448,7 @@ static int32_t run_holo(const HoloBatch& hb, const HoloPlan& pl, char* wbase, co
         a.gam = reinterpret_cast<float2*>(wbase + pl.o_q) + q_at[q];
         hipError_t e = hipSuccess;
         if (hb.algo == kAlgoGSPAT) {
+            a.split = gspat_split(np, mp);
             // kernel C reads an instance's G through 32-bit buffer offsets
             if ((uint64_t)mp * np * sizeof(float2) > 0xFFFFFFFFull) {
                 set_error("holo: GS-PAT supports at most 2^32 / (8 * 128) padded sources");
@@ -2490,6 +2496,7 @@ static int32_t run_holo(const HoloBatch& hb, const HoloPlan& pl, char* wbase, co
     if (merged) {
         // kernels B and C of every bucket, as record_gspat_bucket's tail
         const int closing = hb.precision == AFH_HOLO_MIXED_CLOSING ? 1 : 0;
+        a.split = gspat_split(np, 64);  // every bucket of the set
         const dim3 gb(set_b.first[set_b.n]), gc(set_c.first[set_c.n]), blk(kHB);
         hipLaunchKernelGGL(gspat_iter_set, gb, blk, 0, st, a, set_b, hb.iterations, closing);
         if (closing) hipLaunchKernelGGL(gspat_q_set<false>, gc, blk, 0, st, a, set_c, closing, (int)kQGamma);
