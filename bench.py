@@ -167,9 +167,10 @@ def main():
     ap.add_argument("--serve-qps", default="500,1000,2000,3000")
     ap.add_argument("--serve-seconds", type=float, default=4.0, help="Poisson arrivals per rate, mixed stream")
     ap.add_argument("--serve-gspat-seconds", type=float, default=1.5, help="Poisson arrivals per rate, GS-PAT ramp")
-    ap.add_argument("--serve-gspat-qps",
-                    default="10000,40000,80000,160000,240000,320000,400000,500000,640000,800000,1000000")
-    ap.add_argument("--serve-gspat-3984-qps", default="10000,20000,40000,60000,80000,100000,120000,160000,200000")
+    ap.add_argument("--serve-gspat-qps", default="10000,40000,80000,100000,120000,140000,160000,200000,240000,"
+                                                 "320000,400000,500000,640000,800000,1000000")
+    ap.add_argument("--serve-gspat-3984-qps",
+                    default="10000,20000,40000,60000,80000,90000,100000,110000,120000,140000,160000,200000")
     ap.add_argument("--serve-gspat-cus", type=int, default=32, help="CUs reserved for GS-PAT batches while serving")
     ap.add_argument("--serve-pageable", action="store_true",
                     help="mixed stream: field results into pageable buffers (default: pinned, DMA'd directly)")
