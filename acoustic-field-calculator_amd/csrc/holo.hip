@@ -33,8 +33,11 @@
 //   C  computes q = B p in FP32 from that copy and d_j (reading it back costs
 //      less than generating G again); in MIXED_CLOSING a first pass redoes the
 //      last gamma = R p exactly as G (B p) in FP32 (gspat_q).
-// A and C run a.split workgroups per instance over ranges of source chunks;
-// A's partial R and C's partial gamma are summed in a fixed order.
+// The buckets' kernels A run in turn; B and C run once for every bucket up to
+// MP = 64 (gspat_iter_set, gspat_q_set), the MP = 128 bucket on its own.
+// A and C run a.split workgroups per instance over ranges of source chunks
+// (a function of the array and the bucket); A's partial R and C's partial
+// gamma are summed in a fixed order.
 // Every reduction is per instance and in a fixed order, so an instance's result
 // does not depend on the other instances of the batch.
 #include <math.h>
@@ -1092,7 +1095,8 @@ __global__ __launch_bounds__(kHB, 3) void gspat_wave(HoloArgs a) {
                 const uint32_t jb = (c0 + cb) * kKC + h * KS + SPL * g;  // the lane's first column
                 // 1. terms; cs: the lane's rows in order
                 float re[T][SPL], im[T][SPL], cs[SPL];
-                lane_terms_live<SRC, T, G32>(n_live, s_src[bb], cb * kKC + h * KS + SPL * g, pf, wt, s_cells, re, im, cs);
+                lane_terms_live<SRC, T, G32>(n_live, s_src[bb], cb * kKC + h * KS + SPL * g, pf, wt, s_cells, re, im,
+                                             cs);
                 // 2. d_j over the 16 lanes of the group
                 float d[SPL];
 #pragma unroll
