@@ -735,7 +735,8 @@ __global__ __launch_bounds__(4 * MP, 4) void gspat_fused(HoloArgs a) {
     __shared__ __attribute__((aligned(16))) float s_part[T][kKC];  // per-wave column sums of |G|
     __shared__ __attribute__((aligned(16))) _Float16 s_b[4][MP * RS];  // hi re, hi im, lo re, lo im
     const int t = threadIdx.x;
-    const int w = t >> 6, l = t & 63, r16 = l & 15, g = l >> 4;
+    // the wave index as a scalar: the wave's tile list and liveness in SGPRs
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6), l = t & 63, r16 = l & 15, g = l >> 4;
     const uint32_t k = blockIdx.x / a.split, part = blockIdx.x % a.split, b = a.inst[k];
     const uint32_t n_chunks = a.n_packed / kKC;
     const uint32_t c_begin = part * n_chunks / a.split, c_end = (part + 1) * n_chunks / a.split;
@@ -759,7 +760,9 @@ __global__ __launch_bounds__(4 * MP, 4) void gspat_fused(HoloArgs a) {
     }
     const Split3 pf = split3(px, py, pz);
     const bool live = 16 * w < (int)M;  // wave-uniform
-    const float scale = fmaxf(1.0f, exp2f(floorf(log2f((float)M)) - 1.0f));
+    // uniform: kept in an SGPR
+    const float scale = __builtin_bit_cast(
+        float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, fmaxf(1.0f, exp2f(floorf(log2f((float)M)) - 1.0f)))));
     // the chunk's staged sources: thread t < 3 kKC loads float4 t of the chunk
     constexpr int kStageT = 3 * kKC;
     float4 pre = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -806,6 +809,9 @@ __global__ __launch_bounds__(4 * MP, 4) void gspat_fused(HoloArgs a) {
                 const SrcF sv = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
                 add_terms_exact<SRC, 1>(sv, split3(q0.x, q0.y, q0.z), &pf, s_cells->p, s_cells->q, &re[v], &im[v], &wt,
                                         &am[v]);
+                // one term at a time where the 4-wave bound would otherwise
+                // spill (the FP16-copy variant; the FP32-G one fits as is)
+                if constexpr (!G32) __builtin_amdgcn_sched_barrier(0);
             }
         }
         // 2. column sums over the wave's 16 rows, then over the waves
@@ -1281,6 +1287,10 @@ __global__ __launch_bounds__(kHB, 3) void gspat_pair(HoloArgs a) {
         accIm[s] = accRe[s];
     }
     float* d_out = a.d + (uint64_t)k * a.n_packed;
+    // kernel C's FP16 copy of this instance (MP n_packed words < 2^32 B, checked on the host)
+    const __amdgpu_buffer_rsrc_t gw = __builtin_amdgcn_make_buffer_rsrc(
+        reinterpret_cast<uint32_t*>(a.G) + (uint64_t)k * MP * a.n_packed, (short)0, (int)(MP * a.n_packed * 4u),
+        0x00020000);
     float4 pre[3];
     if (c_begin < c_end) {
         src_block_fetch(a, c_begin, pre);
@@ -1338,10 +1348,13 @@ __global__ __launch_bounds__(kHB, 3) void gspat_pair(HoloArgs a) {
                             for (int v = 0; v < SPL / 2; ++v)
                                 gp[v] = floatx4{re[q][2 * v], im[q][2 * v], re[q][2 * v + 1], im[q][2 * v + 1]};
                         } else {
-                            uintx4* hp = reinterpret_cast<uintx4*>(reinterpret_cast<uint32_t*>(a.G) +
-                                                                   ((uint64_t)k * MP + row) * a.n_packed + jb);
+                            // through the instance's buffer resource (a 32-bit offset per
+                            // lane instead of a 64-bit address: one VGPR, not two)
+                            const uint32_t off = ((uint32_t)row * a.n_packed + jb) * 4u;
 #pragma unroll
-                            for (int v = 0; v < SPL / 4; ++v) hp[v] = interleave_h8(hr[q], hi[q], v);
+                            for (int v = 0; v < SPL / 4; ++v)
+                                __builtin_amdgcn_raw_buffer_store_b128(interleave_h8(hr[q], hi[q], v), gw,
+                                                                       (int)(off + 16u * v), 0, 0);
                         }
                     }
                 }
